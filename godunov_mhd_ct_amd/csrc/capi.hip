@@ -1,0 +1,427 @@
+// capi.hip -- the extern "C" boundary (include/mhdct.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/mhdct.h"
+#include "ctx.hpp"
+
+using mhdct::Context;
+using mhdct::Mesh;
+
+struct mhdct_mesh
+{
+  Mesh m;
+};
+
+struct mhdct_ctx
+{
+  Context c;
+};
+
+namespace mhdct
+{
+int  fused_alloc(Context & c);
+void fused_free(Context & c);
+int  fused_step(Context & c, const double * U, double * U2, const double * B, double * B2, double dt,
+                hipStream_t s);
+} // namespace mhdct
+
+namespace
+{
+
+template <typename T>
+int
+upload(Context & c, T *& dst, const std::vector<T> & src)
+{
+  if (dst)
+    (void)hipFree(dst);
+  dst = nullptr;
+  const size_t bytes = (src.empty() ? 1 : src.size()) * sizeof(T);
+  hipError_t   e = hipMalloc(&dst, bytes);
+  if (e != hipSuccess)
+  {
+    c.err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return -1;
+  }
+  if (!src.empty())
+  {
+    e = hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+      c.err = std::string("hipMemcpy: ") + hipGetErrorString(e);
+      return -1;
+    }
+  }
+  c.mesh_bytes += bytes;
+  return 0;
+}
+
+void
+free_mesh(Context & c)
+{
+  void * ps[] = { c.dm.level, c.dm.nb, c.dm.conformal, c.dm.fix, c.dm.fix_src, c.dm.out_src, c.dm.out_dirs };
+  for (void * p : ps)
+    if (p)
+      (void)hipFree(p);
+  c.dm.level = nullptr;
+  c.dm.nb = nullptr;
+  c.dm.conformal = nullptr;
+  c.dm.fix = nullptr;
+  c.dm.fix_src = nullptr;
+  c.dm.out_src = nullptr;
+  c.dm.out_dirs = nullptr;
+  c.mesh_bytes = 0;
+}
+
+} // namespace
+
+extern "C"
+{
+
+  int
+  mhdct_mesh_create(int dim, int block_size, const int nbrick[3], const int periodic[3], int level,
+                    mhdct_mesh ** out)
+  {
+    if (!out)
+      return -1;
+    *out = nullptr;
+    if (dim != 2 && dim != 3)
+      return -1;
+    if (block_size < 4 || (block_size % 2) || level < 0 || level > 12)
+      return -1;
+    auto * h = new (std::nothrow) mhdct_mesh();
+    if (!h)
+      return -1;
+    h->m.dim = dim;
+    h->m.b = block_size;
+    for (int d = 0; d < 3; ++d)
+    {
+      h->m.nbrick[d] = d < dim ? (nbrick ? nbrick[d] : 1) : 1;
+      h->m.periodic[d] = d < dim ? (periodic ? (periodic[d] != 0) : 1) : 1;
+      if (h->m.nbrick[d] < 1)
+      {
+        delete h;
+        return -1;
+      }
+    }
+    h->m.uniform(level);
+    *out = h;
+    return 0;
+  }
+
+  int
+  mhdct_mesh_set_bc(mhdct_mesh * mesh, const int bc[6])
+  {
+    if (!mesh || !bc)
+      return -1;
+    for (int f = 0; f < 6; ++f)
+    {
+      if (bc[f] != 0 && bc[f] != 1 && bc[f] != 2 && bc[f] != 3)
+      {
+        mesh->m.err = "border condition must be one of NONE(0) PERIODIC(1) ZERO_GRADIENT(2) WALL(3)";
+        return -1;
+      }
+      mesh->m.bc[f] = bc[f];
+    }
+    return 0;
+  }
+
+  int
+  mhdct_mesh_adapt(mhdct_mesh * mesh, const int8_t * flags, int nleaves)
+  {
+    if (!mesh || (nleaves > 0 && !flags))
+      return -1;
+    std::vector<int8_t> f(flags, flags + nleaves);
+    return mesh->m.adapt(f) ? 0 : -1;
+  }
+
+  int
+  mhdct_mesh_balance(mhdct_mesh * mesh)
+  {
+    if (!mesh)
+      return -1;
+    mesh->m.balance();
+    return 0;
+  }
+
+  int
+  mhdct_mesh_finalize(mhdct_mesh * mesh)
+  {
+    if (!mesh)
+      return -1;
+    return mesh->m.finalize() ? 0 : -2;
+  }
+
+  int
+  mhdct_mesh_num_leaves(const mhdct_mesh * mesh)
+  {
+    return mesh ? int(mesh->m.leaves.size()) : -1;
+  }
+
+  int
+  mhdct_mesh_num_outside(const mhdct_mesh * mesh)
+  {
+    return mesh ? int(mesh->m.outside.size()) : -1;
+  }
+
+  int
+  mhdct_mesh_leaves(const mhdct_mesh * mesh, int * levels, int * coords)
+  {
+    if (!mesh)
+      return -1;
+    int n = 0;
+    for (auto & L : mesh->m.leaves)
+    {
+      if (levels)
+        levels[n] = L.level;
+      if (coords)
+      {
+        coords[3 * n] = L.x;
+        coords[3 * n + 1] = L.y;
+        coords[3 * n + 2] = L.z;
+      }
+      ++n;
+    }
+    for (auto & L : mesh->m.outside)
+    {
+      if (levels)
+        levels[n] = L.level;
+      if (coords)
+      {
+        coords[3 * n] = L.x;
+        coords[3 * n + 1] = L.y;
+        coords[3 * n + 2] = L.z;
+      }
+      ++n;
+    }
+    return 0;
+  }
+
+  int
+  mhdct_mesh_outside(const mhdct_mesh * mesh, int * src, int * dirs)
+  {
+    if (!mesh)
+      return -1;
+    for (size_t i = 0; i < mesh->m.outside.size(); ++i)
+    {
+      if (src)
+        src[i] = mesh->m.outside_src[i];
+      if (dirs)
+        for (int d = 0; d < 3; ++d)
+          dirs[3 * i + d] = mesh->m.outside_dirs[3 * i + d];
+    }
+    return 0;
+  }
+
+  const char *
+  mhdct_mesh_last_error(const mhdct_mesh * mesh)
+  {
+    return mesh ? mesh->m.err.c_str() : "null mesh handle";
+  }
+
+  void
+  mhdct_mesh_destroy(mhdct_mesh * mesh)
+  {
+    delete mesh;
+  }
+
+  int
+  mhdct_create(const mhdct_settings * s, int dim, const int block_size[3], int impl_version, mhdct_ctx ** out)
+  {
+    if (!out)
+      return -1;
+    *out = nullptr;
+    if (!s || !block_size || (dim != 2 && dim != 3))
+      return -1;
+    if (impl_version != MHDCT_IMPL_PLAIN && impl_version != MHDCT_IMPL_FUSED)
+      return -4;
+    if (s->riemann != 5 || s->prolongation != 0)
+      return -5;
+    for (int d = 1; d < dim; ++d)
+      if (block_size[d] != block_size[0])
+        return -6; // dx == dy == dz (ComputeDtMHDFunctor.cpp:98)
+    auto * h = new (std::nothrow) mhdct_ctx();
+    if (!h)
+      return -1;
+    Context & c = h->c;
+    c.dim = dim;
+    c.b = block_size[0];
+    c.impl = impl_version;
+    c.st.gamma0 = s->gamma0;
+    c.st.cfl = s->cfl;
+    c.st.slope_type = s->slope_type;
+    c.st.smallr = s->smallr;
+    c.st.smallc = s->smallc;
+    c.st.scaling = s->scaling_factor;
+    c.st.abort_when_negative_eint = s->abort_when_negative_eint;
+    if (hipGetDevice(&c.device) != hipSuccess)
+    {
+      delete h;
+      return -7;
+    }
+    if (hipMalloc(&c.red, 2 * sizeof(unsigned long long)) != hipSuccess)
+    {
+      delete h;
+      return -7;
+    }
+    (void)hipMemset(c.red, 0, 2 * sizeof(unsigned long long));
+    *out = h;
+    return 0;
+  }
+
+  int
+  mhdct_set_mesh(mhdct_ctx * h, const mhdct_mesh * mesh)
+  {
+    if (!h || !mesh)
+      return -1;
+    Context & c = h->c;
+    if (mesh->m.dim != c.dim || mesh->m.b != c.b)
+    {
+      c.err = "mesh dimension / block size differ from the context's";
+      return -1;
+    }
+    if (mesh->m.nb.empty() && !mesh->m.leaves.empty())
+    {
+      c.err = "mesh not finalized (call mhdct_mesh_finalize)";
+      return -1;
+    }
+    c.mesh = mesh->m;
+    c.has_mesh = true;
+    free_mesh(c);
+    c.dm.dim = c.dim;
+    c.dm.b = c.b;
+    c.dm.nowned = int(c.mesh.leaves.size());
+    c.dm.ntotal = c.mesh.ntotal();
+    c.dm.noutside = int(c.mesh.outside.size());
+    c.dm.nfix = int(c.mesh.emf_fix.size());
+    for (int f = 0; f < 6; ++f)
+      c.dm.bc[f] = c.mesh.bc[f];
+    std::vector<int> lv;
+    for (auto & L : c.mesh.leaves)
+      lv.push_back(L.level);
+    for (auto & L : c.mesh.outside)
+      lv.push_back(L.level);
+    if (upload(c, c.dm.level, lv) || upload(c, c.dm.nb, c.mesh.nb) || upload(c, c.dm.conformal, c.mesh.conformal) ||
+        upload(c, c.dm.fix, c.mesh.emf_fix) || upload(c, c.dm.fix_src, c.mesh.emf_src) ||
+        upload(c, c.dm.out_src, c.mesh.outside_src) || upload(c, c.dm.out_dirs, c.mesh.outside_dirs))
+      return -1;
+    int rc = 0;
+    if (c.impl == MHDCT_IMPL_PLAIN)
+      rc = mhdct::plain_alloc(c);
+    else
+      rc = mhdct::fused_alloc(c);
+    return rc;
+  }
+
+  uint64_t
+  mhdct_mem_bytes(const mhdct_ctx * h)
+  {
+    if (!h)
+      return 0;
+    return uint64_t(h->c.ps.bytes + h->c.mesh_bytes);
+  }
+
+  int
+  mhdct_do_time_step(mhdct_ctx * h, const double * U, double * U2, const double * B, double * B2, double dt,
+                     void * stream)
+  {
+    if (!h)
+      return -1;
+    Context & c = h->c;
+    if (!c.has_mesh)
+    {
+      c.err = "no mesh set";
+      return -1;
+    }
+    if (c.dm.nowned > 0 && (!U || !U2 || !B || !B2))
+    {
+      c.err = "null state pointer";
+      return -1;
+    }
+    if (!(dt > 0.0) || dt != dt)
+    {
+      c.err = "dt must be positive and finite";
+      return -1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (c.impl == MHDCT_IMPL_PLAIN)
+      return mhdct::plain_step(c, U, U2, B, B2, dt, s);
+    return mhdct::fused_step(c, U, U2, B, B2, dt, s);
+  }
+
+  int
+  mhdct_compute_dt(mhdct_ctx * h, const double * U, const double * B, double * dt_out, void * stream)
+  {
+    if (!h || !dt_out)
+      return -1;
+    Context & c = h->c;
+    if (!c.has_mesh)
+    {
+      c.err = "no mesh set";
+      return -1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long host[2] = { 0, 0 };
+    if (hipMemsetAsync(c.red, 0, sizeof(unsigned long long), s) != hipSuccess)
+      return -1;
+    if (mhdct::launch_inv_dt(c, U, B, s))
+      return -1;
+    if (hipMemcpyAsync(host, c.red, sizeof(host), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+    {
+      c.err = "dt readback failed";
+      return -1;
+    }
+    double inv;
+    std::memcpy(&inv, &host[0], sizeof(inv));
+    if (host[1] != 0)
+      (void)hipMemset(c.red + 1, 0, sizeof(unsigned long long));
+    *dt_out = c.st.cfl / inv;
+    if (host[1] & 2ull)
+    {
+      c.err = "NaN state met in the dt reduction";
+      return -3;
+    }
+    if (host[1] & 1ull)
+    {
+      c.err = "negative internal energy detected (abort_when_negative_eint)";
+      return -2;
+    }
+    return 0;
+  }
+
+  int
+  mhdct_fill_outside(mhdct_ctx * h, double * U, double * B, void * stream)
+  {
+    if (!h)
+      return -1;
+    if (!h->c.has_mesh)
+    {
+      h->c.err = "no mesh set";
+      return -1;
+    }
+    return mhdct::launch_fill_outside(h->c, U, B, static_cast<hipStream_t>(stream));
+  }
+
+  const char *
+  mhdct_last_error(const mhdct_ctx * h)
+  {
+    return h ? h->c.err.c_str() : "null context";
+  }
+
+  void
+  mhdct_destroy(mhdct_ctx * h)
+  {
+    if (!h)
+      return;
+    mhdct::plain_free(h->c);
+    mhdct::fused_free(h->c);
+    free_mesh(h->c);
+    if (h->c.red)
+      (void)hipFree(h->c.red);
+    delete h;
+  }
+}

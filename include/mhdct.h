@@ -1,0 +1,111 @@
+/*
+ * mhdct.h -- C ABI of the MI355X MHD-CT time step (libmhdct.so).
+ *
+ * The entry points replace the reference's GodunovImplemBase<dim,device_t> boundary
+ * (scheme/GodunovImplemBase.h:45-170) and the two solver-side kernels next to it:
+ *
+ *   mhdct_create        GodunovImplemBase ctor + SolverGodunovMHD::create_godunov_implem
+ *                       (scheme/GodunovImplemBase.h:54-73, SolverGodunovMHD.h:326-340)
+ *   mhdct_set_mesh      GodunovImplemV0::resize_auxiliary_data (scheme/GodunovImplemV0.cpp:40-75)
+ *                       after the host octree changed (SolverGodunovMHD.cpp:307)
+ *   mhdct_mem_bytes     GodunovImplemV0::total_mem_size_in_bytes (scheme/GodunovImplemV0.cpp:79-97)
+ *   mhdct_do_time_step  GodunovImplemV0::do_time_step (scheme/GodunovImplemBase.h:85-90,
+ *                       scheme/GodunovImplemV0.cpp:101-207)
+ *   mhdct_compute_dt    SolverGodunovMHD::compute_dt_local (SolverGodunovMHD.cpp:377-403,
+ *                       scheme/ComputeDtMHDFunctor.cpp:44-151)
+ *   mhdct_fill_outside  SolverGodunovMHD::fill_outside_quadrants (SolverGodunovMHD.cpp:1068-1088,
+ *                       border_conditions/FillOutside.cpp:47-395)
+ *
+ * The mesh entry points stand in for the p4est / kalypsso::core AMR manager the reference
+ * keeps on the host (refine, coarsen and 2:1 balance stay host-side).
+ *
+ * Data layout (both sides of the ABI, the reference's DataArrayBlock / FaceDataArrayBlock):
+ *   U[block][var][cell]      var = rho, E, rho*u, rho*v, rho*w; cell = i + b*(j + b*k)
+ *   B[block][face]           per block: the x-face grid (b+1) x b x b, then the y-face grid
+ *                            b x (b+1) x b, then the z-face grid b x b x (b+1), each x-fastest;
+ *                            in 2D: (b+1) x b, b x (b+1) and the cell-centred Bz b x b.
+ *   blocks are the owned leaves in mesh order followed by the outside blocks.
+ * All pointers passed to the step functions are device pointers (FP64).  Streams are
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Every function returns 0 on success and a negative code on error; the message is in
+ * mhdct_last_error / mhdct_mesh_last_error.  Nothing aborts the process.
+ */
+#ifndef MHDCT_H
+#define MHDCT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhdct_ctx  mhdct_ctx;
+typedef struct mhdct_mesh mhdct_mesh;
+
+/* [hydro] / [mhd] settings the hot path reads (models/HydroSettings.cpp:19-30). */
+typedef struct mhdct_settings
+{
+  double gamma0;         /* [hydro] gamma0, default 1.4 */
+  double cfl;            /* [hydro] cfl, default 0.5 */
+  double slope_type;     /* [hydro] slope_type: 1 minmod, 2 MC, default 1 */
+  double smallr;         /* [hydro] smallr, default 1e-10 */
+  double smallc;         /* [hydro] smallc, default 1e-10 (smallp = smallc^2/gamma0) */
+  double scaling_factor; /* [mesh] scaling_factor, default 1 */
+  int    riemann;        /* RiemannSolverType (models/riemann_solver_types.h): 5 = HLLD, the only one supported */
+  int    prolongation;   /* [amr] cell_centered_prolongation_type: 0 = SIMPLE_COPY, the only one supported */
+  int    abort_when_negative_eint; /* [hydro] abort_when_negative_eint: report eint < 0 as an error */
+} mhdct_settings;
+
+/* implementation_version values ([hydro] implementation_version, SolverGodunovMHD.cpp:76) */
+#define MHDCT_IMPL_PLAIN 10 /* one kernel per stage, drop-in for GodunovImplemV0 */
+#define MHDCT_IMPL_FUSED 11 /* fused per-block kernels */
+
+/* ---- host block octree (p4est brick forest replacement) ---------------------------- */
+
+/* Uniform forest at `level` over an nbrick[0] x nbrick[1] (x nbrick[2]) brick of trees. */
+int mhdct_mesh_create(int dim, int block_size, const int nbrick[3], const int periodic[3], int level,
+                      mhdct_mesh ** out);
+/* Border-condition type per face (XMIN,XMAX,YMIN,YMAX,ZMIN,ZMAX), BC_MHD enum values
+ * (border_conditions/FillOutside.h:55-61): 2 ZERO_GRADIENT, 3 WALL.  Only read for
+ * non-periodic directions. */
+int mhdct_mesh_set_bc(mhdct_mesh * mesh, const int bc[6]);
+/* Per-leaf flags: +1 refine, -1 coarsen (applied when all siblings ask), 0 keep. */
+int mhdct_mesh_adapt(mhdct_mesh * mesh, const int8_t * flags, int nleaves);
+/* 2:1 balance across faces, edges and corners. */
+int mhdct_mesh_balance(mhdct_mesh * mesh);
+/* Outside blocks and neighbour tables; call after the last adapt/balance. */
+int mhdct_mesh_finalize(mhdct_mesh * mesh);
+int mhdct_mesh_num_leaves(const mhdct_mesh * mesh);
+int mhdct_mesh_num_outside(const mhdct_mesh * mesh);
+/* levels[n] and coords[3n] (block units at that level) of owned leaves then outside blocks */
+int mhdct_mesh_leaves(const mhdct_mesh * mesh, int * levels, int * coords);
+/* source leaf and outward direction (3 ints) of every outside block */
+int mhdct_mesh_outside(const mhdct_mesh * mesh, int * src, int * dirs);
+const char * mhdct_mesh_last_error(const mhdct_mesh * mesh);
+void         mhdct_mesh_destroy(mhdct_mesh * mesh);
+
+/* ---- GodunovImplemBase ---------------------------------------------------------------- */
+
+int      mhdct_create(const mhdct_settings * settings, int dim, const int block_size[3], int impl_version,
+                      mhdct_ctx ** out);
+int      mhdct_set_mesh(mhdct_ctx * ctx, const mhdct_mesh * mesh);
+uint64_t mhdct_mem_bytes(const mhdct_ctx * ctx);
+/* U2 == U and B2 == B on entry (the solver's contract); on exit U2, B2 hold t + dt for the
+ * owned blocks.  Asynchronous on `stream`. */
+int mhdct_do_time_step(mhdct_ctx * ctx, const double * U, double * U2, const double * B, double * B2, double dt,
+                       void * stream);
+/* dt = cfl / max over owned cells of sum_d (|v_d| + c_fast,d) / dx.  Synchronises `stream`;
+ * returns -3 if a NaN state was met and -2 if eint < 0 was flagged by a previous step with
+ * abort_when_negative_eint set. */
+int mhdct_compute_dt(mhdct_ctx * ctx, const double * U, const double * B, double * dt_out, void * stream);
+/* Fill the outside blocks from their sources by the border conditions (no-op when periodic). */
+int mhdct_fill_outside(mhdct_ctx * ctx, double * U, double * B, void * stream);
+const char * mhdct_last_error(const mhdct_ctx * ctx);
+void         mhdct_destroy(mhdct_ctx * ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MHDCT_H */

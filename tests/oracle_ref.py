@@ -1,0 +1,128 @@
+"""Test-side binding of the CPU oracle (oracle/mhd_oracle.cpp, built into oracle/_build).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "_build" / "libmhd_oracle.so"
+
+_lib = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_LIB.exists():
+            build_oracle()
+        L = C.CDLL(str(ORACLE_LIB))
+        P = C.POINTER
+        L.oracle_create.restype = C.c_void_p
+        L.oracle_create.argtypes = [C.c_int, C.c_int, P(C.c_double), C.c_int, C.c_int, P(C.c_int), P(C.c_int),
+                                    P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_int]
+        L.oracle_destroy.argtypes = [C.c_void_p]
+        L.oracle_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_do_time_step.restype = C.c_int
+        L.oracle_do_time_step.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double),
+                                          C.c_double]
+        L.oracle_compute_inv_dt.restype = C.c_double
+        L.oracle_compute_inv_dt.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        L.oracle_fill_outside.restype = C.c_int
+        L.oracle_fill_outside.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_last_error.argtypes = [C.c_void_p]
+        L.oracle_riemann_hlld.argtypes = [P(C.c_double)] * 4
+        L.oracle_compute_emf.restype = C.c_double
+        L.oracle_compute_emf.argtypes = [P(C.c_double), P(C.c_double), C.c_int]
+        L.oracle_compute_primitives.argtypes = [P(C.c_double)] * 3
+        L.oracle_slope.restype = C.c_double
+        L.oracle_slope.argtypes = [C.c_double] * 4
+        L.oracle_speed_info.argtypes = [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a.ctypes.data_as(C.POINTER(C.c_int)), a
+
+
+def settings_array(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling=1.0,
+                   abort_eint=0):
+    return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint], dtype=np.float64)
+
+
+class OracleSolver:
+    """The oracle's GodunovImplemV0 restatement on a given mesh (leaves from the product's
+    host octree; the oracle does its own neighbour search on them)."""
+
+    def __init__(self, mesh, settings=None, nthreads=1, bc=None):
+        self.dim = mesh.dim
+        self.b = mesh.b
+        self.settings = settings if settings is not None else settings_array()
+        lv, co = mesh.leaves()
+        src, dirs = mesh.outside()
+        self.nowned = mesh.num_leaves
+        self.noutside = mesh.num_outside
+        keep = []
+        lvp, a1 = iptr(lv)
+        cop, a2 = iptr(co.reshape(-1))
+        nbp, a3 = iptr(mesh.nbrick)
+        pep, a4 = iptr(mesh.periodic)
+        bcv = bc if bc is not None else [0] * 6
+        bcp, a5 = iptr(bcv)
+        srcp, a6 = iptr(src if len(src) else np.zeros(1))
+        dirp, a7 = iptr(dirs.reshape(-1) if len(dirs) else np.zeros(3))
+        keep += [a1, a2, a3, a4, a5, a6, a7]
+        self._h = lib().oracle_create(self.dim, self.b, dptr(self.settings), self.nowned, self.noutside, lvp, cop,
+                                      nbp, pep, bcp, srcp, dirp, int(nthreads))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_destroy(self._h)
+            self._h = None
+
+    def set_threads(self, n):
+        lib().oracle_set_threads(self._h, int(n))
+
+    def do_time_step(self, U, U2, B, B2, dt):
+        rc = lib().oracle_do_time_step(self._h, dptr(U), dptr(U2), dptr(B), dptr(B2), float(dt))
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error(self._h).decode())
+
+    def inv_dt(self, U, B):
+        return lib().oracle_compute_inv_dt(self._h, dptr(U), dptr(B))
+
+    def compute_dt(self, U, B):
+        return self.settings[1] / self.inv_dt(U, B)
+
+    def fill_outside(self, U, B):
+        lib().oracle_fill_outside(self._h, dptr(U), dptr(B))
+
+    def step(self, U, B, dt=None):
+        """One solver step (SolverGodunovMHD::next_iteration_impl, HANCOCK): dt, outside
+        fill, U2 = U, B2 = B, do_time_step; returns (U2, B2, dt)."""
+        if self.noutside:
+            self.fill_outside(U, B)
+        if dt is None:
+            dt = self.compute_dt(U, B)
+        U2 = U.copy()
+        B2 = B.copy()
+        self.do_time_step(U, U2, B, B2, dt)
+        return U2, B2, dt
