@@ -236,7 +236,7 @@ extern "C"
     *out = nullptr;
     if (!s || !block_size || (dim != 2 && dim != 3))
       return -1;
-    if (impl_version != MHDCT_IMPL_PLAIN && impl_version != MHDCT_IMPL_FUSED)
+    if (impl_version != MHDCT_IMPL_PLAIN && impl_version != MHDCT_IMPL_FUSED && impl_version != MHDCT_IMPL_FUSED_FP32)
       return -4;
     if (s->riemann != 5 || s->prolongation != 0)
       return -5;

@@ -1,0 +1,271 @@
+// common.hpp -- device helpers shared by the kernel forms: index arithmetic of the reference
+// layout, the neighbour-table lookup and the face-B ghost rule.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ctx.hpp"
+
+namespace mhdct
+{
+
+// ---------------------------------------------------------------------------
+// index helpers (layout A0: block slowest, then variable, then cell)
+// ---------------------------------------------------------------------------
+struct Idx
+{
+  int dim, b;
+
+  __device__ long cells() const { return dim == 3 ? long(b) * b * b : long(b) * b; }
+  __device__ long cell(int i, int j, int k) const { return i + long(b) * (j + long(b) * k); }
+  __device__ long gn(int g) const { return b + 2 * g; }
+  __device__ long gcells(int g) const
+  {
+    const long n = b + 2 * g;
+    return dim == 3 ? n * n * n : n * n;
+  }
+  __device__ long gcell(int g, int i, int j, int k) const
+  {
+    const long n = b + 2 * g;
+    return (i + g) + n * ((j + g) + n * (dim == 3 ? (k + g) : 0));
+  }
+  __device__ long fsize(int c) const
+  {
+    if (dim == 2)
+      return c == 2 ? long(b) * b : long(b + 1) * b;
+    return long(b + 1) * b * b;
+  }
+  __device__ long foff(int c) const { return c == 0 ? 0 : (c == 1 ? fsize(0) : fsize(0) + fsize(1)); }
+  __device__ long fper() const { return fsize(0) + fsize(1) + fsize(2); }
+  __device__ long face(int c, int i, int j, int k) const
+  {
+    const long nx = c == 0 ? b + 1 : b;
+    const long ny = c == 1 ? b + 1 : b;
+    return foff(c) + i + nx * (j + ny * (dim == 3 ? k : 0));
+  }
+  __device__ long gfsize(int c) const
+  {
+    const long n = b + 4;
+    if (dim == 2)
+      return c == 2 ? n * n : (n + 1) * n;
+    return (n + 1) * n * n;
+  }
+  __device__ long gfoff(int c) const { return c == 0 ? 0 : (c == 1 ? gfsize(0) : gfsize(0) + gfsize(1)); }
+  __device__ long gfper() const { return gfsize(0) + gfsize(1) + gfsize(2); }
+  __device__ long gface(int c, int i, int j, int k) const
+  {
+    const long n = b + 4;
+    const long nx = c == 0 ? n + 1 : n;
+    const long ny = c == 1 ? n + 1 : n;
+    return gfoff(c) + (i + 2) + nx * ((j + 2) + ny * (dim == 3 ? (k + 2) : 0));
+  }
+  __device__ long ecells() const
+  {
+    const long n = b + 1;
+    return dim == 3 ? n * n * n : n * n;
+  }
+  __device__ long edge(int i, int j, int k) const
+  {
+    const long n = b + 1;
+    return i + n * (j + n * k);
+  }
+};
+
+struct Hit
+{
+  int oct;
+  int rel;
+  int l[3];
+};
+
+// Leaf holding ghost cell c of block `oct` (c outside [0,b)^dim) from the neighbour table.
+__device__ inline Hit
+neighbour(const DevMesh & m, int oct, const int c[3])
+{
+  const int b = m.b;
+  int       dir = 0, sub = 0, mul = 1;
+  for (int d = 0; d < m.dim; ++d)
+  {
+    const int dl = c[d] < 0 ? -1 : (c[d] >= b ? 1 : 0);
+    dir += (dl + 1) * mul;
+    mul *= 3;
+    if (dl == 0 && c[d] >= b / 2)
+      sub |= 1 << d;
+  }
+  const int     ndir = m.dim == 3 ? 27 : 9;
+  const int     nsub = 1 << m.dim;
+  const NbEntry e = m.nb[(long(oct) * ndir + dir) * nsub + sub];
+  Hit           h;
+  h.oct = e.oct;
+  h.rel = e.rel;
+  for (int d = 0; d < 3; ++d)
+  {
+    const int cd = d < m.dim ? c[d] : 0;
+    if (d >= m.dim)
+      h.l[d] = 0;
+    else if (e.rel == 0)
+      h.l[d] = cd + e.off[d];
+    else if (e.rel < 0)
+      h.l[d] = (cd + e.off[d]) >> 1; // floor division by 2
+    else
+      h.l[d] = 2 * cd + e.off[d];
+  }
+  return h;
+}
+
+__device__ inline double
+dx_of(const DevMesh & m, double scaling, int oct)
+{
+  return 1.0 / double(long(m.b) << m.level[oct]) * scaling;
+}
+
+// ---------------------------------------------------------------------------
+// Face B of block `o` at face (c, f) of its 2-ghost-layer face grid: the block's own face when
+// inside, else the ghost rule (FillBlockGhostFacesFunctor, GodunovImplemV0.cpp:224-235):
+// same level -> copy; coarser -> coarse face on a coarse face plane, else the mean of the two
+// bracketing coarse faces; finer -> mean of the 2^(dim-1) fine faces.  A face beside the block
+// that lies on a leaf boundary takes the leaf at this block's level, else the finer one.
+// ---------------------------------------------------------------------------
+__device__ inline double
+face_value_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__ B, int o, int c, const int fin[3])
+{
+  const int b = m.b, dim = m.dim;
+  int       f[3] = { fin[0], fin[1], fin[2] };
+  const bool cellc = (dim == 2 && c == 2);
+  bool       inner = true;
+  for (int d = 0; d < dim; ++d)
+  {
+    const int hi = (d == c && !cellc) ? b : b - 1;
+    if (f[d] < 0 || f[d] > hi)
+      inner = false;
+  }
+  const double * Bo = B + long(o) * I.fper();
+  double         v;
+  if (inner)
+  {
+    v = Bo[I.face(c, f[0], f[1], f[2])];
+  }
+  else if (cellc)
+  {
+    const Hit      h = neighbour(m, o, f);
+    const double * Bn = B + long(h.oct) * I.fper();
+    if (h.rel <= 0)
+      v = Bn[I.face(2, h.l[0], h.l[1], 0)];
+    else
+    {
+      double s = 0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di)
+          s += Bn[I.face(2, h.l[0] + di, h.l[1] + dj, 0)];
+      v = s / 4.0;
+    }
+  }
+  else
+  {
+    // the face separates cells lo (f[c]-1) and hi (f[c]); beyond the block along c take the
+    // nearer one, beside it prefer the leaf at this block's level, then the finer (ties: hi)
+    int  cc[3] = { f[0], f[1], f[2] };
+    bool high = true;
+    Hit  h;
+    if (f[c] > b)
+    {
+      cc[c] = f[c] - 1;
+      high = false;
+      h = neighbour(m, o, cc);
+    }
+    else if (f[c] < 0)
+      h = neighbour(m, o, cc);
+    else
+    {
+      const Hit hh = neighbour(m, o, cc);
+      int       lo[3] = { f[0], f[1], f[2] };
+      lo[c] -= 1;
+      const Hit hl = neighbour(m, o, lo);
+      // same level first, then finer, then coarser
+      const int rl = hl.rel == 0 ? 2 : (hl.rel > 0 ? 1 : 0);
+      const int rh = hh.rel == 0 ? 2 : (hh.rel > 0 ? 1 : 0);
+      if (rl > rh)
+      {
+        h = hl;
+        cc[c] = lo[c];
+        high = false;
+      }
+      else
+        h = hh;
+    }
+    const double * Bn = B + long(h.oct) * I.fper();
+    if (h.rel == 0)
+    {
+      int q[3] = { h.l[0], h.l[1], h.l[2] };
+      if (!high)
+        q[c] += 1;
+      v = Bn[I.face(c, q[0], q[1], q[2])];
+    }
+    else if (h.rel < 0)
+    {
+      // fine face position relative to the coarse block origin, in fine units
+      const int    ndir = dim == 3 ? 27 : 9;
+      int          dir = 0, sub = 0, mul = 1;
+      for (int d = 0; d < dim; ++d)
+      {
+        const int dl = cc[d] < 0 ? -1 : (cc[d] >= b ? 1 : 0);
+        dir += (dl + 1) * mul;
+        mul *= 3;
+        if (dl == 0 && cc[d] >= b / 2)
+          sub |= 1 << d;
+      }
+      const NbEntry e = m.nb[(long(o) * ndir + dir) * (1 << dim) + sub];
+      const int     F = f[c] + e.off[c];
+      int           q[3] = { h.l[0], h.l[1], h.l[2] };
+      if ((F & 1) == 0)
+      {
+        q[c] = F >> 1;
+        v = Bn[I.face(c, q[0], q[1], q[2])];
+      }
+      else
+      {
+        q[c] = (F - 1) >> 1;
+        const double lo = Bn[I.face(c, q[0], q[1], q[2])];
+        q[c] += 1;
+        const double hi = Bn[I.face(c, q[0], q[1], q[2])];
+        v = 0.5 * (lo + hi);
+      }
+    }
+    else
+    {
+      int base[3] = { h.l[0], h.l[1], h.l[2] };
+      if (!high)
+        base[c] += 2;
+      if (dim == 2)
+      {
+        const int t2 = c == 0 ? 1 : 0;
+        double    s = 0;
+        for (int a = 0; a < 2; ++a)
+        {
+          int q[3] = { base[0], base[1], 0 };
+          q[t2] += a;
+          s += Bn[I.face(c, q[0], q[1], 0)];
+        }
+        v = s / 2.0;
+      }
+      else
+      {
+        const int t1 = c == 0 ? 1 : 0;
+        const int t2 = c == 2 ? 1 : 2;
+        double    s = 0;
+        for (int a2 = 0; a2 < 2; ++a2)
+          for (int a1 = 0; a1 < 2; ++a1)
+          {
+            int q[3] = { base[0], base[1], base[2] };
+            q[t1] += a1;
+            q[t2] += a2;
+            s += Bn[I.face(c, q[0], q[1], q[2])];
+          }
+        v = s / 4.0;
+      }
+    }
+  }
+  return v;
+}
+
+} // namespace mhdct

@@ -1,29 +1,900 @@
-// fused.hip -- fused per-block form (placeholder until the fused kernels land).
+// fused.hip -- the fused form of the MHD-CT time step.
+//
+// The 16 device operations of GodunovImplemV0::do_time_step (scheme/GodunovImplemV0.cpp:101-207)
+// are regrouped so that each intermediate crosses HBM at most once:
+//
+//   k_prim         face-B ghost fill + primitive conversion (GodunovImplemV0.cpp:128-139)
+//   k_cell         limited slopes, Hancock predictor, edge E and face-B source, transverse
+//                  face-B slopes: one compact record per cell (:141-156, and the slopes the EMF
+//                  reconstruction reads, ComputeEmfAndStoreFunctor.h:363-419)
+//   k_border_flux  HLLD on block-border faces only (what a coarser neighbour sums)
+//   k_flux_tile    HLLD on the faces of an 8^dim tile, one direction at a time through LDS, and
+//                  the conservative update U2 = ((U + dX) + dY) + dZ in FP64 registers (:161-177)
+//   k_edge         2D-HLLD EMF, one thread per edge, every edge computed once (:197)
+//   k_emf_fix      CorrectEmfAtBlockBorder from the precomputed gather list (:628-638)
+//   k_emf_scatter  corrected values back into the EMF array
+//   k_bupdate      face-B update by the discrete curl (:641)
+//
+// T is the arithmetic type of the bulk (slopes, predictor, HLLD, EMF): double for the FP64
+// form, float for the mixed-precision form; primitive recovery from total energy, the
+// conservative accumulation and the face-B accumulation are FP64 in both.
+//
+// Every kernel walks a work list that may span several meshes ("slots") with different block
+// sizes: element kernels index (slot, block, element) by prefix offsets, tile kernels
+// (slot, block, tile).  Launches use a bounded, persistent grid (grid-stride loops).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <vector>
+
+#include "common.hpp"
 #include "ctx.hpp"
+#include "fused.hpp"
 
 namespace mhdct
 {
 
-int
-fused_alloc(Context & c)
+namespace
 {
-  c.err = "fused form not built yet";
-  return -4;
+
+constexpr int kTS = 8; // tile edge of the flux kernel
+
+template <typename T>
+__device__ inline int
+slot_of(const long * off, int nslots, long t)
+{
+  int s = 0;
+  while (s + 1 < nslots && t >= off[s + 1])
+    ++s;
+  return s;
 }
 
-void
-fused_free(Context & c)
+// read-only views of one slot's arrays ------------------------------------------------------
+template <typename T, int DIM>
+struct View
 {
-  (void)c;
+  // one block of one slot: base pointers and 32-bit strides (indices stay within a block)
+  const FusedSlot<T> & S;
+  int                  b, n2, n1, G2, G1, gf, gs;
+  const T *            p_qp;
+  const T *            p_bg;
+  const T *            p_q2;
+  const T *            p_sl;
+  const T *            p_sf;
+  const T *            p_fs;
+  __device__ View(const FusedSlot<T> & s, int o)
+    : S(s)
+  {
+    b = s.m.b;
+    n2 = b + 4;
+    n1 = b + 2;
+    G2 = DIM == 3 ? n2 * n2 * n2 : n2 * n2;
+    G1 = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
+    gs = DIM == 3 ? (n2 + 1) * n2 * n2 : (n2 + 1) * n2;
+    gf = DIM == 3 ? 3 * gs : 2 * gs + n2 * n2;
+    p_qp = s.Qp + long(o) * 5 * G2;
+    p_bg = s.Bg + long(o) * gf;
+    p_q2 = s.Q2 + long(o) * 8 * G1;
+    p_sl = s.S + long(o) * DIM * 8 * G1;
+    p_sf = s.sF + long(o) * DIM * G1;
+    p_fs = s.FS + long(o) * (DIM == 3 ? 6 : 2) * G1;
+  }
+  __device__ int c2(int i, int j, int k) const { return (i + 2) + n2 * ((j + 2) + (DIM == 3 ? n2 * (k + 2) : 0)); }
+  __device__ int c1(int i, int j, int k) const { return (i + 1) + n1 * ((j + 1) + (DIM == 3 ? n1 * (k + 1) : 0)); }
+  __device__ int gface(int c, int i, int j, int k) const
+  {
+    const int nx = c == 0 ? n2 + 1 : n2;
+    const int ny = c == 1 ? n2 + 1 : n2;
+    return c * gs + (i + 2) + nx * ((j + 2) + (DIM == 3 ? ny * (k + 2) : 0));
+  }
+  __device__ T qp(int v, int i, int j, int k) const { return p_qp[v * G2 + c2(i, j, k)]; }
+  __device__ T bf(int c, int i, int j, int k) const { return p_bg[gface(c, i, j, k)]; }
+  __device__ T q2(int v, int c) const { return p_q2[v * G1 + c]; }
+  __device__ T sl(int d, int v, int c) const { return p_sl[(d * 8 + v) * G1 + c]; }
+  __device__ T sf(int d, int c) const { return p_sf[d * G1 + c]; }
+  __device__ T fs(int n, int c) const { return p_fs[n * G1 + c]; }
+};
+
+__device__ inline double
+dxd(const DevMesh & m, double scaling, int o)
+{
+  return 1.0 / double(long(m.b) << m.level[o]) * scaling;
 }
 
-int
-fused_step(Context & c, const double *, double *, const double *, double *, double, hipStream_t)
+// ---------------------------------------------------------------------------------------------
+// k_prim: U, B -> primitive (rho, p, u, v, w) on the 2-ghost tile and the ghosted face field.
+// The cell owns its low faces (and its high faces on the last ghost layer).
+// ---------------------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total,
+       unsigned long long * flags)
 {
-  c.err = "fused form not built yet";
-  return -4;
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const DevMesh &      m = S.m;
+    const Idx            I{ DIM, m.b };
+    const long           nc = I.gcells(2);
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[0])
+      continue;
+    const int            o = int(r0 / nc);
+    const long           ci = r0 - long(o) * nc;
+    const int            b = m.b, n = b + 4;
+    const int i = int(ci % n) - 2, j = int((ci / n) % n) - 2, k = DIM == 3 ? int(ci / (long(n) * n)) - 2 : 0;
+    const long ncell = I.cells();
+    const double * __restrict__ U = S.U;
+    const double * __restrict__ B = S.B;
+    double     u[8];
+    const bool inside = i >= 0 && i < b && j >= 0 && j < b && (DIM == 2 || (k >= 0 && k < b));
+    auto       gfv = [&](int c, int a0, int a1, int a2) {
+      const int f[3] = { a0, a1, a2 };
+      return face_value_ghosted(m, I, B, o, c, f);
+    };
+    if (inside)
+    {
+      const long     cidx = I.cell(i, j, k);
+      const double * Bo = B + long(o) * I.fper();
+      for (int v = 0; v < 5; ++v)
+        u[v] = U[(long(o) * 5 + v) * ncell + cidx];
+      if (DIM == 2)
+      {
+        u[kA] = 0.5 * (Bo[I.face(0, i, j, 0)] + Bo[I.face(0, i + 1, j, 0)]);
+        u[kB] = 0.5 * (Bo[I.face(1, i, j, 0)] + Bo[I.face(1, i, j + 1, 0)]);
+        u[kC] = 0.5 * (Bo[I.face(2, i, j, 0)] + Bo[I.face(2, i, j, 0)]);
+      }
+      else
+      {
+        u[kA] = 0.5 * (Bo[I.face(0, i, j, k)] + Bo[I.face(0, i + 1, j, k)]);
+        u[kB] = 0.5 * (Bo[I.face(1, i, j, k)] + Bo[I.face(1, i, j + 1, k)]);
+        u[kC] = 0.5 * (Bo[I.face(2, i, j, k)] + Bo[I.face(2, i, j, k + 1)]);
+      }
+    }
+    else
+    {
+      const int c[3] = { i, j, k };
+      const Hit h = neighbour(m, o, c);
+      if (h.rel > 0)
+      {
+        for (int v = 0; v < 5; ++v)
+        {
+          const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+          double         sum = 0;
+          for (int dk = 0; dk < (DIM == 3 ? 2 : 1); ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+              for (int di = 0; di < 2; ++di)
+                sum += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+          u[v] = sum / double(1 << DIM);
+        }
+        if (DIM == 2)
+        {
+          u[kA] = 0.5 * (gfv(0, i, j, 0) + gfv(0, i + 1, j, 0));
+          u[kB] = 0.5 * (gfv(1, i, j, 0) + gfv(1, i, j + 1, 0));
+          u[kC] = gfv(2, i, j, 0);
+        }
+        else
+        {
+          u[kA] = 0.5 * (gfv(0, i, j, k) + gfv(0, i + 1, j, k));
+          u[kB] = 0.5 * (gfv(1, i, j, k) + gfv(1, i, j + 1, k));
+          u[kC] = 0.5 * (gfv(2, i, j, k) + gfv(2, i, j, k + 1));
+        }
+      }
+      else
+      {
+        const long cidx = I.cell(h.l[0], h.l[1], h.l[2]);
+        for (int v = 0; v < 5; ++v)
+          u[v] = U[(long(h.oct) * 5 + v) * ncell + cidx];
+        const double * Bn = B + long(h.oct) * I.fper();
+        const int      a = h.l[0], bb = h.l[1], cc = h.l[2];
+        if (DIM == 2)
+        {
+          u[kA] = 0.5 * (Bn[I.face(0, a, bb, 0)] + Bn[I.face(0, a + 1, bb, 0)]);
+          u[kB] = 0.5 * (Bn[I.face(1, a, bb, 0)] + Bn[I.face(1, a, bb + 1, 0)]);
+          u[kC] = 0.5 * (Bn[I.face(2, a, bb, 0)] + Bn[I.face(2, a, bb, 0)]);
+        }
+        else
+        {
+          u[kA] = 0.5 * (Bn[I.face(0, a, bb, cc)] + Bn[I.face(0, a + 1, bb, cc)]);
+          u[kB] = 0.5 * (Bn[I.face(1, a, bb, cc)] + Bn[I.face(1, a, bb + 1, cc)]);
+          u[kC] = 0.5 * (Bn[I.face(2, a, bb, cc)] + Bn[I.face(2, a, bb, cc + 1)]);
+        }
+      }
+    }
+    double       q[8];
+    const double eint = to_primitive(u, q, S.gd);
+    if (S.abort_eint && eint < 0)
+      atomicOr(flags, 1ull);
+    T * Qo = S.Qp + long(o) * 5 * nc;
+    for (int v = 0; v < 5; ++v)
+      Qo[v * nc + ci] = T(q[v]);
+    // face field: low faces of this cell, and the high faces on the last ghost layer
+    T * Go = S.Bg + long(o) * I.gfper();
+    if (DIM == 2)
+    {
+      Go[I.gface(0, i, j, 0)] = T(gfv(0, i, j, 0));
+      Go[I.gface(1, i, j, 0)] = T(gfv(1, i, j, 0));
+      Go[I.gface(2, i, j, 0)] = T(gfv(2, i, j, 0));
+      if (i == b + 1)
+        Go[I.gface(0, b + 2, j, 0)] = T(gfv(0, b + 2, j, 0));
+      if (j == b + 1)
+        Go[I.gface(1, i, b + 2, 0)] = T(gfv(1, i, b + 2, 0));
+    }
+    else
+    {
+      Go[I.gface(0, i, j, k)] = T(gfv(0, i, j, k));
+      Go[I.gface(1, i, j, k)] = T(gfv(1, i, j, k));
+      Go[I.gface(2, i, j, k)] = T(gfv(2, i, j, k));
+      if (i == b + 1)
+        Go[I.gface(0, b + 2, j, k)] = T(gfv(0, b + 2, j, k));
+      if (j == b + 1)
+        Go[I.gface(1, i, b + 2, k)] = T(gfv(1, i, b + 2, k));
+      if (k == b + 1)
+        Go[I.gface(2, i, j, b + 2)] = T(gfv(2, i, j, b + 2));
+    }
+  }
 }
+
+// ---------------------------------------------------------------------------------------------
+// k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const DevMesh &      m = S.m;
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[1])
+      continue;
+    const int            n = m.b + 2;
+    const int            g1 = DIM == 3 ? n * n * n : n * n;
+    const int            o = int(r0 / g1);
+    const int            ci = int(r0 - long(o) * g1);
+    const View<T, DIM>   V(S, o);
+    const int            b = m.b;
+    const int i = int(ci % n) - 1, j = int((ci / n) % n) - 1, k = DIM == 3 ? int(ci / (long(n) * n)) - 1 : 0;
+    const T   st = S.gt.slope_type;
+    // the 11-variable primitive state of cell (a, c, e): 5 hydro + low/high face fields
+    auto Q = [&](int v, int a, int c, int e) -> T {
+      if (v < 5)
+        return V.qp(v, a, c, e);
+      const int fc = (v - 5) % 3;
+      const int hi = v >= 8;
+      if (DIM == 2 && fc == 2)
+        return V.bf(2, a, c, 0);
+      return V.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
+    };
+    const long c1 = ci;
+    T *        Sd = S.S + long(o) * DIM * 8 * V.G1;
+    T          sl[3][8];
+    const int  nh = DIM == 3 ? 5 : 4;
+    const int  e3 = DIM == 3 ? 1 : 0;
+    for (int d = 0; d < DIM; ++d)
+    {
+      const int di = d == 0, dj = d == 1, dk = d == 2;
+      for (int v = 0; v < 8; ++v)
+        sl[d][v] = T(0);
+      for (int v = 0; v < nh; ++v)
+        sl[d][v] = limited_slope(Q(v, i, j, k), Q(v, i + di, j + dj, k + dk), Q(v, i - di, j - dj, k - dk), st);
+      for (int mc = 0; mc < 3; ++mc)
+      {
+        const int vL = 5 + mc, vR = 8 + mc;
+        if (mc == d)
+          sl[d][5 + mc] = Q(vR, i, j, k) - Q(vL, i, j, k);
+        else
+        {
+          const T dc = (Q(vL, i, j, k) + Q(vR, i, j, k)) * T(0.5);
+          const T dp = (Q(vL, i + di, j + dj, k + dk) + Q(vR, i + di, j + dj, k + dk)) * T(0.5);
+          const T dm = (Q(vL, i - di, j - dj, k - dk) + Q(vR, i - di, j - dj, k - dk)) * T(0.5);
+          sl[d][5 + mc] = limited_slope(dc, dp, dm, st);
+        }
+      }
+      for (int v = 0; v < 8; ++v)
+        Sd[(d * 8 + v) * V.G1 + c1] = sl[d][v];
+    }
+    (void)e3;
+    // Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
+    const double dxo = dxd(m, S.scaling, o);
+    const T      dtdx = T(S.dt / dxo), dtdy = dtdx, dtdz = dtdx;
+    const T      gamma = S.gt.gamma0;
+    const T r = Q(0, i, j, k), p = Q(1, i, j, k), u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
+    const T A = (Q(5, i, j, k) + Q(8, i, j, k)) * T(0.5);
+    const T B = (Q(6, i, j, k) + Q(9, i, j, k)) * T(0.5);
+    const T C = (Q(7, i, j, k) + Q(10, i, j, k)) * T(0.5);
+    const T drx = sl[0][0], dpx = sl[0][1], dux = sl[0][2], dvx = sl[0][3], dwx = sl[0][4], dAx = sl[0][5],
+            dBx = sl[0][6], dCx = sl[0][7];
+    const T dry = sl[1][0], dpy = sl[1][1], duy = sl[1][2], dvy = sl[1][3], dwy = sl[1][4], dAy = sl[1][5],
+            dBy = sl[1][6], dCy = sl[1][7];
+    T sr0, su0, sv0, sw0, sp0, sA0, sB0, sC0;
+    if (DIM == 2)
+    {
+      sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy;
+      su0 = (-u * dux - dpx / r - B * dBx / r - C * dCx / r) * dtdx + (-v * duy + B * dAy / r) * dtdy;
+      sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - dpy / r - A * dAy / r - C * dCy / r) * dtdy;
+      sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy;
+      sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy;
+      sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy;
+      sB0 = (-u * dBx - B * dux + v * dAx + A * dvx) * dtdx;
+      sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (-v * dCy - C * dvy + w * dBy + B * dwy) * dtdy;
+    }
+    else
+    {
+      const T drz = sl[2][0], dpz = sl[2][1], duz = sl[2][2], dvz = sl[2][3], dwz = sl[2][4], dAz = sl[2][5],
+              dBz = sl[2][6], dCz = sl[2][7];
+      sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-w * drz - dwz * r) * dtdz;
+      su0 = (-u * dux - (dpx + B * dBx + C * dCx) / r) * dtdx + (-v * duy + B * dAy / r) * dtdy +
+            (-w * duz + C * dAz / r) * dtdz;
+      sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - (dpy + A * dAy + C * dCy) / r) * dtdy +
+            (-w * dvz + C * dBz / r) * dtdz;
+      sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy +
+            (-w * dwz - (dpz + A * dAz + B * dBz) / r) * dtdz;
+      sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy + (-w * dpz - dwz * gamma * p) * dtdz;
+      sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
+      sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
+      sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+    }
+    T * q2 = S.Q2 + long(o) * 8 * V.G1 + c1;
+    q2[0 * V.G1] = r + T(0.5) * sr0;
+    q2[2 * V.G1] = u + T(0.5) * su0;
+    q2[3 * V.G1] = v + T(0.5) * sv0;
+    q2[4 * V.G1] = w + T(0.5) * sw0;
+    q2[1 * V.G1] = p + T(0.5) * sp0;
+    q2[5 * V.G1] = A + T(0.5) * sA0;
+    q2[6 * V.G1] = B + T(0.5) * sB0;
+    q2[7 * V.G1] = C + T(0.5) * sC0;
+    // edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235), evaluated where
+    // the face source needs it
+    auto Ex = [&](int a, int c, int e) -> T {
+      const T vv = T(0.25) * (Q(3, a, c - 1, e - 1) + Q(3, a, c - 1, e) + Q(3, a, c, e - 1) + Q(3, a, c, e));
+      const T ww = T(0.25) * (Q(4, a, c - 1, e - 1) + Q(4, a, c - 1, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
+      const T BB = T(0.5) * (Q(6, a, c, e - 1) + Q(6, a, c, e));
+      const T CC = T(0.5) * (Q(7, a, c - 1, e) + Q(7, a, c, e));
+      return vv * CC - ww * BB;
+    };
+    auto Ey = [&](int a, int c, int e) -> T {
+      const T uu = T(0.25) * (Q(2, a - 1, c, e - 1) + Q(2, a - 1, c, e) + Q(2, a, c, e - 1) + Q(2, a, c, e));
+      const T ww = T(0.25) * (Q(4, a - 1, c, e - 1) + Q(4, a - 1, c, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
+      const T AA = T(0.5) * (Q(5, a, c, e - 1) + Q(5, a, c, e));
+      const T CC = T(0.5) * (Q(7, a - 1, c, e) + Q(7, a, c, e));
+      return ww * AA - uu * CC;
+    };
+    auto Ez = [&](int a, int c, int e) -> T {
+      const T uu = T(0.25) * (Q(2, a - 1, c - 1, e) + Q(2, a - 1, c, e) + Q(2, a, c - 1, e) + Q(2, a, c, e));
+      const T vv = T(0.25) * (Q(3, a - 1, c - 1, e) + Q(3, a - 1, c, e) + Q(3, a, c - 1, e) + Q(3, a, c, e));
+      const T AA = T(0.5) * (Q(5, a, c - 1, e) + Q(5, a, c, e));
+      const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
+      return uu * BB - vv * AA;
+    };
+    T * sF = S.sF + long(o) * DIM * V.G1 + c1;
+    if (DIM == 2)
+    {
+      const T e00 = Ez(i, j, 0);
+      sF[0] = (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy;
+      sF[V.G1] = -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx;
+    }
+    else
+    {
+      const T ex = Ex(i, j, k), ey = Ey(i, j, k), ez = Ez(i, j, k);
+      sF[0] = (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz;
+      sF[V.G1] = -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz;
+      sF[2 * V.G1] = (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy;
+    }
+    // transverse slopes of the face field at this cell's low faces (dBx/dy, dBx/dz, dBy/dx,
+    // dBy/dz, dBz/dx, dBz/dy; 2D: dBx/dy, dBy/dx)
+    T * fs = S.FS + long(o) * (DIM == 3 ? 6 : 2) * V.G1 + c1;
+    if (DIM == 2)
+    {
+      fs[0] = limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st);
+      fs[V.G1] = limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st);
+    }
+    else
+    {
+      fs[0] = limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st);
+      fs[1 * V.G1] = limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st);
+      fs[2 * V.G1] = limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st);
+      fs[3 * V.G1] = limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st);
+      fs[4 * V.G1] = limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st);
+      fs[5 * V.G1] = limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st);
+    }
+  }
+}
+
+// HLLD flux (times dt/dx) through the low face of cell (i,j,k) along `dir`
+// (ComputeHydroFluxesAndStoreFunctor.cpp:218-503); out in the reference variable order.
+template <typename T, int DIM>
+__device__ inline void
+face_flux(const View<T, DIM> & V, int o, int dir, int i, int j, int k, T dtdS, T * f)
+{
+  const int cR = V.c1(i, j, k);
+  const int cL = V.c1(i - (dir == 0), j - (dir == 1), k - (dir == 2));
+  T          qL[8], qR[8];
+  for (int v = 0; v < 8; ++v)
+  {
+    qL[v] = V.q2(v, cL) + T(0.5) * V.sl(dir, v, cL);
+    qR[v] = V.q2(v, cR) + T(-0.5) * V.sl(dir, v, cR);
+  }
+  const Gas<T> & g = V.S.gt;
+  qL[kD] = fmax(g.smallr, qL[kD]);
+  qL[kP] = fmax(g.smallp * qL[kD], qL[kP]);
+  qR[kD] = fmax(g.smallr, qR[kD]);
+  qR[kP] = fmax(g.smallp * qR[kD], qR[kP]);
+  const T bn = V.bf(DIM == 2 && dir == 2 ? 2 : dir, i, j, k) + V.sf(dir, cR);
+  qR[kA + dir] = bn;
+  qL[kA + dir] = bn;
+  if (dir != 0)
+  {
+    const int vt = kU + dir, bt = kA + dir;
+    T         x;
+    x = qL[kU]; qL[kU] = qL[vt]; qL[vt] = x;
+    x = qR[kU]; qR[kU] = qR[vt]; qR[vt] = x;
+    x = qL[kA]; qL[kA] = qL[bt]; qL[bt] = x;
+    x = qR[kA]; qR[kA] = qR[bt]; qR[bt] = x;
+  }
+  hlld(qL, qR, f, g);
+  if (dir != 0)
+  {
+    const int vt = kU + dir, bt = kA + dir;
+    T         x;
+    x = f[kU]; f[kU] = f[vt]; f[vt] = x;
+    x = f[kA]; f[kA] = f[bt]; f[bt] = x;
+  }
+  for (int v = 0; v < 8; ++v)
+    f[v] = f[v] * dtdS;
+}
+
+// the six stored flux components: rho, E, mx, my, mz, Bz (2D, cell-centred Bz)
+__device__ constexpr int kFluxVar[6] = { kD, kP, kU, kV, kW, kC };
+
+template <typename T, int DIM>
+__device__ inline long
+bf_index(int b, int o, int dir, int side, int v, long tpos)
+{
+  const long nt = DIM == 3 ? long(b) * b : long(b);
+  return ((((long(o) * DIM + dir) * 2 + side) * 6 + v) * nt) + tpos;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_border_flux: HLLD on every block-border face (both sides, every direction)
+// ---------------------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const int            b = S.m.b;
+    const long           nt = DIM == 3 ? long(b) * b : long(b);
+    const long           per = DIM * 2 * nt;
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[2])
+      continue;
+    const int            o = int(r0 / per);
+    const View<T, DIM>   V(S, o);
+    long                 r = r0 - long(o) * per;
+    const int            dir = int(r / (2 * nt));
+    r -= long(dir) * 2 * nt;
+    const int  side = int(r / nt);
+    const long tp = r - long(side) * nt;
+    int        tt[2], ntt = 0;
+    for (int d = 0; d < DIM; ++d)
+      if (d != dir)
+        tt[ntt++] = d;
+    int c[3] = { 0, 0, 0 };
+    c[tt[0]] = int(tp % b);
+    if (DIM == 3)
+      c[tt[1]] = int(tp / b);
+    c[dir] = side ? b : 0;
+    T         f[8];
+    const T   dtdS = T(S.dt / dxd(S.m, S.scaling, o));
+    face_flux<T, DIM>(V, o, dir, c[0], c[1], c[2], dtdS, f);
+    for (int v = 0; v < (DIM == 2 ? 6 : 5); ++v)
+      S.BF[bf_index<T, DIM>(b, o, dir, side, v, tp)] = f[kFluxVar[v]];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_flux_tile: one workgroup per 8^dim tile; HLLD on the tile faces, one direction at a time
+// through LDS; U2 = ((U + dX) + dY) + dZ accumulated in FP64 registers.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void __launch_bounds__(DIM == 3 ? 512 : 64)
+k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  constexpr int TS = kTS;
+  constexpr int NT = DIM == 3 ? TS * TS * TS : TS * TS;
+  constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
+  constexpr int NV = DIM == 2 ? 6 : 5;
+  __shared__ T  lf[NV][NF];
+  for (long t = blockIdx.x; t < total; t += gridDim.x)
+  {
+    const int            s = slot_of<T>(off, nslots, t);
+    const FusedSlot<T> & S = slots[s];
+    const DevMesh &      m = S.m;
+    const int            b = m.b;
+    const int            nb1 = b / TS;
+    const int            tpb = DIM == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[3])
+      continue;
+    const int            o = int(r0 / tpb);
+    const View<T, DIM>   V(S, o);
+    const int            tl = int(r0 - long(o) * tpb);
+    const int org[3] = { (tl % nb1) * TS, ((tl / nb1) % nb1) * TS, DIM == 3 ? (tl / (nb1 * nb1)) * TS : 0 };
+    const int tid = threadIdx.x;
+    const int li = tid % TS, lj = (tid / TS) % TS, lk = DIM == 3 ? tid / (TS * TS) : 0;
+    const int ci = org[0] + li, cj = org[1] + lj, ck = org[2] + lk;
+    const Idx I{ DIM, b };
+    const long ncell = I.cells();
+    const long cidx = I.cell(ci, cj, ck);
+    double     acc[NV];
+    for (int v = 0; v < 5; ++v)
+      acc[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
+    if (DIM == 2)
+      acc[5] = S.B[long(o) * I.fper() + I.face(2, ci, cj, 0)];
+    const T      dtdS = T(S.dt / dxd(m, S.scaling, o));
+    const double vr = double(1 << DIM);
+    for (int dir = 0; dir < DIM; ++dir)
+    {
+      // extent of the face set along each axis
+      int ext[3] = { TS, TS, DIM == 3 ? TS : 1 };
+      ext[dir] = TS + 1;
+      const bool lo_fine = m.conformal[long(o) * 2 * DIM + 2 * dir] != 0;
+      const bool hi_fine = m.conformal[long(o) * 2 * DIM + 2 * dir + 1] != 0;
+      int        tt[2], ntt = 0;
+      for (int d = 0; d < DIM; ++d)
+        if (d != dir)
+          tt[ntt++] = d;
+      for (int fl = tid; fl < NF; fl += NT)
+      {
+        const int a0 = fl % ext[0], a1 = (fl / ext[0]) % ext[1], a2 = DIM == 3 ? fl / (ext[0] * ext[1]) : 0;
+        int       c[3] = { org[0] + a0, org[1] + a1, org[2] + a2 };
+        T         out[NV];
+        if (c[dir] == 0 || c[dir] == b)
+        {
+          const int  side = c[dir] == b;
+          const long tp = DIM == 3 ? long(c[tt[0]]) + long(b) * c[tt[1]] : long(c[tt[0]]);
+          if ((side == 0 && lo_fine) || (side == 1 && hi_fine))
+          {
+            // sum of the fine faces covering this coarse face, divided by 2^dim
+            int cc[3] = { c[0], c[1], c[2] };
+            cc[dir] = side == 0 ? -1 : b;
+            const Hit h = neighbour(m, o, cc);
+            for (int vi = 0; vi < NV; ++vi)
+            {
+              double sum = 0;
+              if (DIM == 2)
+              {
+                for (int a = 0; a < 2; ++a)
+                  sum += S.BF[bf_index<T, DIM>(b, h.oct, dir, 1 - side, vi, h.l[tt[0]] + a)];
+              }
+              else
+              {
+                for (int q1 = 0; q1 < 2; ++q1)
+                  for (int q0 = 0; q0 < 2; ++q0)
+                    sum += S.BF[bf_index<T, DIM>(b, h.oct, dir, 1 - side, vi,
+                                                 long(h.l[tt[0]] + q0) + long(b) * (h.l[tt[1]] + q1))];
+              }
+              out[vi] = T(sum / vr);
+            }
+          }
+          else
+          {
+            for (int vi = 0; vi < NV; ++vi)
+              out[vi] = S.BF[bf_index<T, DIM>(b, o, dir, side, vi, tp)];
+          }
+        }
+        else
+        {
+          T f[8];
+          face_flux<T, DIM>(V, o, dir, c[0], c[1], c[2], dtdS, f);
+          for (int vi = 0; vi < NV; ++vi)
+            out[vi] = f[kFluxVar[vi]];
+        }
+        for (int vi = 0; vi < NV; ++vi)
+          lf[vi][fl] = out[vi];
+      }
+      __syncthreads();
+      {
+        int a[3] = { li, lj, lk };
+        const int fL = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
+        a[dir] += 1;
+        const int fR = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
+        for (int vi = 0; vi < NV; ++vi)
+        {
+          // (0 + F_left) - F_right, as ReadFluxesAndConservativeUpdateFunctor accumulates it
+          const double d = double(lf[vi][fL]) - double(lf[vi][fR]);
+          acc[vi] = acc[vi] + d;
+        }
+      }
+      __syncthreads();
+    }
+    for (int v = 0; v < 5; ++v)
+      S.U2[(long(o) * 5 + v) * ncell + cidx] = acc[v];
+    if (DIM == 2)
+      S.B2[long(o) * I.fper() + I.face(2, ci, cj, 0)] = acc[5];
+  }
+}
+
+// corner state of an edge (ComputeEmfAndStoreFunctor.cpp:105-367); kind 0 LB, 1 RT, 2 RB, 3 LT
+template <typename T, int DIM>
+__device__ inline void
+edge_state(const View<T, DIM> & V, int o, int is, int js, int ks, int kind, int d0, int d1, T * q)
+{
+  const int c0 = V.c1(is, js, ks);
+  for (int v = 0; v < 8; ++v)
+    q[v] = V.q2(v, c0);
+  int s0 = 1, s1 = 1, b0 = 1, b1 = 1;
+  int p0[3] = { is, js, ks }, p1[3] = { is, js, ks };
+  if (kind == 0)
+  {
+    s0 = -1; s1 = -1; b0 = -1; b1 = -1;
+  }
+  else if (kind == 1)
+  {
+    p0[d0] += 1; p1[d1] += 1;
+  }
+  else if (kind == 2)
+  {
+    p0[d0] += 1; s1 = -1; b0 = -1;
+  }
+  else
+  {
+    p1[d1] += 1; s0 = -1; b1 = -1;
+  }
+  const int cp0 = V.c1(p0[0], p0[1], p0[2]);
+  const int cp1 = V.c1(p1[0], p1[1], p1[2]);
+  T          B0 = V.bf(d0, p0[0], p0[1], p0[2]);
+  T          B1 = V.bf(d1, p1[0], p1[1], p1[2]);
+  B0 += V.sf(d0, cp0);
+  B1 += V.sf(d1, cp1);
+  // face-slope record index of (component, direction)
+  auto fsi = [](int comp, int dir) { return DIM == 2 ? comp : (comp == 0 ? dir - 1 : (comp == 1 ? (dir == 0 ? 2 : 3) : (dir == 0 ? 4 : 5))); };
+  const T dB0d1 = V.fs(fsi(d0, d1), cp0);
+  const T dB1d0 = V.fs(fsi(d1, d0), cp1);
+  const T f0 = T(s0), f1 = T(s1);
+  for (int v = 0; v < 5; ++v)
+    q[v] += T(0.5) * (f0 * V.sl(d0, v, c0) + f1 * V.sl(d1, v, c0));
+  auto tr = [&](int v) { return T(0.5) * (f0 * V.sl(d0, v, c0) + f1 * V.sl(d1, v, c0)); };
+  if (DIM == 2)
+  {
+    q[kA] = B0 + T(0.5) * (T(b0) * dB0d1);
+    q[kB] = B1 + T(0.5) * (T(b1) * dB1d0);
+    q[kC] += tr(kC);
+  }
+  else
+  {
+    if (d0 == 0)
+      q[kA] = B0 + T(0.5) * (T(b0) * dB0d1);
+    else
+      q[kA] += tr(kA);
+    if (d0 == 1)
+      q[kB] = B0 + T(0.5) * (T(b0) * dB0d1);
+    else
+      q[kB] += tr(kB);
+    if (d1 == 1)
+      q[kB] = B1 + T(0.5) * (T(b1) * dB1d0);
+    else
+      q[kB] += tr(kB);
+    if (d1 == 2)
+      q[kC] = B1 + T(0.5) * (T(b1) * dB1d0);
+    else
+      q[kC] += tr(kC);
+  }
+  q[kD] = fmax(V.S.gt.smallr, q[kD]);
+  q[kP] = fmax(V.S.gt.smallp * q[kD], q[kP]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_edge: one thread per edge of a block (all three directions), every edge once
+// ---------------------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const int            b = S.m.b;
+    const long           n1 = b + 1;
+    // edges per block: 2D (b+1)^2 z-edges; 3D for each axis a: b (b+1)^2
+    const long per_dir = DIM == 3 ? long(b) * n1 * n1 : n1 * n1;
+    const long per = DIM == 3 ? 3 * per_dir : per_dir;
+    const long r0 = t - off[s];
+    if (r0 >= S.cnt[4])
+      continue;
+    const int  o = int(r0 / per);
+    const View<T, DIM> V(S, o);
+    long       r = r0 - long(o) * per;
+    const int  a = DIM == 3 ? int(r / per_dir) : 2;
+    r -= DIM == 3 ? long(a) * per_dir : 0;
+    int e[3];
+    if (DIM == 2)
+    {
+      e[0] = int(r % n1);
+      e[1] = int(r / n1);
+      e[2] = 0;
+    }
+    else
+    {
+      // along-axis index runs over b, the two others over b+1 (x fastest)
+      int ext[3] = { int(n1), int(n1), int(n1) };
+      ext[a] = b;
+      e[0] = int(r % ext[0]);
+      e[1] = int((r / ext[0]) % ext[1]);
+      e[2] = int(r / (long(ext[0]) * ext[1]));
+    }
+    const T dtdx = T(S.dt / dxd(S.m, S.scaling, o));
+    T       qe[4][8];
+    const int i = e[0], j = e[1], k = e[2];
+    if (a == 2)
+    {
+      edge_state<T, DIM>(V, o, i, j, k, 0, 0, 1, qe[kLB]);
+      edge_state<T, DIM>(V, o, i - 1, j - 1, k, 1, 0, 1, qe[kRT]);
+      edge_state<T, DIM>(V, o, i - 1, j, k, 2, 0, 1, qe[kRB]);
+      edge_state<T, DIM>(V, o, i, j - 1, k, 3, 0, 1, qe[kLT]);
+    }
+    else if (a == 1)
+    {
+      edge_state<T, DIM>(V, o, i, j, k, 0, 0, 2, qe[kLB]);
+      edge_state<T, DIM>(V, o, i - 1, j, k - 1, 1, 0, 2, qe[kRT]);
+      edge_state<T, DIM>(V, o, i - 1, j, k, 2, 0, 2, qe[kLT]);
+      edge_state<T, DIM>(V, o, i, j, k - 1, 3, 0, 2, qe[kRB]);
+    }
+    else
+    {
+      edge_state<T, DIM>(V, o, i, j, k, 0, 1, 2, qe[kLB]);
+      edge_state<T, DIM>(V, o, i, j - 1, k - 1, 1, 1, 2, qe[kRT]);
+      edge_state<T, DIM>(V, o, i, j - 1, k, 2, 1, 2, qe[kRB]);
+      edge_state<T, DIM>(V, o, i, j, k - 1, 3, 1, 2, qe[kLT]);
+    }
+    const T   E = edge_emf(qe, a, S.gt) * dtdx;
+    const int comp = DIM == 3 ? a : 0;
+    const long ne = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
+    S.emf[(long(o) * (DIM == 3 ? 3 : 1) + comp) * ne + i + n1 * (j + n1 * k)] = E;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_fix(int nfix, const EmfFix * __restrict__ fix, const int64_t * __restrict__ src, const T * __restrict__ emf,
+      T * __restrict__ val)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < nfix; t += long(gridDim.x) * blockDim.x)
+  {
+    const EmfFix f = fix[t];
+    T            s = 0;
+    for (int n = 0; n < f.nsrc; ++n)
+      s += emf[src[f.first + n]];
+    val[t] = s / T(f.cnt) / T(f.ratio);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_scatter(int nfix, const EmfFix * __restrict__ fix, const T * __restrict__ val, T * __restrict__ emf)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < nfix; t += long(gridDim.x) * blockDim.x)
+    emf[fix[t].dst] = val[t];
+}
+
+// face-B update (ReadEmfAndUpdateFunctor.cpp:58-163), FP64 accumulation
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const int            b = S.m.b;
+    const Idx            I{ DIM, b };
+    const long           per = I.fper();
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[5])
+      continue;
+    const int            o = int(r0 / per);
+    const long           fi = r0 - long(o) * per;
+    long                 r = fi;
+    int                  c = 0;
+    while (c < 2 && r >= I.fsize(c))
+    {
+      r -= I.fsize(c);
+      ++c;
+    }
+    const int  nx = c == 0 ? b + 1 : b;
+    const int  ny = c == 1 ? b + 1 : b;
+    const int  i = int(r % nx), j = int((r / nx) % ny), k = DIM == 3 ? int(r / (long(nx) * ny)) : 0;
+    const long ne = I.ecells();
+    const double Bv = S.B[long(o) * per + fi];
+    if (DIM == 2)
+    {
+      const T * e = S.emf + long(o) * ne;
+      if (c == 0)
+        S.B2[long(o) * per + fi] = Bv + (double(e[I.edge(i, j + 1, 0)]) - double(e[I.edge(i, j, 0)]));
+      else if (c == 1)
+        S.B2[long(o) * per + fi] = Bv - (double(e[I.edge(i + 1, j, 0)]) - double(e[I.edge(i, j, 0)]));
+      continue;
+    }
+    const T * ex = S.emf + (long(o) * 3 + 0) * ne;
+    const T * ey = S.emf + (long(o) * 3 + 1) * ne;
+    const T * ez = S.emf + (long(o) * 3 + 2) * ne;
+    double    dB = 0;
+    if (c == 0)
+    {
+      dB += (double(ez[I.edge(i, j + 1, k)]) - double(ez[I.edge(i, j, k)]));
+      dB -= (double(ey[I.edge(i, j, k + 1)]) - double(ey[I.edge(i, j, k)]));
+    }
+    else if (c == 1)
+    {
+      dB += (double(ex[I.edge(i, j, k + 1)]) - double(ex[I.edge(i, j, k)]));
+      dB -= (double(ez[I.edge(i + 1, j, k)]) - double(ez[I.edge(i, j, k)]));
+    }
+    else
+    {
+      dB += (double(ey[I.edge(i + 1, j, k)]) - double(ey[I.edge(i, j, k)]));
+      dB -= (double(ex[I.edge(i, j + 1, k)]) - double(ex[I.edge(i, j, k)]));
+    }
+    S.B2[long(o) * per + fi] = Bv + dB;
+  }
+}
+
+inline unsigned
+grid_for(long n, int threads, int cap)
+{
+  long g = (n + threads - 1) / threads;
+  if (g > cap)
+    g = cap;
+  if (g < 1)
+    g = 1;
+  return unsigned(g);
+}
+
+template <typename T, int DIM>
+int
+launch_step(FusedBatch<T> & bt, hipStream_t st, int cap)
+{
+  const int   n = bt.nslots;
+  const auto * sl = bt.d_slots;
+  // element kernels
+  hipLaunchKernelGGL((k_prim<T, DIM>), grid_for(bt.tot[0], 256, cap), 256, 0, st, sl, n, bt.d_off[0], bt.tot[0],
+                     bt.flags);
+  hipLaunchKernelGGL((k_cell<T, DIM>), grid_for(bt.tot[1], 256, cap), 256, 0, st, sl, n, bt.d_off[1], bt.tot[1]);
+  hipLaunchKernelGGL((k_border_flux<T, DIM>), grid_for(bt.tot[2], 256, cap), 256, 0, st, sl, n, bt.d_off[2],
+                     bt.tot[2]);
+  hipLaunchKernelGGL((k_flux_tile<T, DIM>), grid_for(bt.tot[3], 1, cap), DIM == 3 ? 512 : 64, 0, st, sl, n,
+                     bt.d_off[3], bt.tot[3]);
+  hipLaunchKernelGGL((k_edge<T, DIM>), grid_for(bt.tot[4], 256, cap), 256, 0, st, sl, n, bt.d_off[4], bt.tot[4]);
+  if (bt.nfix > 0)
+  {
+    hipLaunchKernelGGL((k_fix<T>), grid_for(bt.nfix, 256, cap), 256, 0, st, bt.nfix, bt.fix, bt.fix_src,
+                       bt.emf_all, bt.fixval);
+    hipLaunchKernelGGL((k_scatter<T>), grid_for(bt.nfix, 256, cap), 256, 0, st, bt.nfix, bt.fix, bt.fixval,
+                       bt.emf_all);
+  }
+  hipLaunchKernelGGL((k_bupdate<T, DIM>), grid_for(bt.tot[5], 256, cap), 256, 0, st, sl, n, bt.d_off[5],
+                     bt.tot[5]);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+} // namespace
+
+template <typename T>
+int
+fused_launch(FusedBatch<T> & bt, hipStream_t st)
+{
+  const int cap = bt.grid_cap > 0 ? bt.grid_cap : 1 << 20;
+  return bt.dim == 3 ? launch_step<T, 3>(bt, st, cap) : launch_step<T, 2>(bt, st, cap);
+}
+
+template int fused_launch<double>(FusedBatch<double> &, hipStream_t);
+template int fused_launch<float>(FusedBatch<float> &, hipStream_t);
 
 } // namespace mhdct

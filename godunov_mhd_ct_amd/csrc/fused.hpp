@@ -1,0 +1,83 @@
+// fused.hpp -- work-list description of the fused form: one slot per mesh, a batch of slots.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace mhdct
+{
+
+// One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
+template <typename T>
+struct FusedSlot
+{
+  DevMesh        m;
+  Gas<T>         gt;  // bulk settings
+  Gas<double>    gd;  // primitive recovery
+  double         dt;
+  double         scaling;
+  int            abort_eint;
+  const double * U;
+  double *       U2;
+  const double * B;
+  double *       B2;
+  T *            Qp;  // [block][5][(b+4)^d]       primitive, 2 ghost layers
+  T *            Bg;  // [block][ghosted faces]     face field, 2 ghost layers
+  T *            Q2;  // [block][8][(b+2)^d]       predictor
+  T *            S;   // [block][dim][8][(b+2)^d]  limited slopes
+  T *            sF;  // [block][dim][(b+2)^d]     face-B half-step source
+  T *            FS;  // [block][6|2][(b+2)^d]     transverse slopes of the face field
+  T *            BF;  // [block][dim][2][6][b^(d-1)] border-face fluxes
+  T *            emf; // [block][3|1][(b+1)^d]
+  long           cnt[6]; // work items per stage (the stage offsets are padded to workgroup multiples)
+};
+
+// Scratch element counts per block, for a block edge b.
+struct FusedSizes
+{
+  long qp, bg, q2, s, sf, fs, bf, emf;
+  FusedSizes(int dim, int b)
+  {
+    const long g2 = dim == 3 ? long(b + 4) * (b + 4) * (b + 4) : long(b + 4) * (b + 4);
+    const long g1 = dim == 3 ? long(b + 2) * (b + 2) * (b + 2) : long(b + 2) * (b + 2);
+    const long n = b + 4;
+    bg = dim == 3 ? 3 * (n + 1) * n * n : 2 * (n + 1) * n + n * n;
+    qp = 5 * g2;
+    q2 = 8 * g1;
+    s = long(dim) * 8 * g1;
+    sf = long(dim) * g1;
+    fs = (dim == 3 ? 6 : 2) * g1;
+    bf = long(dim) * 2 * 6 * (dim == 3 ? long(b) * b : long(b));
+    emf = dim == 3 ? 3L * (b + 1) * (b + 1) * (b + 1) : long(b + 1) * (b + 1);
+  }
+  long total() const { return qp + bg + q2 + s + sf + fs + bf + emf; }
+};
+
+// The device work list: per stage, the prefix offsets of every slot's work items.
+//   stage 0 k_prim (ghost-2 cells), 1 k_cell (ghost-1 cells), 2 k_border_flux (border faces),
+//   3 k_flux_tile (8^dim tiles), 4 k_edge (edges), 5 k_bupdate (faces)
+template <typename T>
+struct FusedBatch
+{
+  int              dim = 3;
+  int              nslots = 0;
+  FusedSlot<T> *   d_slots = nullptr;
+  long *           d_off[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
+  long             tot[6] = { 0, 0, 0, 0, 0, 0 };
+  int              nfix = 0;
+  EmfFix *         fix = nullptr;     // concatenated over slots, dst shifted to the batch EMF array
+  int64_t *        fix_src = nullptr;
+  T *              fixval = nullptr;
+  T *              emf_all = nullptr; // start of the batch EMF array (slot arrays are contiguous in it)
+  unsigned long long * flags = nullptr;
+  int              grid_cap = 0;      // persistent grid: at most this many workgroups per launch
+};
+
+template <typename T>
+int fused_launch(FusedBatch<T> & bt, hipStream_t st);
+
+} // namespace mhdct

@@ -1,0 +1,176 @@
+// fused_host.hip -- scratch and work lists of the fused form for one context (one mesh slot).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+#include "fused.hpp"
+
+namespace mhdct
+{
+
+void fused_free(Context & c);
+
+namespace
+{
+
+struct FusedState
+{
+  bool                      fp32 = false;
+  FusedBatch<double>        bd;
+  FusedBatch<float>         bf;
+  FusedSlot<double>         hd{};
+  FusedSlot<float>          hf{};
+  std::vector<void *>       allocs;
+  size_t                    bytes = 0;
+};
+
+template <typename T>
+int
+build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
+{
+  const int  dim = c.dim, b = c.b;
+  const long nq = c.dm.nowned;
+  if (b % 8)
+  {
+    c.err = "the fused form tiles blocks by 8 cells: block size must be a multiple of 8";
+    return -4;
+  }
+  auto alloc = [&](size_t bytes, void ** p) -> int {
+    if (bytes == 0)
+      bytes = 8;
+    if (hipMalloc(p, bytes) != hipSuccess)
+    {
+      c.err = "hipMalloc failed for fused scratch";
+      return -1;
+    }
+    (void)hipMemset(*p, 0, bytes);
+    F.allocs.push_back(*p);
+    F.bytes += bytes;
+    return 0;
+  };
+  const FusedSizes z(dim, b);
+  T *              arena = nullptr;
+  if (alloc(size_t(nq) * z.total() * sizeof(T), reinterpret_cast<void **>(&arena)))
+    return -1;
+  std::memset(&sl, 0, sizeof(sl));
+  sl.m = c.dm;
+  sl.gd = gas_of(c.st);
+  sl.gt.gamma0 = T(c.st.gamma0);
+  sl.gt.smallr = T(c.st.smallr);
+  sl.gt.smallc = T(c.st.smallc);
+  sl.gt.smallp = T(sl.gd.smallp);
+  sl.gt.slope_type = T(c.st.slope_type);
+  sl.scaling = c.st.scaling;
+  sl.abort_eint = c.st.abort_when_negative_eint;
+  T * p = arena;
+  sl.Qp = p;  p += nq * z.qp;
+  sl.Bg = p;  p += nq * z.bg;
+  sl.Q2 = p;  p += nq * z.q2;
+  sl.S = p;   p += nq * z.s;
+  sl.sF = p;  p += nq * z.sf;
+  sl.FS = p;  p += nq * z.fs;
+  sl.BF = p;  p += nq * z.bf;
+  sl.emf = p; p += nq * z.emf;
+  bt.dim = dim;
+  bt.nslots = 1;
+  bt.emf_all = sl.emf;
+  bt.flags = c.red + 1;
+  const long g2 = dim == 3 ? long(b + 4) * (b + 4) * (b + 4) : long(b + 4) * (b + 4);
+  const long g1 = dim == 3 ? long(b + 2) * (b + 2) * (b + 2) : long(b + 2) * (b + 2);
+  const long nt = dim == 3 ? long(b) * b : long(b);
+  const long ntile = dim == 3 ? long(b / 8) * (b / 8) * (b / 8) : long(b / 8) * (b / 8);
+  const long n1 = b + 1;
+  const long nedge = dim == 3 ? 3 * long(b) * n1 * n1 : n1 * n1;
+  const long nface = dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
+  const long per[6] = { g2, g1, dim * 2 * nt, ntile, nedge, nface };
+  for (int k = 0; k < 6; ++k)
+  {
+    sl.cnt[k] = nq * per[k];
+    bt.tot[k] = (sl.cnt[k] + 511) / 512 * 512; // whole workgroups per slot: the slot is wave-uniform
+    long off[2] = { 0, bt.tot[k] };
+    if (alloc(sizeof(off), reinterpret_cast<void **>(&bt.d_off[k])))
+      return -1;
+    if (hipMemcpy(bt.d_off[k], off, sizeof(off), hipMemcpyHostToDevice) != hipSuccess)
+      return -1;
+  }
+  if (alloc(sizeof(FusedSlot<T>), reinterpret_cast<void **>(&bt.d_slots)))
+    return -1;
+  bt.nfix = c.dm.nfix;
+  bt.fix = c.dm.fix;
+  bt.fix_src = c.dm.fix_src;
+  if (alloc(size_t(bt.nfix > 0 ? bt.nfix : 1) * sizeof(T), reinterpret_cast<void **>(&bt.fixval)))
+    return -1;
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  bt.grid_cap = ncu > 0 ? ncu * 16 : 4096;
+  return 0;
+}
+
+template <typename T>
+int
+step(Context & c, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, double * U2, const double * B,
+     double * B2, double dt, hipStream_t s)
+{
+  sl.U = U;
+  sl.U2 = U2;
+  sl.B = B;
+  sl.B2 = B2;
+  sl.dt = dt;
+  if (hipMemcpyAsync(bt.d_slots, &sl, sizeof(sl), hipMemcpyHostToDevice, s) != hipSuccess)
+  {
+    c.err = "slot upload failed";
+    return -1;
+  }
+  if (fused_launch<T>(bt, s))
+  {
+    c.err = std::string("fused launch failed: ") + hipGetErrorString(hipGetLastError());
+    return -1;
+  }
+  return 0;
+}
+
+} // namespace
+
+int
+fused_alloc(Context & c)
+{
+  fused_free(c);
+  auto * F = new FusedState();
+  F->fp32 = c.impl == 12;
+  c.fused = F;
+  const int rc = F->fp32 ? build<float>(c, *F, F->bf, F->hf) : build<double>(c, *F, F->bd, F->hd);
+  c.ps.bytes = F->bytes;
+  return rc;
+}
+
+void
+fused_free(Context & c)
+{
+  auto * F = static_cast<FusedState *>(c.fused);
+  if (!F)
+    return;
+  for (void * p : F->allocs)
+    (void)hipFree(p);
+  delete F;
+  c.fused = nullptr;
+}
+
+int
+fused_step(Context & c, const double * U, double * U2, const double * B, double * B2, double dt, hipStream_t s)
+{
+  auto * F = static_cast<FusedState *>(c.fused);
+  if (!F)
+  {
+    c.err = "fused form not allocated (set a mesh first)";
+    return -1;
+  }
+  if (c.dm.nowned == 0)
+    return 0;
+  return F->fp32 ? step<float>(c, F->bf, F->hf, U, U2, B, B2, dt, s) : step<double>(c, F->bd, F->hd, U, U2, B, B2, dt, s);
+}
+
+} // namespace mhdct

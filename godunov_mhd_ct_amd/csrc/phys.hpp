@@ -18,6 +18,13 @@
 namespace mhdct
 {
 
+// type-preserving overloads for both double and float bulk types
+using std::copysign;
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+
 // cell state component order (MHD::VarId, models/MHD.h:35-62)
 enum : int
 {

@@ -7,6 +7,7 @@
 
 #include <cstdio>
 
+#include "common.hpp"
 #include "ctx.hpp"
 
 namespace mhdct
@@ -24,119 +25,9 @@ grid_of(long n)
 }
 
 // ---------------------------------------------------------------------------
-// index helpers (layout A0: block slowest, then variable, then cell)
-// ---------------------------------------------------------------------------
-struct Idx
-{
-  int dim, b;
-
-  __device__ long cells() const { return dim == 3 ? long(b) * b * b : long(b) * b; }
-  __device__ long cell(int i, int j, int k) const { return i + long(b) * (j + long(b) * k); }
-  __device__ long gn(int g) const { return b + 2 * g; }
-  __device__ long gcells(int g) const
-  {
-    const long n = b + 2 * g;
-    return dim == 3 ? n * n * n : n * n;
-  }
-  __device__ long gcell(int g, int i, int j, int k) const
-  {
-    const long n = b + 2 * g;
-    return (i + g) + n * ((j + g) + n * (dim == 3 ? (k + g) : 0));
-  }
-  __device__ long fsize(int c) const
-  {
-    if (dim == 2)
-      return c == 2 ? long(b) * b : long(b + 1) * b;
-    return long(b + 1) * b * b;
-  }
-  __device__ long foff(int c) const { return c == 0 ? 0 : (c == 1 ? fsize(0) : fsize(0) + fsize(1)); }
-  __device__ long fper() const { return fsize(0) + fsize(1) + fsize(2); }
-  __device__ long face(int c, int i, int j, int k) const
-  {
-    const long nx = c == 0 ? b + 1 : b;
-    const long ny = c == 1 ? b + 1 : b;
-    return foff(c) + i + nx * (j + ny * (dim == 3 ? k : 0));
-  }
-  __device__ long gfsize(int c) const
-  {
-    const long n = b + 4;
-    if (dim == 2)
-      return c == 2 ? n * n : (n + 1) * n;
-    return (n + 1) * n * n;
-  }
-  __device__ long gfoff(int c) const { return c == 0 ? 0 : (c == 1 ? gfsize(0) : gfsize(0) + gfsize(1)); }
-  __device__ long gfper() const { return gfsize(0) + gfsize(1) + gfsize(2); }
-  __device__ long gface(int c, int i, int j, int k) const
-  {
-    const long n = b + 4;
-    const long nx = c == 0 ? n + 1 : n;
-    const long ny = c == 1 ? n + 1 : n;
-    return gfoff(c) + (i + 2) + nx * ((j + 2) + ny * (dim == 3 ? (k + 2) : 0));
-  }
-  __device__ long ecells() const
-  {
-    const long n = b + 1;
-    return dim == 3 ? n * n * n : n * n;
-  }
-  __device__ long edge(int i, int j, int k) const
-  {
-    const long n = b + 1;
-    return i + n * (j + n * k);
-  }
-};
-
-struct Hit
-{
-  int oct;
-  int rel;
-  int l[3];
-};
-
-// Leaf holding ghost cell c of block `oct` (c outside [0,b)^dim) from the neighbour table.
-__device__ inline Hit
-neighbour(const DevMesh & m, int oct, const int c[3])
-{
-  const int b = m.b;
-  int       dir = 0, sub = 0, mul = 1;
-  for (int d = 0; d < m.dim; ++d)
-  {
-    const int dl = c[d] < 0 ? -1 : (c[d] >= b ? 1 : 0);
-    dir += (dl + 1) * mul;
-    mul *= 3;
-    if (dl == 0 && c[d] >= b / 2)
-      sub |= 1 << d;
-  }
-  const int     ndir = m.dim == 3 ? 27 : 9;
-  const int     nsub = 1 << m.dim;
-  const NbEntry e = m.nb[(long(oct) * ndir + dir) * nsub + sub];
-  Hit           h;
-  h.oct = e.oct;
-  h.rel = e.rel;
-  for (int d = 0; d < 3; ++d)
-  {
-    const int cd = d < m.dim ? c[d] : 0;
-    if (d >= m.dim)
-      h.l[d] = 0;
-    else if (e.rel == 0)
-      h.l[d] = cd + e.off[d];
-    else if (e.rel < 0)
-      h.l[d] = (cd + e.off[d]) >> 1; // floor division by 2
-    else
-      h.l[d] = 2 * cd + e.off[d];
-  }
-  return h;
-}
-
-__device__ inline double
-dx_of(const DevMesh & m, double scaling, int oct)
-{
-  return 1.0 / double(long(m.b) << m.level[oct]) * scaling;
-}
-
-// ---------------------------------------------------------------------------
 // K1: face B with two ghost layers (FillBlockGhostFacesFunctor, GodunovImplemV0.cpp:224-235)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_fill_bface_ghosted(DevMesh m, const double * __restrict__ B, double * __restrict__ Bg)
 {
   const Idx  I{ m.dim, m.b };
@@ -157,148 +48,14 @@ k_fill_bface_ghosted(DevMesh m, const double * __restrict__ B, double * __restri
   const int nx = c == 0 ? n + 1 : n;
   const int ny = c == 1 ? n + 1 : n;
   int       f[3] = { int(r % nx) - 2, int((r / nx) % ny) - 2, dim == 3 ? int(r / (long(nx) * ny)) - 2 : 0 };
-  const bool cellc = (dim == 2 && c == 2);
-  bool       inner = true;
-  for (int d = 0; d < dim; ++d)
-  {
-    const int hi = (d == c && !cellc) ? b : b - 1;
-    if (f[d] < 0 || f[d] > hi)
-      inner = false;
-  }
-  const double * Bo = B + long(o) * I.fper();
-  double         v;
-  if (inner)
-  {
-    v = Bo[I.face(c, f[0], f[1], f[2])];
-  }
-  else if (cellc)
-  {
-    const Hit      h = neighbour(m, o, f);
-    const double * Bn = B + long(h.oct) * I.fper();
-    if (h.rel <= 0)
-      v = Bn[I.face(2, h.l[0], h.l[1], 0)];
-    else
-    {
-      double s = 0;
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di)
-          s += Bn[I.face(2, h.l[0] + di, h.l[1] + dj, 0)];
-      v = s / 4.0;
-    }
-  }
-  else
-  {
-    // the face separates cells lo (f[c]-1) and hi (f[c]); beyond the block along c take the
-    // nearer one, beside it prefer the leaf at this block's level, then the finer (ties: hi)
-    int  cc[3] = { f[0], f[1], f[2] };
-    bool high = true;
-    Hit  h;
-    if (f[c] > b)
-    {
-      cc[c] = f[c] - 1;
-      high = false;
-      h = neighbour(m, o, cc);
-    }
-    else if (f[c] < 0)
-      h = neighbour(m, o, cc);
-    else
-    {
-      const Hit hh = neighbour(m, o, cc);
-      int       lo[3] = { f[0], f[1], f[2] };
-      lo[c] -= 1;
-      const Hit hl = neighbour(m, o, lo);
-      // same level first, then finer, then coarser
-      const int rl = hl.rel == 0 ? 2 : (hl.rel > 0 ? 1 : 0);
-      const int rh = hh.rel == 0 ? 2 : (hh.rel > 0 ? 1 : 0);
-      if (rl > rh)
-      {
-        h = hl;
-        cc[c] = lo[c];
-        high = false;
-      }
-      else
-        h = hh;
-    }
-    const double * Bn = B + long(h.oct) * I.fper();
-    if (h.rel == 0)
-    {
-      int q[3] = { h.l[0], h.l[1], h.l[2] };
-      if (!high)
-        q[c] += 1;
-      v = Bn[I.face(c, q[0], q[1], q[2])];
-    }
-    else if (h.rel < 0)
-    {
-      // fine face position relative to the coarse block origin, in fine units
-      const int    ndir = dim == 3 ? 27 : 9;
-      int          dir = 0, sub = 0, mul = 1;
-      for (int d = 0; d < dim; ++d)
-      {
-        const int dl = cc[d] < 0 ? -1 : (cc[d] >= b ? 1 : 0);
-        dir += (dl + 1) * mul;
-        mul *= 3;
-        if (dl == 0 && cc[d] >= b / 2)
-          sub |= 1 << d;
-      }
-      const NbEntry e = m.nb[(long(o) * ndir + dir) * (1 << dim) + sub];
-      const int     F = f[c] + e.off[c];
-      int           q[3] = { h.l[0], h.l[1], h.l[2] };
-      if ((F & 1) == 0)
-      {
-        q[c] = F >> 1;
-        v = Bn[I.face(c, q[0], q[1], q[2])];
-      }
-      else
-      {
-        q[c] = (F - 1) >> 1;
-        const double lo = Bn[I.face(c, q[0], q[1], q[2])];
-        q[c] += 1;
-        const double hi = Bn[I.face(c, q[0], q[1], q[2])];
-        v = 0.5 * (lo + hi);
-      }
-    }
-    else
-    {
-      int base[3] = { h.l[0], h.l[1], h.l[2] };
-      if (!high)
-        base[c] += 2;
-      if (dim == 2)
-      {
-        const int t2 = c == 0 ? 1 : 0;
-        double    s = 0;
-        for (int a = 0; a < 2; ++a)
-        {
-          int q[3] = { base[0], base[1], 0 };
-          q[t2] += a;
-          s += Bn[I.face(c, q[0], q[1], 0)];
-        }
-        v = s / 2.0;
-      }
-      else
-      {
-        const int t1 = c == 0 ? 1 : 0;
-        const int t2 = c == 2 ? 1 : 2;
-        double    s = 0;
-        for (int a2 = 0; a2 < 2; ++a2)
-          for (int a1 = 0; a1 < 2; ++a1)
-          {
-            int q[3] = { base[0], base[1], base[2] };
-            q[t1] += a1;
-            q[t2] += a2;
-            s += Bn[I.face(c, q[0], q[1], q[2])];
-          }
-        v = s / 4.0;
-      }
-    }
-  }
-  Bg[long(o) * per + I.gface(c, f[0], f[1], f[2])] = v;
+  Bg[long(o) * per + I.gface(c, f[0], f[1], f[2])] = face_value_ghosted(m, I, B, o, c, f);
 }
 
 // ---------------------------------------------------------------------------
 // K3: conservative -> primitive with two ghost layers (ConvertToPrimitivesVariablesFunctor,
 // fill_ghosts .cpp:639-719; prolongation SIMPLE_COPY)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_convert_to_primitives(DevMesh m, Gas<double> g, int abort_eint, const double * __restrict__ U,
                         const double * __restrict__ B, const double * __restrict__ Bg,
                         double * __restrict__ Q, unsigned long long * flags)
@@ -456,7 +213,7 @@ slope_mag(const QView & q, int i, int j, int k, int dir, int v1, int v2, int o, 
   return limited_slope(dc, dp, dm, st);
 }
 
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_slopes(DevMesh m, double st, const double * __restrict__ Qp, double * Sx, double * Sy, double * Sz)
 {
   const Idx  I{ m.dim, m.b };
@@ -491,7 +248,7 @@ k_slopes(DevMesh m, double st, const double * __restrict__ Qp, double * Sx, doub
 // ---------------------------------------------------------------------------
 // K6: Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_predictor(DevMesh m, double scaling, double gamma, double dt, const double * __restrict__ Qp,
             const double * __restrict__ Sx, const double * __restrict__ Sy, const double * __restrict__ Sz,
             double * __restrict__ Q2)
@@ -561,7 +318,7 @@ k_predictor(DevMesh m, double scaling, double gamma, double dt, const double * _
 // ---------------------------------------------------------------------------
 // K7: edge electric field E = v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_elec_field(DevMesh m, const double * __restrict__ Qp, double * __restrict__ E)
 {
   const Idx  I{ m.dim, m.b };
@@ -611,7 +368,7 @@ k_elec_field(DevMesh m, const double * __restrict__ Qp, double * __restrict__ E)
 // ---------------------------------------------------------------------------
 // K8: face-B half-step source (ComputeSourceFaceMagFunctor.cpp:114-203)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_sfacemag(DevMesh m, double scaling, double dt, const double * __restrict__ E, double * __restrict__ sF)
 {
   const Idx  I{ m.dim, m.b };
@@ -645,7 +402,7 @@ k_sfacemag(DevMesh m, double scaling, double dt, const double * __restrict__ E, 
 // ---------------------------------------------------------------------------
 // K9: HLLD fluxes along one direction (ComputeHydroFluxesAndStoreFunctor.cpp:218-503)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_fluxes(DevMesh m, Gas<double> g, double scaling, double dt, int dir, const double * __restrict__ Qp,
          const double * __restrict__ Q2, const double * __restrict__ Sd, const double * __restrict__ sF,
          double * __restrict__ Fl)
@@ -711,7 +468,7 @@ k_fluxes(DevMesh m, Gas<double> g, double scaling, double dt, int dir, const dou
 // ---------------------------------------------------------------------------
 // K10: conservative update (ReadFluxesAndConservativeUpdateFunctor.cpp:105-436)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_update(DevMesh m, int dir, const double * __restrict__ Fl, double * __restrict__ U2, double * __restrict__ B2)
 {
   const Idx  I{ m.dim, m.b };
@@ -915,7 +672,7 @@ edge_state_3d(const EmfIn & in, const Idx & I, int is, int js, int ks, int o, in
   qo[kP] = fmax(in.g.smallp * qo[kD], qo[kP]);
 }
 
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_emf(DevMesh m, Gas<double> g, double scaling, double dt, const double * __restrict__ Qp,
       const double * __restrict__ Q2, const double * __restrict__ Sx, const double * __restrict__ Sy,
       const double * __restrict__ Sz, const double * __restrict__ sF, double * __restrict__ emf)
@@ -973,7 +730,7 @@ k_emf(DevMesh m, Gas<double> g, double scaling, double dt, const double * __rest
 // K13: border-edge EMF correction from the precomputed gather list
 // (CorrectEmfAtBlockBorderFunctor.cpp:94-360)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_emf_fix(int nfix, const EmfFix * __restrict__ fix, const int64_t * __restrict__ src,
           const double * __restrict__ emf, double * __restrict__ emf2)
 {
@@ -990,7 +747,7 @@ k_emf_fix(int nfix, const EmfFix * __restrict__ fix, const int64_t * __restrict_
 // ---------------------------------------------------------------------------
 // K14: face-B update by the discrete curl of the EMF (ReadEmfAndUpdateFunctor.cpp:58-163)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_emf_update(DevMesh m, const double * __restrict__ emf2, double * __restrict__ B2)
 {
   const Idx  I{ m.dim, m.b };
@@ -1048,7 +805,7 @@ k_emf_update(DevMesh m, const double * __restrict__ emf2, double * __restrict__ 
 // max, one atomic per workgroup on the bit pattern (inv_dt >= 0 so bits order like values).
 // A NaN candidate is left out of the max and raises error bit 2.
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_inv_dt(DevMesh m, Gas<double> g, double scaling, const double * __restrict__ U, const double * __restrict__ B,
          unsigned long long * red)
 {
@@ -1121,7 +878,7 @@ k_inv_dt(DevMesh m, Gas<double> g, double scaling, const double * __restrict__ U
 // ---------------------------------------------------------------------------
 // K16: outside blocks (FillOutsideCellFunctor, border_conditions/FillOutside.cpp:97-395)
 // ---------------------------------------------------------------------------
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_fill_outside_hydro(DevMesh m, double * __restrict__ U)
 {
   const Idx  I{ m.dim, m.b };
@@ -1158,7 +915,7 @@ k_fill_outside_hydro(DevMesh m, double * __restrict__ U)
   }
 }
 
-__global__ void
+__global__ void __launch_bounds__(kThreads)
 k_fill_outside_mag(DevMesh m, double * __restrict__ B)
 {
   const Idx  I{ m.dim, m.b };

@@ -15,6 +15,7 @@ from .mesh import Mesh
 
 IMPL_PLAIN = 10
 IMPL_FUSED = 11
+IMPL_FUSED_FP32 = 12
 
 RIEMANN = {"UNKNOWN": 0, "APPROX": 1, "LLF": 2, "HLL": 3, "HLLC": 4, "HLLD": 5}
 PROLONGATION = {"SIMPLE_COPY": 0, "EXTRAPOLATE_LINEAR_MINMOD": 1}
@@ -112,8 +113,8 @@ class GodunovImplemHIP:
 
 
 def create_godunov_implem(settings: Settings, dim: int, block_size: int, implementation_version: int):
-    if implementation_version not in (IMPL_PLAIN, IMPL_FUSED):
+    if implementation_version not in (IMPL_PLAIN, IMPL_FUSED, IMPL_FUSED_FP32):
         raise MhdctError(
             f"Wrong value for input parameter hydro/implementation_version={implementation_version} "
-            f"(allowed: {IMPL_PLAIN} plain, {IMPL_FUSED} fused)")
+            f"(allowed: {IMPL_PLAIN} plain, {IMPL_FUSED} fused, {IMPL_FUSED_FP32} fused FP32 bulk)")
     return GodunovImplemHIP(settings, dim, block_size, implementation_version)

@@ -59,7 +59,8 @@ typedef struct mhdct_settings
 
 /* implementation_version values ([hydro] implementation_version, SolverGodunovMHD.cpp:76) */
 #define MHDCT_IMPL_PLAIN 10 /* one kernel per stage, drop-in for GodunovImplemV0 */
-#define MHDCT_IMPL_FUSED 11 /* fused per-block kernels */
+#define MHDCT_IMPL_FUSED 11 /* fused stages, FP64 */
+#define MHDCT_IMPL_FUSED_FP32 12 /* fused stages, FP32 slopes/HLLD/EMF, FP64 primitive recovery, accumulation and dt */
 
 /* ---- host block octree (p4est brick forest replacement) ---------------------------- */
 

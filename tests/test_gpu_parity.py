@@ -72,7 +72,7 @@ def amr_mesh_3d(b=8):
     return refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.5 and y0 < 0.5 and z0 < 0.5, 2)
 
 
-IMPLS = [IMPL_PLAIN]
+IMPLS = [IMPL_PLAIN, IMPL_FUSED]
 
 
 @pytest.mark.gpu
