@@ -71,6 +71,51 @@ struct Idx
   }
 };
 
+// The same index arithmetic with the block size fixed at compile time (32-bit, constant
+// divisors); one instantiation per block size class.
+template <int DIM, int BS>
+struct IdxC
+{
+  static constexpr int dim = DIM;
+  static constexpr int b = BS;
+
+  __host__ __device__ static constexpr int cells() { return DIM == 3 ? BS * BS * BS : BS * BS; }
+  __host__ __device__ static constexpr int cell(int i, int j, int k) { return i + BS * (j + BS * k); }
+  __host__ __device__ static constexpr int gcells(int g)
+  {
+    return DIM == 3 ? (BS + 2 * g) * (BS + 2 * g) * (BS + 2 * g) : (BS + 2 * g) * (BS + 2 * g);
+  }
+  __host__ __device__ static constexpr int gcell(int g, int i, int j, int k)
+  {
+    return (i + g) + (BS + 2 * g) * ((j + g) + (BS + 2 * g) * (DIM == 3 ? (k + g) : 0));
+  }
+  __host__ __device__ static constexpr int fsize(int c)
+  {
+    return DIM == 2 ? (c == 2 ? BS * BS : (BS + 1) * BS) : (BS + 1) * BS * BS;
+  }
+  __host__ __device__ static constexpr int foff(int c) { return c == 0 ? 0 : (c == 1 ? fsize(0) : fsize(0) + fsize(1)); }
+  __host__ __device__ static constexpr int fper() { return fsize(0) + fsize(1) + fsize(2); }
+  __host__ __device__ static constexpr int face(int c, int i, int j, int k)
+  {
+    return foff(c) + i + (c == 0 ? BS + 1 : BS) * (j + (c == 1 ? BS + 1 : BS) * (DIM == 3 ? k : 0));
+  }
+  __host__ __device__ static constexpr int gfsize(int c)
+  {
+    return DIM == 2 ? (c == 2 ? (BS + 4) * (BS + 4) : (BS + 5) * (BS + 4)) : (BS + 5) * (BS + 4) * (BS + 4);
+  }
+  __host__ __device__ static constexpr int gfoff(int c)
+  {
+    return c == 0 ? 0 : (c == 1 ? gfsize(0) : gfsize(0) + gfsize(1));
+  }
+  __host__ __device__ static constexpr int gfper() { return gfsize(0) + gfsize(1) + gfsize(2); }
+  __host__ __device__ static constexpr int gface(int c, int i, int j, int k)
+  {
+    return gfoff(c) + (i + 2) + (c == 0 ? BS + 5 : BS + 4) * ((j + 2) + (c == 1 ? BS + 5 : BS + 4) * (DIM == 3 ? (k + 2) : 0));
+  }
+  __host__ __device__ static constexpr int ecells() { return DIM == 3 ? (BS + 1) * (BS + 1) * (BS + 1) : (BS + 1) * (BS + 1); }
+  __host__ __device__ static constexpr int edge(int i, int j, int k) { return i + (BS + 1) * (j + (BS + 1) * k); }
+};
+
 struct Hit
 {
   int oct;
@@ -79,12 +124,14 @@ struct Hit
 };
 
 // Leaf holding ghost cell c of block `oct` (c outside [0,b)^dim) from the neighbour table.
+// IX is the index type (Idx with run-time sizes, or IdxC with compile-time ones).
+template <class IX>
 __device__ inline Hit
-neighbour(const DevMesh & m, int oct, const int c[3])
+neighbour(const DevMesh & m, const IX & I, int oct, const int c[3])
 {
-  const int b = m.b;
+  const int b = I.b;
   int       dir = 0, sub = 0, mul = 1;
-  for (int d = 0; d < m.dim; ++d)
+  for (int d = 0; d < I.dim; ++d)
   {
     const int dl = c[d] < 0 ? -1 : (c[d] >= b ? 1 : 0);
     dir += (dl + 1) * mul;
@@ -92,16 +139,16 @@ neighbour(const DevMesh & m, int oct, const int c[3])
     if (dl == 0 && c[d] >= b / 2)
       sub |= 1 << d;
   }
-  const int     ndir = m.dim == 3 ? 27 : 9;
-  const int     nsub = 1 << m.dim;
+  const int     ndir = I.dim == 3 ? 27 : 9;
+  const int     nsub = 1 << I.dim;
   const NbEntry e = m.nb[(long(oct) * ndir + dir) * nsub + sub];
   Hit           h;
   h.oct = e.oct;
   h.rel = e.rel;
   for (int d = 0; d < 3; ++d)
   {
-    const int cd = d < m.dim ? c[d] : 0;
-    if (d >= m.dim)
+    const int cd = d < I.dim ? c[d] : 0;
+    if (d >= I.dim)
       h.l[d] = 0;
     else if (e.rel == 0)
       h.l[d] = cd + e.off[d];
@@ -111,6 +158,12 @@ neighbour(const DevMesh & m, int oct, const int c[3])
       h.l[d] = 2 * cd + e.off[d];
   }
   return h;
+}
+
+__device__ inline Hit
+neighbour(const DevMesh & m, int oct, const int c[3])
+{
+  return neighbour(m, Idx{ m.dim, m.b }, oct, c);
 }
 
 __device__ inline double
@@ -126,10 +179,11 @@ dx_of(const DevMesh & m, double scaling, int oct)
 // bracketing coarse faces; finer -> mean of the 2^(dim-1) fine faces.  A face beside the block
 // that lies on a leaf boundary takes the leaf at this block's level, else the finer one.
 // ---------------------------------------------------------------------------
+template <class IX>
 __device__ inline double
-face_value_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__ B, int o, int c, const int fin[3])
+face_value_ghosted(const DevMesh & m, const IX & I, const double * __restrict__ B, int o, int c, const int fin[3])
 {
-  const int b = m.b, dim = m.dim;
+  const int b = I.b, dim = I.dim;
   int       f[3] = { fin[0], fin[1], fin[2] };
   const bool cellc = (dim == 2 && c == 2);
   bool       inner = true;
@@ -147,7 +201,7 @@ face_value_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__
   }
   else if (cellc)
   {
-    const Hit      h = neighbour(m, o, f);
+    const Hit      h = neighbour(m, I, o, f);
     const double * Bn = B + long(h.oct) * I.fper();
     if (h.rel <= 0)
       v = Bn[I.face(2, h.l[0], h.l[1], 0)];
@@ -171,16 +225,16 @@ face_value_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__
     {
       cc[c] = f[c] - 1;
       high = false;
-      h = neighbour(m, o, cc);
+      h = neighbour(m, I, o, cc);
     }
     else if (f[c] < 0)
-      h = neighbour(m, o, cc);
+      h = neighbour(m, I, o, cc);
     else
     {
-      const Hit hh = neighbour(m, o, cc);
+      const Hit hh = neighbour(m, I, o, cc);
       int       lo[3] = { f[0], f[1], f[2] };
       lo[c] -= 1;
-      const Hit hl = neighbour(m, o, lo);
+      const Hit hl = neighbour(m, I, o, lo);
       // same level first, then finer, then coarser
       const int rl = hl.rel == 0 ? 2 : (hl.rel > 0 ? 1 : 0);
       const int rh = hh.rel == 0 ? 2 : (hh.rel > 0 ? 1 : 0);

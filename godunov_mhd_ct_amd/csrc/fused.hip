@@ -19,9 +19,11 @@
 // form, float for the mixed-precision form; primitive recovery from total energy, the
 // conservative accumulation and the face-B accumulation are FP64 in both.
 //
-// Every kernel walks a work list that may span several meshes ("slots") with different block
-// sizes: element kernels index (slot, block, element) by prefix offsets, tile kernels
-// (slot, block, tile).  Launches use a bounded, persistent grid (grid-stride loops).
+// Every kernel is instantiated per (dimension, block size): the block geometry is a set of
+// compile-time constants, so index arithmetic is 32-bit with constant divisors.  A launch walks
+// a work list of one size class that may span several meshes ("slots"): each slot's range of
+// work items is padded to whole 512-item groups, so the slot of a wavefront is uniform.
+// Launches use a bounded, persistent grid (grid-stride loops).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -39,7 +41,28 @@ namespace
 
 constexpr int kTS = 8; // tile edge of the flux kernel
 
-template <typename T>
+// compile-time geometry of one block size class
+template <int DIM, int BS>
+struct Geo
+{
+  using I = IdxC<DIM, BS>;
+  static constexpr int N2 = BS + 4;  // cells per axis with 2 ghost layers
+  static constexpr int N1 = BS + 2;  // with 1 ghost layer
+  static constexpr int NE = BS + 1;  // edge-grid points per axis
+  static constexpr int G2 = DIM == 3 ? N2 * N2 * N2 : N2 * N2;
+  static constexpr int G1 = DIM == 3 ? N1 * N1 * N1 : N1 * N1;
+  static constexpr int NC = I::cells();
+  static constexpr int NFS = DIM == 3 ? 6 : 2;           // transverse face-B slopes per cell
+  static constexpr int NT = DIM == 3 ? BS * BS : BS;      // faces on one block side
+  static constexpr int NCE = DIM == 3 ? 3 : 1;            // EMF components
+  static constexpr int NEC = I::ecells();                 // edge-grid points per component
+  static constexpr int EPD = DIM == 3 ? BS * NE * NE : NE * NE; // edges per direction
+  static constexpr int EPDP = (EPD + 63) / 64 * 64;       // ... padded to whole wavefronts
+  static constexpr int NB1 = BS / kTS;                    // flux tiles per axis
+  static constexpr int TPB = DIM == 3 ? NB1 * NB1 * NB1 : NB1 * NB1;
+  static_assert(BS % kTS == 0, "block size must be a multiple of the flux tile");
+};
+
 __device__ inline int
 slot_of(const long * off, int nslots, long t)
 {
@@ -49,13 +72,12 @@ slot_of(const long * off, int nslots, long t)
   return s;
 }
 
-// read-only views of one slot's arrays ------------------------------------------------------
-template <typename T, int DIM>
+// read-only views of one block of one slot
+template <typename T, int DIM, int BS>
 struct View
 {
-  // one block of one slot: base pointers and 32-bit strides (indices stay within a block)
+  using G = Geo<DIM, BS>;
   const FusedSlot<T> & S;
-  int                  b, n2, n1, G2, G1, gf, gs;
   const T *            p_qp;
   const T *            p_bg;
   const T *            p_q2;
@@ -64,67 +86,62 @@ struct View
   const T *            p_fs;
   __device__ View(const FusedSlot<T> & s, int o)
     : S(s)
+    , p_qp(s.Qp + long(o) * 5 * G::G2)
+    , p_bg(s.Bg + long(o) * G::I::gfper())
+    , p_q2(s.Q2 + long(o) * 8 * G::G1)
+    , p_sl(s.S + long(o) * DIM * 8 * G::G1)
+    , p_sf(s.sF + long(o) * DIM * G::G1)
+    , p_fs(s.FS + long(o) * G::NFS * G::G1)
   {
-    b = s.m.b;
-    n2 = b + 4;
-    n1 = b + 2;
-    G2 = DIM == 3 ? n2 * n2 * n2 : n2 * n2;
-    G1 = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
-    gs = DIM == 3 ? (n2 + 1) * n2 * n2 : (n2 + 1) * n2;
-    gf = DIM == 3 ? 3 * gs : 2 * gs + n2 * n2;
-    p_qp = s.Qp + long(o) * 5 * G2;
-    p_bg = s.Bg + long(o) * gf;
-    p_q2 = s.Q2 + long(o) * 8 * G1;
-    p_sl = s.S + long(o) * DIM * 8 * G1;
-    p_sf = s.sF + long(o) * DIM * G1;
-    p_fs = s.FS + long(o) * (DIM == 3 ? 6 : 2) * G1;
   }
-  __device__ int c2(int i, int j, int k) const { return (i + 2) + n2 * ((j + 2) + (DIM == 3 ? n2 * (k + 2) : 0)); }
-  __device__ int c1(int i, int j, int k) const { return (i + 1) + n1 * ((j + 1) + (DIM == 3 ? n1 * (k + 1) : 0)); }
-  __device__ int gface(int c, int i, int j, int k) const
+  __device__ static int c2(int i, int j, int k)
   {
-    const int nx = c == 0 ? n2 + 1 : n2;
-    const int ny = c == 1 ? n2 + 1 : n2;
-    return c * gs + (i + 2) + nx * ((j + 2) + (DIM == 3 ? ny * (k + 2) : 0));
+    return (i + 2) + G::N2 * ((j + 2) + (DIM == 3 ? G::N2 * (k + 2) : 0));
   }
-  __device__ T qp(int v, int i, int j, int k) const { return p_qp[v * G2 + c2(i, j, k)]; }
-  __device__ T bf(int c, int i, int j, int k) const { return p_bg[gface(c, i, j, k)]; }
-  __device__ T q2(int v, int c) const { return p_q2[v * G1 + c]; }
-  __device__ T sl(int d, int v, int c) const { return p_sl[(d * 8 + v) * G1 + c]; }
-  __device__ T sf(int d, int c) const { return p_sf[d * G1 + c]; }
-  __device__ T fs(int n, int c) const { return p_fs[n * G1 + c]; }
+  __device__ static int c1(int i, int j, int k)
+  {
+    return (i + 1) + G::N1 * ((j + 1) + (DIM == 3 ? G::N1 * (k + 1) : 0));
+  }
+  __device__ T qp(int v, int i, int j, int k) const { return p_qp[v * G::G2 + c2(i, j, k)]; }
+  __device__ T bf(int c, int i, int j, int k) const { return p_bg[G::I::gface(c, i, j, k)]; }
+  __device__ T q2(int v, int c) const { return p_q2[v * G::G1 + c]; }
+  __device__ T sl(int d, int v, int c) const { return p_sl[(d * 8 + v) * G::G1 + c]; }
+  __device__ T sf(int d, int c) const { return p_sf[d * G::G1 + c]; }
+  __device__ T fs(int n, int c) const { return p_fs[n * G::G1 + c]; }
 };
 
+template <int BS>
 __device__ inline double
 dxd(const DevMesh & m, double scaling, int o)
 {
-  return 1.0 / double(long(m.b) << m.level[o]) * scaling;
+  return 1.0 / double(long(BS) << m.level[o]) * scaling;
 }
 
 // ---------------------------------------------------------------------------------------------
 // k_prim: U, B -> primitive (rho, p, u, v, w) on the 2-ghost tile and the ghosted face field.
 // The cell owns its low faces (and its high faces on the last ghost layer).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM>
+template <typename T, int DIM, int BS>
 __global__ void __launch_bounds__(256)
 k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total,
        unsigned long long * flags)
 {
+  using G = Geo<DIM, BS>;
+  using IX = IdxC<DIM, BS>;
+  constexpr IX I{};
   for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
     const DevMesh &      m = S.m;
-    const Idx            I{ DIM, m.b };
-    const long           nc = I.gcells(2);
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[0])
       continue;
-    const int            o = int(r0 / nc);
-    const long           ci = r0 - long(o) * nc;
-    const int            b = m.b, n = b + 4;
-    const int i = int(ci % n) - 2, j = int((ci / n) % n) - 2, k = DIM == 3 ? int(ci / (long(n) * n)) - 2 : 0;
-    const long ncell = I.cells();
+    const int o = int(r0) / G::G2;
+    const int ci = int(r0) - o * G::G2;
+    constexpr int n = G::N2, b = BS;
+    const int i = ci % n - 2, j = (ci / n) % n - 2, k = DIM == 3 ? ci / (n * n) - 2 : 0;
+    constexpr int ncell = IX::cells();
     const double * __restrict__ U = S.U;
     const double * __restrict__ B = S.B;
     double     u[8];
@@ -135,27 +152,27 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     };
     if (inside)
     {
-      const long     cidx = I.cell(i, j, k);
-      const double * Bo = B + long(o) * I.fper();
+      const int      cidx = IX::cell(i, j, k);
+      const double * Bo = B + long(o) * IX::fper();
       for (int v = 0; v < 5; ++v)
         u[v] = U[(long(o) * 5 + v) * ncell + cidx];
       if (DIM == 2)
       {
-        u[kA] = 0.5 * (Bo[I.face(0, i, j, 0)] + Bo[I.face(0, i + 1, j, 0)]);
-        u[kB] = 0.5 * (Bo[I.face(1, i, j, 0)] + Bo[I.face(1, i, j + 1, 0)]);
-        u[kC] = 0.5 * (Bo[I.face(2, i, j, 0)] + Bo[I.face(2, i, j, 0)]);
+        u[kA] = 0.5 * (Bo[IX::face(0, i, j, 0)] + Bo[IX::face(0, i + 1, j, 0)]);
+        u[kB] = 0.5 * (Bo[IX::face(1, i, j, 0)] + Bo[IX::face(1, i, j + 1, 0)]);
+        u[kC] = 0.5 * (Bo[IX::face(2, i, j, 0)] + Bo[IX::face(2, i, j, 0)]);
       }
       else
       {
-        u[kA] = 0.5 * (Bo[I.face(0, i, j, k)] + Bo[I.face(0, i + 1, j, k)]);
-        u[kB] = 0.5 * (Bo[I.face(1, i, j, k)] + Bo[I.face(1, i, j + 1, k)]);
-        u[kC] = 0.5 * (Bo[I.face(2, i, j, k)] + Bo[I.face(2, i, j, k + 1)]);
+        u[kA] = 0.5 * (Bo[IX::face(0, i, j, k)] + Bo[IX::face(0, i + 1, j, k)]);
+        u[kB] = 0.5 * (Bo[IX::face(1, i, j, k)] + Bo[IX::face(1, i, j + 1, k)]);
+        u[kC] = 0.5 * (Bo[IX::face(2, i, j, k)] + Bo[IX::face(2, i, j, k + 1)]);
       }
     }
     else
     {
       const int c[3] = { i, j, k };
-      const Hit h = neighbour(m, o, c);
+      const Hit h = neighbour(m, I, o, c);
       if (h.rel > 0)
       {
         for (int v = 0; v < 5; ++v)
@@ -165,7 +182,7 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
           for (int dk = 0; dk < (DIM == 3 ? 2 : 1); ++dk)
             for (int dj = 0; dj < 2; ++dj)
               for (int di = 0; di < 2; ++di)
-                sum += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+                sum += Un[IX::cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
           u[v] = sum / double(1 << DIM);
         }
         if (DIM == 2)
@@ -183,22 +200,22 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       }
       else
       {
-        const long cidx = I.cell(h.l[0], h.l[1], h.l[2]);
+        const int cidx = IX::cell(h.l[0], h.l[1], h.l[2]);
         for (int v = 0; v < 5; ++v)
           u[v] = U[(long(h.oct) * 5 + v) * ncell + cidx];
-        const double * Bn = B + long(h.oct) * I.fper();
+        const double * Bn = B + long(h.oct) * IX::fper();
         const int      a = h.l[0], bb = h.l[1], cc = h.l[2];
         if (DIM == 2)
         {
-          u[kA] = 0.5 * (Bn[I.face(0, a, bb, 0)] + Bn[I.face(0, a + 1, bb, 0)]);
-          u[kB] = 0.5 * (Bn[I.face(1, a, bb, 0)] + Bn[I.face(1, a, bb + 1, 0)]);
-          u[kC] = 0.5 * (Bn[I.face(2, a, bb, 0)] + Bn[I.face(2, a, bb, 0)]);
+          u[kA] = 0.5 * (Bn[IX::face(0, a, bb, 0)] + Bn[IX::face(0, a + 1, bb, 0)]);
+          u[kB] = 0.5 * (Bn[IX::face(1, a, bb, 0)] + Bn[IX::face(1, a, bb + 1, 0)]);
+          u[kC] = 0.5 * (Bn[IX::face(2, a, bb, 0)] + Bn[IX::face(2, a, bb, 0)]);
         }
         else
         {
-          u[kA] = 0.5 * (Bn[I.face(0, a, bb, cc)] + Bn[I.face(0, a + 1, bb, cc)]);
-          u[kB] = 0.5 * (Bn[I.face(1, a, bb, cc)] + Bn[I.face(1, a, bb + 1, cc)]);
-          u[kC] = 0.5 * (Bn[I.face(2, a, bb, cc)] + Bn[I.face(2, a, bb, cc + 1)]);
+          u[kA] = 0.5 * (Bn[IX::face(0, a, bb, cc)] + Bn[IX::face(0, a + 1, bb, cc)]);
+          u[kB] = 0.5 * (Bn[IX::face(1, a, bb, cc)] + Bn[IX::face(1, a, bb + 1, cc)]);
+          u[kC] = 0.5 * (Bn[IX::face(2, a, bb, cc)] + Bn[IX::face(2, a, bb, cc + 1)]);
         }
       }
     }
@@ -206,32 +223,32 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     const double eint = to_primitive(u, q, S.gd);
     if (S.abort_eint && eint < 0)
       atomicOr(flags, 1ull);
-    T * Qo = S.Qp + long(o) * 5 * nc;
+    T * Qo = S.Qp + long(o) * 5 * G::G2;
     for (int v = 0; v < 5; ++v)
-      Qo[v * nc + ci] = T(q[v]);
+      Qo[v * G::G2 + ci] = T(q[v]);
     // face field: low faces of this cell, and the high faces on the last ghost layer
-    T * Go = S.Bg + long(o) * I.gfper();
+    T * Go = S.Bg + long(o) * IX::gfper();
     if (DIM == 2)
     {
-      Go[I.gface(0, i, j, 0)] = T(gfv(0, i, j, 0));
-      Go[I.gface(1, i, j, 0)] = T(gfv(1, i, j, 0));
-      Go[I.gface(2, i, j, 0)] = T(gfv(2, i, j, 0));
+      Go[IX::gface(0, i, j, 0)] = T(gfv(0, i, j, 0));
+      Go[IX::gface(1, i, j, 0)] = T(gfv(1, i, j, 0));
+      Go[IX::gface(2, i, j, 0)] = T(gfv(2, i, j, 0));
       if (i == b + 1)
-        Go[I.gface(0, b + 2, j, 0)] = T(gfv(0, b + 2, j, 0));
+        Go[IX::gface(0, b + 2, j, 0)] = T(gfv(0, b + 2, j, 0));
       if (j == b + 1)
-        Go[I.gface(1, i, b + 2, 0)] = T(gfv(1, i, b + 2, 0));
+        Go[IX::gface(1, i, b + 2, 0)] = T(gfv(1, i, b + 2, 0));
     }
     else
     {
-      Go[I.gface(0, i, j, k)] = T(gfv(0, i, j, k));
-      Go[I.gface(1, i, j, k)] = T(gfv(1, i, j, k));
-      Go[I.gface(2, i, j, k)] = T(gfv(2, i, j, k));
+      Go[IX::gface(0, i, j, k)] = T(gfv(0, i, j, k));
+      Go[IX::gface(1, i, j, k)] = T(gfv(1, i, j, k));
+      Go[IX::gface(2, i, j, k)] = T(gfv(2, i, j, k));
       if (i == b + 1)
-        Go[I.gface(0, b + 2, j, k)] = T(gfv(0, b + 2, j, k));
+        Go[IX::gface(0, b + 2, j, k)] = T(gfv(0, b + 2, j, k));
       if (j == b + 1)
-        Go[I.gface(1, i, b + 2, k)] = T(gfv(1, i, b + 2, k));
+        Go[IX::gface(1, i, b + 2, k)] = T(gfv(1, i, b + 2, k));
       if (k == b + 1)
-        Go[I.gface(2, i, j, b + 2)] = T(gfv(2, i, j, b + 2));
+        Go[IX::gface(2, i, j, b + 2)] = T(gfv(2, i, j, b + 2));
     }
   }
 }
@@ -239,25 +256,24 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
 // ---------------------------------------------------------------------------------------------
 // k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM>
+template <typename T, int DIM, int BS>
 __global__ void __launch_bounds__(256)
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
+  using G = Geo<DIM, BS>;
   for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
     const DevMesh &      m = S.m;
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[1])
       continue;
-    const int            n = m.b + 2;
-    const int            g1 = DIM == 3 ? n * n * n : n * n;
-    const int            o = int(r0 / g1);
-    const int            ci = int(r0 - long(o) * g1);
-    const View<T, DIM>   V(S, o);
-    const int            b = m.b;
-    const int i = int(ci % n) - 1, j = int((ci / n) % n) - 1, k = DIM == 3 ? int(ci / (long(n) * n)) - 1 : 0;
+    constexpr int             n = G::N1;
+    const int                 o = int(r0) / G::G1;
+    const int                 ci = int(r0) - o * G::G1;
+    const View<T, DIM, BS>    V(S, o);
+    const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
     const T   st = S.gt.slope_type;
     // the 11-variable primitive state of cell (a, c, e): 5 hydro + low/high face fields
     auto Q = [&](int v, int a, int c, int e) -> T {
@@ -269,11 +285,10 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
         return V.bf(2, a, c, 0);
       return V.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
     };
-    const long c1 = ci;
-    T *        Sd = S.S + long(o) * DIM * 8 * V.G1;
-    T          sl[3][8];
-    const int  nh = DIM == 3 ? 5 : 4;
-    const int  e3 = DIM == 3 ? 1 : 0;
+    const int c1 = ci;
+    T *       Sd = S.S + long(o) * DIM * 8 * G::G1;
+    T         sl[3][8];
+    constexpr int nh = DIM == 3 ? 5 : 4;
     for (int d = 0; d < DIM; ++d)
     {
       const int di = d == 0, dj = d == 1, dk = d == 2;
@@ -295,11 +310,10 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
         }
       }
       for (int v = 0; v < 8; ++v)
-        Sd[(d * 8 + v) * V.G1 + c1] = sl[d][v];
+        Sd[(d * 8 + v) * G::G1 + c1] = sl[d][v];
     }
-    (void)e3;
     // Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
-    const double dxo = dxd(m, S.scaling, o);
+    const double dxo = dxd<BS>(m, S.scaling, o);
     const T      dtdx = T(S.dt / dxo), dtdy = dtdx, dtdz = dtdx;
     const T      gamma = S.gt.gamma0;
     const T r = Q(0, i, j, k), p = Q(1, i, j, k), u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
@@ -338,15 +352,15 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
       sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
     }
-    T * q2 = S.Q2 + long(o) * 8 * V.G1 + c1;
-    q2[0 * V.G1] = r + T(0.5) * sr0;
-    q2[2 * V.G1] = u + T(0.5) * su0;
-    q2[3 * V.G1] = v + T(0.5) * sv0;
-    q2[4 * V.G1] = w + T(0.5) * sw0;
-    q2[1 * V.G1] = p + T(0.5) * sp0;
-    q2[5 * V.G1] = A + T(0.5) * sA0;
-    q2[6 * V.G1] = B + T(0.5) * sB0;
-    q2[7 * V.G1] = C + T(0.5) * sC0;
+    T * q2 = S.Q2 + long(o) * 8 * G::G1 + c1;
+    q2[0 * G::G1] = r + T(0.5) * sr0;
+    q2[2 * G::G1] = u + T(0.5) * su0;
+    q2[3 * G::G1] = v + T(0.5) * sv0;
+    q2[4 * G::G1] = w + T(0.5) * sw0;
+    q2[1 * G::G1] = p + T(0.5) * sp0;
+    q2[5 * G::G1] = A + T(0.5) * sA0;
+    q2[6 * G::G1] = B + T(0.5) * sB0;
+    q2[7 * G::G1] = C + T(0.5) * sC0;
     // edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235), evaluated where
     // the face source needs it
     auto Ex = [&](int a, int c, int e) -> T {
@@ -370,76 +384,77 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
       return uu * BB - vv * AA;
     };
-    T * sF = S.sF + long(o) * DIM * V.G1 + c1;
+    T * sF = S.sF + long(o) * DIM * G::G1 + c1;
     if (DIM == 2)
     {
       const T e00 = Ez(i, j, 0);
       sF[0] = (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy;
-      sF[V.G1] = -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx;
+      sF[G::G1] = -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx;
     }
     else
     {
       const T ex = Ex(i, j, k), ey = Ey(i, j, k), ez = Ez(i, j, k);
       sF[0] = (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz;
-      sF[V.G1] = -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz;
-      sF[2 * V.G1] = (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy;
+      sF[G::G1] = -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz;
+      sF[2 * G::G1] = (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy;
     }
     // transverse slopes of the face field at this cell's low faces (dBx/dy, dBx/dz, dBy/dx,
     // dBy/dz, dBz/dx, dBz/dy; 2D: dBx/dy, dBy/dx)
-    T * fs = S.FS + long(o) * (DIM == 3 ? 6 : 2) * V.G1 + c1;
+    T * fs = S.FS + long(o) * G::NFS * G::G1 + c1;
     if (DIM == 2)
     {
       fs[0] = limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st);
-      fs[V.G1] = limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st);
+      fs[G::G1] = limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st);
     }
     else
     {
       fs[0] = limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st);
-      fs[1 * V.G1] = limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st);
-      fs[2 * V.G1] = limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st);
-      fs[3 * V.G1] = limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st);
-      fs[4 * V.G1] = limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st);
-      fs[5 * V.G1] = limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st);
+      fs[1 * G::G1] = limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st);
+      fs[2 * G::G1] = limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st);
+      fs[3 * G::G1] = limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st);
+      fs[4 * G::G1] = limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st);
+      fs[5 * G::G1] = limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st);
     }
   }
 }
 
-// HLLD flux (times dt/dx) through the low face of cell (i,j,k) along `dir`
+// HLLD flux (times dt/dx) through the low face of cell (i,j,k) along DIR
 // (ComputeHydroFluxesAndStoreFunctor.cpp:218-503); out in the reference variable order.
-template <typename T, int DIM>
+template <typename T, int DIM, int BS, int DIR>
 __device__ inline void
-face_flux(const View<T, DIM> & V, int o, int dir, int i, int j, int k, T dtdS, T * f)
+face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
 {
-  const int cR = V.c1(i, j, k);
-  const int cL = V.c1(i - (dir == 0), j - (dir == 1), k - (dir == 2));
-  T          qL[8], qR[8];
+  using VW = View<T, DIM, BS>;
+  const int cR = VW::c1(i, j, k);
+  const int cL = VW::c1(i - (DIR == 0), j - (DIR == 1), k - (DIR == 2));
+  T         qL[8], qR[8];
   for (int v = 0; v < 8; ++v)
   {
-    qL[v] = V.q2(v, cL) + T(0.5) * V.sl(dir, v, cL);
-    qR[v] = V.q2(v, cR) + T(-0.5) * V.sl(dir, v, cR);
+    qL[v] = V.q2(v, cL) + T(0.5) * V.sl(DIR, v, cL);
+    qR[v] = V.q2(v, cR) + T(-0.5) * V.sl(DIR, v, cR);
   }
   const Gas<T> & g = V.S.gt;
   qL[kD] = fmax(g.smallr, qL[kD]);
   qL[kP] = fmax(g.smallp * qL[kD], qL[kP]);
   qR[kD] = fmax(g.smallr, qR[kD]);
   qR[kP] = fmax(g.smallp * qR[kD], qR[kP]);
-  const T bn = V.bf(DIM == 2 && dir == 2 ? 2 : dir, i, j, k) + V.sf(dir, cR);
-  qR[kA + dir] = bn;
-  qL[kA + dir] = bn;
-  if (dir != 0)
+  const T bn = V.bf(DIM == 2 && DIR == 2 ? 2 : DIR, i, j, k) + V.sf(DIR, cR);
+  qR[kA + DIR] = bn;
+  qL[kA + DIR] = bn;
+  if (DIR != 0)
   {
-    const int vt = kU + dir, bt = kA + dir;
-    T         x;
+    constexpr int vt = kU + DIR, bt = kA + DIR;
+    T             x;
     x = qL[kU]; qL[kU] = qL[vt]; qL[vt] = x;
     x = qR[kU]; qR[kU] = qR[vt]; qR[vt] = x;
     x = qL[kA]; qL[kA] = qL[bt]; qL[bt] = x;
     x = qR[kA]; qR[kA] = qR[bt]; qR[bt] = x;
   }
   hlld(qL, qR, f, g);
-  if (dir != 0)
+  if (DIR != 0)
   {
-    const int vt = kU + dir, bt = kA + dir;
-    T         x;
+    constexpr int vt = kU + DIR, bt = kA + DIR;
+    T             x;
     x = f[kU]; f[kU] = f[vt]; f[vt] = x;
     x = f[kA]; f[kA] = f[bt]; f[bt] = x;
   }
@@ -450,158 +465,192 @@ face_flux(const View<T, DIM> & V, int o, int dir, int i, int j, int k, T dtdS, T
 // the six stored flux components: rho, E, mx, my, mz, Bz (2D, cell-centred Bz)
 __device__ constexpr int kFluxVar[6] = { kD, kP, kU, kV, kW, kC };
 
-template <typename T, int DIM>
+template <int DIM, int BS>
 __device__ inline long
-bf_index(int b, int o, int dir, int side, int v, long tpos)
+bf_index(int o, int dir, int side, int v, int tpos)
 {
-  const long nt = DIM == 3 ? long(b) * b : long(b);
-  return ((((long(o) * DIM + dir) * 2 + side) * 6 + v) * nt) + tpos;
+  constexpr int nt = Geo<DIM, BS>::NT;
+  return long((((o * DIM + dir) * 2 + side) * 6 + v)) * nt + tpos;
 }
 
 // ---------------------------------------------------------------------------------------------
 // k_border_flux: HLLD on every block-border face (both sides, every direction)
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM>
+template <typename T, int DIM, int BS, int DIR>
+__device__ inline void
+border_face(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int side, int tp)
+{
+  constexpr int t0 = DIR == 0 ? 1 : 0;
+  constexpr int t1 = DIR == 2 ? 1 : 2;
+  int           c[3] = { 0, 0, 0 };
+  c[t0] = tp % BS;
+  if (DIM == 3)
+    c[t1] = tp / BS;
+  c[DIR] = side ? BS : 0;
+  T       f[8];
+  const T dtdS = T(S.dt / dxd<BS>(S.m, S.scaling, o));
+  face_flux<T, DIM, BS, DIR>(V, c[0], c[1], c[2], dtdS, f);
+  for (int v = 0; v < (DIM == 2 ? 6 : 5); ++v)
+    S.BF[bf_index<DIM, BS>(o, DIR, side, v, tp)] = f[kFluxVar[v]];
+}
+
+template <typename T, int DIM, int BS>
 __global__ void __launch_bounds__(256)
 k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
+  using G = Geo<DIM, BS>;
   for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
-    const int            b = S.m.b;
-    const long           nt = DIM == 3 ? long(b) * b : long(b);
-    const long           per = DIM * 2 * nt;
+    constexpr int        per = DIM * 2 * G::NT;
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[2])
       continue;
-    const int            o = int(r0 / per);
-    const View<T, DIM>   V(S, o);
-    long                 r = r0 - long(o) * per;
-    const int            dir = int(r / (2 * nt));
-    r -= long(dir) * 2 * nt;
-    const int  side = int(r / nt);
-    const long tp = r - long(side) * nt;
-    int        tt[2], ntt = 0;
-    for (int d = 0; d < DIM; ++d)
-      if (d != dir)
-        tt[ntt++] = d;
-    int c[3] = { 0, 0, 0 };
-    c[tt[0]] = int(tp % b);
-    if (DIM == 3)
-      c[tt[1]] = int(tp / b);
-    c[dir] = side ? b : 0;
-    T         f[8];
-    const T   dtdS = T(S.dt / dxd(S.m, S.scaling, o));
-    face_flux<T, DIM>(V, o, dir, c[0], c[1], c[2], dtdS, f);
-    for (int v = 0; v < (DIM == 2 ? 6 : 5); ++v)
-      S.BF[bf_index<T, DIM>(b, o, dir, side, v, tp)] = f[kFluxVar[v]];
+    const int              o = int(r0) / per;
+    const View<T, DIM, BS> V(S, o);
+    int                    r = int(r0) - o * per;
+    const int              dir = r / (2 * G::NT);
+    r -= dir * 2 * G::NT;
+    const int side = r / G::NT;
+    const int tp = r - side * G::NT;
+    if (dir == 0)
+      border_face<T, DIM, BS, 0>(S, V, o, side, tp);
+    else if (dir == 1)
+      border_face<T, DIM, BS, 1>(S, V, o, side, tp);
+    else if (DIM == 3)
+      border_face<T, DIM, BS, 2>(S, V, o, side, tp);
   }
 }
 
 // ---------------------------------------------------------------------------------------------
 // k_flux_tile: one workgroup per 8^dim tile; HLLD on the tile faces, one direction at a time
-// through LDS; U2 = ((U + dX) + dY) + dZ accumulated in FP64 registers.
+// through LDS; U2 = ((U + dX) + dY) + dZ accumulated in FP64 registers.  The workgroup has one
+// thread per face of a direction (3D: 9*8*8 = 576), the first 8^dim of them also own a cell.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM>
-__global__ void __launch_bounds__(DIM == 3 ? 512 : 64)
-k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+template <int DIM>
+constexpr int
+tile_threads()
+{
+  return DIM == 3 ? (kTS + 1) * kTS * kTS : 128;
+}
+
+template <typename T, int DIM, int BS, int DIR>
+__device__ inline void
+tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int org[3], T dtdS, T (*lf)[(kTS + 1) * (DIM == 3 ? kTS * kTS : kTS)])
 {
   constexpr int TS = kTS;
-  constexpr int NT = DIM == 3 ? TS * TS * TS : TS * TS;
+  constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
+  constexpr int NV = DIM == 2 ? 6 : 5;
+  constexpr int e0 = DIR == 0 ? TS + 1 : TS;
+  constexpr int e1 = DIR == 1 ? TS + 1 : TS;
+  constexpr int t0 = DIR == 0 ? 1 : 0;
+  constexpr int t1 = DIR == 2 ? 1 : 2;
+  constexpr int NTHR = tile_threads<DIM>();
+  const DevMesh & m = S.m;
+  for (int fl = threadIdx.x; fl < NF; fl += NTHR)
+  {
+    const int a0 = fl % e0, a1 = (fl / e0) % e1, a2 = DIM == 3 ? fl / (e0 * e1) : 0;
+    int       c[3] = { org[0] + a0, org[1] + a1, org[2] + a2 };
+    T         out[NV];
+    if (c[DIR] == 0 || c[DIR] == BS)
+    {
+      const int side = c[DIR] == BS;
+      const int tp = DIM == 3 ? c[t0] + BS * c[t1] : c[t0];
+      if (m.conformal[long(o) * 2 * DIM + 2 * DIR + side] != 0)
+      {
+        // sum of the fine faces covering this coarse face, divided by 2^dim
+        int cc[3] = { c[0], c[1], c[2] };
+        cc[DIR] = side == 0 ? -1 : BS;
+        const Hit    h = neighbour(m, IdxC<DIM, BS>{}, o, cc);
+        const double vr = double(1 << DIM);
+        for (int vi = 0; vi < NV; ++vi)
+        {
+          double sum = 0;
+          if (DIM == 2)
+          {
+            for (int a = 0; a < 2; ++a)
+              sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, h.l[t0] + a)];
+          }
+          else
+          {
+            for (int q1 = 0; q1 < 2; ++q1)
+              for (int q0 = 0; q0 < 2; ++q0)
+                sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, (h.l[t0] + q0) + BS * (h.l[t1] + q1))];
+          }
+          out[vi] = T(sum / vr);
+        }
+      }
+      else
+      {
+        for (int vi = 0; vi < NV; ++vi)
+          out[vi] = S.BF[bf_index<DIM, BS>(o, DIR, side, vi, tp)];
+      }
+    }
+    else
+    {
+      T f[8];
+      face_flux<T, DIM, BS, DIR>(V, c[0], c[1], c[2], dtdS, f);
+      for (int vi = 0; vi < NV; ++vi)
+        out[vi] = f[kFluxVar[vi]];
+    }
+    for (int vi = 0; vi < NV; ++vi)
+      lf[vi][fl] = out[vi];
+  }
+}
+
+template <typename T, int DIM, int BS>
+__global__ void __launch_bounds__(tile_threads<DIM>())
+k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  using G = Geo<DIM, BS>;
+  using IX = IdxC<DIM, BS>;
+  constexpr int TS = kTS;
+  constexpr int NCT = DIM == 3 ? TS * TS * TS : TS * TS;
   constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
   constexpr int NV = DIM == 2 ? 6 : 5;
   __shared__ T  lf[NV][NF];
   for (long t = blockIdx.x; t < total; t += gridDim.x)
   {
-    const int            s = slot_of<T>(off, nslots, t);
+    const int            s = slot_of(off, nslots, t);
     const FusedSlot<T> & S = slots[s];
     const DevMesh &      m = S.m;
-    const int            b = m.b;
-    const int            nb1 = b / TS;
-    const int            tpb = DIM == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[3])
       continue;
-    const int            o = int(r0 / tpb);
-    const View<T, DIM>   V(S, o);
-    const int            tl = int(r0 - long(o) * tpb);
+    const int              o = int(r0) / G::TPB;
+    const View<T, DIM, BS> V(S, o);
+    const int              tl = int(r0) - o * G::TPB;
+    constexpr int          nb1 = G::NB1;
     const int org[3] = { (tl % nb1) * TS, ((tl / nb1) % nb1) * TS, DIM == 3 ? (tl / (nb1 * nb1)) * TS : 0 };
     const int tid = threadIdx.x;
+    const bool owner = tid < NCT;
     const int li = tid % TS, lj = (tid / TS) % TS, lk = DIM == 3 ? tid / (TS * TS) : 0;
     const int ci = org[0] + li, cj = org[1] + lj, ck = org[2] + lk;
-    const Idx I{ DIM, b };
-    const long ncell = I.cells();
-    const long cidx = I.cell(ci, cj, ck);
-    double     acc[NV];
-    for (int v = 0; v < 5; ++v)
-      acc[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
-    if (DIM == 2)
-      acc[5] = S.B[long(o) * I.fper() + I.face(2, ci, cj, 0)];
-    const T      dtdS = T(S.dt / dxd(m, S.scaling, o));
-    const double vr = double(1 << DIM);
+    constexpr int ncell = IX::cells();
+    const int     cidx = IX::cell(ci, cj, ck);
+    double        acc[NV];
+    if (owner)
+    {
+      for (int v = 0; v < 5; ++v)
+        acc[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
+      if (DIM == 2)
+        acc[5] = S.B[long(o) * IX::fper() + IX::face(2, ci, cj, 0)];
+    }
+    const T dtdS = T(S.dt / dxd<BS>(m, S.scaling, o));
     for (int dir = 0; dir < DIM; ++dir)
     {
-      // extent of the face set along each axis
-      int ext[3] = { TS, TS, DIM == 3 ? TS : 1 };
-      ext[dir] = TS + 1;
-      const bool lo_fine = m.conformal[long(o) * 2 * DIM + 2 * dir] != 0;
-      const bool hi_fine = m.conformal[long(o) * 2 * DIM + 2 * dir + 1] != 0;
-      int        tt[2], ntt = 0;
-      for (int d = 0; d < DIM; ++d)
-        if (d != dir)
-          tt[ntt++] = d;
-      for (int fl = tid; fl < NF; fl += NT)
-      {
-        const int a0 = fl % ext[0], a1 = (fl / ext[0]) % ext[1], a2 = DIM == 3 ? fl / (ext[0] * ext[1]) : 0;
-        int       c[3] = { org[0] + a0, org[1] + a1, org[2] + a2 };
-        T         out[NV];
-        if (c[dir] == 0 || c[dir] == b)
-        {
-          const int  side = c[dir] == b;
-          const long tp = DIM == 3 ? long(c[tt[0]]) + long(b) * c[tt[1]] : long(c[tt[0]]);
-          if ((side == 0 && lo_fine) || (side == 1 && hi_fine))
-          {
-            // sum of the fine faces covering this coarse face, divided by 2^dim
-            int cc[3] = { c[0], c[1], c[2] };
-            cc[dir] = side == 0 ? -1 : b;
-            const Hit h = neighbour(m, o, cc);
-            for (int vi = 0; vi < NV; ++vi)
-            {
-              double sum = 0;
-              if (DIM == 2)
-              {
-                for (int a = 0; a < 2; ++a)
-                  sum += S.BF[bf_index<T, DIM>(b, h.oct, dir, 1 - side, vi, h.l[tt[0]] + a)];
-              }
-              else
-              {
-                for (int q1 = 0; q1 < 2; ++q1)
-                  for (int q0 = 0; q0 < 2; ++q0)
-                    sum += S.BF[bf_index<T, DIM>(b, h.oct, dir, 1 - side, vi,
-                                                 long(h.l[tt[0]] + q0) + long(b) * (h.l[tt[1]] + q1))];
-              }
-              out[vi] = T(sum / vr);
-            }
-          }
-          else
-          {
-            for (int vi = 0; vi < NV; ++vi)
-              out[vi] = S.BF[bf_index<T, DIM>(b, o, dir, side, vi, tp)];
-          }
-        }
-        else
-        {
-          T f[8];
-          face_flux<T, DIM>(V, o, dir, c[0], c[1], c[2], dtdS, f);
-          for (int vi = 0; vi < NV; ++vi)
-            out[vi] = f[kFluxVar[vi]];
-        }
-        for (int vi = 0; vi < NV; ++vi)
-          lf[vi][fl] = out[vi];
-      }
+      if (dir == 0)
+        tile_faces<T, DIM, BS, 0>(S, V, o, org, dtdS, lf);
+      else if (dir == 1)
+        tile_faces<T, DIM, BS, 1>(S, V, o, org, dtdS, lf);
+      else
+        tile_faces<T, DIM, BS, 2>(S, V, o, org, dtdS, lf);
       __syncthreads();
+      if (owner)
       {
+        int ext[3] = { TS, TS, DIM == 3 ? TS : 1 };
+        ext[dir] = TS + 1;
         int a[3] = { li, lj, lk };
         const int fL = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
         a[dir] += 1;
@@ -615,53 +664,50 @@ k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __
       }
       __syncthreads();
     }
-    for (int v = 0; v < 5; ++v)
-      S.U2[(long(o) * 5 + v) * ncell + cidx] = acc[v];
-    if (DIM == 2)
-      S.B2[long(o) * I.fper() + I.face(2, ci, cj, 0)] = acc[5];
+    if (owner)
+    {
+      for (int v = 0; v < 5; ++v)
+        S.U2[(long(o) * 5 + v) * ncell + cidx] = acc[v];
+      if (DIM == 2)
+        S.B2[long(o) * IX::fper() + IX::face(2, ci, cj, 0)] = acc[5];
+    }
   }
 }
 
-// corner state of an edge (ComputeEmfAndStoreFunctor.cpp:105-367); kind 0 LB, 1 RT, 2 RB, 3 LT
-template <typename T, int DIM>
+// corner state of an edge (ComputeEmfAndStoreFunctor.cpp:105-367); KIND 0 LB, 1 RT, 2 RB, 3 LT,
+// (D0, D1) the two directions across the edge
+template <typename T, int DIM, int BS, int KIND, int D0, int D1>
 __device__ inline void
-edge_state(const View<T, DIM> & V, int o, int is, int js, int ks, int kind, int d0, int d1, T * q)
+edge_state(const View<T, DIM, BS> & V, int is, int js, int ks, T * q)
 {
-  const int c0 = V.c1(is, js, ks);
+  using VW = View<T, DIM, BS>;
+  const int c0 = VW::c1(is, js, ks);
   for (int v = 0; v < 8; ++v)
     q[v] = V.q2(v, c0);
-  int s0 = 1, s1 = 1, b0 = 1, b1 = 1;
+  constexpr int s0 = (KIND == 0 || KIND == 3) ? -1 : 1;
+  constexpr int s1 = (KIND == 0 || KIND == 2) ? -1 : 1;
+  constexpr int b0 = (KIND == 0 || KIND == 2) ? -1 : 1;
+  constexpr int b1 = (KIND == 0 || KIND == 3) ? -1 : 1;
   int p0[3] = { is, js, ks }, p1[3] = { is, js, ks };
-  if (kind == 0)
-  {
-    s0 = -1; s1 = -1; b0 = -1; b1 = -1;
-  }
-  else if (kind == 1)
-  {
-    p0[d0] += 1; p1[d1] += 1;
-  }
-  else if (kind == 2)
-  {
-    p0[d0] += 1; s1 = -1; b0 = -1;
-  }
-  else
-  {
-    p1[d1] += 1; s0 = -1; b1 = -1;
-  }
-  const int cp0 = V.c1(p0[0], p0[1], p0[2]);
-  const int cp1 = V.c1(p1[0], p1[1], p1[2]);
-  T          B0 = V.bf(d0, p0[0], p0[1], p0[2]);
-  T          B1 = V.bf(d1, p1[0], p1[1], p1[2]);
-  B0 += V.sf(d0, cp0);
-  B1 += V.sf(d1, cp1);
+  if (KIND == 1 || KIND == 2)
+    p0[D0] += 1;
+  if (KIND == 1 || KIND == 3)
+    p1[D1] += 1;
+  const int cp0 = VW::c1(p0[0], p0[1], p0[2]);
+  const int cp1 = VW::c1(p1[0], p1[1], p1[2]);
+  T         B0 = V.bf(D0, p0[0], p0[1], p0[2]);
+  T         B1 = V.bf(D1, p1[0], p1[1], p1[2]);
+  B0 += V.sf(D0, cp0);
+  B1 += V.sf(D1, cp1);
   // face-slope record index of (component, direction)
-  auto fsi = [](int comp, int dir) { return DIM == 2 ? comp : (comp == 0 ? dir - 1 : (comp == 1 ? (dir == 0 ? 2 : 3) : (dir == 0 ? 4 : 5))); };
-  const T dB0d1 = V.fs(fsi(d0, d1), cp0);
-  const T dB1d0 = V.fs(fsi(d1, d0), cp1);
+  constexpr int fs01 = DIM == 2 ? D0 : (D0 == 0 ? D1 - 1 : (D0 == 1 ? (D1 == 0 ? 2 : 3) : (D1 == 0 ? 4 : 5)));
+  constexpr int fs10 = DIM == 2 ? D1 : (D1 == 0 ? D0 - 1 : (D1 == 1 ? (D0 == 0 ? 2 : 3) : (D0 == 0 ? 4 : 5)));
+  const T dB0d1 = V.fs(fs01, cp0);
+  const T dB1d0 = V.fs(fs10, cp1);
   const T f0 = T(s0), f1 = T(s1);
   for (int v = 0; v < 5; ++v)
-    q[v] += T(0.5) * (f0 * V.sl(d0, v, c0) + f1 * V.sl(d1, v, c0));
-  auto tr = [&](int v) { return T(0.5) * (f0 * V.sl(d0, v, c0) + f1 * V.sl(d1, v, c0)); };
+    q[v] += T(0.5) * (f0 * V.sl(D0, v, c0) + f1 * V.sl(D1, v, c0));
+  auto tr = [&](int v) { return T(0.5) * (f0 * V.sl(D0, v, c0) + f1 * V.sl(D1, v, c0)); };
   if (DIM == 2)
   {
     q[kA] = B0 + T(0.5) * (T(b0) * dB0d1);
@@ -670,19 +716,19 @@ edge_state(const View<T, DIM> & V, int o, int is, int js, int ks, int kind, int 
   }
   else
   {
-    if (d0 == 0)
+    if (D0 == 0)
       q[kA] = B0 + T(0.5) * (T(b0) * dB0d1);
     else
       q[kA] += tr(kA);
-    if (d0 == 1)
+    if (D0 == 1)
       q[kB] = B0 + T(0.5) * (T(b0) * dB0d1);
     else
       q[kB] += tr(kB);
-    if (d1 == 1)
+    if (D1 == 1)
       q[kB] = B1 + T(0.5) * (T(b1) * dB1d0);
     else
       q[kB] += tr(kB);
-    if (d1 == 2)
+    if (D1 == 2)
       q[kC] = B1 + T(0.5) * (T(b1) * dB1d0);
     else
       q[kC] += tr(kC);
@@ -691,74 +737,74 @@ edge_state(const View<T, DIM> & V, int o, int is, int js, int ks, int kind, int 
   q[kP] = fmax(V.S.gt.smallp * q[kD], q[kP]);
 }
 
+// the EMF along axis AX at edge r (of EPD) of block o
+template <typename T, int DIM, int BS, int AX>
+__device__ inline void
+edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx)
+{
+  using G = Geo<DIM, BS>;
+  constexpr int e0 = (DIM == 3 && AX == 0) ? BS : G::NE;
+  constexpr int e1 = (DIM == 3 && AX == 1) ? BS : G::NE;
+  const int     i = r % e0, j = (r / e0) % e1, k = DIM == 3 ? r / (e0 * e1) : 0;
+  T             qe[4][8];
+  if (AX == 2)
+  {
+    edge_state<T, DIM, BS, 0, 0, 1>(V, i, j, k, qe[kLB]);
+    edge_state<T, DIM, BS, 1, 0, 1>(V, i - 1, j - 1, k, qe[kRT]);
+    edge_state<T, DIM, BS, 2, 0, 1>(V, i - 1, j, k, qe[kRB]);
+    edge_state<T, DIM, BS, 3, 0, 1>(V, i, j - 1, k, qe[kLT]);
+  }
+  else if (AX == 1)
+  {
+    edge_state<T, DIM, BS, 0, 0, 2>(V, i, j, k, qe[kLB]);
+    edge_state<T, DIM, BS, 1, 0, 2>(V, i - 1, j, k - 1, qe[kRT]);
+    edge_state<T, DIM, BS, 2, 0, 2>(V, i - 1, j, k, qe[kLT]);
+    edge_state<T, DIM, BS, 3, 0, 2>(V, i, j, k - 1, qe[kRB]);
+  }
+  else
+  {
+    edge_state<T, DIM, BS, 0, 1, 2>(V, i, j, k, qe[kLB]);
+    edge_state<T, DIM, BS, 1, 1, 2>(V, i, j - 1, k - 1, qe[kRT]);
+    edge_state<T, DIM, BS, 2, 1, 2>(V, i, j - 1, k, qe[kRB]);
+    edge_state<T, DIM, BS, 3, 1, 2>(V, i, j, k - 1, qe[kLT]);
+  }
+  const T       E = edge_emf_ax<AX>(qe, S.gt) * dtdx;
+  constexpr int comp = DIM == 3 ? AX : 0;
+  S.emf[(long(o) * G::NCE + comp) * G::NEC + IdxC<DIM, BS>::edge(i, j, k)] = E;
+}
+
 // ---------------------------------------------------------------------------------------------
-// k_edge: one thread per edge of a block (all three directions), every edge once
+// k_edge: one thread per edge of a block, every edge once.  Each block's edges are enumerated
+// per direction, each direction padded to whole wavefronts, so a wavefront's direction is
+// uniform and compiled in.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM>
+template <typename T, int DIM, int BS>
 __global__ void __launch_bounds__(256)
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
+  using G = Geo<DIM, BS>;
   for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
-    const int            b = S.m.b;
-    const long           n1 = b + 1;
-    // edges per block: 2D (b+1)^2 z-edges; 3D for each axis a: b (b+1)^2
-    const long per_dir = DIM == 3 ? long(b) * n1 * n1 : n1 * n1;
-    const long per = DIM == 3 ? 3 * per_dir : per_dir;
-    const long r0 = t - off[s];
+    constexpr int        per = G::NCE * G::EPDP;
+    const long           r0 = t - off[s];
     if (r0 >= S.cnt[4])
       continue;
-    const int  o = int(r0 / per);
-    const View<T, DIM> V(S, o);
-    long       r = r0 - long(o) * per;
-    const int  a = DIM == 3 ? int(r / per_dir) : 2;
-    r -= DIM == 3 ? long(a) * per_dir : 0;
-    int e[3];
-    if (DIM == 2)
-    {
-      e[0] = int(r % n1);
-      e[1] = int(r / n1);
-      e[2] = 0;
-    }
-    else
-    {
-      // along-axis index runs over b, the two others over b+1 (x fastest)
-      int ext[3] = { int(n1), int(n1), int(n1) };
-      ext[a] = b;
-      e[0] = int(r % ext[0]);
-      e[1] = int((r / ext[0]) % ext[1]);
-      e[2] = int(r / (long(ext[0]) * ext[1]));
-    }
-    const T dtdx = T(S.dt / dxd(S.m, S.scaling, o));
-    T       qe[4][8];
-    const int i = e[0], j = e[1], k = e[2];
-    if (a == 2)
-    {
-      edge_state<T, DIM>(V, o, i, j, k, 0, 0, 1, qe[kLB]);
-      edge_state<T, DIM>(V, o, i - 1, j - 1, k, 1, 0, 1, qe[kRT]);
-      edge_state<T, DIM>(V, o, i - 1, j, k, 2, 0, 1, qe[kRB]);
-      edge_state<T, DIM>(V, o, i, j - 1, k, 3, 0, 1, qe[kLT]);
-    }
+    const int o = int(r0) / per;
+    int       r = int(r0) - o * per;
+    const int a = DIM == 3 ? __builtin_amdgcn_readfirstlane(r / G::EPDP) : 2;
+    r -= DIM == 3 ? a * G::EPDP : 0;
+    if (r >= G::EPD)
+      continue;
+    const View<T, DIM, BS> V(S, o);
+    const T                dtdx = T(S.dt / dxd<BS>(S.m, S.scaling, o));
+    if (DIM == 2 || a == 2)
+      edge_at<T, DIM, BS, 2>(S, V, o, r, dtdx);
     else if (a == 1)
-    {
-      edge_state<T, DIM>(V, o, i, j, k, 0, 0, 2, qe[kLB]);
-      edge_state<T, DIM>(V, o, i - 1, j, k - 1, 1, 0, 2, qe[kRT]);
-      edge_state<T, DIM>(V, o, i - 1, j, k, 2, 0, 2, qe[kLT]);
-      edge_state<T, DIM>(V, o, i, j, k - 1, 3, 0, 2, qe[kRB]);
-    }
+      edge_at<T, DIM, BS, 1>(S, V, o, r, dtdx);
     else
-    {
-      edge_state<T, DIM>(V, o, i, j, k, 0, 1, 2, qe[kLB]);
-      edge_state<T, DIM>(V, o, i, j - 1, k - 1, 1, 1, 2, qe[kRT]);
-      edge_state<T, DIM>(V, o, i, j - 1, k, 2, 1, 2, qe[kRB]);
-      edge_state<T, DIM>(V, o, i, j, k - 1, 3, 1, 2, qe[kLT]);
-    }
-    const T   E = edge_emf(qe, a, S.gt) * dtdx;
-    const int comp = DIM == 3 ? a : 0;
-    const long ne = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
-    S.emf[(long(o) * (DIM == 3 ? 3 : 1) + comp) * ne + i + n1 * (j + n1 * k)] = E;
+      edge_at<T, DIM, BS, 0>(S, V, o, r, dtdx);
   }
 }
 
@@ -786,41 +832,45 @@ k_scatter(int nfix, const EmfFix * __restrict__ fix, const T * __restrict__ val,
 }
 
 // face-B update (ReadEmfAndUpdateFunctor.cpp:58-163), FP64 accumulation
-template <typename T, int DIM>
+template <typename T, int DIM, int BS>
 __global__ void __launch_bounds__(256)
 k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
+  using IX = IdxC<DIM, BS>;
   for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of<T>(off, nslots, t));
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
-    const int            b = S.m.b;
-    const Idx            I{ DIM, b };
-    const long           per = I.fper();
+    constexpr int        per = IX::fper();
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[5])
       continue;
-    const int            o = int(r0 / per);
-    const long           fi = r0 - long(o) * per;
-    long                 r = fi;
-    int                  c = 0;
-    while (c < 2 && r >= I.fsize(c))
+    const int o = int(r0) / per;
+    const int fi = int(r0) - o * per;
+    int       r = fi;
+    int       c = 0;
+    if (r >= IX::fsize(0))
     {
-      r -= I.fsize(c);
-      ++c;
+      r -= IX::fsize(0);
+      c = 1;
+      if (r >= IX::fsize(1))
+      {
+        r -= IX::fsize(1);
+        c = 2;
+      }
     }
-    const int  nx = c == 0 ? b + 1 : b;
-    const int  ny = c == 1 ? b + 1 : b;
-    const int  i = int(r % nx), j = int((r / nx) % ny), k = DIM == 3 ? int(r / (long(nx) * ny)) : 0;
-    const long ne = I.ecells();
-    const double Bv = S.B[long(o) * per + fi];
+    const int     nx = c == 0 ? BS + 1 : BS;
+    const int     ny = c == 1 ? BS + 1 : BS;
+    const int     i = r % nx, j = (r / nx) % ny, k = DIM == 3 ? r / (nx * ny) : 0;
+    constexpr int ne = IX::ecells();
+    const double  Bv = S.B[long(o) * per + fi];
     if (DIM == 2)
     {
       const T * e = S.emf + long(o) * ne;
       if (c == 0)
-        S.B2[long(o) * per + fi] = Bv + (double(e[I.edge(i, j + 1, 0)]) - double(e[I.edge(i, j, 0)]));
+        S.B2[long(o) * per + fi] = Bv + (double(e[IX::edge(i, j + 1, 0)]) - double(e[IX::edge(i, j, 0)]));
       else if (c == 1)
-        S.B2[long(o) * per + fi] = Bv - (double(e[I.edge(i + 1, j, 0)]) - double(e[I.edge(i, j, 0)]));
+        S.B2[long(o) * per + fi] = Bv - (double(e[IX::edge(i + 1, j, 0)]) - double(e[IX::edge(i, j, 0)]));
       continue;
     }
     const T * ex = S.emf + (long(o) * 3 + 0) * ne;
@@ -829,18 +879,18 @@ k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __re
     double    dB = 0;
     if (c == 0)
     {
-      dB += (double(ez[I.edge(i, j + 1, k)]) - double(ez[I.edge(i, j, k)]));
-      dB -= (double(ey[I.edge(i, j, k + 1)]) - double(ey[I.edge(i, j, k)]));
+      dB += (double(ez[IX::edge(i, j + 1, k)]) - double(ez[IX::edge(i, j, k)]));
+      dB -= (double(ey[IX::edge(i, j, k + 1)]) - double(ey[IX::edge(i, j, k)]));
     }
     else if (c == 1)
     {
-      dB += (double(ex[I.edge(i, j, k + 1)]) - double(ex[I.edge(i, j, k)]));
-      dB -= (double(ez[I.edge(i + 1, j, k)]) - double(ez[I.edge(i, j, k)]));
+      dB += (double(ex[IX::edge(i, j, k + 1)]) - double(ex[IX::edge(i, j, k)]));
+      dB -= (double(ez[IX::edge(i + 1, j, k)]) - double(ez[IX::edge(i, j, k)]));
     }
     else
     {
-      dB += (double(ey[I.edge(i + 1, j, k)]) - double(ey[I.edge(i, j, k)]));
-      dB -= (double(ex[I.edge(i, j + 1, k)]) - double(ex[I.edge(i, j, k)]));
+      dB += (double(ey[IX::edge(i + 1, j, k)]) - double(ey[IX::edge(i, j, k)]));
+      dB -= (double(ex[IX::edge(i, j + 1, k)]) - double(ex[IX::edge(i, j, k)]));
     }
     S.B2[long(o) * per + fi] = Bv + dB;
   }
@@ -857,21 +907,22 @@ grid_for(long n, int threads, int cap)
   return unsigned(g);
 }
 
-template <typename T, int DIM>
+template <typename T, int DIM, int BS>
 int
 launch_step(FusedBatch<T> & bt, hipStream_t st, int cap)
 {
-  const int   n = bt.nslots;
+  const int    n = bt.nslots;
   const auto * sl = bt.d_slots;
-  // element kernels
-  hipLaunchKernelGGL((k_prim<T, DIM>), grid_for(bt.tot[0], 256, cap), 256, 0, st, sl, n, bt.d_off[0], bt.tot[0],
-                     bt.flags);
-  hipLaunchKernelGGL((k_cell<T, DIM>), grid_for(bt.tot[1], 256, cap), 256, 0, st, sl, n, bt.d_off[1], bt.tot[1]);
-  hipLaunchKernelGGL((k_border_flux<T, DIM>), grid_for(bt.tot[2], 256, cap), 256, 0, st, sl, n, bt.d_off[2],
+  hipLaunchKernelGGL((k_prim<T, DIM, BS>), grid_for(bt.tot[0], 256, cap), 256, 0, st, sl, n, bt.d_off[0],
+                     bt.tot[0], bt.flags);
+  hipLaunchKernelGGL((k_cell<T, DIM, BS>), grid_for(bt.tot[1], 256, cap), 256, 0, st, sl, n, bt.d_off[1],
+                     bt.tot[1]);
+  hipLaunchKernelGGL((k_border_flux<T, DIM, BS>), grid_for(bt.tot[2], 256, cap), 256, 0, st, sl, n, bt.d_off[2],
                      bt.tot[2]);
-  hipLaunchKernelGGL((k_flux_tile<T, DIM>), grid_for(bt.tot[3], 1, cap), DIM == 3 ? 512 : 64, 0, st, sl, n,
+  hipLaunchKernelGGL((k_flux_tile<T, DIM, BS>), grid_for(bt.tot[3], 1, cap), tile_threads<DIM>(), 0, st, sl, n,
                      bt.d_off[3], bt.tot[3]);
-  hipLaunchKernelGGL((k_edge<T, DIM>), grid_for(bt.tot[4], 256, cap), 256, 0, st, sl, n, bt.d_off[4], bt.tot[4]);
+  hipLaunchKernelGGL((k_edge<T, DIM, BS>), grid_for(bt.tot[4], 256, cap), 256, 0, st, sl, n, bt.d_off[4],
+                     bt.tot[4]);
   if (bt.nfix > 0)
   {
     hipLaunchKernelGGL((k_fix<T>), grid_for(bt.nfix, 256, cap), 256, 0, st, bt.nfix, bt.fix, bt.fix_src,
@@ -879,19 +930,55 @@ launch_step(FusedBatch<T> & bt, hipStream_t st, int cap)
     hipLaunchKernelGGL((k_scatter<T>), grid_for(bt.nfix, 256, cap), 256, 0, st, bt.nfix, bt.fix, bt.fixval,
                        bt.emf_all);
   }
-  hipLaunchKernelGGL((k_bupdate<T, DIM>), grid_for(bt.tot[5], 256, cap), 256, 0, st, sl, n, bt.d_off[5],
+  hipLaunchKernelGGL((k_bupdate<T, DIM, BS>), grid_for(bt.tot[5], 256, cap), 256, 0, st, sl, n, bt.d_off[5],
                      bt.tot[5]);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 } // namespace
 
+bool
+fused_supports(int dim, int b)
+{
+  return (dim == 2 || dim == 3) && (b == 8 || b == 16 || b == 32);
+}
+
+long
+fused_items_per_block(int dim, int b, int stage)
+{
+  const long g2 = dim == 3 ? long(b + 4) * (b + 4) * (b + 4) : long(b + 4) * (b + 4);
+  const long g1 = dim == 3 ? long(b + 2) * (b + 2) * (b + 2) : long(b + 2) * (b + 2);
+  const long nt = dim == 3 ? long(b) * b : long(b);
+  const long nb1 = b / kTS;
+  const long n1 = b + 1;
+  const long epd = dim == 3 ? long(b) * n1 * n1 : n1 * n1;
+  const long epdp = (epd + 63) / 64 * 64;
+  switch (stage)
+  {
+  case 0: return g2;
+  case 1: return g1;
+  case 2: return dim * 2 * nt;
+  case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
+  case 4: return (dim == 3 ? 3 : 1) * epdp;
+  default: return dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
+  }
+}
+
 template <typename T>
 int
 fused_launch(FusedBatch<T> & bt, hipStream_t st)
 {
   const int cap = bt.grid_cap > 0 ? bt.grid_cap : 1 << 20;
-  return bt.dim == 3 ? launch_step<T, 3>(bt, st, cap) : launch_step<T, 2>(bt, st, cap);
+  switch (bt.dim * 100 + bt.bs)
+  {
+  case 208: return launch_step<T, 2, 8>(bt, st, cap);
+  case 216: return launch_step<T, 2, 16>(bt, st, cap);
+  case 232: return launch_step<T, 2, 32>(bt, st, cap);
+  case 308: return launch_step<T, 3, 8>(bt, st, cap);
+  case 316: return launch_step<T, 3, 16>(bt, st, cap);
+  case 332: return launch_step<T, 3, 32>(bt, st, cap);
+  default: return -1;
+  }
 }
 
 template int fused_launch<double>(FusedBatch<double> &, hipStream_t);

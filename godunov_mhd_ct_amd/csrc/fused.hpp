@@ -57,13 +57,15 @@ struct FusedSizes
   long total() const { return qp + bg + q2 + s + sf + fs + bf + emf; }
 };
 
-// The device work list: per stage, the prefix offsets of every slot's work items.
+// The device work list of one size class (dimension, block size): per stage, the prefix
+// offsets of every slot's work items.
 //   stage 0 k_prim (ghost-2 cells), 1 k_cell (ghost-1 cells), 2 k_border_flux (border faces),
-//   3 k_flux_tile (8^dim tiles), 4 k_edge (edges), 5 k_bupdate (faces)
+//   3 k_flux_tile (8^dim tiles), 4 k_edge (edges, each direction padded to 64), 5 k_bupdate (faces)
 template <typename T>
 struct FusedBatch
 {
   int              dim = 3;
+  int              bs = 16;
   int              nslots = 0;
   FusedSlot<T> *   d_slots = nullptr;
   long *           d_off[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
@@ -76,6 +78,11 @@ struct FusedBatch
   unsigned long long * flags = nullptr;
   int              grid_cap = 0;      // persistent grid: at most this many workgroups per launch
 };
+
+// block sizes the fused kernels are instantiated for
+bool fused_supports(int dim, int b);
+// work items of one block in stage `stage` (the launch geometry of the kernels above)
+long fused_items_per_block(int dim, int b, int stage);
 
 template <typename T>
 int fused_launch(FusedBatch<T> & bt, hipStream_t st);

@@ -33,9 +33,9 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
 {
   const int  dim = c.dim, b = c.b;
   const long nq = c.dm.nowned;
-  if (b % 8)
+  if (!fused_supports(dim, b))
   {
-    c.err = "the fused form tiles blocks by 8 cells: block size must be a multiple of 8";
+    c.err = "the fused form is built for block sizes 8, 16 and 32";
     return -4;
   }
   auto alloc = [&](size_t bytes, void ** p) -> int {
@@ -75,20 +75,18 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.BF = p;  p += nq * z.bf;
   sl.emf = p; p += nq * z.emf;
   bt.dim = dim;
+  bt.bs = b;
   bt.nslots = 1;
   bt.emf_all = sl.emf;
   bt.flags = c.red + 1;
-  const long g2 = dim == 3 ? long(b + 4) * (b + 4) * (b + 4) : long(b + 4) * (b + 4);
-  const long g1 = dim == 3 ? long(b + 2) * (b + 2) * (b + 2) : long(b + 2) * (b + 2);
-  const long nt = dim == 3 ? long(b) * b : long(b);
-  const long ntile = dim == 3 ? long(b / 8) * (b / 8) * (b / 8) : long(b / 8) * (b / 8);
-  const long n1 = b + 1;
-  const long nedge = dim == 3 ? 3 * long(b) * n1 * n1 : n1 * n1;
-  const long nface = dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
-  const long per[6] = { g2, g1, dim * 2 * nt, ntile, nedge, nface };
   for (int k = 0; k < 6; ++k)
   {
-    sl.cnt[k] = nq * per[k];
+    sl.cnt[k] = nq * fused_items_per_block(dim, b, k);
+    if (sl.cnt[k] >= (1L << 31))
+    {
+      c.err = "mesh too large for one fused work list (2^31 items per stage)";
+      return -4;
+    }
     bt.tot[k] = (sl.cnt[k] + 511) / 512 * 512; // whole workgroups per slot: the slot is wave-uniform
     long off[2] = { 0, bt.tot[k] };
     if (alloc(sizeof(off), reinterpret_cast<void **>(&bt.d_off[k])))

@@ -274,24 +274,17 @@ enum : int
 
 // 2D HLLD electric field at an edge from the four corner states (RT, RB, LT, LB) given in
 // the lab frame; `axis` is the edge direction (0 = x, 1 = y, 2 = z).
-template <typename T>
+template <int AXIS, typename T>
 __host__ __device__ inline T
-edge_emf(const T (&qe)[4][8], int axis, const Gas<T> & g)
+edge_emf_ax(const T (&qe)[4][8], const Gas<T> & g)
 {
   // in-plane (iu, iv), out-of-plane iw, and the same for B
-  int iu, iv, iw, ia, ib, ic;
-  if (axis == 2)
-  {
-    iu = kU; iv = kV; iw = kW; ia = kA; ib = kB; ic = kC;
-  }
-  else if (axis == 1)
-  {
-    iu = kW; iv = kU; iw = kV; ia = kC; ib = kA; ic = kB;
-  }
-  else
-  {
-    iu = kV; iv = kW; iw = kU; ia = kB; ib = kC; ic = kA;
-  }
+  constexpr int iu = AXIS == 2 ? kU : (AXIS == 1 ? kW : kV);
+  constexpr int iv = AXIS == 2 ? kV : (AXIS == 1 ? kU : kW);
+  constexpr int iw = AXIS == 2 ? kW : (AXIS == 1 ? kV : kU);
+  constexpr int ia = AXIS == 2 ? kA : (AXIS == 1 ? kC : kB);
+  constexpr int ib = AXIS == 2 ? kB : (AXIS == 1 ? kA : kC);
+  constexpr int ic = AXIS == 2 ? kC : (AXIS == 1 ? kB : kA);
   const T * RT = qe[kRT];
   const T * LT = qe[kLT];
   const T * RB = qe[kRB];
@@ -442,6 +435,13 @@ edge_emf(const T (&qe)[4][8], int axis, const Gas<T> & g)
   t = (SAT * EsRLy - SAB * EsRRy - SAT * SAB * (aRR - aRL)) / (SAT - SAB);
   E += (SBn * STp * SLn * SRn) * t;
   return E;
+}
+
+template <typename T>
+__host__ __device__ inline T
+edge_emf(const T (&qe)[4][8], int axis, const Gas<T> & g)
+{
+  return axis == 2 ? edge_emf_ax<2>(qe, g) : (axis == 1 ? edge_emf_ax<1>(qe, g) : edge_emf_ax<0>(qe, g));
 }
 
 } // namespace mhdct
