@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--impl", default="plain", choices=["plain", "fused"])
     ap.add_argument("--workload", default="bl3d", choices=["bl3d", "kh3d", "ot2d"])
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="before the steps, copy a buffer of this many MiB device to device once (the "
+                         "known-size reference for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -72,6 +75,14 @@ def main():
     cells = mesh.owned_cells()
     print(f"[bench] workload={args.workload} impl={args.impl} leaves={mesh.num_leaves} cells={cells}",
           file=sys.stderr, flush=True)
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {args.copy_mb} MiB device to device", file=sys.stderr, flush=True)
+        del src, dst
     for _ in range(args.warmup):
         sol.step()
     torch.cuda.synchronize()

@@ -116,6 +116,18 @@ struct IdxC
   __host__ __device__ static constexpr int edge(int i, int j, int k) { return i + (BS + 1) * (j + (BS + 1) * k); }
 };
 
+// Workgroup w is dispatched to XCD w % 8 (round robin), and each XCD has its own L2.  Grids are
+// launched as multiples of 8 and renumbered so that each XCD takes one contiguous eighth of the
+// work groups: cells whose stencils overlap then run on the same XCD and share its L2.
+constexpr int kXcd = 8;
+
+__device__ inline long
+xcd_group()
+{
+  const unsigned w = blockIdx.x;
+  return long(w % kXcd) * (gridDim.x / kXcd) + w / kXcd;
+}
+
 struct Hit
 {
   int oct;

@@ -4,10 +4,11 @@
 // are regrouped so that each intermediate crosses HBM at most once:
 //
 //   k_prim         face-B ghost fill + primitive conversion (GodunovImplemV0.cpp:128-139)
-//   k_cell         limited slopes, Hancock predictor, edge E and face-B source, transverse
+//   k_efield       edge electric field v x B at t_n (:141-149)
+//   k_cell         limited slopes, Hancock predictor, face-B source, transverse
 //                  face-B slopes: one compact record per cell (:141-156, and the slopes the EMF
 //                  reconstruction reads, ComputeEmfAndStoreFunctor.h:363-419)
-//   k_border_flux  HLLD on block-border faces only (what a coarser neighbour sums)
+//   k_border_flux  HLLD on the block-border faces that a coarser neighbour sums
 //   k_flux_tile    HLLD on the faces of an 8^dim tile, one direction at a time through LDS, and
 //                  the conservative update U2 = ((U + dX) + dY) + dZ in FP64 registers (:161-177)
 //   k_edge         2D-HLLD EMF, one thread per edge, every edge computed once (:197)
@@ -23,7 +24,7 @@
 // compile-time constants, so index arithmetic is 32-bit with constant divisors.  A launch walks
 // a work list of one size class that may span several meshes ("slots"): each slot's range of
 // work items is padded to whole 512-item groups, so the slot of a wavefront is uniform.
-// Launches use a bounded, persistent grid (grid-stride loops).
+// A launch has one workgroup per work group (grid-stride beyond a cap), numbered XCD by XCD.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -58,6 +59,13 @@ struct Geo
   static constexpr int NEC = I::ecells();                 // edge-grid points per component
   static constexpr int EPD = DIM == 3 ? BS * NE * NE : NE * NE; // edges per direction
   static constexpr int EPDP = (EPD + 63) / 64 * 64;       // ... padded to whole wavefronts
+  static constexpr int P1 = (G1 + 63) / 64 * 64;          // k_cell items per block
+  static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
+  static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
+  static constexpr int PRIM = NC + (GSH + 63) / 64 * 64;  // k_prim items: interior, then shell
+  static constexpr int NP = BS + 3;                       // edge points per axis of the E grid
+  static constexpr int NPE = DIM == 3 ? NP * NP * NP : NP * NP;
+  static constexpr int EFP = (NPE + 63) / 64 * 64;        // k_efield items
   static constexpr int NB1 = BS / kTS;                    // flux tiles per axis
   static constexpr int TPB = DIM == 3 ? NB1 * NB1 * NB1 : NB1 * NB1;
   static_assert(BS % kTS == 0, "block size must be a multiple of the flux tile");
@@ -72,26 +80,52 @@ slot_of(const long * off, int nslots, long t)
   return s;
 }
 
-// read-only views of one block of one slot
+// Raw buffer resource over `bytes` bytes at `base` (bounds-checked vector loads with a 32-bit
+// offset; word 3 selects the 32-bit data format of gfx9-class buffer descriptors).
+__device__ inline __amdgpu_buffer_rsrc_t
+rsrc(const void * base, int bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ inline T
+bload(__amdgpu_buffer_rsrc_t r, int idx)
+{
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+}
+
+template <typename T>
+__device__ inline void
+bstore(__amdgpu_buffer_rsrc_t r, int idx, T x)
+{
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r,
+                                          idx * 8, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, idx * 4, 0, 0);
+}
+
+// View of one block of one slot.  The block index must be uniform across the
+// wavefront (every work list pads its per-block range to whole wavefronts): the six arrays are
+// then addressed through buffer descriptors held in scalar registers.
 template <typename T, int DIM, int BS>
 struct View
 {
   using G = Geo<DIM, BS>;
-  const FusedSlot<T> & S;
-  const T *            p_qp;
-  const T *            p_bg;
-  const T *            p_q2;
-  const T *            p_sl;
-  const T *            p_sf;
-  const T *            p_fs;
+  const FusedSlot<T> &   S;
+  __amdgpu_buffer_rsrc_t r_qp, r_bg, r_q2, r_sl, r_sf, r_fs;
   __device__ View(const FusedSlot<T> & s, int o)
     : S(s)
-    , p_qp(s.Qp + long(o) * 5 * G::G2)
-    , p_bg(s.Bg + long(o) * G::I::gfper())
-    , p_q2(s.Q2 + long(o) * 8 * G::G1)
-    , p_sl(s.S + long(o) * DIM * 8 * G::G1)
-    , p_sf(s.sF + long(o) * DIM * G::G1)
-    , p_fs(s.FS + long(o) * G::NFS * G::G1)
+    , r_qp(rsrc(s.Qp + long(o) * 5 * G::G2, 5 * G::G2 * int(sizeof(T))))
+    , r_bg(rsrc(s.Bg + long(o) * G::I::gfper(), G::I::gfper() * int(sizeof(T))))
+    , r_q2(rsrc(s.Q2 + long(o) * 8 * G::G1, 8 * G::G1 * int(sizeof(T))))
+    , r_sl(rsrc(s.S + long(o) * DIM * 8 * G::G1, DIM * 8 * G::G1 * int(sizeof(T))))
+    , r_sf(rsrc(s.sF + long(o) * DIM * G::G1, DIM * G::G1 * int(sizeof(T))))
+    , r_fs(rsrc(s.FS + long(o) * G::NFS * G::G1, G::NFS * G::G1 * int(sizeof(T))))
   {
   }
   __device__ static int c2(int i, int j, int k)
@@ -102,13 +136,58 @@ struct View
   {
     return (i + 1) + G::N1 * ((j + 1) + (DIM == 3 ? G::N1 * (k + 1) : 0));
   }
-  __device__ T qp(int v, int i, int j, int k) const { return p_qp[v * G::G2 + c2(i, j, k)]; }
-  __device__ T bf(int c, int i, int j, int k) const { return p_bg[G::I::gface(c, i, j, k)]; }
-  __device__ T q2(int v, int c) const { return p_q2[v * G::G1 + c]; }
-  __device__ T sl(int d, int v, int c) const { return p_sl[(d * 8 + v) * G::G1 + c]; }
-  __device__ T sf(int d, int c) const { return p_sf[d * G::G1 + c]; }
-  __device__ T fs(int n, int c) const { return p_fs[n * G::G1 + c]; }
+  __device__ T qp(int v, int i, int j, int k) const { return bload<T>(r_qp, v * G::G2 + c2(i, j, k)); }
+  __device__ T bf(int c, int i, int j, int k) const { return bload<T>(r_bg, G::I::gface(c, i, j, k)); }
+  __device__ T q2(int v, int c) const { return bload<T>(r_q2, v * G::G1 + c); }
+  __device__ T sl(int d, int v, int c) const { return bload<T>(r_sl, (d * 8 + v) * G::G1 + c); }
+  __device__ T sf(int d, int c) const { return bload<T>(r_sf, d * G::G1 + c); }
+  __device__ T fs(int n, int c) const { return bload<T>(r_fs, n * G::G1 + c); }
 };
+
+// Cell (i,j,k) of the g-th ghost cell of the 2-ghost tile, g < (b+4)^dim - b^dim.  3D: two
+// full planes below the block, two above, then for each interior plane two full rows below,
+// two above and the four side cells of each interior row; 2D: the same for rows and cells.
+template <int DIM, int BS>
+__host__ __device__ inline void
+shell_cell(int g, int & i, int & j, int & k)
+{
+  constexpr int n = BS + 4;
+  constexpr int ring = 4 * n + 4 * BS; // ghost cells of one interior plane (3D) / the 2D shell
+  int           rem = g;
+  k = 0;
+  if (DIM == 3)
+  {
+    constexpr int P = n * n;
+    if (rem < 4 * P)
+    {
+      k = rem < 2 * P ? rem / P - 2 : BS + (rem - 2 * P) / P;
+      rem %= P;
+      j = rem / n - 2;
+      i = rem % n - 2;
+      return;
+    }
+    rem -= 4 * P;
+    k = rem / ring;
+    rem %= ring;
+  }
+  if (rem < 2 * n)
+  {
+    j = rem / n - 2;
+    i = rem % n - 2;
+  }
+  else if (rem < 4 * n)
+  {
+    j = BS + (rem - 2 * n) / n;
+    i = (rem - 2 * n) % n - 2;
+  }
+  else
+  {
+    rem -= 4 * n;
+    j = rem / 4;
+    const int q = rem % 4;
+    i = q < 2 ? q - 2 : BS + q - 2;
+  }
+}
 
 template <int BS>
 __device__ inline double
@@ -129,7 +208,7 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
   using G = Geo<DIM, BS>;
   using IX = IdxC<DIM, BS>;
   constexpr IX I{};
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
     const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
@@ -137,10 +216,25 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[0])
       continue;
-    const int o = int(r0) / G::G2;
-    const int ci = int(r0) - o * G::G2;
+    // interior cells first, then the ghost shell: every wavefront is one or the other
+    const int o = int(r0) / G::PRIM;
+    int       g = int(r0) - o * G::PRIM;
     constexpr int n = G::N2, b = BS;
-    const int i = ci % n - 2, j = (ci / n) % n - 2, k = DIM == 3 ? ci / (n * n) - 2 : 0;
+    int           i, j, k = 0;
+    if (g < G::NC)
+    {
+      i = g % BS;
+      j = (g / BS) % BS;
+      k = DIM == 3 ? g / (BS * BS) : 0;
+    }
+    else
+    {
+      g -= G::NC;
+      if (g >= G::GSH)
+        continue;
+      shell_cell<DIM, BS>(g, i, j, k);
+    }
+    const int ci = (i + 2) + n * ((j + 2) + (DIM == 3 ? n * (k + 2) : 0));
     constexpr int ncell = IX::cells();
     const double * __restrict__ U = S.U;
     const double * __restrict__ B = S.B;
@@ -254,6 +348,65 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
 }
 
 // ---------------------------------------------------------------------------------------------
+// k_efield: edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235) on the
+// edge points [-1, b+1]^dim of each block, from the primitive state of the 2-ghost tile.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int DIM, int BS>
+__global__ void __launch_bounds__(256)
+k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  using G = Geo<DIM, BS>;
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[6])
+      continue;
+    const int o = __builtin_amdgcn_readfirstlane(int(r0) / G::EFP);
+    const int r = int(r0) - o * G::EFP;
+    if (r >= G::NPE)
+      continue;
+    constexpr int          np = G::NP;
+    const int              a = r % np - 1, c = (r / np) % np - 1, e = DIM == 3 ? r / (np * np) - 1 : 0;
+    const View<T, DIM, BS> V(S, o);
+    auto Q = [&](int v, int x, int y, int z) -> T {
+      if (v < 5)
+        return V.qp(v, x, y, z);
+      const int fc = (v - 5) % 3;
+      const int hi = v >= 8;
+      if (DIM == 2 && fc == 2)
+        return V.bf(2, x, y, 0);
+      return V.bf(fc, x + (hi && fc == 0), y + (hi && fc == 1), z + (hi && fc == 2));
+    };
+    T * Ed = S.Ed + long(o) * G::NCE * G::NPE + r;
+    if (DIM == 3)
+    {
+      const T vv = T(0.25) * (Q(3, a, c - 1, e - 1) + Q(3, a, c - 1, e) + Q(3, a, c, e - 1) + Q(3, a, c, e));
+      const T ww = T(0.25) * (Q(4, a, c - 1, e - 1) + Q(4, a, c - 1, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
+      const T BB = T(0.5) * (Q(6, a, c, e - 1) + Q(6, a, c, e));
+      const T CC = T(0.5) * (Q(7, a, c - 1, e) + Q(7, a, c, e));
+      Ed[0] = vv * CC - ww * BB;
+    }
+    if (DIM == 3)
+    {
+      const T uu = T(0.25) * (Q(2, a - 1, c, e - 1) + Q(2, a - 1, c, e) + Q(2, a, c, e - 1) + Q(2, a, c, e));
+      const T ww = T(0.25) * (Q(4, a - 1, c, e - 1) + Q(4, a - 1, c, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
+      const T AA = T(0.5) * (Q(5, a, c, e - 1) + Q(5, a, c, e));
+      const T CC = T(0.5) * (Q(7, a - 1, c, e) + Q(7, a, c, e));
+      Ed[G::NPE] = ww * AA - uu * CC;
+    }
+    {
+      const T uu = T(0.25) * (Q(2, a - 1, c - 1, e) + Q(2, a - 1, c, e) + Q(2, a, c - 1, e) + Q(2, a, c, e));
+      const T vv = T(0.25) * (Q(3, a - 1, c - 1, e) + Q(3, a - 1, c, e) + Q(3, a, c - 1, e) + Q(3, a, c, e));
+      const T AA = T(0.5) * (Q(5, a, c - 1, e) + Q(5, a, c, e));
+      const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
+      Ed[DIM == 3 ? 2 * G::NPE : 0] = uu * BB - vv * AA;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
@@ -261,7 +414,7 @@ __global__ void __launch_bounds__(256)
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   using G = Geo<DIM, BS>;
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
     const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
@@ -270,8 +423,10 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     if (r0 >= S.cnt[1])
       continue;
     constexpr int             n = G::N1;
-    const int                 o = int(r0) / G::G1;
-    const int                 ci = int(r0) - o * G::G1;
+    const int                 o = __builtin_amdgcn_readfirstlane(int(r0) / G::P1);
+    const int                 ci = int(r0) - o * G::P1;
+    if (ci >= G::G1)
+      continue;
     const View<T, DIM, BS>    V(S, o);
     const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
     const T   st = S.gt.slope_type;
@@ -286,7 +441,6 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       return V.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
     };
     const int c1 = ci;
-    T *       Sd = S.S + long(o) * DIM * 8 * G::G1;
     T         sl[3][8];
     constexpr int nh = DIM == 3 ? 5 : 4;
     for (int d = 0; d < DIM; ++d)
@@ -310,7 +464,7 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
         }
       }
       for (int v = 0; v < 8; ++v)
-        Sd[(d * 8 + v) * G::G1 + c1] = sl[d][v];
+        bstore<T>(V.r_sl, (d * 8 + v) * G::G1 + c1, sl[d][v]);
     }
     // Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
     const double dxo = dxd<BS>(m, S.scaling, o);
@@ -352,68 +506,52 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
       sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
     }
-    T * q2 = S.Q2 + long(o) * 8 * G::G1 + c1;
-    q2[0 * G::G1] = r + T(0.5) * sr0;
-    q2[2 * G::G1] = u + T(0.5) * su0;
-    q2[3 * G::G1] = v + T(0.5) * sv0;
-    q2[4 * G::G1] = w + T(0.5) * sw0;
-    q2[1 * G::G1] = p + T(0.5) * sp0;
-    q2[5 * G::G1] = A + T(0.5) * sA0;
-    q2[6 * G::G1] = B + T(0.5) * sB0;
-    q2[7 * G::G1] = C + T(0.5) * sC0;
-    // edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235), evaluated where
-    // the face source needs it
-    auto Ex = [&](int a, int c, int e) -> T {
-      const T vv = T(0.25) * (Q(3, a, c - 1, e - 1) + Q(3, a, c - 1, e) + Q(3, a, c, e - 1) + Q(3, a, c, e));
-      const T ww = T(0.25) * (Q(4, a, c - 1, e - 1) + Q(4, a, c - 1, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
-      const T BB = T(0.5) * (Q(6, a, c, e - 1) + Q(6, a, c, e));
-      const T CC = T(0.5) * (Q(7, a, c - 1, e) + Q(7, a, c, e));
-      return vv * CC - ww * BB;
+    bstore<T>(V.r_q2, 0 * G::G1 + c1, r + T(0.5) * sr0);
+    bstore<T>(V.r_q2, 2 * G::G1 + c1, u + T(0.5) * su0);
+    bstore<T>(V.r_q2, 3 * G::G1 + c1, v + T(0.5) * sv0);
+    bstore<T>(V.r_q2, 4 * G::G1 + c1, w + T(0.5) * sw0);
+    bstore<T>(V.r_q2, 1 * G::G1 + c1, p + T(0.5) * sp0);
+    bstore<T>(V.r_q2, 5 * G::G1 + c1, A + T(0.5) * sA0);
+    bstore<T>(V.r_q2, 6 * G::G1 + c1, B + T(0.5) * sB0);
+    bstore<T>(V.r_q2, 7 * G::G1 + c1, C + T(0.5) * sC0);
+    // edge electric field v x B at t_n from k_efield
+    const auto r_ed = rsrc(S.Ed + long(o) * G::NCE * G::NPE, G::NCE * G::NPE * int(sizeof(T)));
+    auto       ep = [](int a, int c, int e) {
+      return (a + 1) + G::NP * ((c + 1) + (DIM == 3 ? G::NP * (e + 1) : 0));
     };
-    auto Ey = [&](int a, int c, int e) -> T {
-      const T uu = T(0.25) * (Q(2, a - 1, c, e - 1) + Q(2, a - 1, c, e) + Q(2, a, c, e - 1) + Q(2, a, c, e));
-      const T ww = T(0.25) * (Q(4, a - 1, c, e - 1) + Q(4, a - 1, c, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
-      const T AA = T(0.5) * (Q(5, a, c, e - 1) + Q(5, a, c, e));
-      const T CC = T(0.5) * (Q(7, a - 1, c, e) + Q(7, a, c, e));
-      return ww * AA - uu * CC;
-    };
-    auto Ez = [&](int a, int c, int e) -> T {
-      const T uu = T(0.25) * (Q(2, a - 1, c - 1, e) + Q(2, a - 1, c, e) + Q(2, a, c - 1, e) + Q(2, a, c, e));
-      const T vv = T(0.25) * (Q(3, a - 1, c - 1, e) + Q(3, a - 1, c, e) + Q(3, a, c - 1, e) + Q(3, a, c, e));
-      const T AA = T(0.5) * (Q(5, a, c - 1, e) + Q(5, a, c, e));
-      const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
-      return uu * BB - vv * AA;
-    };
-    T * sF = S.sF + long(o) * DIM * G::G1 + c1;
+    auto Ex = [&](int a, int c, int e) -> T { return bload<T>(r_ed, ep(a, c, e)); };
+    auto Ey = [&](int a, int c, int e) -> T { return bload<T>(r_ed, G::NPE + ep(a, c, e)); };
+    auto Ez = [&](int a, int c, int e) -> T { return bload<T>(r_ed, (DIM == 3 ? 2 * G::NPE : 0) + ep(a, c, e)); };
     if (DIM == 2)
     {
       const T e00 = Ez(i, j, 0);
-      sF[0] = (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy;
-      sF[G::G1] = -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx;
+      bstore<T>(V.r_sf, c1, (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy);
+      bstore<T>(V.r_sf, G::G1 + c1, -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx);
     }
     else
     {
       const T ex = Ex(i, j, k), ey = Ey(i, j, k), ez = Ez(i, j, k);
-      sF[0] = (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz;
-      sF[G::G1] = -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz;
-      sF[2 * G::G1] = (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy;
+      bstore<T>(V.r_sf, c1, (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz);
+      bstore<T>(V.r_sf, G::G1 + c1,
+                -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz);
+      bstore<T>(V.r_sf, 2 * G::G1 + c1,
+                (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy);
     }
     // transverse slopes of the face field at this cell's low faces (dBx/dy, dBx/dz, dBy/dx,
     // dBy/dz, dBz/dx, dBz/dy; 2D: dBx/dy, dBy/dx)
-    T * fs = S.FS + long(o) * G::NFS * G::G1 + c1;
     if (DIM == 2)
     {
-      fs[0] = limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st);
-      fs[G::G1] = limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st);
+      bstore<T>(V.r_fs, c1, limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st));
+      bstore<T>(V.r_fs, G::G1 + c1, limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st));
     }
     else
     {
-      fs[0] = limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st);
-      fs[1 * G::G1] = limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st);
-      fs[2 * G::G1] = limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st);
-      fs[3 * G::G1] = limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st);
-      fs[4 * G::G1] = limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st);
-      fs[5 * G::G1] = limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st);
+      bstore<T>(V.r_fs, c1, limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st));
+      bstore<T>(V.r_fs, 1 * G::G1 + c1, limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st));
+      bstore<T>(V.r_fs, 2 * G::G1 + c1, limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st));
+      bstore<T>(V.r_fs, 3 * G::G1 + c1, limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st));
+      bstore<T>(V.r_fs, 4 * G::G1 + c1, limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st));
+      bstore<T>(V.r_fs, 5 * G::G1 + c1, limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st));
     }
   }
 }
@@ -499,21 +637,23 @@ __global__ void __launch_bounds__(256)
 k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   using G = Geo<DIM, BS>;
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
     const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
-    constexpr int        per = DIM * 2 * G::NT;
+    constexpr int        per = G::NTP;
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[2])
       continue;
-    const int              o = int(r0) / per;
+    const int q = __builtin_amdgcn_readfirstlane(int(r0) / per);
+    const int tp = int(r0) - q * per;
+    if (tp >= G::NT)
+      continue;
+    const int              code = S.bside[q];
+    const int              o = code / (2 * DIM);
+    const int              dir = (code >> 1) % DIM;
+    const int              side = code & 1;
     const View<T, DIM, BS> V(S, o);
-    int                    r = int(r0) - o * per;
-    const int              dir = r / (2 * G::NT);
-    r -= dir * 2 * G::NT;
-    const int side = r / G::NT;
-    const int tp = r - side * G::NT;
     if (dir == 0)
       border_face<T, DIM, BS, 0>(S, V, o, side, tp);
     else if (dir == 1)
@@ -552,39 +692,31 @@ tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int 
   {
     const int a0 = fl % e0, a1 = (fl / e0) % e1, a2 = DIM == 3 ? fl / (e0 * e1) : 0;
     int       c[3] = { org[0] + a0, org[1] + a1, org[2] + a2 };
-    T         out[NV];
-    if (c[DIR] == 0 || c[DIR] == BS)
+    T          out[NV];
+    const bool border = c[DIR] == 0 || c[DIR] == BS;
+    const int  side = c[DIR] == BS;
+    if (border && m.conformal[long(o) * 2 * DIM + 2 * DIR + side] != 0)
     {
-      const int side = c[DIR] == BS;
-      const int tp = DIM == 3 ? c[t0] + BS * c[t1] : c[t0];
-      if (m.conformal[long(o) * 2 * DIM + 2 * DIR + side] != 0)
+      // the neighbour is finer: sum of the fine faces covering this face, divided by 2^dim
+      int cc[3] = { c[0], c[1], c[2] };
+      cc[DIR] = side == 0 ? -1 : BS;
+      const Hit    h = neighbour(m, IdxC<DIM, BS>{}, o, cc);
+      const double vr = double(1 << DIM);
+      for (int vi = 0; vi < NV; ++vi)
       {
-        // sum of the fine faces covering this coarse face, divided by 2^dim
-        int cc[3] = { c[0], c[1], c[2] };
-        cc[DIR] = side == 0 ? -1 : BS;
-        const Hit    h = neighbour(m, IdxC<DIM, BS>{}, o, cc);
-        const double vr = double(1 << DIM);
-        for (int vi = 0; vi < NV; ++vi)
+        double sum = 0;
+        if (DIM == 2)
         {
-          double sum = 0;
-          if (DIM == 2)
-          {
-            for (int a = 0; a < 2; ++a)
-              sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, h.l[t0] + a)];
-          }
-          else
-          {
-            for (int q1 = 0; q1 < 2; ++q1)
-              for (int q0 = 0; q0 < 2; ++q0)
-                sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, (h.l[t0] + q0) + BS * (h.l[t1] + q1))];
-          }
-          out[vi] = T(sum / vr);
+          for (int a = 0; a < 2; ++a)
+            sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, h.l[t0] + a)];
         }
-      }
-      else
-      {
-        for (int vi = 0; vi < NV; ++vi)
-          out[vi] = S.BF[bf_index<DIM, BS>(o, DIR, side, vi, tp)];
+        else
+        {
+          for (int q1 = 0; q1 < 2; ++q1)
+            for (int q0 = 0; q0 < 2; ++q0)
+              sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, (h.l[t0] + q0) + BS * (h.l[t1] + q1))];
+        }
+        out[vi] = T(sum / vr);
       }
     }
     else
@@ -610,7 +742,7 @@ k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __
   constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
   constexpr int NV = DIM == 2 ? 6 : 5;
   __shared__ T  lf[NV][NF];
-  for (long t = blockIdx.x; t < total; t += gridDim.x)
+  for (long t = xcd_group(); t < total; t += gridDim.x)
   {
     const int            s = slot_of(off, nslots, t);
     const FusedSlot<T> & S = slots[s];
@@ -783,7 +915,7 @@ __global__ void __launch_bounds__(256)
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   using G = Geo<DIM, BS>;
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
     const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
@@ -791,7 +923,7 @@ k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[4])
       continue;
-    const int o = int(r0) / per;
+    const int o = __builtin_amdgcn_readfirstlane(int(r0) / per);
     int       r = int(r0) - o * per;
     const int a = DIM == 3 ? __builtin_amdgcn_readfirstlane(r / G::EPDP) : 2;
     r -= DIM == 3 ? a * G::EPDP : 0;
@@ -837,7 +969,7 @@ __global__ void __launch_bounds__(256)
 k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   using IX = IdxC<DIM, BS>;
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
     const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
@@ -896,14 +1028,16 @@ k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __re
   }
 }
 
+// a multiple of 8 workgroups (see xcd_group), at most `cap` (itself a multiple of 8)
 inline unsigned
 grid_for(long n, int threads, int cap)
 {
   long g = (n + threads - 1) / threads;
+  g = (g + kXcd - 1) / kXcd * kXcd;
   if (g > cap)
     g = cap;
-  if (g < 1)
-    g = 1;
+  if (g < kXcd)
+    g = kXcd;
   return unsigned(g);
 }
 
@@ -915,6 +1049,8 @@ launch_step(FusedBatch<T> & bt, hipStream_t st, int cap)
   const auto * sl = bt.d_slots;
   hipLaunchKernelGGL((k_prim<T, DIM, BS>), grid_for(bt.tot[0], 256, cap), 256, 0, st, sl, n, bt.d_off[0],
                      bt.tot[0], bt.flags);
+  hipLaunchKernelGGL((k_efield<T, DIM, BS>), grid_for(bt.tot[6], 256, cap), 256, 0, st, sl, n, bt.d_off[6],
+                     bt.tot[6]);
   hipLaunchKernelGGL((k_cell<T, DIM, BS>), grid_for(bt.tot[1], 256, cap), 256, 0, st, sl, n, bt.d_off[1],
                      bt.tot[1]);
   hipLaunchKernelGGL((k_border_flux<T, DIM, BS>), grid_for(bt.tot[2], 256, cap), 256, 0, st, sl, n, bt.d_off[2],
@@ -953,22 +1089,33 @@ fused_items_per_block(int dim, int b, int stage)
   const long n1 = b + 1;
   const long epd = dim == 3 ? long(b) * n1 * n1 : n1 * n1;
   const long epdp = (epd + 63) / 64 * 64;
+  const long nc = dim == 3 ? long(b) * b * b : long(b) * b;
+  const long np = b + 3;
+  const long npe = dim == 3 ? np * np * np : np * np;
   switch (stage)
   {
-  case 0: return g2;
-  case 1: return g1;
-  case 2: return dim * 2 * nt;
+  case 0: return nc + (g2 - nc + 63) / 64 * 64;
+  case 6: return (npe + 63) / 64 * 64;
+  case 1: return (g1 + 63) / 64 * 64;
+  case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
   case 4: return (dim == 3 ? 3 : 1) * epdp;
   default: return dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
   }
 }
 
+long
+fused_border_items_per_side(int dim, int b)
+{
+  const long nt = dim == 3 ? long(b) * b : long(b);
+  return (nt + 63) / 64 * 64;
+}
+
 template <typename T>
 int
 fused_launch(FusedBatch<T> & bt, hipStream_t st)
 {
-  const int cap = bt.grid_cap > 0 ? bt.grid_cap : 1 << 20;
+  const int cap = (bt.grid_cap > 0 ? bt.grid_cap : 1 << 20) / kXcd * kXcd;
   switch (bt.dim * 100 + bt.bs)
   {
   case 208: return launch_step<T, 2, 8>(bt, st, cap);

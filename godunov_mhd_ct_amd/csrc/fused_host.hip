@@ -74,14 +74,39 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.FS = p;  p += nq * z.fs;
   sl.BF = p;  p += nq * z.bf;
   sl.emf = p; p += nq * z.emf;
+  sl.Ed = p;  p += nq * z.ed;
   bt.dim = dim;
   bt.bs = b;
   bt.nslots = 1;
   bt.emf_all = sl.emf;
   bt.flags = c.red + 1;
-  for (int k = 0; k < 6; ++k)
+  // border fluxes are needed only where a coarser neighbour sums them
   {
-    sl.cnt[k] = nq * fused_items_per_block(dim, b, k);
+    std::vector<int> sides;
+    const int        nd = dim == 3 ? 27 : 9, ns = 1 << dim;
+    for (long o = 0; o < nq; ++o)
+      for (int d = 0; d < dim; ++d)
+        for (int side = 0; side < 2; ++side)
+        {
+          int dl[3] = { 0, 0, 0 };
+          dl[d] = side ? 1 : -1;
+          const int di = (dl[0] + 1) + 3 * (dl[1] + 1) + (dim == 3 ? 9 * (dl[2] + 1) : 0);
+          if (c.mesh.nb[(o * nd + di) * ns].rel < 0)
+            sides.push_back(int(o * 2 * dim + 2 * d + side));
+        }
+    sl.nbside = int(sides.size());
+    int * d_sides = nullptr;
+    if (alloc(sides.size() * sizeof(int), reinterpret_cast<void **>(&d_sides)))
+      return -1;
+    if (!sides.empty() &&
+        hipMemcpy(d_sides, sides.data(), sides.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return -1;
+    sl.bside = d_sides;
+  }
+  for (int k = 0; k < FusedBatch<T>::kStages; ++k)
+  {
+    sl.cnt[k] = k == 2 ? long(sl.nbside) * fused_border_items_per_side(dim, b)
+                       : nq * fused_items_per_block(dim, b, k);
     if (sl.cnt[k] >= (1L << 31))
     {
       c.err = "mesh too large for one fused work list (2^31 items per stage)";
@@ -101,10 +126,7 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   bt.fix_src = c.dm.fix_src;
   if (alloc(size_t(bt.nfix > 0 ? bt.nfix : 1) * sizeof(T), reinterpret_cast<void **>(&bt.fixval)))
     return -1;
-  int dev = 0, ncu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  bt.grid_cap = ncu > 0 ? ncu * 16 : 4096;
+  bt.grid_cap = 1 << 20; // one work group per workgroup below this size
   return 0;
 }
 

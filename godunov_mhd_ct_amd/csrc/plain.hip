@@ -5,6 +5,7 @@
 // 256-thread workgroups, FP64 throughout.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 
 #include "common.hpp"
@@ -805,58 +806,47 @@ k_emf_update(DevMesh m, const double * __restrict__ emf2, double * __restrict__ 
 // max, one atomic per workgroup on the bit pattern (inv_dt >= 0 so bits order like values).
 // A NaN candidate is left out of the max and raises error bit 2.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_inv_dt(DevMesh m, Gas<double> g, double scaling, const double * __restrict__ U, const double * __restrict__ B,
-         unsigned long long * red)
+// signal speed over dx of one cell (ComputeDtMHD); IX is the run-time or compile-time index type
+template <class IX>
+__device__ inline double
+cell_inv_dt(const DevMesh & m, const IX & I, const Gas<double> & g, double scaling, const double * __restrict__ U,
+            const double * __restrict__ B, int o, long ci, int i, int j, int k)
 {
-  const Idx  I{ m.dim, m.b };
-  const int  b = m.b, dim = m.dim;
-  const long ncell = I.cells();
-  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  double     cand = 0.0;
-  bool       nan = false;
-  if (t < long(m.nowned) * ncell)
+  const int      b = I.b, dim = I.dim;
+  const long     ncell = I.cells();
+  const double * Bo = B + long(o) * I.fper();
+  double         u[8], q[8];
+  for (int v = 0; v < 5; ++v)
+    u[v] = U[(long(o) * 5 + v) * ncell + ci];
+  if (dim == 2)
   {
-    const int  o = int(t / ncell);
-    const long ci = t - long(o) * ncell;
-    const int  i = int(ci % b), j = int((ci / b) % b), k = dim == 3 ? int(ci / (long(b) * b)) : 0;
-    const double * Bo = B + long(o) * I.fper();
-    double         u[8], q[8];
-    for (int v = 0; v < 5; ++v)
-      u[v] = U[(long(o) * 5 + v) * ncell + ci];
-    if (dim == 2)
-    {
-      u[kA] = 0.5 * (Bo[I.face(0, i, j, 0)] + Bo[I.face(0, i + 1, j, 0)]);
-      u[kB] = 0.5 * (Bo[I.face(1, i, j, 0)] + Bo[I.face(1, i, j + 1, 0)]);
-      u[kC] = 0.5 * (Bo[I.face(2, i, j, 0)] + Bo[I.face(2, i, j, 0)]);
-    }
-    else
-    {
-      u[kA] = 0.5 * (Bo[I.face(0, i, j, k)] + Bo[I.face(0, i + 1, j, k)]);
-      u[kB] = 0.5 * (Bo[I.face(1, i, j, k)] + Bo[I.face(1, i, j + 1, k)]);
-      u[kC] = 0.5 * (Bo[I.face(2, i, j, k)] + Bo[I.face(2, i, j, k + 1)]);
-    }
-    to_primitive(u, q, g);
-    const double b2 = q[kA] * q[kA] + q[kB] * q[kB] + q[kC] * q[kC];
-    const double c2 = g.gamma0 * q[kP] / q[kD];
-    const double d2 = 0.5 * (b2 / q[kD] + c2);
-    const double dx = 1.0 / double(long(b) << m.level[o]) * scaling;
-    const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
-    const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
-    if (dim == 2)
-      cand = vx / dx + vy / dx;
-    else
-    {
-      const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
-      cand = vx / dx + vy / dx + vz / dx;
-    }
-    if (cand != cand)
-    {
-      nan = true;
-      cand = 0.0;
-    }
+    u[kA] = 0.5 * (Bo[I.face(0, i, j, 0)] + Bo[I.face(0, i + 1, j, 0)]);
+    u[kB] = 0.5 * (Bo[I.face(1, i, j, 0)] + Bo[I.face(1, i, j + 1, 0)]);
+    u[kC] = 0.5 * (Bo[I.face(2, i, j, 0)] + Bo[I.face(2, i, j, 0)]);
   }
-  // wave64 butterfly max
+  else
+  {
+    u[kA] = 0.5 * (Bo[I.face(0, i, j, k)] + Bo[I.face(0, i + 1, j, k)]);
+    u[kB] = 0.5 * (Bo[I.face(1, i, j, k)] + Bo[I.face(1, i, j + 1, k)]);
+    u[kC] = 0.5 * (Bo[I.face(2, i, j, k)] + Bo[I.face(2, i, j, k + 1)]);
+  }
+  to_primitive(u, q, g);
+  const double b2 = q[kA] * q[kA] + q[kB] * q[kB] + q[kC] * q[kC];
+  const double c2 = g.gamma0 * q[kP] / q[kD];
+  const double d2 = 0.5 * (b2 / q[kD] + c2);
+  const double dx = 1.0 / double(long(b) << m.level[o]) * scaling;
+  const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
+  const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
+  if (dim == 2)
+    return vx / dx + vy / dx;
+  const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
+  return vx / dx + vy / dx + vz / dx;
+}
+
+// block max of the candidates, one atomic per workgroup; NaN candidates raise flag bit 2
+__device__ inline void
+reduce_inv_dt(double cand, bool nan, unsigned long long * red)
+{
   for (int off = 32; off > 0; off >>= 1)
     cand = fmax(cand, __shfl_xor(cand, off, 64));
   __shared__ double wmax[kThreads / 64];
@@ -873,6 +863,58 @@ k_inv_dt(DevMesh m, Gas<double> g, double scaling, const double * __restrict__ U
       mx = fmax(mx, wmax[w]);
     atomicMax(red, (unsigned long long)__double_as_longlong(mx));
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_inv_dt(DevMesh m, Gas<double> g, double scaling, const double * __restrict__ U, const double * __restrict__ B,
+         unsigned long long * red)
+{
+  const Idx  I{ m.dim, m.b };
+  const int  b = m.b, dim = m.dim;
+  const long ncell = I.cells();
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  double     cand = 0.0;
+  bool       nan = false;
+  if (t < long(m.nowned) * ncell)
+  {
+    const int  o = int(t / ncell);
+    const long ci = t - long(o) * ncell;
+    const int  i = int(ci % b), j = int((ci / b) % b), k = dim == 3 ? int(ci / (long(b) * b)) : 0;
+    cand = cell_inv_dt(m, I, g, scaling, U, B, o, ci, i, j, k);
+    if (cand != cand)
+    {
+      nan = true;
+      cand = 0.0;
+    }
+  }
+  reduce_inv_dt(cand, nan, red);
+}
+
+// The same with the block size fixed at compile time: a bounded grid of workgroups, each
+// folding many cells before its one atomic, numbered XCD by XCD.
+template <int DIM, int BS>
+__global__ void __launch_bounds__(kThreads)
+k_inv_dt_c(DevMesh m, Gas<double> g, double scaling, const double * __restrict__ U, const double * __restrict__ B,
+           unsigned long long * red)
+{
+  using IX = IdxC<DIM, BS>;
+  constexpr IX  I{};
+  constexpr int ncell = IX::cells();
+  const long    total = long(m.nowned) * ncell;
+  double        cand = 0.0;
+  bool          nan = false;
+  for (long t = xcd_group() * kThreads + threadIdx.x; t < total; t += long(gridDim.x) * kThreads)
+  {
+    const int o = int(t / ncell);
+    const int ci = int(t - long(o) * ncell);
+    const int i = ci % BS, j = (ci / BS) % BS, k = DIM == 3 ? ci / (BS * BS) : 0;
+    const double c = cell_inv_dt(m, I, g, scaling, U, B, o, ci, i, j, k);
+    if (c != c)
+      nan = true;
+    else
+      cand = fmax(cand, c);
+  }
+  reduce_inv_dt(cand, nan, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1068,8 +1110,25 @@ launch_inv_dt(Context & c, const double * U, const double * B, hipStream_t s)
   const long   nq = c.dm.nowned;
   if (nq == 0)
     return 0;
-  hipLaunchKernelGGL(k_inv_dt, grid_of(nq * S.cells()), kThreads, 0, s, c.dm, gas_of(c.st), c.st.scaling, U, B,
-                     c.red);
+  if (c.impl != 10 && (c.b == 8 || c.b == 16 || c.b == 32))
+  {
+    // the fused forms' reduction: compile-time geometry, bounded grid (a multiple of 8)
+    const long groups = (nq * S.cells() + kThreads - 1) / kThreads;
+    const unsigned grid = unsigned(std::min<long>((groups + 7) / 8 * 8, 2048));
+    const Gas<double> g = gas_of(c.st);
+    switch (c.dim * 100 + c.b)
+    {
+    case 208: hipLaunchKernelGGL((k_inv_dt_c<2, 8>), grid, kThreads, 0, s, c.dm, g, c.st.scaling, U, B, c.red); break;
+    case 216: hipLaunchKernelGGL((k_inv_dt_c<2, 16>), grid, kThreads, 0, s, c.dm, g, c.st.scaling, U, B, c.red); break;
+    case 232: hipLaunchKernelGGL((k_inv_dt_c<2, 32>), grid, kThreads, 0, s, c.dm, g, c.st.scaling, U, B, c.red); break;
+    case 308: hipLaunchKernelGGL((k_inv_dt_c<3, 8>), grid, kThreads, 0, s, c.dm, g, c.st.scaling, U, B, c.red); break;
+    case 316: hipLaunchKernelGGL((k_inv_dt_c<3, 16>), grid, kThreads, 0, s, c.dm, g, c.st.scaling, U, B, c.red); break;
+    default: hipLaunchKernelGGL((k_inv_dt_c<3, 32>), grid, kThreads, 0, s, c.dm, g, c.st.scaling, U, B, c.red); break;
+    }
+  }
+  else
+    hipLaunchKernelGGL(k_inv_dt, grid_of(nq * S.cells()), kThreads, 0, s, c.dm, gas_of(c.st), c.st.scaling, U, B,
+                       c.red);
   MHDCT_HIP(hipGetLastError());
   return 0;
 }
