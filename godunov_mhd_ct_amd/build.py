@@ -30,7 +30,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["mesh.cpp", "plain.hip", "fused.hip", "fused_host.hip", "capi.hip"]
+SOURCES = ["mesh.cpp", "plain.hip", "fused.hip", "fused_host.hip", "batch.hip", "capi.hip"]
 
 
 def _compile(src: str) -> Path:

@@ -7,27 +7,13 @@
 
 #include "../../include/mhdct.h"
 #include "ctx.hpp"
+#include "fused_host.hpp"
+#include "handles.hpp"
 
 using mhdct::Context;
 using mhdct::Mesh;
 
-struct mhdct_mesh
-{
-  Mesh m;
-};
 
-struct mhdct_ctx
-{
-  Context c;
-};
-
-namespace mhdct
-{
-int  fused_alloc(Context & c);
-void fused_free(Context & c);
-int  fused_step(Context & c, const double * U, double * U2, const double * B, double * B2, double dt,
-                hipStream_t s);
-} // namespace mhdct
 
 namespace
 {

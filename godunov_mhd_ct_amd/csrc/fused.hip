@@ -42,6 +42,34 @@ namespace
 
 constexpr int kTS = 8; // tile edge of the flux kernel
 
+// Calls CALL with the compile-time block size BS: BSEL when the launch is specialised to one
+// block size, else the one matching the run-time value bs (8, 16 or 32; a slot property,
+// uniform across the wavefront).
+#define MHDCT_BS_DISPATCH(bs, CALL)                                                                                    \
+  do                                                                                                                   \
+  {                                                                                                                    \
+    if constexpr (BSEL != 0)                                                                                           \
+    {                                                                                                                  \
+      constexpr int BS = BSEL;                                                                                         \
+      CALL;                                                                                                            \
+    }                                                                                                                  \
+    else if ((bs) == 8)                                                                                                \
+    {                                                                                                                  \
+      constexpr int BS = 8;                                                                                            \
+      CALL;                                                                                                            \
+    }                                                                                                                  \
+    else if ((bs) == 16)                                                                                               \
+    {                                                                                                                  \
+      constexpr int BS = 16;                                                                                           \
+      CALL;                                                                                                            \
+    }                                                                                                                  \
+    else                                                                                                               \
+    {                                                                                                                  \
+      constexpr int BS = 32;                                                                                           \
+      CALL;                                                                                                            \
+    }                                                                                                                  \
+  } while (0)
+
 // compile-time geometry of one block size class
 template <int DIM, int BS>
 struct Geo
@@ -71,7 +99,7 @@ struct Geo
   static_assert(BS % kTS == 0, "block size must be a multiple of the flux tile");
 };
 
-__device__ inline int
+__device__ __forceinline__ int
 slot_of(const long * off, int nslots, long t)
 {
   int s = 0;
@@ -82,14 +110,14 @@ slot_of(const long * off, int nslots, long t)
 
 // Raw buffer resource over `bytes` bytes at `base` (bounds-checked vector loads with a 32-bit
 // offset; word 3 selects the 32-bit data format of gfx9-class buffer descriptors).
-__device__ inline __amdgpu_buffer_rsrc_t
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t
 rsrc(const void * base, int bytes)
 {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, 0x00020000);
 }
 
 template <typename T>
-__device__ inline T
+__device__ __forceinline__ T
 bload(__amdgpu_buffer_rsrc_t r, int idx)
 {
   if constexpr (sizeof(T) == 8)
@@ -99,7 +127,7 @@ bload(__amdgpu_buffer_rsrc_t r, int idx)
 }
 
 template <typename T>
-__device__ inline void
+__device__ __forceinline__ void
 bstore(__amdgpu_buffer_rsrc_t r, int idx, T x)
 {
   if constexpr (sizeof(T) == 8)
@@ -190,7 +218,7 @@ shell_cell(int g, int & i, int & j, int & k)
 }
 
 template <int BS>
-__device__ inline double
+__device__ __forceinline__ double
 dxd(const DevMesh & m, double scaling, int o)
 {
   return 1.0 / double(long(BS) << m.level[o]) * scaling;
@@ -201,21 +229,14 @@ dxd(const DevMesh & m, double scaling, int o)
 // The cell owns its low faces (and its high faces on the last ghost layer).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(256)
-k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total,
-       unsigned long long * flags)
+__device__ __forceinline__ void
+prim_item(const FusedSlot<T> & S, long r0)
 {
+
   using G = Geo<DIM, BS>;
   using IX = IdxC<DIM, BS>;
   constexpr IX I{};
-  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
-  {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
-    const FusedSlot<T> & S = slots[s];
     const DevMesh &      m = S.m;
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[0])
-      continue;
     // interior cells first, then the ghost shell: every wavefront is one or the other
     const int o = int(r0) / G::PRIM;
     int       g = int(r0) - o * G::PRIM;
@@ -231,7 +252,7 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     {
       g -= G::NC;
       if (g >= G::GSH)
-        continue;
+        return;
       shell_cell<DIM, BS>(g, i, j, k);
     }
     const int ci = (i + 2) + n * ((j + 2) + (DIM == 3 ? n * (k + 2) : 0));
@@ -316,7 +337,7 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     double       q[8];
     const double eint = to_primitive(u, q, S.gd);
     if (S.abort_eint && eint < 0)
-      atomicOr(flags, 1ull);
+      atomicOr(S.red + 1, 1ull);
     T * Qo = S.Qp + long(o) * 5 * G::G2;
     for (int v = 0; v < 5; ++v)
       Qo[v * G::G2 + ci] = T(q[v]);
@@ -344,29 +365,39 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       if (k == b + 1)
         Go[IX::gface(2, i, j, b + 2)] = T(gfv(2, i, j, b + 2));
     }
+}
+
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[0])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (prim_item<T, DIM, BS>(S, r0)));
   }
 }
+
 
 // ---------------------------------------------------------------------------------------------
 // k_efield: edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235) on the
 // edge points [-1, b+1]^dim of each block, from the primitive state of the 2-ghost tile.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(256)
-k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+__device__ __forceinline__ void
+efield_item(const FusedSlot<T> & S, long r0)
 {
+
   using G = Geo<DIM, BS>;
-  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
-  {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
-    const FusedSlot<T> & S = slots[s];
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[6])
-      continue;
+
     const int o = __builtin_amdgcn_readfirstlane(int(r0) / G::EFP);
     const int r = int(r0) - o * G::EFP;
     if (r >= G::NPE)
-      continue;
+      return;
     constexpr int          np = G::NP;
     const int              a = r % np - 1, c = (r / np) % np - 1, e = DIM == 3 ? r / (np * np) - 1 : 0;
     const View<T, DIM, BS> V(S, o);
@@ -403,30 +434,39 @@ k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __res
       const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
       Ed[DIM == 3 ? 2 * G::NPE : 0] = uu * BB - vv * AA;
     }
+}
+
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[6])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (efield_item<T, DIM, BS>(S, r0)));
   }
 }
+
 
 // ---------------------------------------------------------------------------------------------
 // k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(256)
-k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+__device__ __forceinline__ void
+cell_item(const FusedSlot<T> & S, long r0)
 {
+
   using G = Geo<DIM, BS>;
-  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
-  {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
-    const FusedSlot<T> & S = slots[s];
     const DevMesh &      m = S.m;
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[1])
-      continue;
     constexpr int             n = G::N1;
     const int                 o = __builtin_amdgcn_readfirstlane(int(r0) / G::P1);
     const int                 ci = int(r0) - o * G::P1;
     if (ci >= G::G1)
-      continue;
+      return;
     const View<T, DIM, BS>    V(S, o);
     const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
     const T   st = S.gt.slope_type;
@@ -553,13 +593,28 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       bstore<T>(V.r_fs, 4 * G::G1 + c1, limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st));
       bstore<T>(V.r_fs, 5 * G::G1 + c1, limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st));
     }
+}
+
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[1])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS>(S, r0)));
   }
 }
+
 
 // HLLD flux (times dt/dx) through the low face of cell (i,j,k) along DIR
 // (ComputeHydroFluxesAndStoreFunctor.cpp:218-503); out in the reference variable order.
 template <typename T, int DIM, int BS, int DIR>
-__device__ inline void
+__device__ __forceinline__ void
 face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
 {
   using VW = View<T, DIM, BS>;
@@ -604,7 +659,7 @@ face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
 __device__ constexpr int kFluxVar[6] = { kD, kP, kU, kV, kW, kC };
 
 template <int DIM, int BS>
-__device__ inline long
+__device__ __forceinline__ long
 bf_index(int o, int dir, int side, int v, int tpos)
 {
   constexpr int nt = Geo<DIM, BS>::NT;
@@ -615,7 +670,7 @@ bf_index(int o, int dir, int side, int v, int tpos)
 // k_border_flux: HLLD on every block-border face (both sides, every direction)
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS, int DIR>
-__device__ inline void
+__device__ __forceinline__ void
 border_face(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int side, int tp)
 {
   constexpr int t0 = DIR == 0 ? 1 : 0;
@@ -633,22 +688,16 @@ border_face(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int side,
 }
 
 template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(256)
-k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+__device__ __forceinline__ void
+border_flux_item(const FusedSlot<T> & S, long r0)
 {
+
   using G = Geo<DIM, BS>;
-  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
-  {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
-    const FusedSlot<T> & S = slots[s];
     constexpr int        per = G::NTP;
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[2])
-      continue;
     const int q = __builtin_amdgcn_readfirstlane(int(r0) / per);
     const int tp = int(r0) - q * per;
     if (tp >= G::NT)
-      continue;
+      return;
     const int              code = S.bside[q];
     const int              o = code / (2 * DIM);
     const int              dir = (code >> 1) % DIM;
@@ -660,8 +709,23 @@ k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * 
       border_face<T, DIM, BS, 1>(S, V, o, side, tp);
     else if (DIM == 3)
       border_face<T, DIM, BS, 2>(S, V, o, side, tp);
+}
+
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[2])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (border_flux_item<T, DIM, BS>(S, r0)));
   }
 }
+
 
 // ---------------------------------------------------------------------------------------------
 // k_flux_tile: one workgroup per 8^dim tile; HLLD on the tile faces, one direction at a time
@@ -676,7 +740,7 @@ tile_threads()
 }
 
 template <typename T, int DIM, int BS, int DIR>
-__device__ inline void
+__device__ __forceinline__ void
 tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int org[3], T dtdS, T (*lf)[(kTS + 1) * (DIM == 3 ? kTS * kTS : kTS)])
 {
   constexpr int TS = kTS;
@@ -732,24 +796,18 @@ tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int 
 }
 
 template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(tile_threads<DIM>())
-k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+__device__ __forceinline__ void
+flux_tile_item(const FusedSlot<T> & S, long r0, T (*lf)[(kTS + 1) * (DIM == 3 ? kTS * kTS : kTS)])
 {
+
   using G = Geo<DIM, BS>;
   using IX = IdxC<DIM, BS>;
   constexpr int TS = kTS;
   constexpr int NCT = DIM == 3 ? TS * TS * TS : TS * TS;
   constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
   constexpr int NV = DIM == 2 ? 6 : 5;
-  __shared__ T  lf[NV][NF];
-  for (long t = xcd_group(); t < total; t += gridDim.x)
+  const DevMesh & m = S.m;
   {
-    const int            s = slot_of(off, nslots, t);
-    const FusedSlot<T> & S = slots[s];
-    const DevMesh &      m = S.m;
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[3])
-      continue;
     const int              o = int(r0) / G::TPB;
     const View<T, DIM, BS> V(S, o);
     const int              tl = int(r0) - o * G::TPB;
@@ -806,10 +864,29 @@ k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __
   }
 }
 
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(tile_threads<DIM>())
+k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  constexpr int NF = DIM == 3 ? (kTS + 1) * kTS * kTS : (kTS + 1) * kTS;
+  constexpr int NV = DIM == 2 ? 6 : 5;
+  __shared__ T  lf[NV][NF];
+  for (long t = xcd_group(); t < total; t += gridDim.x)
+  {
+    const int            s = slot_of(off, nslots, t);
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[3])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (flux_tile_item<T, DIM, BS>(S, r0, lf)));
+  }
+}
+
+
 // corner state of an edge (ComputeEmfAndStoreFunctor.cpp:105-367); KIND 0 LB, 1 RT, 2 RB, 3 LT,
 // (D0, D1) the two directions across the edge
 template <typename T, int DIM, int BS, int KIND, int D0, int D1>
-__device__ inline void
+__device__ __forceinline__ void
 edge_state(const View<T, DIM, BS> & V, int is, int js, int ks, T * q)
 {
   using VW = View<T, DIM, BS>;
@@ -871,7 +948,7 @@ edge_state(const View<T, DIM, BS> & V, int is, int js, int ks, T * q)
 
 // the EMF along axis AX at edge r (of EPD) of block o
 template <typename T, int DIM, int BS, int AX>
-__device__ inline void
+__device__ __forceinline__ void
 edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx)
 {
   using G = Geo<DIM, BS>;
@@ -911,24 +988,18 @@ edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx
 // uniform and compiled in.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(256)
-k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+__device__ __forceinline__ void
+edge_item(const FusedSlot<T> & S, long r0)
 {
+
   using G = Geo<DIM, BS>;
-  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
-  {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
-    const FusedSlot<T> & S = slots[s];
     constexpr int        per = G::NCE * G::EPDP;
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[4])
-      continue;
     const int o = __builtin_amdgcn_readfirstlane(int(r0) / per);
     int       r = int(r0) - o * per;
     const int a = DIM == 3 ? __builtin_amdgcn_readfirstlane(r / G::EPDP) : 2;
     r -= DIM == 3 ? a * G::EPDP : 0;
     if (r >= G::EPD)
-      continue;
+      return;
     const View<T, DIM, BS> V(S, o);
     const T                dtdx = T(S.dt / dxd<BS>(S.m, S.scaling, o));
     if (DIM == 2 || a == 2)
@@ -937,46 +1008,180 @@ k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       edge_at<T, DIM, BS, 1>(S, V, o, r, dtdx);
     else
       edge_at<T, DIM, BS, 0>(S, V, o, r, dtdx);
-  }
 }
 
-template <typename T>
+template <typename T, int DIM, int BSEL>
 __global__ void __launch_bounds__(256)
-k_fix(int nfix, const EmfFix * __restrict__ fix, const int64_t * __restrict__ src, const T * __restrict__ emf,
-      T * __restrict__ val)
+k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < nfix; t += long(gridDim.x) * blockDim.x)
-  {
-    const EmfFix f = fix[t];
-    T            s = 0;
-    for (int n = 0; n < f.nsrc; ++n)
-      s += emf[src[f.first + n]];
-    val[t] = s / T(f.cnt) / T(f.ratio);
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_scatter(int nfix, const EmfFix * __restrict__ fix, const T * __restrict__ val, T * __restrict__ emf)
-{
-  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < nfix; t += long(gridDim.x) * blockDim.x)
-    emf[fix[t].dst] = val[t];
-}
-
-// face-B update (ReadEmfAndUpdateFunctor.cpp:58-163), FP64 accumulation
-template <typename T, int DIM, int BS>
-__global__ void __launch_bounds__(256)
-k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
-{
-  using IX = IdxC<DIM, BS>;
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
   {
     const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
     const FusedSlot<T> & S = slots[s];
-    constexpr int        per = IX::fper();
     const long           r0 = t - off[s];
-    if (r0 >= S.cnt[5])
+    if (r0 >= S.cnt[4])
       continue;
+    MHDCT_BS_DISPATCH(S.m.b, (edge_item<T, DIM, BS>(S, r0)));
+  }
+}
+
+
+// CorrectEmfAtBlockBorder from each slot's gather list: dst = (sum of sources) / cnt / ratio,
+// summed in list order; the values are staged in fixval and scattered by k_scatter.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_fix(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.m.nfix)
+      continue;
+    const EmfFix f = S.m.fix[r0];
+    T            v = 0;
+    for (int n = 0; n < f.nsrc; ++n)
+      v += S.emf[S.m.fix_src[f.first + n]];
+    S.fixval[r0] = v / T(f.cnt) / T(f.ratio);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_scatter(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.m.nfix)
+      continue;
+    S.emf[S.m.fix[r0].dst] = S.fixval[r0];
+  }
+}
+
+// ComputeDtMHD over every slot's owned cells (ComputeDtMHDFunctor.cpp:86-151): per workgroup a
+// max of (c_f + |v|)/dx summed over directions, one atomic into the slot's reduction word;
+// a NaN candidate raises flag bit 2.  Every workgroup lies in one slot (slot ranges are
+// padded to 512 items).
+template <typename T, int DIM, int BS>
+__device__ __forceinline__ double
+inv_dt_item(const FusedSlot<T> & S, long r0)
+{
+  using IX = IdxC<DIM, BS>;
+  constexpr int ncell = IX::cells();
+  const int     o = int(r0) / ncell;
+  const int     ci = int(r0) - o * ncell;
+  const int     i = ci % BS, j = (ci / BS) % BS, k = DIM == 3 ? ci / (BS * BS) : 0;
+  const double * Bo = S.B + long(o) * IX::fper();
+  const Gas<double> & g = S.gd;
+  double        u[8], q[8];
+  for (int v = 0; v < 5; ++v)
+    u[v] = S.U[(long(o) * 5 + v) * ncell + ci];
+  if (DIM == 2)
+  {
+    u[kA] = 0.5 * (Bo[IX::face(0, i, j, 0)] + Bo[IX::face(0, i + 1, j, 0)]);
+    u[kB] = 0.5 * (Bo[IX::face(1, i, j, 0)] + Bo[IX::face(1, i, j + 1, 0)]);
+    u[kC] = 0.5 * (Bo[IX::face(2, i, j, 0)] + Bo[IX::face(2, i, j, 0)]);
+  }
+  else
+  {
+    u[kA] = 0.5 * (Bo[IX::face(0, i, j, k)] + Bo[IX::face(0, i + 1, j, k)]);
+    u[kB] = 0.5 * (Bo[IX::face(1, i, j, k)] + Bo[IX::face(1, i, j + 1, k)]);
+    u[kC] = 0.5 * (Bo[IX::face(2, i, j, k)] + Bo[IX::face(2, i, j, k + 1)]);
+  }
+  to_primitive(u, q, g);
+  const double b2 = q[kA] * q[kA] + q[kB] * q[kB] + q[kC] * q[kC];
+  const double c2 = g.gamma0 * q[kP] / q[kD];
+  const double d2 = 0.5 * (b2 / q[kD] + c2);
+  const double dx = dxd<BS>(S.m, S.scaling, o);
+  const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
+  const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
+  if (DIM == 2)
+    return vx / dx + vy / dx;
+  const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
+  return vx / dx + vy / dx + vz / dx;
+}
+
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_inv_dt(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  __shared__ double wmax[4];
+  for (long g0 = xcd_group() * blockDim.x; g0 < total; g0 += long(gridDim.x) * blockDim.x)
+  {
+    const long           t = g0 + threadIdx.x;
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, g0));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    double               cand = 0.0;
+    bool                 nan = false;
+    if (t < total && r0 < S.cnt[kStageInvDt])
+    {
+      MHDCT_BS_DISPATCH(S.m.b, (cand = inv_dt_item<T, DIM, BS>(S, r0)));
+      if (cand != cand)
+      {
+        nan = true;
+        cand = 0.0;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+      cand = fmax(cand, __shfl_xor(cand, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+      wmax[wave] = cand;
+    if (__any(nan) && lane == 0)
+      atomicOr(S.red + 1, 2ull);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+      double mx = wmax[0];
+      for (int w = 1; w < 4; ++w)
+        mx = fmax(mx, wmax[w]);
+      atomicMax(S.red, (unsigned long long)__double_as_longlong(mx));
+    }
+    __syncthreads();
+  }
+}
+
+// Per slot: clear the reduction words before k_inv_dt (a kernel, so that the step can be
+// captured in a graph), then dt = cfl / inv_dt into both parities of the slot array, and the
+// slot's clock and step count advanced.
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_dt_clear(FusedSlot<T> * __restrict__ slots, int nslots)
+{
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += gridDim.x * blockDim.x)
+  {
+    slots[s].red[0] = 0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_dt_set(FusedSlot<T> * __restrict__ even, FusedSlot<T> * __restrict__ odd, int nslots, double * __restrict__ clock)
+{
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += gridDim.x * blockDim.x)
+  {
+    const double inv = __longlong_as_double((long long)even[s].red[0]);
+    const double dt = even[s].cfl / inv;
+    even[s].dt = dt;
+    odd[s].dt = dt;
+    clock[2 * s] += dt;
+    clock[2 * s + 1] += 1.0;
+  }
+}
+
+// face-B update (ReadEmfAndUpdateFunctor.cpp:58-163), FP64 accumulation
+template <typename T, int DIM, int BS>
+__device__ __forceinline__ void
+bupdate_item(const FusedSlot<T> & S, long r0)
+{
+
+  using IX = IdxC<DIM, BS>;
+    constexpr int        per = IX::fper();
     const int o = int(r0) / per;
     const int fi = int(r0) - o * per;
     int       r = fi;
@@ -1003,7 +1208,7 @@ k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __re
         S.B2[long(o) * per + fi] = Bv + (double(e[IX::edge(i, j + 1, 0)]) - double(e[IX::edge(i, j, 0)]));
       else if (c == 1)
         S.B2[long(o) * per + fi] = Bv - (double(e[IX::edge(i + 1, j, 0)]) - double(e[IX::edge(i, j, 0)]));
-      continue;
+      return;
     }
     const T * ex = S.emf + (long(o) * 3 + 0) * ne;
     const T * ey = S.emf + (long(o) * 3 + 1) * ne;
@@ -1025,8 +1230,23 @@ k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __re
       dB -= (double(ex[IX::edge(i, j + 1, k)]) - double(ex[IX::edge(i, j, k)]));
     }
     S.B2[long(o) * per + fi] = Bv + dB;
+}
+
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_bupdate(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (r0 >= S.cnt[5])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (bupdate_item<T, DIM, BS>(S, r0)));
   }
 }
+
 
 // a multiple of 8 workgroups (see xcd_group), at most `cap` (itself a multiple of 8)
 inline unsigned
@@ -1041,33 +1261,48 @@ grid_for(long n, int threads, int cap)
   return unsigned(g);
 }
 
-template <typename T, int DIM, int BS>
+template <typename T, int DIM, int BSEL>
 int
-launch_step(FusedBatch<T> & bt, hipStream_t st, int cap)
+launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st)
 {
-  const int    n = bt.nslots;
-  const auto * sl = bt.d_slots;
-  hipLaunchKernelGGL((k_prim<T, DIM, BS>), grid_for(bt.tot[0], 256, cap), 256, 0, st, sl, n, bt.d_off[0],
-                     bt.tot[0], bt.flags);
-  hipLaunchKernelGGL((k_efield<T, DIM, BS>), grid_for(bt.tot[6], 256, cap), 256, 0, st, sl, n, bt.d_off[6],
-                     bt.tot[6]);
-  hipLaunchKernelGGL((k_cell<T, DIM, BS>), grid_for(bt.tot[1], 256, cap), 256, 0, st, sl, n, bt.d_off[1],
-                     bt.tot[1]);
-  hipLaunchKernelGGL((k_border_flux<T, DIM, BS>), grid_for(bt.tot[2], 256, cap), 256, 0, st, sl, n, bt.d_off[2],
-                     bt.tot[2]);
-  hipLaunchKernelGGL((k_flux_tile<T, DIM, BS>), grid_for(bt.tot[3], 1, cap), tile_threads<DIM>(), 0, st, sl, n,
-                     bt.d_off[3], bt.tot[3]);
-  hipLaunchKernelGGL((k_edge<T, DIM, BS>), grid_for(bt.tot[4], 256, cap), 256, 0, st, sl, n, bt.d_off[4],
-                     bt.tot[4]);
-  if (bt.nfix > 0)
+  const int            n = bt.nslots;
+  const FusedSlot<T> * sl = bt.d_slots[parity];
+  const int            cap = bt.tune.grid_cap / kXcd * kXcd;
+  const int            tcap = bt.tune.tile_cap / kXcd * kXcd;
+  auto                 grid = [&](int stage) { return grid_for(bt.tot[stage], 256, cap); };
+  hipLaunchKernelGGL((k_prim<T, DIM, BSEL>), grid(kStagePrim), 256, 0, st, sl, n, bt.d_off[kStagePrim],
+                     bt.tot[kStagePrim]);
+  hipLaunchKernelGGL((k_efield<T, DIM, BSEL>), grid(kStageEfield), 256, 0, st, sl, n, bt.d_off[kStageEfield],
+                     bt.tot[kStageEfield]);
+  hipLaunchKernelGGL((k_cell<T, DIM, BSEL>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
+                     bt.tot[kStageCell]);
+  if (bt.tot[kStageBorder] > 0)
+    hipLaunchKernelGGL((k_border_flux<T, DIM, BSEL>), grid(kStageBorder), 256, 0, st, sl, n, bt.d_off[kStageBorder],
+                       bt.tot[kStageBorder]);
+  hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL>), grid_for(bt.tot[kStageTile], 1, tcap), tile_threads<DIM>(), 0, st, sl,
+                     n, bt.d_off[kStageTile], bt.tot[kStageTile]);
+  hipLaunchKernelGGL((k_edge<T, DIM, BSEL>), grid(kStageEdge), 256, 0, st, sl, n, bt.d_off[kStageEdge],
+                     bt.tot[kStageEdge]);
+  if (bt.tot[kStageFix] > 0)
   {
-    hipLaunchKernelGGL((k_fix<T>), grid_for(bt.nfix, 256, cap), 256, 0, st, bt.nfix, bt.fix, bt.fix_src,
-                       bt.emf_all, bt.fixval);
-    hipLaunchKernelGGL((k_scatter<T>), grid_for(bt.nfix, 256, cap), 256, 0, st, bt.nfix, bt.fix, bt.fixval,
-                       bt.emf_all);
+    hipLaunchKernelGGL((k_fix<T>), grid(kStageFix), 256, 0, st, sl, n, bt.d_off[kStageFix], bt.tot[kStageFix]);
+    hipLaunchKernelGGL((k_scatter<T>), grid(kStageFix), 256, 0, st, sl, n, bt.d_off[kStageFix],
+                       bt.tot[kStageFix]);
   }
-  hipLaunchKernelGGL((k_bupdate<T, DIM, BS>), grid_for(bt.tot[5], 256, cap), 256, 0, st, sl, n, bt.d_off[5],
-                     bt.tot[5]);
+  hipLaunchKernelGGL((k_bupdate<T, DIM, BSEL>), grid(kStageBupdate), 256, 0, st, sl, n, bt.d_off[kStageBupdate],
+                     bt.tot[kStageBupdate]);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T, int DIM, int BSEL>
+int
+launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
+{
+  const int n = bt.nslots;
+  hipLaunchKernelGGL((k_dt_clear<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[parity], n);
+  hipLaunchKernelGGL((k_inv_dt<T, DIM, BSEL>), grid_for(bt.tot[kStageInvDt], 256, bt.tune.grid_cap / kXcd * kXcd), 256, 0,
+                     st, bt.d_slots[parity], n, bt.d_off[kStageInvDt], bt.tot[kStageInvDt]);
+  hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], n, bt.clock);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1096,6 +1331,7 @@ fused_items_per_block(int dim, int b, int stage)
   {
   case 0: return nc + (g2 - nc + 63) / 64 * 64;
   case 6: return (npe + 63) / 64 * 64;
+  case 8: return nc;
   case 1: return (g1 + 63) / 64 * 64;
   case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
@@ -1113,22 +1349,47 @@ fused_border_items_per_side(int dim, int b)
 
 template <typename T>
 int
-fused_launch(FusedBatch<T> & bt, hipStream_t st)
+fused_launch(const FusedBatch<T> & bt, int parity, hipStream_t st)
 {
-  const int cap = (bt.grid_cap > 0 ? bt.grid_cap : 1 << 20) / kXcd * kXcd;
-  switch (bt.dim * 100 + bt.bs)
+  if (bt.nslots == 0)
+    return 0;
+  switch (bt.dim * 100 + bt.bsel)
   {
-  case 208: return launch_step<T, 2, 8>(bt, st, cap);
-  case 216: return launch_step<T, 2, 16>(bt, st, cap);
-  case 232: return launch_step<T, 2, 32>(bt, st, cap);
-  case 308: return launch_step<T, 3, 8>(bt, st, cap);
-  case 316: return launch_step<T, 3, 16>(bt, st, cap);
-  case 332: return launch_step<T, 3, 32>(bt, st, cap);
+  case 200: return launch_stages<T, 2, 0>(bt, parity, st);
+  case 208: return launch_stages<T, 2, 8>(bt, parity, st);
+  case 216: return launch_stages<T, 2, 16>(bt, parity, st);
+  case 232: return launch_stages<T, 2, 32>(bt, parity, st);
+  case 300: return launch_stages<T, 3, 0>(bt, parity, st);
+  case 308: return launch_stages<T, 3, 8>(bt, parity, st);
+  case 316: return launch_stages<T, 3, 16>(bt, parity, st);
+  case 332: return launch_stages<T, 3, 32>(bt, parity, st);
   default: return -1;
   }
 }
 
-template int fused_launch<double>(FusedBatch<double> &, hipStream_t);
-template int fused_launch<float>(FusedBatch<float> &, hipStream_t);
+template <typename T>
+int
+fused_launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
+{
+  if (bt.nslots == 0)
+    return 0;
+  switch (bt.dim * 100 + bt.bsel)
+  {
+  case 200: return launch_dt<T, 2, 0>(bt, parity, st);
+  case 208: return launch_dt<T, 2, 8>(bt, parity, st);
+  case 216: return launch_dt<T, 2, 16>(bt, parity, st);
+  case 232: return launch_dt<T, 2, 32>(bt, parity, st);
+  case 300: return launch_dt<T, 3, 0>(bt, parity, st);
+  case 308: return launch_dt<T, 3, 8>(bt, parity, st);
+  case 316: return launch_dt<T, 3, 16>(bt, parity, st);
+  case 332: return launch_dt<T, 3, 32>(bt, parity, st);
+  default: return -1;
+  }
+}
+
+template int fused_launch<double>(const FusedBatch<double> &, int, hipStream_t);
+template int fused_launch<float>(const FusedBatch<float> &, int, hipStream_t);
+template int fused_launch_dt<double>(const FusedBatch<double> &, int, hipStream_t);
+template int fused_launch_dt<float>(const FusedBatch<float> &, int, hipStream_t);
 
 } // namespace mhdct

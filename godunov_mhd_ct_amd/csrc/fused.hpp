@@ -11,32 +11,51 @@
 namespace mhdct
 {
 
+// Stages of one fused step, in launch order: prim, efield, cell, border, tile, edge, fix
+// (k_fix + k_scatter), bupdate; the dt reduction is a separate launch before them.
+enum : int
+{
+  kStagePrim = 0,    // interior cells, then the ghost shell of the 2-ghost tile (padded to 64)
+  kStageCell = 1,    // ghost-1 cells (padded to 64 per block)
+  kStageBorder = 2,  // faces of the coarse-facing block sides (padded to 64 per side)
+  kStageTile = 3,    // 8^dim flux tiles (one workgroup each)
+  kStageEdge = 4,    // edges, each direction padded to 64
+  kStageBupdate = 5, // faces
+  kStageEfield = 6,  // edge points of the E grid (padded to 64 per block)
+  kStageFix = 7,     // corrected border edges (the mesh's gather list)
+  kStageInvDt = 8,   // owned cells (dt reduction)
+  kStages = 9
+};
+
 // One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
 template <typename T>
 struct FusedSlot
 {
-  DevMesh        m;
-  Gas<T>         gt;  // bulk settings
-  Gas<double>    gd;  // primitive recovery
-  double         dt;
-  double         scaling;
-  int            abort_eint;
-  const double * U;
-  double *       U2;
-  const double * B;
-  double *       B2;
-  T *            Qp;  // [block][5][(b+4)^d]       primitive, 2 ghost layers
-  T *            Bg;  // [block][ghosted faces]     face field, 2 ghost layers
-  T *            Q2;  // [block][8][(b+2)^d]       predictor
-  T *            S;   // [block][dim][8][(b+2)^d]  limited slopes
-  T *            sF;  // [block][dim][(b+2)^d]     face-B half-step source
-  T *            FS;  // [block][6|2][(b+2)^d]     transverse slopes of the face field
-  T *            BF;  // [block][dim][2][6][b^(d-1)] border-face fluxes
-  T *            emf; // [block][3|1][(b+1)^d]
-  T *            Ed;  // [block][3|1][(b+3)^d]     edge electric field v x B at t_n
-  const int *    bside; // sides (block*2*dim + 2*dir + side) whose face neighbour is coarser
-  int            nbside;
-  long           cnt[7]; // work items per stage (the stage offsets are padded to workgroup multiples)
+  DevMesh              m;
+  Gas<T>               gt; // bulk settings
+  Gas<double>          gd; // primitive recovery and dt
+  double               dt;
+  double               cfl;
+  double               scaling;
+  int                  abort_eint;
+  const double *       U;
+  double *             U2;
+  const double *       B;
+  double *             B2;
+  T *                  Qp;     // [block][5][(b+4)^d]       primitive, 2 ghost layers
+  T *                  Bg;     // [block][ghosted faces]     face field, 2 ghost layers
+  T *                  Q2;     // [block][8][(b+2)^d]       predictor
+  T *                  S;      // [block][dim][8][(b+2)^d]  limited slopes
+  T *                  sF;     // [block][dim][(b+2)^d]     face-B half-step source
+  T *                  FS;     // [block][6|2][(b+2)^d]     transverse slopes of the face field
+  T *                  BF;     // [block][dim][2][6][b^(d-1)] border-face fluxes
+  T *                  emf;    // [block][3|1][(b+1)^d]
+  T *                  Ed;     // [block][3|1][(b+3)^d]     edge electric field v x B at t_n
+  T *                  fixval; // [nfix]                    staged corrected border EMFs
+  const int *          bside;  // sides (block*2*dim + 2*dir + side) whose face neighbour is coarser
+  int                  nbside;
+  unsigned long long * red; // [2]: inv_dt bits (max), error flags (bit 0 eint < 0, bit 1 NaN speed)
+  long                 cnt[kStages]; // work items per stage
 };
 
 // Scratch element counts per block, for a block edge b.
@@ -61,28 +80,27 @@ struct FusedSizes
   long total() const { return qp + bg + q2 + s + sf + fs + bf + emf + ed; }
 };
 
-// The device work list of one size class (dimension, block size): per stage, the prefix
-// offsets of every slot's work items.
-//   stage 0 k_prim (interior cells, then the ghost shell padded to 64), 1 k_cell (ghost-1 cells),
-//   2 k_border_flux (faces of the sides in bside, each side padded to 64), 3 k_flux_tile (8^dim tiles), 4 k_edge (edges, each direction
-//   padded to 64), 5 k_bupdate (faces), 6 k_efield (edge points of the E grid, padded to 64)
+// Launch tunables of a batch (picked by the autotuner or set by hand).
+struct FusedTune
+{
+  int grid_cap = 1 << 20; // element kernels: at most this many workgroups (multiple of 8)
+  int tile_cap = 1 << 20; // flux tiles: at most this many workgroups (multiple of 8)
+};
+
+// The device work list of a batch: per stage, the prefix offsets of every slot's work items
+// (each slot's range padded to 512), and the slot array in two parities: parity 0 advances
+// U -> U2, parity 1 advances U2 -> U.
 template <typename T>
 struct FusedBatch
 {
-  int              dim = 3;
-  int              bs = 16;
-  int              nslots = 0;
-  FusedSlot<T> *   d_slots = nullptr;
-  static constexpr int kStages = 7;
-  long *           d_off[kStages] = {};
-  long             tot[kStages] = {};
-  int              nfix = 0;
-  EmfFix *         fix = nullptr;     // concatenated over slots, dst shifted to the batch EMF array
-  int64_t *        fix_src = nullptr;
-  T *              fixval = nullptr;
-  T *              emf_all = nullptr; // start of the batch EMF array (slot arrays are contiguous in it)
-  unsigned long long * flags = nullptr;
-  int              grid_cap = 0;      // persistent grid: at most this many workgroups per launch
+  int            dim = 3;
+  int            bsel = 0; // the block size of every slot, or 0 when they differ
+  int            nslots = 0;
+  FusedSlot<T> * d_slots[2] = { nullptr, nullptr };
+  long *         d_off[kStages] = {};
+  long           tot[kStages] = {};
+  double *       clock = nullptr; // [nslots][2]: simulated time, steps (device-side dt only)
+  FusedTune      tune;
 };
 
 // block sizes the fused kernels are instantiated for
@@ -91,8 +109,19 @@ bool fused_supports(int dim, int b);
 long fused_items_per_block(int dim, int b, int stage);
 // work items of one coarse-facing side in the border-flux stage
 long fused_border_items_per_side(int dim, int b);
+// a slot's work range, padded so that every workgroup and wavefront lies in one slot
+inline long
+fused_pad_slot(long n)
+{
+  return (n + 511) / 512 * 512;
+}
 
+// the fused stages of one step for the slot array of `parity` (dt already in the slots)
 template <typename T>
-int fused_launch(FusedBatch<T> & bt, hipStream_t st);
+int fused_launch(const FusedBatch<T> & bt, int parity, hipStream_t st);
+// dt on the device: clear the slots' reduction words, reduce over the owned cells of the state
+// of `parity`, then dt = cfl / inv_dt into both slot arrays and the clock advanced
+template <typename T>
+int fused_launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st);
 
 } // namespace mhdct

@@ -1,4 +1,9 @@
-// fused_host.hip -- scratch and work lists of the fused form for one context (one mesh slot).
+// fused_host.hip -- scratch and work list of the fused form for one context (one mesh slot).
+//
+// A context of the fused form owns its slot: the scratch arena, the staging of corrected
+// border EMFs, the list of coarse-facing block sides and the reduction words.  It advances
+// through a one-slot work list with the caller's dt (mhdct_do_time_step), and the same slot can
+// be lent to a batch (batch.hip), which advances many slots with per-slot dt on the device.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -7,24 +12,23 @@
 
 #include "ctx.hpp"
 #include "fused.hpp"
+#include "fused_host.hpp"
 
 namespace mhdct
 {
-
-void fused_free(Context & c);
 
 namespace
 {
 
 struct FusedState
 {
-  bool                      fp32 = false;
-  FusedBatch<double>        bd;
-  FusedBatch<float>         bf;
-  FusedSlot<double>         hd{};
-  FusedSlot<float>          hf{};
-  std::vector<void *>       allocs;
-  size_t                    bytes = 0;
+  bool                fp32 = false;
+  FusedSlot<double>   hd{};
+  FusedSlot<float>    hf{};
+  FusedBatch<double>  bd;
+  FusedBatch<float>   bf;
+  std::vector<void *> allocs;
+  size_t              bytes = 0;
 };
 
 template <typename T>
@@ -63,8 +67,10 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.gt.smallc = T(c.st.smallc);
   sl.gt.smallp = T(sl.gd.smallp);
   sl.gt.slope_type = T(c.st.slope_type);
+  sl.cfl = c.st.cfl;
   sl.scaling = c.st.scaling;
   sl.abort_eint = c.st.abort_when_negative_eint;
+  sl.red = c.red;
   T * p = arena;
   sl.Qp = p;  p += nq * z.qp;
   sl.Bg = p;  p += nq * z.bg;
@@ -75,11 +81,8 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.BF = p;  p += nq * z.bf;
   sl.emf = p; p += nq * z.emf;
   sl.Ed = p;  p += nq * z.ed;
-  bt.dim = dim;
-  bt.bs = b;
-  bt.nslots = 1;
-  bt.emf_all = sl.emf;
-  bt.flags = c.red + 1;
+  if (alloc(size_t(c.dm.nfix > 0 ? c.dm.nfix : 1) * sizeof(T), reinterpret_cast<void **>(&sl.fixval)))
+    return -1;
   // border fluxes are needed only where a coarser neighbour sums them
   {
     std::vector<int> sides;
@@ -103,30 +106,38 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
       return -1;
     sl.bside = d_sides;
   }
-  for (int k = 0; k < FusedBatch<T>::kStages; ++k)
+  for (int k = 0; k < kStages; ++k)
   {
-    sl.cnt[k] = k == 2 ? long(sl.nbside) * fused_border_items_per_side(dim, b)
-                       : nq * fused_items_per_block(dim, b, k);
+    if (k == kStageBorder)
+      sl.cnt[k] = long(sl.nbside) * fused_border_items_per_side(dim, b);
+    else if (k == kStageFix)
+      sl.cnt[k] = c.dm.nfix;
+    else
+      sl.cnt[k] = nq * fused_items_per_block(dim, b, k);
     if (sl.cnt[k] >= (1L << 31))
     {
       c.err = "mesh too large for one fused work list (2^31 items per stage)";
       return -4;
     }
-    bt.tot[k] = (sl.cnt[k] + 511) / 512 * 512; // whole workgroups per slot: the slot is wave-uniform
-    long off[2] = { 0, bt.tot[k] };
+  }
+  // the one-slot work list of this context
+  bt.dim = dim;
+  bt.bsel = b;
+  bt.nslots = 1;
+  for (int k = 0; k < kStages; ++k)
+  {
+    bt.tot[k] = fused_pad_slot(sl.cnt[k]);
+    const long off[2] = { 0, bt.tot[k] };
     if (alloc(sizeof(off), reinterpret_cast<void **>(&bt.d_off[k])))
       return -1;
     if (hipMemcpy(bt.d_off[k], off, sizeof(off), hipMemcpyHostToDevice) != hipSuccess)
       return -1;
   }
-  if (alloc(sizeof(FusedSlot<T>), reinterpret_cast<void **>(&bt.d_slots)))
+  if (alloc(sizeof(FusedSlot<T>), reinterpret_cast<void **>(&bt.d_slots[0])))
     return -1;
-  bt.nfix = c.dm.nfix;
-  bt.fix = c.dm.fix;
-  bt.fix_src = c.dm.fix_src;
-  if (alloc(size_t(bt.nfix > 0 ? bt.nfix : 1) * sizeof(T), reinterpret_cast<void **>(&bt.fixval)))
+  bt.d_slots[1] = bt.d_slots[0];
+  if (alloc(2 * sizeof(double), reinterpret_cast<void **>(&bt.clock)))
     return -1;
-  bt.grid_cap = 1 << 20; // one work group per workgroup below this size
   return 0;
 }
 
@@ -140,12 +151,12 @@ step(Context & c, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, doubl
   sl.B = B;
   sl.B2 = B2;
   sl.dt = dt;
-  if (hipMemcpyAsync(bt.d_slots, &sl, sizeof(sl), hipMemcpyHostToDevice, s) != hipSuccess)
+  if (hipMemcpyAsync(bt.d_slots[0], &sl, sizeof(sl), hipMemcpyHostToDevice, s) != hipSuccess)
   {
     c.err = "slot upload failed";
     return -1;
   }
-  if (fused_launch<T>(bt, s))
+  if (fused_launch<T>(bt, 0, s))
   {
     c.err = std::string("fused launch failed: ") + hipGetErrorString(hipGetLastError());
     return -1;
@@ -191,6 +202,22 @@ fused_step(Context & c, const double * U, double * U2, const double * B, double 
   if (c.dm.nowned == 0)
     return 0;
   return F->fp32 ? step<float>(c, F->bf, F->hf, U, U2, B, B2, dt, s) : step<double>(c, F->bd, F->hd, U, U2, B, B2, dt, s);
+}
+
+template <>
+const FusedSlot<double> *
+fused_slot<double>(const Context & c)
+{
+  const auto * F = static_cast<const FusedState *>(c.fused);
+  return (F && !F->fp32) ? &F->hd : nullptr;
+}
+
+template <>
+const FusedSlot<float> *
+fused_slot<float>(const Context & c)
+{
+  const auto * F = static_cast<const FusedState *>(c.fused);
+  return (F && F->fp32) ? &F->hf : nullptr;
 }
 
 } // namespace mhdct

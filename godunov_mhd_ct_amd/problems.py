@@ -165,3 +165,104 @@ def field_loop_2d(mesh: Mesh, gamma=5.0 / 3.0, scaling=1.0, xmin=(-1.0, -0.5, 0.
     p = np.ones_like(x)
     mom = [rho * v[0], rho * v[1], np.zeros_like(x)]
     return _assemble(mesh, rho, mom, p, [Bx.reshape(n, -1), By.reshape(n, -1), np.zeros((n, b * b))], gamma)
+
+
+# ---------------------------------------------------------------------------------------------
+# random turbulent-like state (the batching workload); evaluated with torch so that large
+# meshes are set up in seconds (on the GPU when there is one)
+# ---------------------------------------------------------------------------------------------
+def _modes(rng, kmax, n, slope):
+    """n random integer wavevectors with 1 <= |k| <= kmax, amplitudes ~ |k|^slope (unit RMS sum),
+    random phases."""
+    ks = []
+    while len(ks) < n:
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        kk = math.sqrt(float((k * k).sum()))
+        if 1.0 <= kk <= kmax:
+            ks.append(k)
+    ks = np.array(ks, dtype=np.float64)
+    amp = np.linalg.norm(ks, axis=1) ** slope
+    amp = amp / math.sqrt(0.5 * (amp ** 2).sum())
+    phase = rng.uniform(0.0, 2.0 * math.pi, size=n)
+    return ks, amp * rng.standard_normal(n), phase
+
+
+def _series(torch, ks, amp, phase, x, y, z):
+    out = torch.zeros_like(x)
+    for k, a, ph in zip(ks, amp, phase):
+        out += a * torch.cos(2.0 * math.pi * (k[0] * x + k[1] * y + k[2] * z) + ph)
+    return out
+
+
+def _line_integral(torch, ks, amp, phase, p0, axis, h):
+    """Exact integral of sum_k a cos(2 pi k.x + ph) along `axis` from p0 to p0 + h e_axis."""
+    x, y, z = p0
+    out = torch.zeros_like(x)
+    for k, a, ph in zip(ks, amp, phase):
+        arg = 2.0 * math.pi * (k[0] * x + k[1] * y + k[2] * z) + ph
+        ka = k[axis]
+        if ka == 0:
+            out += a * torch.cos(arg) * h
+        else:
+            w = 2.0 * math.pi * ka
+            out += a * (torch.sin(arg + w * h) - torch.sin(arg)) / w
+    return out
+
+
+def random_mhd_3d(mesh: Mesh, seed=0, gamma=5.0 / 3.0, nmodes=16, kmax=8, device=None):
+    """Smooth random MHD state on a periodic [0,1]^3 mesh: rho = exp(0.3 g1), p = max(1 + 0.1 g2,
+    0.05), v = 0.5 (g3, g4, g5), with g_i unit-variance random Fourier fields (|k| <= 4, a
+    Gaussian-smoothed white-noise spectrum), and face B = curl A from the exact edge line
+    integrals of a random Fourier vector potential (|k| <= kmax, k^-2 spectrum, amplitude 0.5),
+    so the discrete div B of every cell vanishes to roundoff."""
+    import torch
+
+    assert mesh.dim == 3
+    dev = device or ("cuda" if torch.cuda.is_available() else "cpu")
+    rng = np.random.default_rng(seed)
+    lv, co = mesh.leaves()
+    b, n = mesh.b, mesh.num_total
+    h = 1.0 / (b * (1 << lv).astype(np.float64))   # cell size per block
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)  # noqa: E731
+    hb = T(h)[:, None]
+    ob = [T(co[:, d].astype(np.float64) * b * h)[:, None] for d in range(3)]  # block origins
+
+    def pts(shape, off):
+        idx = np.indices(shape[::-1])[::-1]
+        return [ob[d] + (T(idx[d].reshape(1, -1).astype(np.float64)) + off[d]) * hb for d in range(3)]
+
+    # scalar fields at cell centres
+    xc = pts((b, b, b), (0.5, 0.5, 0.5))
+    g = []
+    for _ in range(5):
+        ks, a, ph = _modes(rng, 4, nmodes, 0.0)
+        a = a * np.exp(-0.5 * (np.linalg.norm(ks, axis=1) / 2.0) ** 2)
+        a = a / math.sqrt(0.5 * (a ** 2).sum())
+        g.append(_series(torch, ks, a, ph, *xc))
+    rho = torch.exp(0.3 * g[0])
+    p = torch.clamp(1.0 + 0.1 * g[1], min=0.05)
+    v = [0.5 * g[2], 0.5 * g[3], 0.5 * g[4]]
+    # vector potential modes
+    Am = []
+    for _ in range(3):
+        ks, a, ph = _modes(rng, kmax, nmodes, -2.0)
+        Am.append((ks, 0.5 * a, ph))
+    # edge line integrals: I_c along axis c on the edge grid of each block
+    def edge_int(c):
+        shape = [b + 1] * 3
+        shape[c] = b
+        p0 = pts(tuple(shape), (0.0, 0.0, 0.0))
+        I = _line_integral(torch, *Am[c], p0, c, hb)
+        return I.reshape(n, *shape[::-1])  # [n, z, y, x]
+    Ix, Iy, Iz = edge_int(0), edge_int(1), edge_int(2)
+    area = (hb * hb).reshape(n, 1, 1, 1)
+    # x-faces (n, b, b, b+1): Iz(j+1) - Iz(j) - Iy(k+1) + Iy(k)
+    Bx = (Iz[:, :, 1:, :] - Iz[:, :, :-1, :] - Iy[:, 1:, :, :] + Iy[:, :-1, :, :]) / area
+    # y-faces (n, b, b+1, b): Ix(k+1) - Ix(k) - Iz(i+1) + Iz(i)
+    By = (Ix[:, 1:, :, :] - Ix[:, :-1, :, :] - Iz[:, :, :, 1:] + Iz[:, :, :, :-1]) / area
+    # z-faces (n, b+1, b, b): Iy(i+1) - Iy(i) - Ix(j+1) + Ix(j)
+    Bz = (Iy[:, :, :, 1:] - Iy[:, :, :, :-1] - Ix[:, :, 1:, :] + Ix[:, :, :-1, :]) / area
+    np_ = lambda t: t.reshape(n, -1).cpu().numpy()  # noqa: E731
+    rho_n = np_(rho)
+    mom = [rho_n * np_(vc) for vc in v]
+    return _assemble(mesh, rho_n, mom, np_(p), [np_(Bx), np_(By), np_(Bz)], gamma)

@@ -80,4 +80,43 @@ def kh3d(level=4, b=16, seed=0):
     return mesh, U, B, st
 
 
+def random_amr_mesh(b, target, seed, level_min=2, level_max=6) -> Mesh:
+    """Periodic 2:1-balanced octree of about `target` leaves of b^3 cells over levels
+    level_min..level_max: leaves are refined, highest score first, where a smooth random
+    indicator (seeded) is large, until the leaf count reaches the target."""
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(-3, 4, size=(6, 3)).astype(np.float64)
+    ph = rng.uniform(0, 2 * math.pi, size=6)
+    mesh = Mesh(3, b, level_min)
+    while mesh.num_leaves < target:
+        lv, co = mesh.leaves()
+        n = mesh.num_leaves
+        c = (co[:n].astype(np.float64) + 0.5) / (1 << lv[:n]).astype(np.float64)[:, None]
+        score = sum(np.cos(2 * math.pi * (c @ k) + p) for k, p in zip(ks, ph))
+        score = score + 1e-3 * rng.random(n)
+        score[lv[:n] >= level_max] = -np.inf
+        if not np.isfinite(score).any():
+            break
+        nref = max(1, min(int((target - n) / 7 / 2), int(np.isfinite(score).sum())))
+        flags = np.zeros(n, dtype=np.int8)
+        flags[np.argsort(-score)[:nref]] = 1
+        mesh.adapt(flags)
+        mesh.balance()
+    return mesh.finalize()
+
+
+def mix3d(seed=0, counts=((8, 2048), (16, 1434), (32, 614)), level_min=2, level_max=6):
+    """The batching workload: three independent periodic meshes of 8^3, 16^3 and 32^3 blocks
+    (about 2048, 1434 and 614 leaves, 27 M cells), levels 2-6, each with a smooth random MHD
+    state (problems.random_mhd_3d), gamma 5/3, minmod, HLLD, CFL 0.8.  Returns a list of
+    (mesh, U, B, settings)."""
+    out = []
+    for n, (b, target) in enumerate(counts):
+        mesh = random_amr_mesh(b, target, seed + 101 * n, level_min, level_max)
+        U, B = problems.random_mhd_3d(mesh, seed=seed + 7 * n)
+        st = hydro_settings(gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0)
+        out.append((mesh, U, B, st))
+    return out
+
+
 WORKLOADS = {"ot2d": ot2d, "bl3d": bl3d, "kh3d": kh3d}

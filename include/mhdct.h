@@ -105,6 +105,46 @@ int mhdct_fill_outside(mhdct_ctx * ctx, double * U, double * B, void * stream);
 const char * mhdct_last_error(const mhdct_ctx * ctx);
 void         mhdct_destroy(mhdct_ctx * ctx);
 
+/* ---- batches ---------------------------------------------------------------------------
+ * No counterpart in the reference, which advances one mesh per process
+ * (SolverGodunovMHD::next_iteration_impl, SolverGodunovMHD.cpp:409-484): a batch advances
+ * many fused-form contexts (impl 11 or 12, one dimension) together, each with its own CFL dt
+ * (the ComputeDtMHD reduction and dt = cfl / inv_dt on the device, SolverGodunovMHD.cpp:377-403),
+ * through one kernel per stage over the concatenated work list of every slot and block size.
+ * A slot's results are the same alone and in any batch.
+ *
+ * Each slot's state ping-pongs between two caller-owned buffer pairs: it starts in (U, B);
+ * after n batch steps it is in (U, B) if n is even, else in (U2, B2).  A context lent to a
+ * batch must not be given a new mesh or destroyed before it is removed. */
+typedef struct mhdct_batch mhdct_batch;
+
+int mhdct_batch_create(int dim, int impl_version, mhdct_batch ** out);
+/* Add a context (mesh set, same dim and impl) with its state buffers; the slot starts at the
+ * next batch step.  Synchronises the device. */
+int mhdct_batch_add(mhdct_batch * batch, mhdct_ctx * ctx, double * U, double * U2, double * B, double * B2,
+                    int * slot_id);
+/* Remove a slot after the steps enqueued so far.  Synchronises the device. */
+int mhdct_batch_remove(mhdct_batch * batch, int slot_id);
+/* Enqueue `nsteps` steps of every slot on `stream` (outside fill, dt, stages). */
+int mhdct_batch_step(mhdct_batch * batch, int nsteps, void * stream);
+/* Simulated time, steps taken and which buffer pair holds the state (0: U/B, 1: U2/B2).
+ * Synchronises the device. */
+int mhdct_batch_slot_state(mhdct_batch * batch, int slot_id, int * current, double * time, int64_t * steps);
+/* Error flags of every slot (eint < 0 with abort_when_negative_eint: -2; NaN speed: -3, with
+ * the slot in bad_slot).  Synchronises the device. */
+int mhdct_batch_check(mhdct_batch * batch, int * bad_slot);
+/* Capture each step (per ping-pong parity) in a HIP graph and replay it. */
+int mhdct_batch_set_graph(mhdct_batch * batch, int enable);
+/* Launch tunables: at most grid_cap workgroups per element kernel and tile_cap per flux-tile
+ * kernel (multiples of 8; below the work size the kernels loop). */
+int mhdct_batch_set_tune(mhdct_batch * batch, int grid_cap, int tile_cap);
+/* Time each candidate (grid_cap, tile_cap) over `reps` steps that leave the slots' states and
+ * clocks unchanged, keep the fastest and return it. */
+int mhdct_batch_autotune(mhdct_batch * batch, int reps, void * stream, int * grid_cap, int * tile_cap);
+int          mhdct_batch_num_slots(const mhdct_batch * batch);
+const char * mhdct_batch_last_error(const mhdct_batch * batch);
+void         mhdct_batch_destroy(mhdct_batch * batch);
+
 #ifdef __cplusplus
 }
 #endif

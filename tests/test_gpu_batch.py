@@ -1,0 +1,191 @@
+"""GPU tests of batches: several meshes of different block sizes advanced together.
+
+* every slot of a batch matches the CPU oracle step by step (each slot with its own CFL dt,
+  reduced on the device) at the parity bar, relative L-inf <= 1e-12;
+* a slot's result is the same alone and in a mix (bit for bit), also when it joins a running
+  batch, with graph replay on, and after the autotuner ran;
+* the FP32-bulk batch stays within a stated L1 error of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from godunov_mhd_ct_amd import problems
+from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32, hydro_settings
+from godunov_mhd_ct_amd.mesh import Mesh, refine_where, uniform_mesh
+from oracle_ref import OracleSolver, settings_array
+
+TOL = 1e-12
+
+
+def rel_linf(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def meshes_3d():
+    """Three small 3D cases: uniform 8^3 blocks, AMR 8^3 blocks, AMR 16^3 blocks."""
+    out = []
+    m = uniform_mesh(3, 8, 1)
+    U, B = problems.blast(m, radius=0.25)
+    out.append((m, U, B, 1.0))
+    m = Mesh(3, 8, 1)
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.5 and y0 < 0.5 and z0 < 0.5, 2)
+    U, B = problems.random_mhd_3d(m, seed=5, nmodes=6, device="cpu")
+    out.append((m, U, B, 1.0))
+    m = Mesh(3, 16, 1)
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x1 > 0.5 and y0 < 0.5, 2)
+    U, B = problems.kelvin_helmholtz_3d(m, seed=2)
+    out.append((m, U, B, 2.0))
+    return out
+
+
+def settings_for(slope):
+    return hydro_settings(gamma0=5.0 / 3.0, cfl=0.8, slope_type=slope)
+
+
+def run_batch(cases, nsteps, impl=IMPL_FUSED, graph=False, join_late=None, tune=False):
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+
+    bt = BatchGodunovMHD(3, impl)
+    if graph:
+        bt.set_graph(True)
+    ids = {}
+    for n, (m, U, B, slope) in enumerate(cases):
+        if join_late is not None and n == join_late:
+            continue
+        ids[n] = bt.add(m, U, B, settings_for(slope))
+    if tune:
+        bt.autotune(reps=1)
+    if join_late is not None:
+        bt.step(2)
+        m, U, B, slope = cases[join_late]
+        ids[join_late] = bt.add(m, U, B, settings_for(slope))
+        # run the late slot nsteps, the others two more
+        bt.step(nsteps)
+    else:
+        bt.step(nsteps)
+    bt.check()
+    res = {}
+    for n, sid in ids.items():
+        cur, t, steps = bt.slot_state(sid)
+        res[n] = (bt.state(sid), t, steps)
+    return res
+
+
+@pytest.mark.gpu
+def test_batch_matches_oracle_per_slot():
+    cases = meshes_3d()
+    nsteps = 3
+    print(f"[batch] oracle parity, {len(cases)} slots, {nsteps} steps", flush=True)
+    res = run_batch(cases, nsteps)
+    for n, (m, U, B, slope) in enumerate(cases):
+        ora = OracleSolver(m, settings_array(gamma0=5.0 / 3.0, cfl=0.8, slope_type=slope), nthreads=4)
+        Uo = U.reshape(m.num_total, 5, -1).copy()
+        Bo = B.reshape(m.num_total, -1).copy()
+        t = 0.0
+        for _ in range(nsteps):
+            Uo, Bo, dt = ora.step(Uo, Bo)
+            t += dt
+        (Ug, Bg), tg, steps = res[n]
+        assert steps == nsteps
+        assert abs(tg - t) <= 1e-14 * t, f"slot {n}: time {tg!r} vs oracle {t!r}"
+        no = m.num_leaves
+        for v in range(5):
+            e = rel_linf(Ug[:no, v], Uo[:no, v])
+            assert e <= TOL, f"slot {n} var {v}: {e}"
+        e = rel_linf(Bg[:no], Bo[:no])
+        assert e <= TOL, f"slot {n} B: {e}"
+
+
+@pytest.mark.gpu
+def test_batch_alone_equals_mix():
+    cases = meshes_3d()
+    nsteps = 4
+    print(f"[batch] alone vs mix, {nsteps} steps", flush=True)
+    mix = run_batch(cases, nsteps)
+    for n in range(len(cases)):
+        alone = run_batch([cases[n]], nsteps)[0]
+        (Ua, Ba), ta, _ = alone
+        (Um, Bm), tm, _ = mix[n]
+        assert ta == tm
+        assert np.array_equal(Ua, Um) and np.array_equal(Ba, Bm), f"slot {n} differs alone vs mix"
+
+
+@pytest.mark.gpu
+def test_batch_late_join_graph_and_tune_equal():
+    cases = meshes_3d()
+    nsteps = 3
+    print("[batch] late join + graph + autotune", flush=True)
+    ref = run_batch(cases, nsteps)
+    late = run_batch(cases, nsteps, graph=True, join_late=1, tune=True)
+    (Ua, Ba), _, _ = ref[1]
+    (Ul, Bl), _, steps = late[1]
+    assert steps == nsteps
+    assert np.array_equal(Ua, Ul) and np.array_equal(Ba, Bl)
+    # the other slots ran two more steps than in `ref`: compare with a plain 5-step run
+    ref5 = run_batch(cases, nsteps + 2)
+    for n in (0, 2):
+        (Ua, Ba), _, _ = ref5[n]
+        (Ul, Bl), _, _ = late[n]
+        assert np.array_equal(Ua, Ul) and np.array_equal(Ba, Bl), f"slot {n}"
+
+
+@pytest.mark.gpu
+def test_batch_2d_mixed_block_sizes():
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+
+    print("[batch] 2D mixed block sizes vs oracle", flush=True)
+    cases = []
+    m = uniform_mesh(2, 8, 2)
+    U, B = problems.orszag_tang_2d(m)
+    cases.append((m, U, B))
+    m = Mesh(2, 16, 1)
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.6 and x1 > 0.4 and y0 < 0.6 and y1 > 0.4, 2)
+    U, B = problems.blast(m, radius=0.15, center=(0.5, 0.5, 0.0))
+    cases.append((m, U, B))
+    bt = BatchGodunovMHD(2, IMPL_FUSED)
+    ids = [bt.add(m, U, B, settings_for(1.0)) for (m, U, B) in cases]
+    nsteps = 4
+    bt.step(nsteps)
+    bt.check()
+    for (m, U, B), sid in zip(cases, ids):
+        ora = OracleSolver(m, settings_array(gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0), nthreads=4)
+        Uo = U.reshape(m.num_total, 5, -1).copy()
+        Bo = B.reshape(m.num_total, -1).copy()
+        for _ in range(nsteps):
+            Uo, Bo, _ = ora.step(Uo, Bo)
+        Ug, Bg = bt.state(sid)
+        no = m.num_leaves
+        assert rel_linf(Ug[:no], Uo[:no]) <= TOL
+        assert rel_linf(Bg[:no], Bo[:no]) <= TOL
+
+
+# FP32 bulk: slopes, predictor, HLLD and EMF in FP32; primitive recovery, accumulation and dt in
+# FP64.  Stated bound on the relative L1 error per field after 10 steps: 1e-5 for rho, E, the
+# momentum vector and the face field B (vectors normalised by the L1 norm of all their
+# components, so that a component that is zero initially is not divided by its own zero norm).
+FP32_L1 = 1e-5
+
+
+def rel_l1(a, b):
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+@pytest.mark.gpu
+def test_batch_fp32_bulk_l1_error():
+    cases = meshes_3d()
+    nsteps = 10
+    print(f"[batch] FP32 bulk vs FP64 oracle, {nsteps} steps", flush=True)
+    res = run_batch(cases, nsteps, impl=IMPL_FUSED_FP32)
+    for n, (m, U, B, slope) in enumerate(cases):
+        ora = OracleSolver(m, settings_array(gamma0=5.0 / 3.0, cfl=0.8, slope_type=slope), nthreads=4)
+        Uo = U.reshape(m.num_total, 5, -1).copy()
+        Bo = B.reshape(m.num_total, -1).copy()
+        for _ in range(nsteps):
+            Uo, Bo, _ = ora.step(Uo, Bo)
+        (Ug, Bg), _, _ = res[n]
+        no = m.num_leaves
+        errs = {"rho": rel_l1(Ug[:no, 0], Uo[:no, 0]), "E": rel_l1(Ug[:no, 1], Uo[:no, 1]),
+                "momentum": rel_l1(Ug[:no, 2:5], Uo[:no, 2:5]), "B": rel_l1(Bg[:no], Bo[:no])}
+        print(f"  slot {n}: " + ", ".join(f"{k} L1 {v:.3e}" for k, v in errs.items()))
+        bad = {k: v for k, v in errs.items() if not v <= FP32_L1}
+        assert not bad, f"slot {n}: relative L1 above {FP32_L1}: {bad}"
