@@ -1,10 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark: cell-updates/s of the MHD-CT time step on one MI355X.
+"""Benchmark: cell-updates/s of the MHD-CT time step on one MI355X.  Prints one JSON line.
 
-A step is one SolverGodunovMHD iteration on the workload (dt reduction, outside fill, the
-U2 <- U copy and do_time_step), inputs resident in HBM.  Prints one JSON line.
+  python bench.py [--workload bl3d|mix|kh3d|ot2d|srv] [--steps K] [--warmup W] [--impl fused|fp32|plain]
 
-  python bench.py [--steps K] [--warmup W] [--impl plain|fused] [--workload bl3d|kh3d|ot2d]
+A step advances every owned cell of the workload once: the outside-block fill, the CFL dt
+reduction (dt = cfl / inv_dt, on the device) and the time step, inputs resident in HBM.
+
+  bl3d (default)  3D MHD blast, 16^3 blocks, levels 3-6 (2808 leaves, 11.5 M cells), FP64
+  mix             three periodic AMR meshes of 8^3, 16^3 and 32^3 blocks (27 M cells) in one batch
+  kh3d            3D Kelvin-Helmholtz, 256^3 as 16^3 blocks
+  ot2d            2D Orszag-Tang, 128^2 as 8^2 blocks
+  srv             the serving loop: a seeded Poisson request stream, p50/p99 latency
+
+The fused forms run through the batch engine (one kernel per stage over the work list, device
+dt).  The timed steps record HIP events between the stage launches on the launch stream, so
+the dominant kernel's duration comes from the timed region itself.  The plain form (one kernel
+per stage, the GodunovImplemV0 drop-in) is timed in the same run on the same workload, and the
+CPU restatement of the step (oracle/, OpenMP) on a bounded sample of it.
 """
 from __future__ import annotations
 
@@ -20,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 ALGO_BYTES_PER_CELL = 128  # FP64 state read + written once per cell-update (5 + 3 doubles each way)
+METRIC = "cell-updates/s FP64 on 1 GPU"
 
 
 def parse():
@@ -27,35 +40,119 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--impl", default="plain", choices=["plain", "fused"])
-    ap.add_argument("--workload", default="bl3d", choices=["bl3d", "kh3d", "ot2d"])
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--impl", default="fused", choices=["fused", "fp32", "plain"])
+    ap.add_argument("--workload", default="bl3d", choices=["bl3d", "mix", "kh3d", "ot2d", "srv"])
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
+    ap.add_argument("--requests", type=int, default=40, help="srv: number of requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="srv: arrivals per second")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="before the steps, copy a buffer of this many MiB device to device once (the "
                          "known-size reference for HBM byte counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(mesh, U, B, st, steps):
-    """The CPU restatement (oracle) timed on the host cores on `steps` steps of the same
-    workload; reported beside the GPU figure, not a target."""
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cases, steps):
+    """The CPU restatement (oracle) timed on the host cores over `steps` steps of the workload's
+    meshes; reported beside the GPU figure, not a target."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_ref import OracleSolver, settings_array  # test infrastructure: the CPU baseline leg
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ora = OracleSolver(mesh, settings_array(gamma0=st.gamma0, cfl=st.cfl, slope_type=st.slope_type),
-                       nthreads=threads)
-    nt = mesh.num_total
-    Uo = U.reshape(nt, 5, -1).copy()
-    Bo = B.reshape(nt, -1).copy()
+    cells = 0
+    el = 0.0
+    for mesh, U, B, st in cases:
+        ora = OracleSolver(mesh, settings_array(gamma0=st.gamma0, cfl=st.cfl, slope_type=st.slope_type),
+                           nthreads=threads)
+        nt = mesh.num_total
+        Uo = U.reshape(nt, 5, -1).copy()
+        Bo = B.reshape(nt, -1).copy()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            Uo, Bo, _ = ora.step(Uo, Bo)
+        el += time.perf_counter() - t0
+        cells += mesh.owned_cells() * steps
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} full step(s) of every mesh of the workload ({cells // steps} cells), "
+                      f"C++ FP64 restatement with OpenMP over {threads} threads, {el:.2f} s"}
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, scaled by a copy of known size), or None."""
+    p = ROOT / "profiles" / "hbm_traffic.json"
+    try:
+        d = json.loads(p.read_text())
+        return d[workload][kernel]["bytes_per_launch"]
+    except Exception:
+        return None
+
+
+def run_plain(cases, steps, warmup):
+    import torch
+    from godunov_mhd_ct_amd.implem import IMPL_PLAIN
+    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+
+    sols = [SolverGodunovMHD(m, U, B, st, impl_version=IMPL_PLAIN) for (m, U, B, st) in cases]
+    for _ in range(warmup):
+        for s in sols:
+            s.step()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        Uo, Bo, _ = ora.step(Uo, Bo)
-    el = time.perf_counter() - t0
-    cells = mesh.owned_cells() * steps
-    return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} full step(s) of the same workload ({mesh.owned_cells()} cells), "
-                      f"C++ FP64 restatement with OpenMP, {el:.2f} s"}
+        for s in sols:
+            s.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = sum(m.owned_cells() for (m, _, _, _) in cases)
+    del sols
+    torch.cuda.empty_cache()
+    return {"value": cells * steps / wall, "ms_per_step": wall * 1e3 / steps}
+
+
+def cases_for(name):
+    from godunov_mhd_ct_amd import workloads
+
+    if name == "mix":
+        return workloads.mix3d(seed=0)
+    return [workloads.WORKLOADS[name]()]
+
+
+def bench_batch(args, cases, impl):
+    import torch
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+
+    dim = cases[0][0].dim
+    bt = BatchGodunovMHD(dim, impl)
+    for (m, U, B, st) in cases:
+        bt.add(m, U, B, st)
+    cells = bt.cells()
+    tune = None
+    if not args.no_tune:
+        tune = bt.autotune(reps=2)
+        log(f"[bench] autotuned grid_cap={tune[0]} tile_cap={tune[1]}")
+    bt.step(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stage_ms = bt.step_timed(args.steps)       # synchronises at its end
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    bt.check()
+    # the same steps replayed from HIP graphs
+    bt.set_graph(True)
+    bt.step(1)
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    bt.step(args.steps)
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - g0
+    bt.check()
+    return cells, wall, stage_ms, gwall, tune
 
 
 def main():
@@ -65,69 +162,87 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
-    from godunov_mhd_ct_amd import workloads
-    from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_PLAIN
-    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32
 
-    mesh, U, B, st = workloads.WORKLOADS[args.workload]()
-    impl = IMPL_PLAIN if args.impl == "plain" else IMPL_FUSED
-    sol = SolverGodunovMHD(mesh, U, B, st, impl_version=impl)
-    cells = mesh.owned_cells()
-    print(f"[bench] workload={args.workload} impl={args.impl} leaves={mesh.num_leaves} cells={cells}",
-          file=sys.stderr, flush=True)
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 17, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(f"[bench] copied {args.copy_mb} MiB device to device", file=sys.stderr, flush=True)
+        log(f"[bench] copied {args.copy_mb} MiB device to device")
         del src, dst
-    for _ in range(args.warmup):
-        sol.step()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    step_ms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ev0.record()
-        sol.step()
-        ev1.record()
-        ev1.synchronize()
-        step_ms.append(ev0.elapsed_time(ev1))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = wall * 1e3 / args.steps
-    value = cells * args.steps / wall
-    step_ms.sort()
-    p50 = step_ms[len(step_ms) // 2]
-    p99 = step_ms[min(len(step_ms) - 1, int(0.99 * len(step_ms)))]
-    achieved = cells * ALGO_BYTES_PER_CELL / (p50 * 1e-3)
-    out = {
-        "metric": "cell-updates/s FP64 on 1 GPU",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "impl": args.impl, "leaves": mesh.num_leaves, "cells": cells,
-                   "block": mesh.b, "dim": mesh.dim},
-        "step_ms_p50": p50,
-        "step_ms_p99": p99,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "scope": "whole step (128 B algorithmic per cell-update / median step time)"},
-    }
+
+    if args.workload == "srv":
+        return bench_serving(args)
+
+    cases = cases_for(args.workload)
+    cells = sum(m.owned_cells() for (m, _, _, _) in cases)
+    log(f"[bench] workload={args.workload} impl={args.impl} meshes={len(cases)} "
+        f"leaves={sum(m.num_leaves for (m, _, _, _) in cases)} cells={cells}")
+    out = {"metric": METRIC, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if args.impl != "fp32" else "f32 bulk / f64 accumulation", "data": "synthetic"}
+    config = {"workload": args.workload, "impl": args.impl, "meshes": len(cases),
+              "leaves": sum(m.num_leaves for (m, _, _, _) in cases), "cells": cells,
+              "blocks": sorted({m.b for (m, _, _, _) in cases}), "dim": cases[0][0].dim}
+    if args.impl == "plain":
+        p = run_plain(cases, args.steps, args.warmup)
+        out.update({"value": p["value"], "ms_per_step": p["ms_per_step"]})
+        out["roofline"] = {"bound": "hbm", "achieved": p["value"] * ALGO_BYTES_PER_CELL / 1e9,
+                           "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": p["value"] * ALGO_BYTES_PER_CELL / HBM_PEAK, "traffic": None,
+                           "kernel": "whole step (plain form: no per-kernel events)"}
+    else:
+        impl = IMPL_FUSED if args.impl == "fused" else IMPL_FUSED_FP32
+        ncells, wall, stage_ms, gwall, tune = bench_batch(args, cases, impl)
+        value = ncells * args.steps / wall
+        kern = max((k for k in stage_ms if k.startswith("k_")), key=lambda k: stage_ms[k])
+        kms = stage_ms[kern]
+        achieved = ALGO_BYTES_PER_CELL * ncells / (kms * 1e-3)
+        out.update({"value": value, "ms_per_step": wall * 1e3 / args.steps,
+                    "ms_per_step_graph": gwall * 1e3 / args.steps})
+        out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK, "traffic": load_traffic(kern, args.workload),
+                           "kernel": kern, "kernel_ms": kms,
+                           "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B x {ncells} cell-updates per launch"}
+        out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
+        if tune:
+            config["tune"] = {"grid_cap": tune[0], "tile_cap": tune[1]}
+        if not args.no_plain:
+            p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
+            out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"],
+                            "speedup": value / p["value"]}
+    out["config"] = config
     if args.cpu_steps > 0:
-        out["cpu_baseline"] = cpu_baseline(mesh, U, B, st, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
+    print(json.dumps(out))
+
+
+def bench_serving(args):
+    import torch
+    from godunov_mhd_ct_amd.serving import Server, request_stream
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED
+
+    reqs = request_stream(args.requests, rate=args.rate, seed=0, nsteps=200)
+    free, total = torch.cuda.mem_get_info()
+    budget = int(0.5 * free)
+    log(f"[bench] serving {len(reqs)} requests at {args.rate}/s, HBM budget {budget / 2**30:.1f} GiB")
+    # warm-up: compile-free, but the first batch steps page code in
+    Server(hbm_budget=budget).run(request_stream(2, rate=1000.0, seed=99, nsteps=5, sizes2d=(32,), sizes3d=(32,)))
+    srv = Server(hbm_budget=budget, impl=IMPL_FUSED, max_tick=20)
+    res = srv.run(reqs)
+    out = {"metric": METRIC, "value": res["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": 200, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "latency_ms_p50": res["latency_ms_p50"], "latency_ms_p99": res["latency_ms_p99"],
+           "requests_per_s": res["requests"] / res["wall_s"],
+           "config": {"workload": "srv", "requests": res["requests"], "rate_per_s": args.rate,
+                      "steps_per_request": 200, "hbm_budget_gib": budget / 2 ** 30},
+           "roofline": {"bound": "hbm", "achieved": res["cell_updates_per_s"] * ALGO_BYTES_PER_CELL / 1e9,
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": res["cell_updates_per_s"] * ALGO_BYTES_PER_CELL / HBM_PEAK, "traffic": None,
+                        "kernel": "whole serving run"}}
     print(json.dumps(out))
 
 

@@ -60,6 +60,8 @@ SIGNATURES = {
                                   C.POINTER(C.c_int)]),
     "mhdct_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
     "mhdct_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "mhdct_batch_step_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_double)]),
+    "mhdct_stage_name": (C.c_char_p, [C.c_int]),
     "mhdct_batch_slot_state": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                          C.POINTER(C.c_int64)]),
     "mhdct_batch_check": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -95,6 +95,15 @@ class BatchGodunovMHD:
     def step(self, nsteps: int = 1, stream=None):
         self._check(lib().mhdct_batch_step(self._h, int(nsteps), C.c_void_p(_stream(stream))), "batch_step")
 
+    def step_timed(self, nsteps: int = 1, stream=None) -> dict:
+        """Steps with HIP events between the stage launches; returns the average milliseconds
+        per step of each stage (synchronises)."""
+        n = 10
+        ms = (C.c_double * n)()
+        self._check(lib().mhdct_batch_step_timed(self._h, int(nsteps), C.c_void_p(_stream(stream)), ms),
+                    "batch_step_timed")
+        return {lib().mhdct_stage_name(k).decode(): ms[k] for k in range(n)}
+
     def slot_state(self, slot: int):
         cur, t, n = C.c_int(), C.c_double(), C.c_int64()
         self._check(lib().mhdct_batch_slot_state(self._h, int(slot), C.byref(cur), C.byref(t), C.byref(n)),

@@ -276,6 +276,68 @@ step(mhdct_batch * b, FusedBatch<T> & bt, int nsteps, hipStream_t st)
   return 0;
 }
 
+// Steps with HIP events recorded between the stage launches on `st`: the average milliseconds
+// per step of each timed stage go to ms[MHDCT_NUM_STAGES] (outside fill, dt, then the fused
+// stages in launch order).
+template <typename T>
+int
+step_timed(mhdct_batch * b, FusedBatch<T> & bt, int nsteps, hipStream_t st, double * ms)
+{
+  if (b->dirty && rebuild<T>(b, bt, st))
+    return -1;
+  constexpr int ns = MHDCT_NUM_STAGES;
+  for (int k = 0; k < ns; ++k)
+    ms[k] = 0.0;
+  if (bt.nslots == 0 || nsteps < 1)
+    return 0;
+  std::vector<hipEvent_t> ev(size_t(nsteps) * (ns + 1));
+  for (auto & e : ev)
+    if (hipEventCreate(&e) != hipSuccess)
+    {
+      b->err = "event creation failed";
+      return -1;
+    }
+  int rc = 0;
+  for (int n = 0; n < nsteps && rc == 0; ++n)
+  {
+    const int    p = int(b->gstep & 1);
+    hipEvent_t * E = &ev[size_t(n) * (ns + 1)];
+    (void)hipEventRecord(E[0], st);
+    for (auto & e : b->ent)
+      if (rc == 0 && e.ctx->c.dm.noutside > 0)
+      {
+        const int c = cur_index(e, p);
+        rc = mhdct::launch_fill_outside(e.ctx->c, e.X[c], e.XB[c], st);
+      }
+    (void)hipEventRecord(E[1], st);
+    rc = rc ? rc : mhdct::fused_launch_dt<T>(bt, p, st);
+    (void)hipEventRecord(E[2], st);
+    for (int k = 0; k < 8 && rc == 0; ++k)
+    {
+      rc = mhdct::fused_launch_stage<T>(bt, p, mhdct::fused_stage_order[k], st);
+      (void)hipEventRecord(E[3 + k], st);
+    }
+    ++b->gstep;
+  }
+  if (rc == 0 && hipStreamSynchronize(st) == hipSuccess)
+  {
+    for (int n = 0; n < nsteps; ++n)
+      for (int k = 0; k < ns; ++k)
+      {
+        float f = 0.0f;
+        (void)hipEventElapsedTime(&f, ev[size_t(n) * (ns + 1) + k], ev[size_t(n) * (ns + 1) + k + 1]);
+        ms[k] += double(f) / nsteps;
+      }
+  }
+  else if (rc == 0)
+    rc = -1;
+  for (auto & e : ev)
+    (void)hipEventDestroy(e);
+  if (rc)
+    b->err = "timed step failed";
+  return rc;
+}
+
 // time `reps` steps at the current parity without advancing it (the current state is only
 // read; the other buffer and the slot dt are overwritten, the clocks are not touched)
 template <typename T>
@@ -461,6 +523,26 @@ extern "C"
     b->err.clear();
     hipStream_t st = static_cast<hipStream_t>(stream);
     return fp32(b) ? step<float>(b, b->bf, nsteps, st) : step<double>(b, b->bd, nsteps, st);
+  }
+
+  int
+  mhdct_batch_step_timed(mhdct_batch * b, int nsteps, void * stream, double * stage_ms)
+  {
+    if (!b || !stage_ms)
+      return -1;
+    b->err.clear();
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return fp32(b) ? step_timed<float>(b, b->bf, nsteps, st, stage_ms)
+                   : step_timed<double>(b, b->bd, nsteps, st, stage_ms);
+  }
+
+  const char *
+  mhdct_stage_name(int stage)
+  {
+    static const char * names[MHDCT_NUM_STAGES] = { "fill_outside", "dt",  "k_prim", "k_efield", "k_cell",
+                                                    "k_border_flux", "k_flux_tile", "k_edge", "k_fix+k_scatter",
+                                                    "k_bupdate" };
+    return (stage >= 0 && stage < MHDCT_NUM_STAGES) ? names[stage] : "";
   }
 
   int

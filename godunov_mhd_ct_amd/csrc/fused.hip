@@ -1261,36 +1261,44 @@ grid_for(long n, int threads, int cap)
   return unsigned(g);
 }
 
+// Launch the fused stages of one step (which < 0), or only stage `which` of fused_stage_order.
 template <typename T, int DIM, int BSEL>
 int
-launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st)
+launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
 {
   const int            n = bt.nslots;
   const FusedSlot<T> * sl = bt.d_slots[parity];
   const int            cap = bt.tune.grid_cap / kXcd * kXcd;
   const int            tcap = bt.tune.tile_cap / kXcd * kXcd;
   auto                 grid = [&](int stage) { return grid_for(bt.tot[stage], 256, cap); };
-  hipLaunchKernelGGL((k_prim<T, DIM, BSEL>), grid(kStagePrim), 256, 0, st, sl, n, bt.d_off[kStagePrim],
-                     bt.tot[kStagePrim]);
-  hipLaunchKernelGGL((k_efield<T, DIM, BSEL>), grid(kStageEfield), 256, 0, st, sl, n, bt.d_off[kStageEfield],
-                     bt.tot[kStageEfield]);
-  hipLaunchKernelGGL((k_cell<T, DIM, BSEL>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
-                     bt.tot[kStageCell]);
-  if (bt.tot[kStageBorder] > 0)
-    hipLaunchKernelGGL((k_border_flux<T, DIM, BSEL>), grid(kStageBorder), 256, 0, st, sl, n, bt.d_off[kStageBorder],
-                       bt.tot[kStageBorder]);
-  hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL>), grid_for(bt.tot[kStageTile], 1, tcap), tile_threads<DIM>(), 0, st, sl,
-                     n, bt.d_off[kStageTile], bt.tot[kStageTile]);
-  hipLaunchKernelGGL((k_edge<T, DIM, BSEL>), grid(kStageEdge), 256, 0, st, sl, n, bt.d_off[kStageEdge],
-                     bt.tot[kStageEdge]);
-  if (bt.tot[kStageFix] > 0)
+  auto                 on = [&](int stage) { return which < 0 || which == stage; };
+  if (on(kStagePrim))
+    hipLaunchKernelGGL((k_prim<T, DIM, BSEL>), grid(kStagePrim), 256, 0, st, sl, n, bt.d_off[kStagePrim],
+                       bt.tot[kStagePrim]);
+  if (on(kStageEfield))
+    hipLaunchKernelGGL((k_efield<T, DIM, BSEL>), grid(kStageEfield), 256, 0, st, sl, n, bt.d_off[kStageEfield],
+                       bt.tot[kStageEfield]);
+  if (on(kStageCell))
+    hipLaunchKernelGGL((k_cell<T, DIM, BSEL>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
+                       bt.tot[kStageCell]);
+  if (on(kStageBorder) && bt.tot[kStageBorder] > 0)
+    hipLaunchKernelGGL((k_border_flux<T, DIM, BSEL>), grid(kStageBorder), 256, 0, st, sl, n,
+                       bt.d_off[kStageBorder], bt.tot[kStageBorder]);
+  if (on(kStageTile))
+    hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL>), grid_for(bt.tot[kStageTile], 1, tcap), tile_threads<DIM>(), 0,
+                       st, sl, n, bt.d_off[kStageTile], bt.tot[kStageTile]);
+  if (on(kStageEdge))
+    hipLaunchKernelGGL((k_edge<T, DIM, BSEL>), grid(kStageEdge), 256, 0, st, sl, n, bt.d_off[kStageEdge],
+                       bt.tot[kStageEdge]);
+  if (on(kStageFix) && bt.tot[kStageFix] > 0)
   {
     hipLaunchKernelGGL((k_fix<T>), grid(kStageFix), 256, 0, st, sl, n, bt.d_off[kStageFix], bt.tot[kStageFix]);
     hipLaunchKernelGGL((k_scatter<T>), grid(kStageFix), 256, 0, st, sl, n, bt.d_off[kStageFix],
                        bt.tot[kStageFix]);
   }
-  hipLaunchKernelGGL((k_bupdate<T, DIM, BSEL>), grid(kStageBupdate), 256, 0, st, sl, n, bt.d_off[kStageBupdate],
-                     bt.tot[kStageBupdate]);
+  if (on(kStageBupdate))
+    hipLaunchKernelGGL((k_bupdate<T, DIM, BSEL>), grid(kStageBupdate), 256, 0, st, sl, n, bt.d_off[kStageBupdate],
+                       bt.tot[kStageBupdate]);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1349,22 +1357,29 @@ fused_border_items_per_side(int dim, int b)
 
 template <typename T>
 int
-fused_launch(const FusedBatch<T> & bt, int parity, hipStream_t st)
+fused_launch_stage(const FusedBatch<T> & bt, int parity, int which, hipStream_t st)
 {
   if (bt.nslots == 0)
     return 0;
   switch (bt.dim * 100 + bt.bsel)
   {
-  case 200: return launch_stages<T, 2, 0>(bt, parity, st);
-  case 208: return launch_stages<T, 2, 8>(bt, parity, st);
-  case 216: return launch_stages<T, 2, 16>(bt, parity, st);
-  case 232: return launch_stages<T, 2, 32>(bt, parity, st);
-  case 300: return launch_stages<T, 3, 0>(bt, parity, st);
-  case 308: return launch_stages<T, 3, 8>(bt, parity, st);
-  case 316: return launch_stages<T, 3, 16>(bt, parity, st);
-  case 332: return launch_stages<T, 3, 32>(bt, parity, st);
+  case 200: return launch_stages<T, 2, 0>(bt, parity, st, which);
+  case 208: return launch_stages<T, 2, 8>(bt, parity, st, which);
+  case 216: return launch_stages<T, 2, 16>(bt, parity, st, which);
+  case 232: return launch_stages<T, 2, 32>(bt, parity, st, which);
+  case 300: return launch_stages<T, 3, 0>(bt, parity, st, which);
+  case 308: return launch_stages<T, 3, 8>(bt, parity, st, which);
+  case 316: return launch_stages<T, 3, 16>(bt, parity, st, which);
+  case 332: return launch_stages<T, 3, 32>(bt, parity, st, which);
   default: return -1;
   }
+}
+
+template <typename T>
+int
+fused_launch(const FusedBatch<T> & bt, int parity, hipStream_t st)
+{
+  return fused_launch_stage<T>(bt, parity, -1, st);
 }
 
 template <typename T>
@@ -1388,6 +1403,8 @@ fused_launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
 }
 
 template int fused_launch<double>(const FusedBatch<double> &, int, hipStream_t);
+template int fused_launch_stage<double>(const FusedBatch<double> &, int, int, hipStream_t);
+template int fused_launch_stage<float>(const FusedBatch<float> &, int, int, hipStream_t);
 template int fused_launch<float>(const FusedBatch<float> &, int, hipStream_t);
 template int fused_launch_dt<double>(const FusedBatch<double> &, int, hipStream_t);
 template int fused_launch_dt<float>(const FusedBatch<float> &, int, hipStream_t);

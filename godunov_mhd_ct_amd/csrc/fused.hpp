@@ -119,6 +119,13 @@ fused_pad_slot(long n)
 // the fused stages of one step for the slot array of `parity` (dt already in the slots)
 template <typename T>
 int fused_launch(const FusedBatch<T> & bt, int parity, hipStream_t st);
+// only stage `which` (kStagePrim ... kStageBupdate; kStageFix launches k_fix and k_scatter)
+template <typename T>
+int fused_launch_stage(const FusedBatch<T> & bt, int parity, int which, hipStream_t st);
+// the launch order of the stages of one step
+constexpr int fused_stage_order[8] = { kStagePrim, kStageEfield, kStageCell, kStageBorder,
+                                       kStageTile, kStageEdge, kStageFix,
+                                       kStageBupdate };
 // dt on the device: clear the slots' reduction words, reduce over the owned cells of the state
 // of `parity`, then dt = cfl / inv_dt into both slot arrays and the clock advanced
 template <typename T>

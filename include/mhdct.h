@@ -127,6 +127,12 @@ int mhdct_batch_add(mhdct_batch * batch, mhdct_ctx * ctx, double * U, double * U
 int mhdct_batch_remove(mhdct_batch * batch, int slot_id);
 /* Enqueue `nsteps` steps of every slot on `stream` (outside fill, dt, stages). */
 int mhdct_batch_step(mhdct_batch * batch, int nsteps, void * stream);
+/* The same as mhdct_batch_step without graphs, with HIP events recorded on `stream` between the
+ * stage launches: stage_ms[MHDCT_NUM_STAGES] receives the average milliseconds per step of each
+ * stage (names from mhdct_stage_name).  Synchronises `stream`. */
+#define MHDCT_NUM_STAGES 10
+int          mhdct_batch_step_timed(mhdct_batch * batch, int nsteps, void * stream, double * stage_ms);
+const char * mhdct_stage_name(int stage);
 /* Simulated time, steps taken and which buffer pair holds the state (0: U/B, 1: U2/B2).
  * Synchronises the device. */
 int mhdct_batch_slot_state(mhdct_batch * batch, int slot_id, int * current, double * time, int64_t * steps);
