@@ -1,0 +1,115 @@
+"""CPU tests of the C ABI library and of the host block octree (no GPU needed).
+
+* libmhdct.so loads and exports every function include/mhdct.h declares, and the Python
+  binding declares a signature for each of them;
+* the octree: leaves tile the domain exactly, refinement keeps 2:1 balance across faces,
+  edges and corners, outside blocks appear only on non-periodic borders, and the error paths
+  return codes with a message instead of aborting.
+"""
+import ctypes as C
+import itertools
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from godunov_mhd_ct_amd import MhdctError, lib
+from godunov_mhd_ct_amd._lib import SIGNATURES
+from godunov_mhd_ct_amd.mesh import Mesh, refine_where, uniform_mesh
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "mhdct.h"
+
+
+def declared():
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mhdct_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_abi():
+    names = declared()
+    for must in ["mhdct_create", "mhdct_set_mesh", "mhdct_do_time_step", "mhdct_compute_dt", "mhdct_fill_outside",
+                 "mhdct_mem_bytes", "mhdct_last_error", "mhdct_destroy", "mhdct_batch_step"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib()
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert sorted(SIGNATURES) == declared()
+
+
+def leaf_boxes(m):
+    lv, co = m.leaves()
+    n = m.num_leaves
+    h = 1.0 / (1 << lv[:n]).astype(np.float64)
+    lo = co[:n, :m.dim] * h[:, None]
+    return lo, lo + h[:, None], lv[:n]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_leaves_tile_the_domain(dim):
+    m = Mesh(dim, 4, 1)
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.3 and y0 < 0.4, 3)
+    lo, hi, lv = leaf_boxes(m)
+    vol = np.prod(hi - lo, axis=1)
+    assert abs(vol.sum() - 1.0) < 1e-12
+    # no two leaves overlap: sample points at the centres of the finest cells
+    f = 1 << int(lv.max())
+    pts = (np.stack(np.meshgrid(*[np.arange(f)] * dim, indexing="ij"), -1).reshape(-1, dim) + 0.5) / f
+    inside = np.zeros(len(pts), dtype=int)
+    for a, b in zip(lo, hi):
+        inside += np.all((pts >= a) & (pts < b), axis=1)
+    assert np.all(inside == 1)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_two_to_one_balance(dim):
+    m = Mesh(dim, 4, 1)
+    # refine towards a point repeatedly: without balancing this would break 2:1
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 <= 0.01 and y0 <= 0.01 and (dim == 2 or z0 <= 0.01), 5)
+    lo, hi, lv = leaf_boxes(m)
+    n = len(lv)
+    eps = 1e-12
+    for a, b in itertools.combinations(range(n), 2):
+        # leaves touching across a face, an edge or a corner (periodic images included)
+        touch = True
+        for d in range(dim):
+            gap = min(abs(lo[a, d] - hi[b, d]), abs(lo[b, d] - hi[a, d]),
+                      abs(lo[a, d] - hi[b, d] + 1.0), abs(lo[b, d] - hi[a, d] + 1.0))
+            overlap = lo[a, d] < hi[b, d] - eps and lo[b, d] < hi[a, d] - eps
+            if not (overlap or gap < eps):
+                touch = False
+                break
+        if touch:
+            assert abs(int(lv[a]) - int(lv[b])) <= 1, (a, b, lv[a], lv[b])
+
+
+def test_outside_blocks_only_on_bounded_borders():
+    m = uniform_mesh(2, 4, 2)
+    assert m.num_outside == 0
+    m = Mesh(2, 4, 2, periodic=(0, 1, 1))
+    m.set_bc(["ZERO_GRADIENT", "WALL", "ZERO_GRADIENT", "ZERO_GRADIENT", "ZERO_GRADIENT", "ZERO_GRADIENT"])
+    m.finalize()
+    # one column of outside blocks on each x border, 4 blocks high
+    assert m.num_outside == 2 * 4
+    src, dirs = m.outside()
+    assert set(np.abs(dirs[:, 0]).tolist()) == {1} and not dirs[:, 1].any()
+
+
+def test_error_paths_return_codes():
+    h = C.c_void_p()
+    nb = (C.c_int * 3)(1, 1, 1)
+    pe = (C.c_int * 3)(1, 1, 1)
+    assert lib().mhdct_mesh_create(3, 5, nb, pe, 1, C.byref(h)) != 0  # odd block size
+    with pytest.raises(MhdctError):
+        Mesh(3, 5, 1)
+    m = Mesh(2, 4, 1)
+    with pytest.raises(MhdctError):
+        m.adapt(np.zeros(m.num_leaves + 3, dtype=np.int8))
+    assert lib().mhdct_last_error(None) is not None
