@@ -1011,7 +1011,7 @@ edge_item(const FusedSlot<T> & S, long r0)
 }
 
 template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -1109,41 +1109,55 @@ template <typename T, int DIM, int BSEL>
 __global__ void __launch_bounds__(256)
 k_inv_dt(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
+  // a bounded grid: each workgroup folds many groups of 256 cells into a running maximum and
+  // flushes it (block reduction + one atomic) whenever its next group lies in another slot
   __shared__ double wmax[4];
-  for (long g0 = xcd_group() * blockDim.x; g0 < total; g0 += long(gridDim.x) * blockDim.x)
-  {
-    const long           t = g0 + threadIdx.x;
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, g0));
-    const FusedSlot<T> & S = slots[s];
-    const long           r0 = t - off[s];
-    double               cand = 0.0;
-    bool                 nan = false;
-    if (t < total && r0 < S.cnt[kStageInvDt])
-    {
-      MHDCT_BS_DISPATCH(S.m.b, (cand = inv_dt_item<T, DIM, BS>(S, r0)));
-      if (cand != cand)
-      {
-        nan = true;
-        cand = 0.0;
-      }
-    }
+  double cand = 0.0;
+  bool   nan = false;
+  int    cur = -1;
+  auto   flush = [&](int s) {
     for (int o = 32; o > 0; o >>= 1)
       cand = fmax(cand, __shfl_xor(cand, o, 64));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0)
       wmax[wave] = cand;
     if (__any(nan) && lane == 0)
-      atomicOr(S.red + 1, 2ull);
+      atomicOr(slots[s].red + 1, 2ull);
     __syncthreads();
     if (threadIdx.x == 0)
     {
       double mx = wmax[0];
       for (int w = 1; w < 4; ++w)
         mx = fmax(mx, wmax[w]);
-      atomicMax(S.red, (unsigned long long)__double_as_longlong(mx));
+      atomicMax(slots[s].red, (unsigned long long)__double_as_longlong(mx));
     }
     __syncthreads();
+    cand = 0.0;
+    nan = false;
+  };
+  for (long g0 = xcd_group() * blockDim.x; g0 < total; g0 += long(gridDim.x) * blockDim.x)
+  {
+    const int s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, g0));
+    if (s != cur)
+    {
+      if (cur >= 0)
+        flush(cur);
+      cur = s;
+    }
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = g0 + threadIdx.x - off[s];
+    if (r0 < S.cnt[kStageInvDt])
+    {
+      double c = 0.0;
+      MHDCT_BS_DISPATCH(S.m.b, (c = inv_dt_item<T, DIM, BS>(S, r0)));
+      if (c != c)
+        nan = true;
+      else
+        cand = fmax(cand, c);
+    }
   }
+  if (cur >= 0)
+    flush(cur);
 }
 
 // Per slot: clear the reduction words before k_inv_dt (a kernel, so that the step can be
@@ -1308,8 +1322,8 @@ launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
 {
   const int n = bt.nslots;
   hipLaunchKernelGGL((k_dt_clear<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[parity], n);
-  hipLaunchKernelGGL((k_inv_dt<T, DIM, BSEL>), grid_for(bt.tot[kStageInvDt], 256, bt.tune.grid_cap / kXcd * kXcd), 256, 0,
-                     st, bt.d_slots[parity], n, bt.d_off[kStageInvDt], bt.tot[kStageInvDt]);
+  hipLaunchKernelGGL((k_inv_dt<T, DIM, BSEL>), grid_for(bt.tot[kStageInvDt], 256, 2048), 256, 0, st,
+                     bt.d_slots[parity], n, bt.d_off[kStageInvDt], bt.tot[kStageInvDt]);
   hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], n, bt.clock);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
