@@ -66,6 +66,7 @@ SIGNATURES = {
                                          C.POINTER(C.c_int64)]),
     "mhdct_batch_check": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mhdct_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhdct_batch_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "mhdct_batch_set_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mhdct_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mhdct_batch_num_slots": (C.c_int, [C.c_void_p]),

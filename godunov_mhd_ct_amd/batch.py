@@ -131,6 +131,9 @@ class BatchGodunovMHD:
     def set_graph(self, enable: bool = True):
         self._check(lib().mhdct_batch_set_graph(self._h, int(bool(enable))), "batch_set_graph")
 
+    def set_overlap(self, enable: bool = True):
+        self._check(lib().mhdct_batch_set_overlap(self._h, int(bool(enable))), "batch_set_overlap")
+
     def set_tune(self, grid_cap: int, tile_cap: int):
         self._check(lib().mhdct_batch_set_tune(self._h, int(grid_cap), int(tile_cap)), "batch_set_tune")
 

@@ -50,6 +50,9 @@ struct mhdct_batch
   bool                  use_graph = false;
   hipGraphExec_t        gx[2] = { nullptr, nullptr };
   hipStream_t           cap = nullptr; // capture stream (graphs cannot be captured on the null stream)
+  bool                  fork = false;  // run k_edge on a side stream beside the flux stages
+  hipStream_t           side = nullptr;
+  hipEvent_t            ev_fork = nullptr, ev_join = nullptr;
   std::string           err;
 };
 
@@ -210,7 +213,26 @@ enqueue(mhdct_batch * b, FusedBatch<T> & bt, int p, hipStream_t st)
         return -1;
       }
     }
-  if (mhdct::fused_launch_dt<T>(bt, p, st) || mhdct::fused_launch<T>(bt, p, st))
+  int rc = mhdct::fused_launch_dt<T>(bt, p, st);
+  if (rc == 0 && b->fork)
+  {
+    // k_edge needs only k_cell's records: it runs on the side stream beside k_border_flux and
+    // k_flux_tile, and the EMF correction waits for both branches
+    using namespace mhdct;
+    for (int k : { kStagePrim, kStageEfield, kStageCell })
+      rc = rc ? rc : fused_launch_stage<T>(bt, p, k, st);
+    rc = rc ? rc : (hipEventRecord(b->ev_fork, st) != hipSuccess || hipStreamWaitEvent(b->side, b->ev_fork, 0) != hipSuccess);
+    rc = rc ? rc : fused_launch_stage<T>(bt, p, kStageEdge, b->side);
+    rc = rc ? rc : (hipEventRecord(b->ev_join, b->side) != hipSuccess);
+    for (int k : { kStageBorder, kStageTile })
+      rc = rc ? rc : fused_launch_stage<T>(bt, p, k, st);
+    rc = rc ? rc : (hipStreamWaitEvent(st, b->ev_join, 0) != hipSuccess);
+    for (int k : { kStageFix, kStageBupdate })
+      rc = rc ? rc : fused_launch_stage<T>(bt, p, k, st);
+  }
+  else if (rc == 0)
+    rc = mhdct::fused_launch<T>(bt, p, st);
+  if (rc)
   {
     b->err = std::string("batch launch failed: ") + hipGetErrorString(hipGetLastError());
     return -1;
@@ -290,42 +312,63 @@ step_timed(mhdct_batch * b, FusedBatch<T> & bt, int nsteps, hipStream_t st, doub
     ms[k] = 0.0;
   if (bt.nslots == 0 || nsteps < 1)
     return 0;
-  std::vector<hipEvent_t> ev(size_t(nsteps) * (ns + 1));
+  // per step: 2 events around each of the 10 stages, on the stream the stage runs on
+  std::vector<hipEvent_t> ev(size_t(nsteps) * 2 * ns);
   for (auto & e : ev)
     if (hipEventCreate(&e) != hipSuccess)
     {
       b->err = "event creation failed";
       return -1;
     }
+  using namespace mhdct;
   int rc = 0;
   for (int n = 0; n < nsteps && rc == 0; ++n)
   {
     const int    p = int(b->gstep & 1);
-    hipEvent_t * E = &ev[size_t(n) * (ns + 1)];
-    (void)hipEventRecord(E[0], st);
-    for (auto & e : b->ent)
-      if (rc == 0 && e.ctx->c.dm.noutside > 0)
-      {
-        const int c = cur_index(e, p);
-        rc = mhdct::launch_fill_outside(e.ctx->c, e.X[c], e.XB[c], st);
-      }
-    (void)hipEventRecord(E[1], st);
-    rc = rc ? rc : mhdct::fused_launch_dt<T>(bt, p, st);
-    (void)hipEventRecord(E[2], st);
-    for (int k = 0; k < 8 && rc == 0; ++k)
+    hipEvent_t * E = &ev[size_t(n) * 2 * ns];
+    auto         timed = [&](int slot, hipStream_t s, auto && launch) {
+      (void)hipEventRecord(E[2 * slot], s);
+      rc = rc ? rc : launch(s);
+      (void)hipEventRecord(E[2 * slot + 1], s);
+    };
+    timed(0, st, [&](hipStream_t s) {
+      int r = 0;
+      for (auto & e : b->ent)
+        if (r == 0 && e.ctx->c.dm.noutside > 0)
+        {
+          const int c = cur_index(e, p);
+          r = launch_fill_outside(e.ctx->c, e.X[c], e.XB[c], s);
+        }
+      return r;
+    });
+    timed(1, st, [&](hipStream_t s) { return fused_launch_dt<T>(bt, p, s); });
+    timed(2, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStagePrim, s); });
+    timed(3, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageEfield, s); });
+    timed(4, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageCell, s); });
+    hipStream_t es = st;
+    if (b->fork)
     {
-      rc = mhdct::fused_launch_stage<T>(bt, p, mhdct::fused_stage_order[k], st);
-      (void)hipEventRecord(E[3 + k], st);
+      rc = rc ? rc : (hipEventRecord(b->ev_fork, st) != hipSuccess || hipStreamWaitEvent(b->side, b->ev_fork, 0) != hipSuccess);
+      es = b->side;
     }
+    timed(7, es, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageEdge, s); });
+    if (b->fork)
+      rc = rc ? rc : (hipEventRecord(b->ev_join, b->side) != hipSuccess);
+    timed(5, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageBorder, s); });
+    timed(6, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageTile, s); });
+    if (b->fork)
+      rc = rc ? rc : (hipStreamWaitEvent(st, b->ev_join, 0) != hipSuccess);
+    timed(8, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageFix, s); });
+    timed(9, st, [&](hipStream_t s) { return fused_launch_stage<T>(bt, p, kStageBupdate, s); });
     ++b->gstep;
   }
-  if (rc == 0 && hipStreamSynchronize(st) == hipSuccess)
+  if (rc == 0 && hipStreamSynchronize(st) == hipSuccess && hipStreamSynchronize(b->side) == hipSuccess)
   {
     for (int n = 0; n < nsteps; ++n)
       for (int k = 0; k < ns; ++k)
       {
         float f = 0.0f;
-        (void)hipEventElapsedTime(&f, ev[size_t(n) * (ns + 1) + k], ev[size_t(n) * (ns + 1) + k + 1]);
+        (void)hipEventElapsedTime(&f, ev[size_t(n) * 2 * ns + 2 * k], ev[size_t(n) * 2 * ns + 2 * k + 1]);
         ms[k] += double(f) / nsteps;
       }
   }
@@ -435,7 +478,9 @@ extern "C"
     auto * b = new mhdct_batch();
     b->dim = dim;
     b->impl = impl_version;
-    if (hipStreamCreate(&b->cap) != hipSuccess)
+    if (hipStreamCreate(&b->cap) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
     {
       delete b;
       return -1;
@@ -612,6 +657,16 @@ extern "C"
   }
 
   int
+  mhdct_batch_set_overlap(mhdct_batch * b, int enable)
+  {
+    if (!b)
+      return -1;
+    b->fork = enable != 0;
+    drop_graphs(b);
+    return 0;
+  }
+
+  int
   mhdct_batch_set_tune(mhdct_batch * b, int grid_cap, int tile_cap)
   {
     if (!b || grid_cap < 8 || tile_cap < 8)
@@ -657,6 +712,12 @@ extern "C"
     free_worklist(b);
     if (b->cap)
       (void)hipStreamDestroy(b->cap);
+    if (b->side)
+      (void)hipStreamDestroy(b->side);
+    if (b->ev_fork)
+      (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join)
+      (void)hipEventDestroy(b->ev_join);
     delete b;
   }
 }

@@ -127,9 +127,10 @@ int mhdct_batch_add(mhdct_batch * batch, mhdct_ctx * ctx, double * U, double * U
 int mhdct_batch_remove(mhdct_batch * batch, int slot_id);
 /* Enqueue `nsteps` steps of every slot on `stream` (outside fill, dt, stages). */
 int mhdct_batch_step(mhdct_batch * batch, int nsteps, void * stream);
-/* The same as mhdct_batch_step without graphs, with HIP events recorded on `stream` between the
- * stage launches: stage_ms[MHDCT_NUM_STAGES] receives the average milliseconds per step of each
- * stage (names from mhdct_stage_name).  Synchronises `stream`. */
+/* The same as mhdct_batch_step without graphs, with HIP events recorded around each stage on
+ * the stream it runs on: stage_ms[MHDCT_NUM_STAGES] receives the average milliseconds per step
+ * of each stage (names from mhdct_stage_name; with the overlap on, k_edge runs beside
+ * k_border_flux and k_flux_tile).  Synchronises `stream`. */
 #define MHDCT_NUM_STAGES 10
 int          mhdct_batch_step_timed(mhdct_batch * batch, int nsteps, void * stream, double * stage_ms);
 const char * mhdct_stage_name(int stage);
@@ -141,6 +142,10 @@ int mhdct_batch_slot_state(mhdct_batch * batch, int slot_id, int * current, doub
 int mhdct_batch_check(mhdct_batch * batch, int * bad_slot);
 /* Capture each step (per ping-pong parity) in a HIP graph and replay it. */
 int mhdct_batch_set_graph(mhdct_batch * batch, int enable);
+/* Run k_edge on a second stream beside the flux stages (default off: k_edge fills every CU's
+ * registers, so on MI355X the two branches measured no overlap), joined before the EMF
+ * correction; the results do not depend on it. */
+int mhdct_batch_set_overlap(mhdct_batch * batch, int enable);
 /* Launch tunables: at most grid_cap workgroups per element kernel and tile_cap per flux-tile
  * kernel (multiples of 8; below the work size the kernels loop). */
 int mhdct_batch_set_tune(mhdct_batch * batch, int grid_cap, int tile_cap);
