@@ -135,7 +135,7 @@ def bench_batch(args, cases, impl):
     tune = None
     if not args.no_tune:
         tune = bt.autotune(reps=2)
-        log(f"[bench] autotuned grid_cap={tune[0]} tile_cap={tune[1]}")
+        log(f"[bench] autotuned grid_cap={tune[0]} tile_cap={tune[1]} tile_z={tune[2]}")
     bt.step(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -208,7 +208,7 @@ def main():
                            "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B x {ncells} cell-updates per launch"}
         out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
         if tune:
-            config["tune"] = {"grid_cap": tune[0], "tile_cap": tune[1]}
+            config["tune"] = {"grid_cap": tune[0], "tile_cap": tune[1], "tile_z": tune[2]}
         if not args.no_plain:
             p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
             out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"],

@@ -67,8 +67,9 @@ SIGNATURES = {
     "mhdct_batch_check": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mhdct_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mhdct_batch_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
-    "mhdct_batch_set_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
-    "mhdct_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mhdct_batch_set_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mhdct_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
     "mhdct_batch_num_slots": (C.c_int, [C.c_void_p]),
     "mhdct_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "mhdct_batch_destroy": (None, [C.c_void_p]),

@@ -134,14 +134,16 @@ class BatchGodunovMHD:
     def set_overlap(self, enable: bool = True):
         self._check(lib().mhdct_batch_set_overlap(self._h, int(bool(enable))), "batch_set_overlap")
 
-    def set_tune(self, grid_cap: int, tile_cap: int):
-        self._check(lib().mhdct_batch_set_tune(self._h, int(grid_cap), int(tile_cap)), "batch_set_tune")
+    def set_tune(self, grid_cap: int, tile_cap: int, tile_z: int = 8):
+        self._check(lib().mhdct_batch_set_tune(self._h, int(grid_cap), int(tile_cap), int(tile_z)),
+                    "batch_set_tune")
 
     def autotune(self, reps: int = 2, stream=None):
-        g, t = C.c_int(), C.c_int()
+        """Pick the launch caps and the flux-tile depth by timing; returns (grid_cap, tile_cap, tile_z)."""
+        g, t, z = C.c_int(), C.c_int(), C.c_int()
         self._check(lib().mhdct_batch_autotune(self._h, int(reps), C.c_void_p(_stream(stream)), C.byref(g),
-                                               C.byref(t)), "batch_autotune")
-        return g.value, t.value
+                                               C.byref(t), C.byref(z)), "batch_autotune")
+        return g.value, t.value, z.value
 
     def num_slots(self) -> int:
         return int(lib().mhdct_batch_num_slots(self._h))

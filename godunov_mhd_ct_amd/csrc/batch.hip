@@ -419,7 +419,7 @@ trial(mhdct_batch * b, FusedBatch<T> & bt, int reps, hipStream_t st)
 
 template <typename T>
 int
-autotune(mhdct_batch * b, FusedBatch<T> & bt, int reps, hipStream_t st, int * grid_cap, int * tile_cap)
+autotune(mhdct_batch * b, FusedBatch<T> & bt, int reps, hipStream_t st, int * grid_cap, int * tile_cap, int * tile_z)
 {
   if (b->dirty && rebuild<T>(b, bt, st))
     return -1;
@@ -433,18 +433,21 @@ autotune(mhdct_batch * b, FusedBatch<T> & bt, int reps, hipStream_t st, int * gr
   const std::vector<int> tiles = { ncu, ncu * 2, ncu * 4, full };
   double            best = -1.0;
   FusedTune         pick = bt.tune;
-  for (int g : grids)
-    for (int t : tiles)
-    {
-      bt.tune.grid_cap = g;
-      bt.tune.tile_cap = t;
-      const double ms = trial<T>(b, bt, reps, st);
-      if (ms > 0 && (best < 0 || ms < best))
+  const std::vector<int> depths = b->dim == 3 ? std::vector<int>{ 8, 4 } : std::vector<int>{ 8 };
+  for (int z : depths)
+    for (int g : grids)
+      for (int t : tiles)
       {
-        best = ms;
-        pick = bt.tune;
+        bt.tune.grid_cap = g;
+        bt.tune.tile_cap = t;
+        bt.tune.tile_z = z;
+        const double ms = trial<T>(b, bt, reps, st);
+        if (ms > 0 && (best < 0 || ms < best))
+        {
+          best = ms;
+          pick = bt.tune;
+        }
       }
-    }
   if (best < 0)
   {
     b->err = "autotune: no configuration ran";
@@ -458,6 +461,8 @@ autotune(mhdct_batch * b, FusedBatch<T> & bt, int reps, hipStream_t st, int * gr
     *grid_cap = pick.grid_cap;
   if (tile_cap)
     *tile_cap = pick.tile_cap;
+  if (tile_z)
+    *tile_z = pick.tile_z;
   return 0;
 }
 
@@ -667,12 +672,13 @@ extern "C"
   }
 
   int
-  mhdct_batch_set_tune(mhdct_batch * b, int grid_cap, int tile_cap)
+  mhdct_batch_set_tune(mhdct_batch * b, int grid_cap, int tile_cap, int tile_z)
   {
-    if (!b || grid_cap < 8 || tile_cap < 8)
+    if (!b || grid_cap < 8 || tile_cap < 8 || (tile_z != 8 && tile_z != 4))
       return -1;
     b->tune.grid_cap = grid_cap;
     b->tune.tile_cap = tile_cap;
+    b->tune.tile_z = tile_z;
     b->bd.tune = b->tune;
     b->bf.tune = b->tune;
     drop_graphs(b);
@@ -680,14 +686,14 @@ extern "C"
   }
 
   int
-  mhdct_batch_autotune(mhdct_batch * b, int reps, void * stream, int * grid_cap, int * tile_cap)
+  mhdct_batch_autotune(mhdct_batch * b, int reps, void * stream, int * grid_cap, int * tile_cap, int * tile_z)
   {
     if (!b || reps < 1)
       return -1;
     b->err.clear();
     hipStream_t st = static_cast<hipStream_t>(stream);
-    return fp32(b) ? autotune<float>(b, b->bf, reps, st, grid_cap, tile_cap)
-                   : autotune<double>(b, b->bd, reps, st, grid_cap, tile_cap);
+    return fp32(b) ? autotune<float>(b, b->bf, reps, st, grid_cap, tile_cap, tile_z)
+                   : autotune<double>(b, b->bd, reps, st, grid_cap, tile_cap, tile_z);
   }
 
   int

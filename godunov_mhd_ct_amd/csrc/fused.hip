@@ -744,27 +744,42 @@ k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * 
 // through LDS; U2 = ((U + dX) + dY) + dZ accumulated in FP64 registers.  The workgroup has one
 // thread per face of a direction (3D: 9*8*8 = 576), the first 8^dim of them also own a cell.
 // ---------------------------------------------------------------------------------------------
-template <int DIM>
+// Flux tile of kTS x kTS x TZ cells (3D; 2D: kTS x kTS): the faces of one direction, one per
+// thread, and the cells of the tile owned by the first threads.
+template <int DIM, int TZ>
+struct TileG
+{
+  static constexpr int TX = kTS, TY = kTS, TD = DIM == 3 ? TZ : 1;
+  static constexpr int NCT = TX * TY * TD;
+  static constexpr int NF0 = (TX + 1) * TY * TD;
+  static constexpr int NF1 = TX * (TY + 1) * TD;
+  static constexpr int NF2 = DIM == 3 ? TX * TY * (TD + 1) : 0;
+  static constexpr int NFMAX = NF0 > NF1 ? (NF0 > NF2 ? NF0 : NF2) : (NF1 > NF2 ? NF1 : NF2);
+  static constexpr int NTHR = DIM == 3 ? (NFMAX + 63) / 64 * 64 : 128;
+  static constexpr int SPLIT = DIM == 3 ? kTS / TZ : 1; // tiles per 8^dim work item
+};
+
+template <int DIM, int TZ>
 constexpr int
 tile_threads()
 {
-  return DIM == 3 ? (kTS + 1) * kTS * kTS : 128;
+  return TileG<DIM, TZ>::NTHR;
 }
 
-template <typename T, int DIM, int BS, int DIR>
+template <typename T, int DIM, int BS, int TZ, int DIR>
 __device__ __forceinline__ void
-tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int org[3], T dtdS, T (*lf)[(kTS + 1) * (DIM == 3 ? kTS * kTS : kTS)])
+tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int org[3], T dtdS,
+           T (*lf)[TileG<DIM, TZ>::NFMAX])
 {
-  constexpr int TS = kTS;
-  constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
+  using TG = TileG<DIM, TZ>;
+  constexpr int NF = DIR == 0 ? TG::NF0 : (DIR == 1 ? TG::NF1 : TG::NF2);
   constexpr int NV = DIM == 2 ? 6 : 5;
-  constexpr int e0 = DIR == 0 ? TS + 1 : TS;
-  constexpr int e1 = DIR == 1 ? TS + 1 : TS;
+  constexpr int e0 = DIR == 0 ? TG::TX + 1 : TG::TX;
+  constexpr int e1 = DIR == 1 ? TG::TY + 1 : TG::TY;
   constexpr int t0 = DIR == 0 ? 1 : 0;
   constexpr int t1 = DIR == 2 ? 1 : 2;
-  constexpr int NTHR = tile_threads<DIM>();
   const DevMesh & m = S.m;
-  for (int fl = threadIdx.x; fl < NF; fl += NTHR)
+  for (int fl = threadIdx.x; fl < NF; fl += TG::NTHR)
   {
     const int a0 = fl % e0, a1 = (fl / e0) % e1, a2 = DIM == 3 ? fl / (e0 * e1) : 0;
     int       c[3] = { org[0] + a0, org[1] + a1, org[2] + a2 };
@@ -807,90 +822,91 @@ tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int 
   }
 }
 
-template <typename T, int DIM, int BS>
+// one tile: r0 is the 8^dim work item within the slot, part selects the TZ-deep slab of it
+template <typename T, int DIM, int BS, int TZ>
 __device__ __forceinline__ void
-flux_tile_item(const FusedSlot<T> & S, long r0, T (*lf)[(kTS + 1) * (DIM == 3 ? kTS * kTS : kTS)])
+flux_tile_item(const FusedSlot<T> & S, long r0, int part, T (*lf)[TileG<DIM, TZ>::NFMAX])
 {
-
   using G = Geo<DIM, BS>;
   using IX = IdxC<DIM, BS>;
-  constexpr int TS = kTS;
-  constexpr int NCT = DIM == 3 ? TS * TS * TS : TS * TS;
-  constexpr int NF = DIM == 3 ? (TS + 1) * TS * TS : (TS + 1) * TS;
+  using TG = TileG<DIM, TZ>;
   constexpr int NV = DIM == 2 ? 6 : 5;
   const DevMesh & m = S.m;
+  const int              o = int(r0) / G::TPB;
+  const View<T, DIM, BS> V(S, o);
+  const int              tl = int(r0) - o * G::TPB;
+  constexpr int          nb1 = G::NB1;
+  const int org[3] = { (tl % nb1) * kTS, ((tl / nb1) % nb1) * kTS,
+                       DIM == 3 ? (tl / (nb1 * nb1)) * kTS + part * TZ : 0 };
+  const int     tid = threadIdx.x;
+  const bool    owner = tid < TG::NCT;
+  const int     li = tid % TG::TX, lj = (tid / TG::TX) % TG::TY, lk = DIM == 3 ? tid / (TG::TX * TG::TY) : 0;
+  const int     ci = org[0] + li, cj = org[1] + lj, ck = org[2] + lk;
+  constexpr int ncell = IX::cells();
+  const int     cidx = IX::cell(ci, cj, ck);
+  double        acc[NV];
+  if (owner)
   {
-    const int              o = int(r0) / G::TPB;
-    const View<T, DIM, BS> V(S, o);
-    const int              tl = int(r0) - o * G::TPB;
-    constexpr int          nb1 = G::NB1;
-    const int org[3] = { (tl % nb1) * TS, ((tl / nb1) % nb1) * TS, DIM == 3 ? (tl / (nb1 * nb1)) * TS : 0 };
-    const int tid = threadIdx.x;
-    const bool owner = tid < NCT;
-    const int li = tid % TS, lj = (tid / TS) % TS, lk = DIM == 3 ? tid / (TS * TS) : 0;
-    const int ci = org[0] + li, cj = org[1] + lj, ck = org[2] + lk;
-    constexpr int ncell = IX::cells();
-    const int     cidx = IX::cell(ci, cj, ck);
-    double        acc[NV];
+    for (int v = 0; v < 5; ++v)
+      acc[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
+    if (DIM == 2)
+      acc[5] = S.B[long(o) * IX::fper() + IX::face(2, ci, cj, 0)];
+  }
+  const T dtdS = T(S.dt / dxd<BS>(m, S.scaling, o));
+  for (int dir = 0; dir < DIM; ++dir)
+  {
+    if (dir == 0)
+      tile_faces<T, DIM, BS, TZ, 0>(S, V, o, org, dtdS, lf);
+    else if (dir == 1)
+      tile_faces<T, DIM, BS, TZ, 1>(S, V, o, org, dtdS, lf);
+    else
+      tile_faces<T, DIM, BS, TZ, 2>(S, V, o, org, dtdS, lf);
+    __syncthreads();
     if (owner)
     {
-      for (int v = 0; v < 5; ++v)
-        acc[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
-      if (DIM == 2)
-        acc[5] = S.B[long(o) * IX::fper() + IX::face(2, ci, cj, 0)];
-    }
-    const T dtdS = T(S.dt / dxd<BS>(m, S.scaling, o));
-    for (int dir = 0; dir < DIM; ++dir)
-    {
-      if (dir == 0)
-        tile_faces<T, DIM, BS, 0>(S, V, o, org, dtdS, lf);
-      else if (dir == 1)
-        tile_faces<T, DIM, BS, 1>(S, V, o, org, dtdS, lf);
-      else
-        tile_faces<T, DIM, BS, 2>(S, V, o, org, dtdS, lf);
-      __syncthreads();
-      if (owner)
+      int ext[3] = { TG::TX, TG::TY, TG::TD };
+      ext[dir] += 1;
+      int       a[3] = { li, lj, lk };
+      const int fL = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
+      a[dir] += 1;
+      const int fR = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
+      for (int vi = 0; vi < NV; ++vi)
       {
-        int ext[3] = { TS, TS, DIM == 3 ? TS : 1 };
-        ext[dir] = TS + 1;
-        int a[3] = { li, lj, lk };
-        const int fL = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
-        a[dir] += 1;
-        const int fR = a[0] + ext[0] * (a[1] + ext[1] * a[2]);
-        for (int vi = 0; vi < NV; ++vi)
-        {
-          // (0 + F_left) - F_right, as ReadFluxesAndConservativeUpdateFunctor accumulates it
-          const double d = double(lf[vi][fL]) - double(lf[vi][fR]);
-          acc[vi] = acc[vi] + d;
-        }
+        // (0 + F_left) - F_right, as ReadFluxesAndConservativeUpdateFunctor accumulates it
+        const double d = double(lf[vi][fL]) - double(lf[vi][fR]);
+        acc[vi] = acc[vi] + d;
       }
-      __syncthreads();
     }
-    if (owner)
-    {
-      for (int v = 0; v < 5; ++v)
-        S.U2[(long(o) * 5 + v) * ncell + cidx] = acc[v];
-      if (DIM == 2)
-        S.B2[long(o) * IX::fper() + IX::face(2, ci, cj, 0)] = acc[5];
-    }
+    __syncthreads();
+  }
+  if (owner)
+  {
+    for (int v = 0; v < 5; ++v)
+      S.U2[(long(o) * 5 + v) * ncell + cidx] = acc[v];
+    if (DIM == 2)
+      S.B2[long(o) * IX::fper() + IX::face(2, ci, cj, 0)] = acc[5];
   }
 }
 
-template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(tile_threads<DIM>())
+// k_flux_tile: one workgroup per tile; `total` counts 8^dim work items, each split into
+// TileG::SPLIT tiles of depth TZ
+template <typename T, int DIM, int BSEL, int TZ>
+__global__ void __launch_bounds__((tile_threads<DIM, TZ>()))
 k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
-  constexpr int NF = DIM == 3 ? (kTS + 1) * kTS * kTS : (kTS + 1) * kTS;
+  using TG = TileG<DIM, TZ>;
   constexpr int NV = DIM == 2 ? 6 : 5;
-  __shared__ T  lf[NV][NF];
-  for (long t = xcd_group(); t < total; t += gridDim.x)
+  __shared__ T  lf[NV][TG::NFMAX];
+  for (long g = xcd_group(); g < total * TG::SPLIT; g += gridDim.x)
   {
+    const long           t = g / TG::SPLIT;
+    const int            part = int(g - t * TG::SPLIT);
     const int            s = slot_of(off, nslots, t);
     const FusedSlot<T> & S = slots[s];
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[3])
       continue;
-    MHDCT_BS_DISPATCH(S.m.b, (flux_tile_item<T, DIM, BS>(S, r0, lf)));
+    MHDCT_BS_DISPATCH(S.m.b, (flux_tile_item<T, DIM, BS, TZ>(S, r0, part, lf)));
   }
 }
 
@@ -1311,8 +1327,14 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
     hipLaunchKernelGGL((k_border_flux<T, DIM, BSEL>), grid(kStageBorder), 256, 0, st, sl, n,
                        bt.d_off[kStageBorder], bt.tot[kStageBorder]);
   if (on(kStageTile))
-    hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL>), grid_for(bt.tot[kStageTile], 1, tcap), tile_threads<DIM>(), 0,
-                       st, sl, n, bt.d_off[kStageTile], bt.tot[kStageTile]);
+  {
+    if (DIM == 3 && bt.tune.tile_z == 4)
+      hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL, 4>), grid_for(bt.tot[kStageTile] * 2, 1, tcap),
+                         (tile_threads<DIM, 4>()), 0, st, sl, n, bt.d_off[kStageTile], bt.tot[kStageTile]);
+    else
+      hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL, 8>), grid_for(bt.tot[kStageTile], 1, tcap),
+                         (tile_threads<DIM, 8>()), 0, st, sl, n, bt.d_off[kStageTile], bt.tot[kStageTile]);
+  }
   if (on(kStageEdge))
     hipLaunchKernelGGL((k_edge<T, DIM, BSEL>), grid(kStageEdge), 256, 0, st, sl, n, bt.d_off[kStageEdge],
                        bt.tot[kStageEdge]);

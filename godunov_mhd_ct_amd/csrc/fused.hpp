@@ -85,6 +85,7 @@ struct FusedTune
 {
   int grid_cap = 1 << 20; // element kernels: at most this many workgroups (multiple of 8)
   int tile_cap = 1 << 20; // flux tiles: at most this many workgroups (multiple of 8)
+  int tile_z = 8;         // 3D flux tile depth: 8 (8x8x8 cells, 576 threads) or 4 (8x8x4, 320 threads)
 };
 
 // The device work list of a batch: per stage, the prefix offsets of every slot's work items

@@ -147,11 +147,13 @@ int mhdct_batch_set_graph(mhdct_batch * batch, int enable);
  * correction; the results do not depend on it. */
 int mhdct_batch_set_overlap(mhdct_batch * batch, int enable);
 /* Launch tunables: at most grid_cap workgroups per element kernel and tile_cap per flux-tile
- * kernel (multiples of 8; below the work size the kernels loop). */
-int mhdct_batch_set_tune(mhdct_batch * batch, int grid_cap, int tile_cap);
-/* Time each candidate (grid_cap, tile_cap) over `reps` steps that leave the slots' states and
- * clocks unchanged, keep the fastest and return it. */
-int mhdct_batch_autotune(mhdct_batch * batch, int reps, void * stream, int * grid_cap, int * tile_cap);
+ * kernel (multiples of 8; below the work size the kernels loop), and the 3D flux-tile depth
+ * tile_z: 8 (8x8x8 cells, 576-thread workgroups) or 4 (8x8x4, 320 threads). */
+int mhdct_batch_set_tune(mhdct_batch * batch, int grid_cap, int tile_cap, int tile_z);
+/* Time each candidate (grid_cap, tile_cap, tile_z) over `reps` steps that leave the slots'
+ * states and clocks unchanged, keep the fastest and return it. */
+int mhdct_batch_autotune(mhdct_batch * batch, int reps, void * stream, int * grid_cap, int * tile_cap,
+                         int * tile_z);
 int          mhdct_batch_num_slots(const mhdct_batch * batch);
 const char * mhdct_batch_last_error(const mhdct_batch * batch);
 void         mhdct_batch_destroy(mhdct_batch * batch);

@@ -189,3 +189,23 @@ def test_batch_fp32_bulk_l1_error():
         print(f"  slot {n}: " + ", ".join(f"{k} L1 {v:.3e}" for k, v in errs.items()))
         bad = {k: v for k, v in errs.items() if not v <= FP32_L1}
         assert not bad, f"slot {n}: relative L1 above {FP32_L1}: {bad}"
+
+
+@pytest.mark.gpu
+def test_batch_tunables_do_not_change_results():
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+
+    cases = meshes_3d()
+    nsteps = 3
+    print("[batch] tile depth 4, small launch caps, side-stream overlap vs defaults", flush=True)
+    ref = run_batch(cases, nsteps)
+    bt = BatchGodunovMHD(3, IMPL_FUSED)
+    ids = [bt.add(m, U, B, settings_for(slope)) for (m, U, B, slope) in cases]
+    bt.set_tune(64, 16, 4)
+    bt.set_overlap(True)
+    bt.step(nsteps)
+    bt.check()
+    for n, sid in enumerate(ids):
+        Ug, Bg = bt.state(sid)
+        (Ur, Br), _, _ = ref[n]
+        assert np.array_equal(Ug, Ur) and np.array_equal(Bg, Br), f"slot {n}"
