@@ -744,8 +744,8 @@ k_border_flux(const FusedSlot<T> * __restrict__ slots, int nslots, const long * 
 // through LDS; U2 = ((U + dX) + dY) + dZ accumulated in FP64 registers.  The workgroup has one
 // thread per face of a direction (3D: 9*8*8 = 576), the first 8^dim of them also own a cell.
 // ---------------------------------------------------------------------------------------------
-// Flux tile of kTS x kTS x TZ cells (3D; 2D: kTS x kTS): the faces of one direction, one per
-// thread, and the cells of the tile owned by the first threads.
+// Flux tile of kTS x kTS x TZ cells (3D; 2D: kTS x kTS): the faces of each direction, one per
+// thread, go to an LDS plane per direction; the cells of the tile are owned by the first threads.
 template <int DIM, int TZ>
 struct TileG
 {
@@ -825,7 +825,7 @@ tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int 
 // one tile: r0 is the 8^dim work item within the slot, part selects the TZ-deep slab of it
 template <typename T, int DIM, int BS, int TZ>
 __device__ __forceinline__ void
-flux_tile_item(const FusedSlot<T> & S, long r0, int part, T (*lf)[TileG<DIM, TZ>::NFMAX])
+flux_tile_item(const FusedSlot<T> & S, long r0, int part, T (*lf)[DIM == 3 ? 5 : 6][TileG<DIM, TZ>::NFMAX])
 {
   using G = Geo<DIM, BS>;
   using IX = IdxC<DIM, BS>;
@@ -853,16 +853,15 @@ flux_tile_item(const FusedSlot<T> & S, long r0, int part, T (*lf)[TileG<DIM, TZ>
       acc[5] = S.B[long(o) * IX::fper() + IX::face(2, ci, cj, 0)];
   }
   const T dtdS = T(S.dt / dxd<BS>(m, S.scaling, o));
-  for (int dir = 0; dir < DIM; ++dir)
+  // the faces of every direction first (one LDS plane per direction), then one barrier
+  tile_faces<T, DIM, BS, TZ, 0>(S, V, o, org, dtdS, lf[0]);
+  tile_faces<T, DIM, BS, TZ, 1>(S, V, o, org, dtdS, lf[1]);
+  if (DIM == 3)
+    tile_faces<T, DIM, BS, TZ, 2>(S, V, o, org, dtdS, lf[DIM - 1]);
+  __syncthreads();
+  if (owner)
   {
-    if (dir == 0)
-      tile_faces<T, DIM, BS, TZ, 0>(S, V, o, org, dtdS, lf);
-    else if (dir == 1)
-      tile_faces<T, DIM, BS, TZ, 1>(S, V, o, org, dtdS, lf);
-    else
-      tile_faces<T, DIM, BS, TZ, 2>(S, V, o, org, dtdS, lf);
-    __syncthreads();
-    if (owner)
+    for (int dir = 0; dir < DIM; ++dir)
     {
       int ext[3] = { TG::TX, TG::TY, TG::TD };
       ext[dir] += 1;
@@ -873,12 +872,12 @@ flux_tile_item(const FusedSlot<T> & S, long r0, int part, T (*lf)[TileG<DIM, TZ>
       for (int vi = 0; vi < NV; ++vi)
       {
         // (0 + F_left) - F_right, as ReadFluxesAndConservativeUpdateFunctor accumulates it
-        const double d = double(lf[vi][fL]) - double(lf[vi][fR]);
+        const double d = double(lf[dir][vi][fL]) - double(lf[dir][vi][fR]);
         acc[vi] = acc[vi] + d;
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
   if (owner)
   {
     for (int v = 0; v < 5; ++v)
@@ -896,7 +895,7 @@ k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __
 {
   using TG = TileG<DIM, TZ>;
   constexpr int NV = DIM == 2 ? 6 : 5;
-  __shared__ T  lf[NV][TG::NFMAX];
+  __shared__ T  lf[DIM][NV][TG::NFMAX];
   for (long g = xcd_group(); g < total * TG::SPLIT; g += gridDim.x)
   {
     const long           t = g / TG::SPLIT;
