@@ -515,8 +515,13 @@ cell_item(const FusedSlot<T> & S, long r0)
           sl[d][5 + mc] = limited_slope(dc, dp, dm, st);
         }
       }
+      // every slope but the normal-B one along its own direction, which no consumer reads (the
+      // flux replaces the normal B by the face value; the edge reconstructions use the normal
+      // component only as the quirk of x-edges in 3D, dBy/dy): 3D skips dBx/dx and dBz/dz, 2D
+      // dBx/dx and dBy/dy
       for (int v = 0; v < 8; ++v)
-        bstore<T>(V.r_sl, (d * 8 + v) * G::G1 + c1, sl[d][v]);
+        if (!(v == kA + d && (DIM == 2 || d != 1)))
+          bstore<T>(V.r_sl, (d * 8 + v) * G::G1 + c1, sl[d][v]);
     }
     // Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
     const double dxo = dxd<BS>(m, S.scaling, o);
