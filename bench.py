@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
+    ap.add_argument("--precision-steps", type=int, default=0,
+                    help="with --impl fp32: run the FP32-bulk and the FP64 form this many steps from the same "
+                         "state and report the per-field relative L1 error (the precision config: 500 on kh3d)")
     ap.add_argument("--requests", type=int, default=40, help="srv: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="srv: arrivals per second")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -213,10 +216,52 @@ def main():
             p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
             out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"],
                             "speedup": value / p["value"]}
+    if args.precision_steps > 0 and args.impl == "fp32":
+        out["l1_error_vs_fp64"] = precision_errors(cases, args.precision_steps)
     out["config"] = config
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
     print(json.dumps(out))
+
+
+def precision_errors(cases, steps):
+    """Relative L1 error per field of the FP32-bulk form against the FP64 form after `steps`
+    steps from the same state, each with its own CFL dt (the FP64 form matches the oracle at the
+    1e-12 parity bar).  Momentum and B are normalised as vectors."""
+    import numpy as np
+    import torch
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32
+
+    res = {}
+    states = {}
+    for impl in (IMPL_FUSED, IMPL_FUSED_FP32):
+        bt = BatchGodunovMHD(cases[0][0].dim, impl)
+        ids = [bt.add(m, U, B, st) for (m, U, B, st) in cases]
+        bt.set_graph(True)
+        done = 0
+        while done < steps:
+            n = min(50, steps - done)
+            bt.step(n)
+            done += n
+            torch.cuda.synchronize()
+            log(f"[bench] precision run impl={impl}: {done}/{steps} steps")
+        bt.check()
+        states[impl] = [(bt.state(i), bt.slot_state(i)[1]) for i in ids]
+        del bt
+    l1 = lambda a, b: float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))  # noqa: E731
+    for k, (m, _, _, _) in enumerate(cases):
+        (U64, B64), t64 = states[IMPL_FUSED][k]
+        (U32, B32), t32 = states[IMPL_FUSED_FP32][k]
+        no = m.num_leaves
+        res = {"steps": steps, "time_fp64": t64, "time_fp32": t32,
+               "rho": l1(U32[:no, 0], U64[:no, 0]), "E": l1(U32[:no, 1], U64[:no, 1]),
+               "momentum": l1(U32[:no, 2:5], U64[:no, 2:5]),
+               "mx": l1(U32[:no, 2], U64[:no, 2]), "my": l1(U32[:no, 3], U64[:no, 3]),
+               "mz_abs_over_momentum": float(np.abs(U32[:no, 4] - U64[:no, 4]).sum()
+                                             / max(np.abs(U64[:no, 2:5]).sum(), 1e-300)),
+               "B": l1(B32[:no], B64[:no])}
+    return res
 
 
 def bench_serving(args):
