@@ -50,6 +50,7 @@ def parse():
                          "state and report the per-field relative L1 error (the precision config: 500 on kh3d)")
     ap.add_argument("--requests", type=int, default=40, help="srv: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="srv: arrivals per second")
+    ap.add_argument("--graph", action="store_true", help="srv: replay each batch step from a HIP graph")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="before the steps, copy a buffer of this many MiB device to device once (the "
                          "known-size reference for HBM byte counters)")
@@ -275,7 +276,7 @@ def bench_serving(args):
     log(f"[bench] serving {len(reqs)} requests at {args.rate}/s, HBM budget {budget / 2**30:.1f} GiB")
     # warm-up: compile-free, but the first batch steps page code in
     Server(hbm_budget=budget).run(request_stream(2, rate=1000.0, seed=99, nsteps=5, sizes2d=(32,), sizes3d=(32,)))
-    srv = Server(hbm_budget=budget, impl=IMPL_FUSED, max_tick=20)
+    srv = Server(hbm_budget=budget, impl=IMPL_FUSED, max_tick=20, use_graph=args.graph)
     res = srv.run(reqs)
     out = {"metric": METRIC, "value": res["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
            "steps": 200, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
@@ -283,7 +284,7 @@ def bench_serving(args):
            "latency_ms_p50": res["latency_ms_p50"], "latency_ms_p99": res["latency_ms_p99"],
            "requests_per_s": res["requests"] / res["wall_s"],
            "config": {"workload": "srv", "requests": res["requests"], "rate_per_s": args.rate,
-                      "steps_per_request": 200, "hbm_budget_gib": budget / 2 ** 30},
+                      "steps_per_request": 200, "hbm_budget_gib": budget / 2 ** 30, "graph": args.graph},
            "roofline": {"bound": "hbm", "achieved": res["cell_updates_per_s"] * ALGO_BYTES_PER_CELL / 1e9,
                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": res["cell_updates_per_s"] * ALGO_BYTES_PER_CELL / HBM_PEAK, "traffic": None,
