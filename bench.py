@@ -2,6 +2,7 @@
 """Benchmark: cell-updates/s of the MHD-CT time step on one MI355X.  Prints one JSON line.
 
   python bench.py [--workload bl3d|mix|kh3d|ot2d|srv] [--steps K] [--warmup W] [--impl fused|fp32|plain]
+                  [--integrator hancock|rk2]
 
 A step advances every owned cell of the workload once: the outside-block fill, the CFL dt
 reduction (dt = cfl / inv_dt, on the device) and the time step, inputs resident in HBM.
@@ -17,6 +18,11 @@ dt).  The timed steps record HIP events between the stage launches on the launch
 the dominant kernel's duration comes from the timed region itself.  The plain form (one kernel
 per stage, the GodunovImplemV0 drop-in) is timed in the same run on the same workload, and the
 CPU restatement of the step (oracle/, OpenMP) on a bounded sample of it.
+
+--integrator rk2 times the RK2_SSP integrator (two forward-Euler stages and the stage
+combinations per step, SolverGodunovMHD.cpp:551-647) through the per-mesh solver driver, with the
+CFL dt read back on the host every step as the reference does; a cell-update is still one owned
+cell advanced one full step.
 """
 from __future__ import annotations
 
@@ -42,6 +48,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--impl", default="fused", choices=["fused", "fp32", "plain"])
     ap.add_argument("--workload", default="bl3d", choices=["bl3d", "mix", "kh3d", "ot2d", "srv"])
+    ap.add_argument("--integrator", default="hancock", choices=["hancock", "rk2"])
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
@@ -71,8 +78,8 @@ def cpu_baseline(cases, steps):
     cells = 0
     el = 0.0
     for mesh, U, B, st in cases:
-        ora = OracleSolver(mesh, settings_array(gamma0=st.gamma0, cfl=st.cfl, slope_type=st.slope_type),
-                           nthreads=threads)
+        ora = OracleSolver(mesh, settings_array(gamma0=st.gamma0, cfl=st.cfl, slope_type=st.slope_type,
+                                                time_integrator=st.time_integrator), nthreads=threads)
         nt = mesh.num_total
         Uo = U.reshape(nt, 5, -1).copy()
         Bo = B.reshape(nt, -1).copy()
@@ -97,12 +104,15 @@ def load_traffic(kernel, workload):
         return None
 
 
-def run_plain(cases, steps, warmup):
+def run_plain(cases, steps, warmup, impl=None):
+    """Steps through the per-mesh solver driver (SolverGodunovMHD: host dt read-back each step),
+    the plain form unless `impl` names another."""
     import torch
     from godunov_mhd_ct_amd.implem import IMPL_PLAIN
     from godunov_mhd_ct_amd.solver import SolverGodunovMHD
 
-    sols = [SolverGodunovMHD(m, U, B, st, impl_version=IMPL_PLAIN) for (m, U, B, st) in cases]
+    impl = IMPL_PLAIN if impl is None else impl
+    sols = [SolverGodunovMHD(m, U, B, st, impl_version=impl) for (m, U, B, st) in cases]
     for _ in range(warmup):
         for s in sols:
             s.step()
@@ -181,6 +191,8 @@ def main():
         return bench_serving(args)
 
     cases = cases_for(args.workload)
+    if args.integrator == "rk2":
+        return bench_rk2(args, cases)
     cells = sum(m.owned_cells() for (m, _, _, _) in cases)
     log(f"[bench] workload={args.workload} impl={args.impl} meshes={len(cases)} "
         f"leaves={sum(m.num_leaves for (m, _, _, _) in cases)} cells={cells}")
@@ -220,6 +232,36 @@ def main():
     if args.precision_steps > 0 and args.impl == "fp32":
         out["l1_error_vs_fp64"] = precision_errors(cases, args.precision_steps)
     out["config"] = config
+    if args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
+    print(json.dumps(out))
+
+
+def bench_rk2(args, cases):
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32, IMPL_PLAIN, TIME_INTEGRATOR
+
+    for (_, _, _, st) in cases:
+        st.time_integrator = TIME_INTEGRATOR["RK2_SSP"]
+    impl = {"fused": IMPL_FUSED, "fp32": IMPL_FUSED_FP32, "plain": IMPL_PLAIN}[args.impl]
+    cells = sum(m.owned_cells() for (m, _, _, _) in cases)
+    log(f"[bench] RK2_SSP workload={args.workload} impl={args.impl} meshes={len(cases)} cells={cells}")
+    r = run_plain(cases, args.steps, args.warmup, impl)
+    out = {"metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64" if args.impl != "fp32" else "f32 bulk / f64 accumulation",
+           "data": "synthetic",
+           "config": {"workload": args.workload, "impl": args.impl, "integrator": "RK2_SSP", "meshes": len(cases),
+                      "leaves": sum(m.num_leaves for (m, _, _, _) in cases), "cells": cells,
+                      "driver": "per-mesh solver, host dt each step"}}
+    # two stages per step: the state is read and written twice
+    algo = 2 * ALGO_BYTES_PER_CELL
+    out["roofline"] = {"bound": "hbm", "achieved": r["value"] * algo / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": r["value"] * algo / HBM_PEAK, "traffic": None,
+                       "kernel": "whole step (two stages, per-mesh driver: no per-kernel events)",
+                       "algorithmic_bytes": f"{algo} B per cell-update"}
+    if args.impl != "plain" and not args.no_plain:
+        p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
+        out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"], "speedup": r["value"] / p["value"]}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
     print(json.dumps(out))

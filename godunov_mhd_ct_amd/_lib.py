@@ -29,6 +29,7 @@ class Settings(C.Structure):
         ("riemann", C.c_int),
         ("prolongation", C.c_int),
         ("abort_when_negative_eint", C.c_int),
+        ("time_integrator", C.c_int),
     ]
 
 
@@ -53,6 +54,8 @@ SIGNATURES = {
                                      C.c_void_p]),
     "mhdct_compute_dt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
     "mhdct_fill_outside": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mhdct_linear_combination": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                           C.c_double, C.c_void_p]),
     "mhdct_last_error": (C.c_char_p, [C.c_void_p]),
     "mhdct_destroy": (None, [C.c_void_p]),
     "mhdct_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),

@@ -510,6 +510,11 @@ extern "C"
       b->err = "the context's dimension or implementation differs from the batch";
       return -1;
     }
+    if (ctx->c.st.time_integrator != MHDCT_TIME_HANCOCK)
+    {
+      b->err = "a batch advances HANCOCK contexts only";
+      return -1;
+    }
     if ((fp32(b) ? static_cast<const void *>(mhdct::fused_slot<float>(ctx->c))
                  : static_cast<const void *>(mhdct::fused_slot<double>(ctx->c))) == nullptr)
     {

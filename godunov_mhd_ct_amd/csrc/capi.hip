@@ -226,6 +226,8 @@ extern "C"
       return -4;
     if (s->riemann != 5 || s->prolongation != 0)
       return -5;
+    if (s->time_integrator != MHDCT_TIME_HANCOCK && s->time_integrator != MHDCT_TIME_RK2_SSP)
+      return -5;
     for (int d = 1; d < dim; ++d)
       if (block_size[d] != block_size[0])
         return -6; // dx == dy == dz (ComputeDtMHDFunctor.cpp:98)
@@ -243,6 +245,7 @@ extern "C"
     c.st.smallc = s->smallc;
     c.st.scaling = s->scaling_factor;
     c.st.abort_when_negative_eint = s->abort_when_negative_eint;
+    c.st.time_integrator = s->time_integrator;
     if (hipGetDevice(&c.device) != hipSuccess)
     {
       delete h;
@@ -390,6 +393,26 @@ extern "C"
       return -1;
     }
     return mhdct::launch_fill_outside(h->c, U, B, static_cast<hipStream_t>(stream));
+  }
+
+  int
+  mhdct_linear_combination(mhdct_ctx * h, const double * U, double * U_RK, const double * B, double * B_RK,
+                           double c0, double c1, void * stream)
+  {
+    if (!h)
+      return -1;
+    Context & c = h->c;
+    if (!c.has_mesh)
+    {
+      c.err = "no mesh set";
+      return -1;
+    }
+    if (c.dm.ntotal > 0 && (!U || !U_RK || !B || !B_RK))
+    {
+      c.err = "null state pointer";
+      return -1;
+    }
+    return mhdct::launch_linear_combination(c, U, U_RK, B, B_RK, c0, c1, static_cast<hipStream_t>(stream));
   }
 
   const char *

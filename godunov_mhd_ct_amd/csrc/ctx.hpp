@@ -24,6 +24,7 @@ struct Settings
   double smallc = 1e-10;
   double scaling = 1.0;
   int    abort_when_negative_eint = 0;
+  int    time_integrator = 0; // 0 HANCOCK, 1 RK2_SSP
 };
 
 // Mesh tables resident in HBM.
@@ -121,5 +122,7 @@ int  plain_step(Context & c, const double * U, double * U2, const double * B, do
                 hipStream_t s);
 int  launch_inv_dt(Context & c, const double * U, const double * B, hipStream_t s);
 int  launch_fill_outside(Context & c, double * U, double * B, hipStream_t s);
+int  launch_linear_combination(Context & c, const double * U, double * U_RK, const double * B, double * B_RK,
+                               double c0, double c1, hipStream_t s);
 
 } // namespace mhdct

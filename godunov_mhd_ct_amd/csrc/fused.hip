@@ -465,9 +465,11 @@ k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __res
 
 
 // ---------------------------------------------------------------------------------------------
-// k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.
+// k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.  HK:
+// HANCOCK (half-step predictor), else RK2_SSP (Q2 = the cell-centred state): a kernel template
+// parameter, so that the HANCOCK kernel carries no trace of the other integrator.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM, int BS>
+template <typename T, int DIM, int BS, bool HK>
 __device__ __forceinline__ void
 cell_item(const FusedSlot<T> & S, long r0)
 {
@@ -531,46 +533,62 @@ cell_item(const FusedSlot<T> & S, long r0)
     const T A = (Q(5, i, j, k) + Q(8, i, j, k)) * T(0.5);
     const T B = (Q(6, i, j, k) + Q(9, i, j, k)) * T(0.5);
     const T C = (Q(7, i, j, k) + Q(10, i, j, k)) * T(0.5);
-    const T drx = sl[0][0], dpx = sl[0][1], dux = sl[0][2], dvx = sl[0][3], dwx = sl[0][4], dAx = sl[0][5],
-            dBx = sl[0][6], dCx = sl[0][7];
-    const T dry = sl[1][0], dpy = sl[1][1], duy = sl[1][2], dvy = sl[1][3], dwy = sl[1][4], dAy = sl[1][5],
-            dBy = sl[1][6], dCy = sl[1][7];
-    T sr0, su0, sv0, sw0, sp0, sA0, sB0, sC0;
-    if (DIM == 2)
+    if (!HK)
     {
-      sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy;
-      su0 = (-u * dux - dpx / r - B * dBx / r - C * dCx / r) * dtdx + (-v * duy + B * dAy / r) * dtdy;
-      sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - dpy / r - A * dAy / r - C * dCy / r) * dtdy;
-      sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy;
-      sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy;
-      sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy;
-      sB0 = (-u * dBx - B * dux + v * dAx + A * dvx) * dtdx;
-      sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (-v * dCy - C * dvy + w * dBy + B * dwy) * dtdy;
+      // RK2_SSP: Q2 is the cell-centred state (ComputeUpdatedPrimitiveVariablesFunctor.cpp:396-407,
+      // 531-542)
+      bstore<T>(V.r_q2, 0 * G::G1 + c1, r);
+      bstore<T>(V.r_q2, 2 * G::G1 + c1, u);
+      bstore<T>(V.r_q2, 3 * G::G1 + c1, v);
+      bstore<T>(V.r_q2, 4 * G::G1 + c1, w);
+      bstore<T>(V.r_q2, 1 * G::G1 + c1, p);
+      bstore<T>(V.r_q2, 5 * G::G1 + c1, A);
+      bstore<T>(V.r_q2, 6 * G::G1 + c1, B);
+      bstore<T>(V.r_q2, 7 * G::G1 + c1, C);
     }
     else
     {
-      const T drz = sl[2][0], dpz = sl[2][1], duz = sl[2][2], dvz = sl[2][3], dwz = sl[2][4], dAz = sl[2][5],
-              dBz = sl[2][6], dCz = sl[2][7];
-      sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-w * drz - dwz * r) * dtdz;
-      su0 = (-u * dux - (dpx + B * dBx + C * dCx) / r) * dtdx + (-v * duy + B * dAy / r) * dtdy +
-            (-w * duz + C * dAz / r) * dtdz;
-      sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - (dpy + A * dAy + C * dCy) / r) * dtdy +
-            (-w * dvz + C * dBz / r) * dtdz;
-      sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy +
-            (-w * dwz - (dpz + A * dAz + B * dBz) / r) * dtdz;
-      sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy + (-w * dpz - dwz * gamma * p) * dtdz;
-      sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
-      sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
-      sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+      const T drx = sl[0][0], dpx = sl[0][1], dux = sl[0][2], dvx = sl[0][3], dwx = sl[0][4], dAx = sl[0][5],
+              dBx = sl[0][6], dCx = sl[0][7];
+      const T dry = sl[1][0], dpy = sl[1][1], duy = sl[1][2], dvy = sl[1][3], dwy = sl[1][4], dAy = sl[1][5],
+              dBy = sl[1][6], dCy = sl[1][7];
+      T sr0, su0, sv0, sw0, sp0, sA0, sB0, sC0;
+      if (DIM == 2)
+      {
+        sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy;
+        su0 = (-u * dux - dpx / r - B * dBx / r - C * dCx / r) * dtdx + (-v * duy + B * dAy / r) * dtdy;
+        sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - dpy / r - A * dAy / r - C * dCy / r) * dtdy;
+        sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy;
+        sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy;
+        sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy;
+        sB0 = (-u * dBx - B * dux + v * dAx + A * dvx) * dtdx;
+        sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (-v * dCy - C * dvy + w * dBy + B * dwy) * dtdy;
+      }
+      else
+      {
+        const T drz = sl[2][0], dpz = sl[2][1], duz = sl[2][2], dvz = sl[2][3], dwz = sl[2][4], dAz = sl[2][5],
+                dBz = sl[2][6], dCz = sl[2][7];
+        sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-w * drz - dwz * r) * dtdz;
+        su0 = (-u * dux - (dpx + B * dBx + C * dCx) / r) * dtdx + (-v * duy + B * dAy / r) * dtdy +
+              (-w * duz + C * dAz / r) * dtdz;
+        sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - (dpy + A * dAy + C * dCy) / r) * dtdy +
+              (-w * dvz + C * dBz / r) * dtdz;
+        sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy +
+              (-w * dwz - (dpz + A * dAz + B * dBz) / r) * dtdz;
+        sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy + (-w * dpz - dwz * gamma * p) * dtdz;
+        sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
+        sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
+        sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+      }
+      bstore<T>(V.r_q2, 0 * G::G1 + c1, r + T(0.5) * sr0);
+      bstore<T>(V.r_q2, 2 * G::G1 + c1, u + T(0.5) * su0);
+      bstore<T>(V.r_q2, 3 * G::G1 + c1, v + T(0.5) * sv0);
+      bstore<T>(V.r_q2, 4 * G::G1 + c1, w + T(0.5) * sw0);
+      bstore<T>(V.r_q2, 1 * G::G1 + c1, p + T(0.5) * sp0);
+      bstore<T>(V.r_q2, 5 * G::G1 + c1, A + T(0.5) * sA0);
+      bstore<T>(V.r_q2, 6 * G::G1 + c1, B + T(0.5) * sB0);
+      bstore<T>(V.r_q2, 7 * G::G1 + c1, C + T(0.5) * sC0);
     }
-    bstore<T>(V.r_q2, 0 * G::G1 + c1, r + T(0.5) * sr0);
-    bstore<T>(V.r_q2, 2 * G::G1 + c1, u + T(0.5) * su0);
-    bstore<T>(V.r_q2, 3 * G::G1 + c1, v + T(0.5) * sv0);
-    bstore<T>(V.r_q2, 4 * G::G1 + c1, w + T(0.5) * sw0);
-    bstore<T>(V.r_q2, 1 * G::G1 + c1, p + T(0.5) * sp0);
-    bstore<T>(V.r_q2, 5 * G::G1 + c1, A + T(0.5) * sA0);
-    bstore<T>(V.r_q2, 6 * G::G1 + c1, B + T(0.5) * sB0);
-    bstore<T>(V.r_q2, 7 * G::G1 + c1, C + T(0.5) * sC0);
     // edge electric field v x B at t_n from k_efield
     const auto r_ed = rsrc(S.Ed + long(o) * G::NCE * G::NPE, G::NCE * G::NPE * int(sizeof(T)));
     auto       ep = [](int a, int c, int e) {
@@ -612,7 +630,7 @@ cell_item(const FusedSlot<T> & S, long r0)
     }
 }
 
-template <typename T, int DIM, int BSEL>
+template <typename T, int DIM, int BSEL, bool HK>
 __global__ void __launch_bounds__(256)
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
@@ -623,7 +641,7 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[1])
       continue;
-    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS>(S, r0)));
+    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS, HK>(S, r0)));
   }
 }
 
@@ -916,8 +934,9 @@ k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __
 
 
 // corner state of an edge (ComputeEmfAndStoreFunctor.cpp:105-367); KIND 0 LB, 1 RT, 2 RB, 3 LT,
-// (D0, D1) the two directions across the edge
-template <typename T, int DIM, int BS, int KIND, int D0, int D1>
+// (D0, D1) the two directions across the edge; HK: HANCOCK (the face-B source enters the face
+// values, ComputeEmfAndStoreFunctor.cpp:189-193, 297-301), else RK2_SSP
+template <typename T, int DIM, int BS, int KIND, int D0, int D1, bool HK>
 __device__ __forceinline__ void
 edge_state(const View<T, DIM, BS> & V, int is, int js, int ks, T * q)
 {
@@ -938,8 +957,11 @@ edge_state(const View<T, DIM, BS> & V, int is, int js, int ks, T * q)
   const int cp1 = VW::c1(p1[0], p1[1], p1[2]);
   T         B0 = V.bf(D0, p0[0], p0[1], p0[2]);
   T         B1 = V.bf(D1, p1[0], p1[1], p1[2]);
-  B0 += V.sf(D0, cp0);
-  B1 += V.sf(D1, cp1);
+  if (HK)
+  {
+    B0 += V.sf(D0, cp0);
+    B1 += V.sf(D1, cp1);
+  }
   // face-slope record index of (component, direction)
   constexpr int fs01 = DIM == 2 ? D0 : (D0 == 0 ? D1 - 1 : (D0 == 1 ? (D1 == 0 ? 2 : 3) : (D1 == 0 ? 4 : 5)));
   constexpr int fs10 = DIM == 2 ? D1 : (D1 == 0 ? D0 - 1 : (D1 == 1 ? (D0 == 0 ? 2 : 3) : (D0 == 0 ? 4 : 5)));
@@ -979,7 +1001,7 @@ edge_state(const View<T, DIM, BS> & V, int is, int js, int ks, T * q)
 }
 
 // the EMF along axis AX at edge r (of EPD) of block o
-template <typename T, int DIM, int BS, int AX>
+template <typename T, int DIM, int BS, int AX, bool HK>
 __device__ __forceinline__ void
 edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx)
 {
@@ -990,24 +1012,24 @@ edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx
   T             qe[4][8];
   if (AX == 2)
   {
-    edge_state<T, DIM, BS, 0, 0, 1>(V, i, j, k, qe[kLB]);
-    edge_state<T, DIM, BS, 1, 0, 1>(V, i - 1, j - 1, k, qe[kRT]);
-    edge_state<T, DIM, BS, 2, 0, 1>(V, i - 1, j, k, qe[kRB]);
-    edge_state<T, DIM, BS, 3, 0, 1>(V, i, j - 1, k, qe[kLT]);
+    edge_state<T, DIM, BS, 0, 0, 1, HK>(V, i, j, k, qe[kLB]);
+    edge_state<T, DIM, BS, 1, 0, 1, HK>(V, i - 1, j - 1, k, qe[kRT]);
+    edge_state<T, DIM, BS, 2, 0, 1, HK>(V, i - 1, j, k, qe[kRB]);
+    edge_state<T, DIM, BS, 3, 0, 1, HK>(V, i, j - 1, k, qe[kLT]);
   }
   else if (AX == 1)
   {
-    edge_state<T, DIM, BS, 0, 0, 2>(V, i, j, k, qe[kLB]);
-    edge_state<T, DIM, BS, 1, 0, 2>(V, i - 1, j, k - 1, qe[kRT]);
-    edge_state<T, DIM, BS, 2, 0, 2>(V, i - 1, j, k, qe[kLT]);
-    edge_state<T, DIM, BS, 3, 0, 2>(V, i, j, k - 1, qe[kRB]);
+    edge_state<T, DIM, BS, 0, 0, 2, HK>(V, i, j, k, qe[kLB]);
+    edge_state<T, DIM, BS, 1, 0, 2, HK>(V, i - 1, j, k - 1, qe[kRT]);
+    edge_state<T, DIM, BS, 2, 0, 2, HK>(V, i - 1, j, k, qe[kLT]);
+    edge_state<T, DIM, BS, 3, 0, 2, HK>(V, i, j, k - 1, qe[kRB]);
   }
   else
   {
-    edge_state<T, DIM, BS, 0, 1, 2>(V, i, j, k, qe[kLB]);
-    edge_state<T, DIM, BS, 1, 1, 2>(V, i, j - 1, k - 1, qe[kRT]);
-    edge_state<T, DIM, BS, 2, 1, 2>(V, i, j - 1, k, qe[kRB]);
-    edge_state<T, DIM, BS, 3, 1, 2>(V, i, j, k - 1, qe[kLT]);
+    edge_state<T, DIM, BS, 0, 1, 2, HK>(V, i, j, k, qe[kLB]);
+    edge_state<T, DIM, BS, 1, 1, 2, HK>(V, i, j - 1, k - 1, qe[kRT]);
+    edge_state<T, DIM, BS, 2, 1, 2, HK>(V, i, j - 1, k, qe[kRB]);
+    edge_state<T, DIM, BS, 3, 1, 2, HK>(V, i, j, k - 1, qe[kLT]);
   }
   const T       E = edge_emf_ax<AX>(qe, S.gt) * dtdx;
   constexpr int comp = DIM == 3 ? AX : 0;
@@ -1034,12 +1056,22 @@ edge_item(const FusedSlot<T> & S, long r0)
       return;
     const View<T, DIM, BS> V(S, o);
     const T                dtdx = T(S.dt / dxd<BS>(S.m, S.scaling, o));
-    if (DIM == 2 || a == 2)
-      edge_at<T, DIM, BS, 2>(S, V, o, r, dtdx);
+    // the integrator is uniform per slot, so per wavefront; each variant is compiled on its own
+    if (S.rk2)
+    {
+      if (DIM == 2 || a == 2)
+        edge_at<T, DIM, BS, 2, false>(S, V, o, r, dtdx);
+      else if (a == 1)
+        edge_at<T, DIM, BS, 1, false>(S, V, o, r, dtdx);
+      else
+        edge_at<T, DIM, BS, 0, false>(S, V, o, r, dtdx);
+    }
+    else if (DIM == 2 || a == 2)
+      edge_at<T, DIM, BS, 2, true>(S, V, o, r, dtdx);
     else if (a == 1)
-      edge_at<T, DIM, BS, 1>(S, V, o, r, dtdx);
+      edge_at<T, DIM, BS, 1, true>(S, V, o, r, dtdx);
     else
-      edge_at<T, DIM, BS, 0>(S, V, o, r, dtdx);
+      edge_at<T, DIM, BS, 0, true>(S, V, o, r, dtdx);
 }
 
 template <typename T, int DIM, int BSEL>
@@ -1325,8 +1357,14 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
     hipLaunchKernelGGL((k_efield<T, DIM, BSEL>), grid(kStageEfield), 256, 0, st, sl, n, bt.d_off[kStageEfield],
                        bt.tot[kStageEfield]);
   if (on(kStageCell))
-    hipLaunchKernelGGL((k_cell<T, DIM, BSEL>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
-                       bt.tot[kStageCell]);
+  {
+    if (bt.rk2)
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, false>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
+                         bt.tot[kStageCell]);
+    else
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, true>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
+                         bt.tot[kStageCell]);
+  }
   if (on(kStageBorder) && bt.tot[kStageBorder] > 0)
     hipLaunchKernelGGL((k_border_flux<T, DIM, BSEL>), grid(kStageBorder), 256, 0, st, sl, n,
                        bt.d_off[kStageBorder], bt.tot[kStageBorder]);

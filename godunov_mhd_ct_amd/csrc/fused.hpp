@@ -38,6 +38,7 @@ struct FusedSlot
   double               cfl;
   double               scaling;
   int                  abort_eint;
+  int                  rk2; // RK2_SSP stage: no half-step predictor, no face-B source at the edges
   const double *       U;
   double *             U2;
   const double *       B;
@@ -96,6 +97,7 @@ struct FusedBatch
 {
   int            dim = 3;
   int            bsel = 0; // the block size of every slot, or 0 when they differ
+  int            rk2 = 0;  // every slot is an RK2_SSP stage (per-context steps; batches are HANCOCK)
   int            nslots = 0;
   FusedSlot<T> * d_slots[2] = { nullptr, nullptr };
   long *         d_off[kStages] = {};

@@ -70,6 +70,7 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.cfl = c.st.cfl;
   sl.scaling = c.st.scaling;
   sl.abort_eint = c.st.abort_when_negative_eint;
+  sl.rk2 = c.st.time_integrator == 1;
   sl.red = c.red;
   T * p = arena;
   sl.Qp = p;  p += nq * z.qp;
@@ -123,6 +124,7 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   // the one-slot work list of this context
   bt.dim = dim;
   bt.bsel = b;
+  bt.rk2 = sl.rk2;
   bt.nslots = 1;
   for (int k = 0; k < kStages; ++k)
   {

@@ -250,7 +250,7 @@ k_slopes(DevMesh m, double st, const double * __restrict__ Qp, double * Sx, doub
 // K6: Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_predictor(DevMesh m, double scaling, double gamma, double dt, const double * __restrict__ Qp,
+k_predictor(DevMesh m, double scaling, double gamma, double dt, int rk2, const double * __restrict__ Qp,
             const double * __restrict__ Sx, const double * __restrict__ Sy, const double * __restrict__ Sz,
             double * __restrict__ Q2)
 {
@@ -275,6 +275,21 @@ k_predictor(DevMesh m, double scaling, double gamma, double dt, const double * _
   const double A = (q(i, j, k, 5, o) + q(i, j, k, 8, o)) * 0.5;
   const double B = (q(i, j, k, 6, o) + q(i, j, k, 9, o)) * 0.5;
   const double C = (q(i, j, k, 7, o) + q(i, j, k, 10, o)) * 0.5;
+  if (rk2)
+  {
+    // RK2_SSP: no half-step predictor, Q2 holds the cell-centred state
+    // (ComputeUpdatedPrimitiveVariablesFunctor.cpp:396-407, 531-542)
+    double * out = Q2 + s0;
+    out[0 * nc] = r;
+    out[2 * nc] = u;
+    out[3 * nc] = v;
+    out[4 * nc] = w;
+    out[1 * nc] = p;
+    out[5 * nc] = A;
+    out[6 * nc] = B;
+    out[7 * nc] = C;
+    return;
+  }
   const double drx = SX(0), dpx = SX(1), dux = SX(2), dvx = SX(3), dwx = SX(4), dAx = SX(5), dBx = SX(6), dCx = SX(7);
   const double dry = SY(0), dpy = SY(1), duy = SY(2), dvy = SY(3), dwy = SY(4), dAy = SY(5), dBy = SY(6), dCy = SY(7);
   double sr0, su0, sv0, sw0, sp0, sA0, sB0, sC0;
@@ -559,10 +574,11 @@ struct EmfIn
   QView          q;
   const double * Q2;
   const double * S[3];
-  const double * sF;
+  const double * sF; // face-B half-step source, added to the edge face values under HANCOCK only
   long           n1;
   int            ncS;
   Gas<double>    g;
+  int            hancock;
 };
 
 __device__ inline double
@@ -602,8 +618,11 @@ edge_state_2d(const EmfIn & in, const Idx & I, int is, int js, int o, int kind, 
   }
   double A = in.q(ia, ja, 0, 5, o);
   double B = in.q(ib, jb, 0, 6, o);
-  A += in.sF[(long(o) * 2 + 0) * in.n1 + I.gcell(1, ia, ja, 0)];
-  B += in.sF[(long(o) * 2 + 1) * in.n1 + I.gcell(1, ib, jb, 0)];
+  if (in.hancock) // ComputeEmfAndStoreFunctor.cpp:189-193
+  {
+    A += in.sF[(long(o) * 2 + 0) * in.n1 + I.gcell(1, ia, ja, 0)];
+    B += in.sF[(long(o) * 2 + 1) * in.n1 + I.gcell(1, ib, jb, 0)];
+  }
   const double dAy = qslope(in, ia, ja, 0, 5, o, 1);
   const double dBx = qslope(in, ib, jb, 0, 6, o, 0);
   const double fx = double(sx), fy = double(sy);
@@ -642,8 +661,11 @@ edge_state_3d(const EmfIn & in, const Idx & I, int is, int js, int ks, int o, in
   }
   double B0 = in.q(p0[0], p0[1], p0[2], 5 + d0, o);
   double B1 = in.q(p1[0], p1[1], p1[2], 5 + d1, o);
-  B0 += in.sF[(long(o) * 3 + d0) * in.n1 + I.gcell(1, p0[0], p0[1], p0[2])];
-  B1 += in.sF[(long(o) * 3 + d1) * in.n1 + I.gcell(1, p1[0], p1[1], p1[2])];
+  if (in.hancock) // ComputeEmfAndStoreFunctor.cpp:297-301
+  {
+    B0 += in.sF[(long(o) * 3 + d0) * in.n1 + I.gcell(1, p0[0], p0[1], p0[2])];
+    B1 += in.sF[(long(o) * 3 + d1) * in.n1 + I.gcell(1, p1[0], p1[1], p1[2])];
+  }
   const double dB0d1 = qslope(in, p0[0], p0[1], p0[2], 5 + d0, o, d1);
   const double dB1d0 = qslope(in, p1[0], p1[1], p1[2], 5 + d1, o, d0);
   const double * S0 = in.S[d0];
@@ -676,7 +698,7 @@ edge_state_3d(const EmfIn & in, const Idx & I, int is, int js, int ks, int o, in
 __global__ void __launch_bounds__(kThreads)
 k_emf(DevMesh m, Gas<double> g, double scaling, double dt, const double * __restrict__ Qp,
       const double * __restrict__ Q2, const double * __restrict__ Sx, const double * __restrict__ Sy,
-      const double * __restrict__ Sz, const double * __restrict__ sF, double * __restrict__ emf)
+      const double * __restrict__ Sz, const double * __restrict__ sF, int hancock, double * __restrict__ emf)
 {
   const Idx  I{ m.dim, m.b };
   const int  b = m.b, dim = m.dim;
@@ -688,7 +710,7 @@ k_emf(DevMesh m, Gas<double> g, double scaling, double dt, const double * __rest
   const long ci = t - long(o) * ne;
   const int  n = b + 1;
   const int  i = int(ci % n), j = int((ci / n) % n), k = dim == 3 ? int(ci / (long(n) * n)) : 0;
-  EmfIn      in{ QView{ Qp, I, I.gcells(2) }, Q2, { Sx, Sy, Sz }, sF, I.gcells(1), dim == 3 ? 3 : 2, g };
+  EmfIn      in{ QView{ Qp, I, I.gcells(2) }, Q2, { Sx, Sy, Sz }, sF, I.gcells(1), dim == 3 ? 3 : 2, g, hancock };
   const double dtdx = dt / dx_of(m, scaling, o);
   double       qe[4][8];
   if (dim == 2)
@@ -1082,7 +1104,8 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
                      c.st.abort_when_negative_eint, U, B, c.ps.Bg, c.ps.Q, c.red + 1);
   hipLaunchKernelGGL(k_slopes, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, g.slope_type, c.ps.Q, c.ps.S[0],
                      c.ps.S[1], c.ps.S[2]);
-  hipLaunchKernelGGL(k_predictor, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, g.gamma0, dt, c.ps.Q,
+  const int rk2 = c.st.time_integrator == 1;
+  hipLaunchKernelGGL(k_predictor, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, g.gamma0, dt, rk2, c.ps.Q,
                      c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.Q2);
   hipLaunchKernelGGL(k_elec_field, grid_of(nq * S.gcells(2)), kThreads, 0, s, m, c.ps.Q, c.ps.E);
   hipLaunchKernelGGL(k_sfacemag, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, dt, c.ps.E, c.ps.sF);
@@ -1093,7 +1116,7 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
     hipLaunchKernelGGL(k_update, grid_of(nq * S.cells()), kThreads, 0, s, m, dir, c.ps.Fl, U2, B2);
   }
   hipLaunchKernelGGL(k_emf, grid_of(nq * S.edge_cells()), kThreads, 0, s, m, g, sc, dt, c.ps.Q, c.ps.Q2,
-                     c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.sF, c.ps.emf);
+                     c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.sF, rk2 ? 0 : 1, c.ps.emf);
   MHDCT_HIP(hipMemcpyAsync(c.ps.emf2, c.ps.emf, size_t(nq) * S.ncomp_e() * S.edge_cells() * sizeof(double),
                            hipMemcpyDeviceToDevice, s));
   if (m.nfix > 0)
@@ -1129,6 +1152,37 @@ launch_inv_dt(Context & c, const double * U, const double * B, hipStream_t s)
   else
     hipLaunchKernelGGL(k_inv_dt, grid_of(nq * S.cells()), kThreads, 0, s, c.dm, gas_of(c.st), c.st.scaling, U, B,
                        c.red);
+  MHDCT_HIP(hipGetLastError());
+  return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// RK2-SSP stage combination: y = c0 * x + c1 * y over n values (SolverGodunovMHD.cpp:583-588,
+// 617-622 call kalypsso::core's LinearCombination on every block of U and of Bface).  Two
+// products and one sum, in this order, each one IEEE operation.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_linear_combination(long n, double c0, double c1, const double * __restrict__ x, double * __restrict__ y)
+{
+  for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += long(gridDim.x) * blockDim.x)
+    y[t] = c0 * x[t] + c1 * y[t];
+}
+
+int
+launch_linear_combination(Context & c, const double * U, double * U_RK, const double * B, double * B_RK, double c0,
+                          double c1, hipStream_t s)
+{
+  const Shapes S{ c.dim, c.b };
+  const long   nt = c.dm.ntotal;
+  if (nt == 0)
+    return 0;
+  const long nu = nt * 5 * S.cells(), nb = nt * S.fper();
+  const long cap = 8192; // a grid-stride loop over at most 8192 workgroups
+  hipLaunchKernelGGL(k_linear_combination, dim3(unsigned(std::min(cap, (nu + kThreads - 1) / kThreads))), kThreads,
+                     0, s, nu, c0, c1, U, U_RK);
+  hipLaunchKernelGGL(k_linear_combination, dim3(unsigned(std::min(cap, (nb + kThreads - 1) / kThreads))), kThreads,
+                     0, s, nb, c0, c1, B, B_RK);
   MHDCT_HIP(hipGetLastError());
   return 0;
 }

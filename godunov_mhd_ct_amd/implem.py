@@ -18,12 +18,21 @@ IMPL_FUSED = 11
 IMPL_FUSED_FP32 = 12
 
 RIEMANN = {"UNKNOWN": 0, "APPROX": 1, "LLF": 2, "HLL": 3, "HLLC": 4, "HLLD": 5}
+# [run] time_integrator (SolverGodunovMHD.cpp:494, 551); RK3_SSP is not accepted, as in the
+# reference (SolverGodunovMHD.cpp:174-177)
+TIME_INTEGRATOR = {"HANCOCK": 0, "RK2_SSP": 1}
+# stage coefficients (c0, c1) of y = c0 * x + c1 * y after each forward-Euler stage of the
+# two-stage SSP Runge-Kutta scheme (Shu & Osher 1988): u1 = u + dt L(u),
+# u_new = 1/2 u + 1/2 (u1 + dt L(u1)) (SolverGodunovMHD.cpp:583-588, 617-622)
+RK2_SSP_STAGE1_COEFS = (0.0, 1.0)
+RK2_SSP_STAGE2_COEFS = (0.5, 0.5)
 PROLONGATION = {"SIMPLE_COPY": 0, "EXTRAPOLATE_LINEAR_MINMOD": 1}
 
 
 def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling_factor=1.0,
-                   riemann="HLLD", prolongation="SIMPLE_COPY", abort_when_negative_eint=False) -> Settings:
-    """Defaults of HydroSettings (models/HydroSettings.cpp:19-30)."""
+                   riemann="HLLD", prolongation="SIMPLE_COPY", abort_when_negative_eint=False,
+                   time_integrator="HANCOCK") -> Settings:
+    """Defaults of HydroSettings (models/HydroSettings.cpp:19-30), plus [run] time_integrator."""
     s = Settings()
     s.gamma0 = float(gamma0)
     s.cfl = float(cfl)
@@ -34,6 +43,12 @@ def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-
     s.riemann = RIEMANN[riemann] if isinstance(riemann, str) else int(riemann)
     s.prolongation = PROLONGATION[prolongation] if isinstance(prolongation, str) else int(prolongation)
     s.abort_when_negative_eint = int(bool(abort_when_negative_eint))
+    if isinstance(time_integrator, str):
+        if time_integrator not in TIME_INTEGRATOR:
+            raise MhdctError(f"time_integrator {time_integrator!r} is not supported (HANCOCK or RK2_SSP)")
+        s.time_integrator = TIME_INTEGRATOR[time_integrator]
+    else:
+        s.time_integrator = int(time_integrator)
     return s
 
 
@@ -110,6 +125,12 @@ class GodunovImplemHIP:
     def fill_outside(self, U, B, stream=None):
         self._check(lib().mhdct_fill_outside(self._h, _ptr(U), _ptr(B), C.c_void_p(_stream(stream))),
                     "fill_outside")
+
+    def linear_combination(self, U, U_RK, B, B_RK, coefs, stream=None):
+        """U_RK = c0 U + c1 U_RK, B_RK = c0 B + c1 B_RK (LinearCombination of the RK2-SSP driver)."""
+        c0, c1 = coefs
+        self._check(lib().mhdct_linear_combination(self._h, _ptr(U), _ptr(U_RK), _ptr(B), _ptr(B_RK), float(c0),
+                                                   float(c1), C.c_void_p(_stream(stream))), "linear_combination")
 
 
 def create_godunov_implem(settings: Settings, dim: int, block_size: int, implementation_version: int):

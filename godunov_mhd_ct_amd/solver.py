@@ -1,16 +1,23 @@
-"""SolverGodunovMHD counterpart: owns U, U2, Bface, Bface2 in HBM and drives the step.
+"""SolverGodunovMHD counterpart: owns U, U2, Bface, Bface2 (and U_RK, Bface_RK under RK2_SSP)
+in HBM and drives the step.
 
-One step follows SolverGodunovMHD::next_iteration_impl with the HANCOCK integrator
-(SolverGodunovMHD.cpp:409-484, 490-549): compute_dt, fill the outside blocks, do_time_step
-(with U2 == U, B2 == B on entry), then the new state becomes current.  The reference copies
-U2 back into U; here the buffers swap roles and the copy moves to the start of the next step
-(the same traffic, the same contract).
+One step follows SolverGodunovMHD::next_iteration_impl (SolverGodunovMHD.cpp:409-484):
+
+- HANCOCK (godunov_unsplit, :494-549): compute_dt, fill the outside blocks, do_time_step
+  (with U2 == U, B2 == B on entry), then the new state becomes current.
+- RK2_SSP (:551-647): fill the outside blocks, U_RK = U, do_time_step(U -> U_RK), combine with
+  the stage-1 coefficients, fill U_RK's outside blocks, U2 = U_RK, do_time_step(U_RK -> U2),
+  U2 = 1/2 U + 1/2 U2 (stage-2 coefficients).
+
+The reference copies U2 back into U; here the buffers swap roles and the copy moves to the
+start of the next step (the same traffic, the same contract).
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .implem import IMPL_PLAIN, GodunovImplemHIP, hydro_settings
+from .implem import (IMPL_PLAIN, RK2_SSP_STAGE1_COEFS, RK2_SSP_STAGE2_COEFS, TIME_INTEGRATOR, GodunovImplemHIP,
+                     hydro_settings)
 from .mesh import Mesh
 
 
@@ -33,6 +40,10 @@ class SolverGodunovMHD:
         self.B = torch.from_numpy(np.ascontiguousarray(B, dtype=np.float64).reshape(-1)).to(device)
         self.U2 = self.U.clone()
         self.B2 = self.B.clone()
+        self.rk2 = self.settings.time_integrator == TIME_INTEGRATOR["RK2_SSP"]
+        if self.rk2:
+            self.U_RK = self.U.clone()
+            self.B_RK = self.B.clone()
         self.t = 0.0
         self.iteration = 0
         self.dt = None
@@ -47,9 +58,12 @@ class SolverGodunovMHD:
             self.implem.fill_outside(self.U, self.B)
         if dt is None:
             dt = self.compute_dt()
-        self.U2.copy_(self.U)
-        self.B2.copy_(self.B)
-        self.implem.do_time_step(self.U, self.U2, self.B, self.B2, dt)
+        if self.rk2:
+            self._rk2_stages(dt)
+        else:
+            self.U2.copy_(self.U)
+            self.B2.copy_(self.B)
+            self.implem.do_time_step(self.U, self.U2, self.B, self.B2, dt)
         self.U, self.U2 = self.U2, self.U
         self.B, self.B2 = self.B2, self.B
         self.t += dt
@@ -57,6 +71,20 @@ class SolverGodunovMHD:
         # SolverGodunovMHD.cpp:482
         self.total_num_cell_updates += self.mesh.num_leaves * self.mesh.cells_per_block
         return dt
+
+    def _rk2_stages(self, dt: float):
+        # SolverGodunovMHD.cpp:555-622, after the outside fill of U
+        im = self.implem
+        self.U_RK.copy_(self.U)
+        self.B_RK.copy_(self.B)
+        im.do_time_step(self.U, self.U_RK, self.B, self.B_RK, dt)
+        im.linear_combination(self.U, self.U_RK, self.B, self.B_RK, RK2_SSP_STAGE1_COEFS)
+        if self.mesh.num_outside:
+            im.fill_outside(self.U_RK, self.B_RK)
+        self.U2.copy_(self.U_RK)
+        self.B2.copy_(self.B_RK)
+        im.do_time_step(self.U_RK, self.U2, self.B_RK, self.B2, dt)
+        im.linear_combination(self.U, self.U2, self.B, self.B2, RK2_SSP_STAGE2_COEFS)
 
     def state(self):
         nt = self.mesh.num_total

@@ -55,7 +55,15 @@ typedef struct mhdct_settings
   int    riemann;        /* RiemannSolverType (models/riemann_solver_types.h): 5 = HLLD, the only one supported */
   int    prolongation;   /* [amr] cell_centered_prolongation_type: 0 = SIMPLE_COPY, the only one supported */
   int    abort_when_negative_eint; /* [hydro] abort_when_negative_eint: report eint < 0 as an error */
+  int    time_integrator; /* [run] time_integrator: MHDCT_TIME_HANCOCK or MHDCT_TIME_RK2_SSP */
 } mhdct_settings;
+
+/* [run] time_integrator values (SolverGodunovMHD.cpp:494, 551; RK3_SSP is rejected, as the
+ * reference aborts on it, SolverGodunovMHD.cpp:174-177).  Under RK2_SSP, do_time_step is one
+ * forward-Euler stage: no half-step predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:396-407)
+ * and no face-B source in the edge states (ComputeEmfAndStoreFunctor.cpp:189-193, 297-301). */
+#define MHDCT_TIME_HANCOCK 0
+#define MHDCT_TIME_RK2_SSP 1
 
 /* implementation_version values ([hydro] implementation_version, SolverGodunovMHD.cpp:76) */
 #define MHDCT_IMPL_PLAIN 10 /* one kernel per stage, drop-in for GodunovImplemV0 */
@@ -102,6 +110,12 @@ int mhdct_do_time_step(mhdct_ctx * ctx, const double * U, double * U2, const dou
 int mhdct_compute_dt(mhdct_ctx * ctx, const double * U, const double * B, double * dt_out, void * stream);
 /* Fill the outside blocks from their sources by the border conditions (no-op when periodic). */
 int mhdct_fill_outside(mhdct_ctx * ctx, double * U, double * B, void * stream);
+/* U_RK = c0 * U + c1 * U_RK and B_RK = c0 * B + c1 * B_RK over every block (owned and outside),
+ * each value as two products and a sum.  Replaces LinearCombination<dim>::apply of the RK2-SSP
+ * driver (SolverGodunovMHD.cpp:583-588, 617-622), whose stage coefficients are (0, 1) and
+ * (1/2, 1/2).  Asynchronous on `stream`. */
+int mhdct_linear_combination(mhdct_ctx * ctx, const double * U, double * U_RK, const double * B, double * B_RK,
+                             double c0, double c1, void * stream);
 const char * mhdct_last_error(const mhdct_ctx * ctx);
 void         mhdct_destroy(mhdct_ctx * ctx);
 

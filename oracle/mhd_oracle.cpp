@@ -20,6 +20,9 @@
 // Parity status per function (see tests/test_oracle_*.py for the checks):
 //   compute_primitives, slopes, predictor, E, sFaceMag, HLLD, 2D-HLLD EMF, update
 //   stages, dt          : pinned by closed forms / invariants / brute force in tests.
+//   RK2_SSP stage (no predictor, no face-B source at the edges) and the stage combination:
+//   pinned by the closed-form forward-Euler MUSCL stage of a 1D advected contact and by the
+//   second-order time convergence of the two-stage scheme (tests/test_oracle_rk2.py).
 //   AMR ghost fill, flux and EMF correction (core semantics): parity unpinned against
 //   the true reference (kalypsso::core absent); pinned by invariants only (div B,
 //   conservation, shared-face equality, uniform-state preservation).
@@ -89,6 +92,9 @@ struct Settings
   double smallp = 0; // smallc^2/gamma0
   double scaling = 1.0;
   int    abort_when_negative_eint = 0;
+  // [run] time_integrator: 0 HANCOCK, 1 RK2_SSP (kalypsso::core's TimeIntegrator; read by the
+  // functors through TimeIntegratorConfig::get_time_integrator)
+  int    time_integrator = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1327,8 +1333,9 @@ public:
     }
   }
 
-  // ComputeUpdatedPrimitiveVariablesFunctor::update_primitive_variables (.cpp:284-546),
-  // HANCOCK, no gravity, no viscosity.
+  // ComputeUpdatedPrimitiveVariablesFunctor::update_primitive_variables (.cpp:284-546), no
+  // gravity, no viscosity.  HANCOCK: the half-step predictor; RK2_SSP: q2 = the cell-centred
+  // state (.cpp:396-407, 531-542).
   void
   compute_primitives_predictor(double dt)
   {
@@ -1355,6 +1362,18 @@ public:
       const double A = (q(i, j, k, IAL, o) + q(i, j, k, IAR, o)) * 0.5;
       const double B = (q(i, j, k, IBL, o) + q(i, j, k, IBR, o)) * 0.5;
       const double C = (q(i, j, k, ICL, o) + q(i, j, k, ICR, o)) * 0.5;
+      if (s.time_integrator == 1)
+      {
+        q2(i, j, k, ID, o) = r;
+        q2(i, j, k, IU, o) = u;
+        q2(i, j, k, IV, o) = v;
+        q2(i, j, k, IW, o) = w;
+        q2(i, j, k, IP, o) = p;
+        q2(i, j, k, IA, o) = A;
+        q2(i, j, k, IB, o) = B;
+        q2(i, j, k, IC, o) = C;
+        continue;
+      }
       const double drx = sl(IX, i, j, k, ID, o), dpx = sl(IX, i, j, k, IP, o);
       const double dux = sl(IX, i, j, k, IU, o), dvx = sl(IX, i, j, k, IV, o);
       const double dwx = sl(IX, i, j, k, IW, o), dAx = sl(IX, i, j, k, IA, o);
@@ -1761,8 +1780,11 @@ public:
     }
     double A = q(ia, ja, 0, IAL, o);
     double B = q(ib, jb, 0, IBL, o);
-    A += sfm(ia, ja, 0, IX, o);
-    B += sfm(ib, jb, 0, IY, o);
+    if (s.time_integrator == 0) // HANCOCK only (ComputeEmfAndStoreFunctor.cpp:189-193)
+    {
+      A += sfm(ia, ja, 0, IX, o);
+      B += sfm(ib, jb, 0, IY, o);
+    }
     const double dAy = limited_slope_q(ia, ja, 0, IA, o, IY);
     const double dBx = limited_slope_q(ib, jb, 0, IB, o, IX);
     double       dqX[8], dqY[8];
@@ -1807,8 +1829,11 @@ public:
     }
     double B0 = q(ijk0[0], ijk0[1], ijk0[2], IA + dir0, o);
     double B1 = q(ijk1[0], ijk1[1], ijk1[2], IA + dir1, o);
-    B0 += sfm(ijk0[0], ijk0[1], ijk0[2], dir0, o);
-    B1 += sfm(ijk1[0], ijk1[1], ijk1[2], dir1, o);
+    if (s.time_integrator == 0) // HANCOCK only (ComputeEmfAndStoreFunctor.cpp:297-301)
+    {
+      B0 += sfm(ijk0[0], ijk0[1], ijk0[2], dir0, o);
+      B1 += sfm(ijk1[0], ijk1[1], ijk1[2], dir1, o);
+    }
     const double dB0d1 = limited_slope_q(ijk0[0], ijk0[1], ijk0[2], IA + dir0, o, dir1);
     const double dB1d0 = limited_slope_q(ijk1[0], ijk1[1], ijk1[2], IA + dir1, o, dir0);
     double       dq0[8], dq1[8];
@@ -2342,7 +2367,8 @@ public:
 extern "C"
 {
 
-  // settings: gamma0, cfl, slope_type, smallr, smallc, scaling, abort_when_negative_eint
+  // settings: gamma0, cfl, slope_type, smallr, smallc, scaling, abort_when_negative_eint,
+  // time_integrator (0 HANCOCK, 1 RK2_SSP)
   void *
   oracle_create(int dim, int b, const double * settings, int nowned, int noutside,
                 const int * levels, const int * coords, const int * nbrick, const int * periodic,
@@ -2356,6 +2382,7 @@ extern "C"
     S->s.smallc = settings[4];
     S->s.scaling = settings[5];
     S->s.abort_when_negative_eint = settings[6] != 0.0;
+    S->s.time_integrator = int(settings[7]);
     S->m.dim = dim;
     S->m.b = b;
     S->m.nowned = nowned;
@@ -2410,6 +2437,25 @@ extern "C"
   oracle_fill_outside(void * h, double * U, double * B)
   {
     return static_cast<oracle::Solver *>(h)->fill_outside(U, B);
+  }
+
+  // LinearCombination<dim>::apply of the RK2-SSP driver (SolverGodunovMHD.cpp:583-588,
+  // 617-622).  The functor lives in kalypsso::core, which is not in the reference tree (a CMake
+  // target, kalypsso::core, with no pinned version: CMakeLists.txt:76).  Restated from its call
+  // sites and the two-stage SSP Runge-Kutta scheme it implements (Shu & Osher 1988): the result
+  // is written to the second array, y = c0 * x + c1 * y, over every block (num_octants =
+  // get_nb_total()) of the cell array and of the face array.
+  void
+  oracle_linear_combination(void * h, const double * U, double * U_RK, const double * B, double * B_RK, double c0,
+                            double c1)
+  {
+    auto *     S = static_cast<oracle::Solver *>(h);
+    const long nt = long(S->m.nowned) + S->m.noutside;
+    const long nu = nt * 5 * S->lay.cells(), nb = nt * S->lay.fper();
+    for (long i = 0; i < nu; ++i)
+      U_RK[i] = c0 * U[i] + c1 * U_RK[i];
+    for (long i = 0; i < nb; ++i)
+      B_RK[i] = c0 * B[i] + c1 * B_RK[i];
   }
 
   const char *

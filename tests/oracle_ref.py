@@ -40,6 +40,8 @@ def lib():
         L.oracle_compute_inv_dt.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.oracle_fill_outside.restype = C.c_int
         L.oracle_fill_outside.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        L.oracle_linear_combination.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
+                                                P(C.c_double), C.c_double, C.c_double]
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_last_error.argtypes = [C.c_void_p]
         L.oracle_riemann_hlld.argtypes = [P(C.c_double)] * 4
@@ -63,9 +65,17 @@ def iptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_int)), a
 
 
+TIME_INTEGRATOR = {"HANCOCK": 0, "RK2_SSP": 1}
+# SSP-RK2 stage coefficients (c0, c1) of y = c0 x + c1 y (Shu & Osher 1988; call sites
+# SolverGodunovMHD.cpp:583-588, 617-622)
+RK2_STAGE1 = (0.0, 1.0)
+RK2_STAGE2 = (0.5, 0.5)
+
+
 def settings_array(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling=1.0,
-                   abort_eint=0):
-    return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint], dtype=np.float64)
+                   abort_eint=0, time_integrator="HANCOCK"):
+    ti = TIME_INTEGRATOR[time_integrator] if isinstance(time_integrator, str) else int(time_integrator)
+    return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint, ti], dtype=np.float64)
 
 
 class OracleSolver:
@@ -115,14 +125,37 @@ class OracleSolver:
     def fill_outside(self, U, B):
         lib().oracle_fill_outside(self._h, dptr(U), dptr(B))
 
+    def linear_combination(self, U, U_RK, B, B_RK, coefs):
+        lib().oracle_linear_combination(self._h, dptr(U), dptr(U_RK), dptr(B), dptr(B_RK), float(coefs[0]),
+                                        float(coefs[1]))
+
+    @property
+    def rk2(self):
+        return int(self.settings[7]) == TIME_INTEGRATOR["RK2_SSP"]
+
     def step(self, U, B, dt=None):
-        """One solver step (SolverGodunovMHD::next_iteration_impl, HANCOCK): dt, outside
-        fill, U2 = U, B2 = B, do_time_step; returns (U2, B2, dt)."""
+        """One solver step (SolverGodunovMHD::next_iteration_impl): dt, outside fill, then
+        HANCOCK (godunov_unsplit, SolverGodunovMHD.cpp:494-549): U2 = U, B2 = B, do_time_step;
+        or RK2_SSP (:551-647): U_RK = U, do_time_step(U -> U_RK), stage-1 combination, outside
+        fill of U_RK, U2 = U_RK, do_time_step(U_RK -> U2), stage-2 combination into U2.
+        Returns (U2, B2, dt); U and B are left as the outside fill made them."""
         if self.noutside:
             self.fill_outside(U, B)
         if dt is None:
             dt = self.compute_dt(U, B)
-        U2 = U.copy()
-        B2 = B.copy()
-        self.do_time_step(U, U2, B, B2, dt)
+        if not self.rk2:
+            U2 = U.copy()
+            B2 = B.copy()
+            self.do_time_step(U, U2, B, B2, dt)
+            return U2, B2, dt
+        U_RK = U.copy()
+        B_RK = B.copy()
+        self.do_time_step(U, U_RK, B, B_RK, dt)
+        self.linear_combination(U, U_RK, B, B_RK, RK2_STAGE1)
+        if self.noutside:
+            self.fill_outside(U_RK, B_RK)
+        U2 = U_RK.copy()
+        B2 = B_RK.copy()
+        self.do_time_step(U_RK, U2, B_RK, B2, dt)
+        self.linear_combination(U, U2, B, B2, RK2_STAGE2)
         return U2, B2, dt

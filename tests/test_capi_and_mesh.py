@@ -113,3 +113,19 @@ def test_error_paths_return_codes():
     with pytest.raises(MhdctError):
         m.adapt(np.zeros(m.num_leaves + 3, dtype=np.int8))
     assert lib().mhdct_last_error(None) is not None
+
+
+def test_settings_struct_and_time_integrator():
+    # the ctypes mirror of mhdct_settings has the header's fields in the header's order
+    from godunov_mhd_ct_amd._lib import Settings
+    from godunov_mhd_ct_amd.implem import hydro_settings
+
+    txt = HEADER.read_text()
+    body = re.search(r"typedef struct mhdct_settings\s*\{(.*?)\}\s*mhdct_settings;", txt, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"\b(?:double|int)\s+([a-z0-9_]+)\s*;", body)
+    assert fields == [f[0] for f in Settings._fields_]
+    assert hydro_settings(time_integrator="RK2_SSP").time_integrator == 1
+    assert hydro_settings().time_integrator == 0
+    with pytest.raises(MhdctError, match="RK3_SSP"):
+        hydro_settings(time_integrator="RK3_SSP")  # the reference aborts on it (SolverGodunovMHD.cpp:174-177)
