@@ -33,6 +33,19 @@ class Settings(C.Structure):
     ]
 
 
+class AmrParams(C.Structure):
+    """mhdct_amr_params (include/mhdct.h): the [amr] keys of the refinement criterion."""
+
+    _fields_ = [
+        ("level_min", C.c_int),
+        ("level_max", C.c_int),
+        ("epsilon_refine", C.c_double),
+        ("epsilon_coarsen", C.c_double),
+        ("epsilon_lohner", C.c_double),
+        ("ivar", C.c_int),
+    ]
+
+
 # exported symbols and their signatures (also checked by tests/test_capi_symbols.py)
 SIGNATURES = {
     "mhdct_mesh_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
@@ -73,6 +86,14 @@ SIGNATURES = {
     "mhdct_batch_set_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "mhdct_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
+    "mhdct_refine_flags": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AmrParams), C.POINTER(C.c_int8),
+                                     C.POINTER(C.c_double), C.c_void_p]),
+    "mhdct_mesh_clone": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "mhdct_remap_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "mhdct_remap_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mhdct_remap_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mhdct_remap_last_error": (C.c_char_p, [C.c_void_p]),
+    "mhdct_remap_destroy": (None, [C.c_void_p]),
     "mhdct_batch_num_slots": (C.c_int, [C.c_void_p]),
     "mhdct_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "mhdct_batch_destroy": (None, [C.c_void_p]),

@@ -1,0 +1,604 @@
+// amr.hip -- the device side of the AMR cycle (SolverGodunovMHD::do_amr_cycle,
+// SolverGodunovMHD.cpp:263-319): the refinement indicator and flags of every owned block
+// (compute_refine_flags, :1094-1185), and the remap of U and face B from the leaves before an
+// adaptation to the leaves after it (map_userdata_after_adapt, :1191-1269).
+//
+// The reference delegates both to kalypsso::core (ComputeRefineFlags, UserDataRemapper), which
+// is not in its tree.  Restated here:
+//  * the indicator is Loehner's second-derivative error estimate (R. Loehner, "An adaptive
+//    finite element scheme for transient problems in CFD", CMAME 61, 1987), unsplit (with the
+//    mixed derivatives), with the noise filter epsilon_lohner, on one conserved variable of U;
+//    a block's indicator is the maximum over its cells;
+//  * cells: SIMPLE_COPY prolongation (a fine cell takes its parent cell's value) and
+//    restriction by the mean of the 2^d fine cells (summed in pairs, so that restricting a
+//    prolongation is exact);
+//  * face B: prolongation by injection on the parent's face planes, and inside each parent cell
+//    the mean of the opposite fine faces plus a minimum-norm divergence correction (every fine
+//    cell divergence free; k_prolong_faces); restriction by the mean of the 2^(d-1) fine faces,
+//    summed in pairs (the parent's flux is the fine fluxes' sum).  On a leaf's boundary plane
+//    across which an old leaf of the same level lies, that leaf's faces are kept, so shared
+//    faces stay single valued.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "amr.hpp"
+#include "common.hpp"
+
+namespace mhdct
+{
+
+namespace
+{
+
+constexpr int kThreads = 256;
+
+// conserved variable v of block o at cell c (at most one cell outside [0, b)^dim): the ghost
+// cell rule of the primitive conversion -- same level: copy; finer: mean of the 2^dim fine
+// cells (x fastest, then y, then z); coarser: the containing cell (SIMPLE_COPY)
+__device__ inline double
+u_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__ U, int v, int o, const int c[3])
+{
+  const int  b = I.b;
+  const long ncell = I.cells();
+  bool       inside = true;
+  for (int d = 0; d < I.dim; ++d)
+    inside = inside && c[d] >= 0 && c[d] < b;
+  if (inside)
+    return U[(long(o) * 5 + v) * ncell + I.cell(c[0], c[1], I.dim == 3 ? c[2] : 0)];
+  const Hit      h = neighbour(m, I, o, c);
+  const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+  if (h.rel > 0)
+  {
+    double s = 0;
+    for (int dk = 0; dk < (I.dim == 3 ? 2 : 1); ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di)
+          s += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+    return s / double(1 << I.dim);
+  }
+  return Un[I.cell(h.l[0], h.l[1], h.l[2])];
+}
+
+// Loehner's estimate at cell (i, j, k) of block o from the 3^dim stencil s[(dz+1)*9+(dy+1)*3+dx+1]:
+//   E^2 = sum_kl N_kl^2 / sum_kl D_kl^2
+//   N_kk = (u+ - u0) - (u0 - u-),  D_kk = |u+ - u0| + |u0 - u-| + eps (|u+| + 2|u0| + |u-|)
+//   N_kl = ((u++ - u+-) - (u-+ - u--)) / 4
+//   D_kl = (|u++ - u+-| + |u-+ - u--|) / 4 + eps (|u++| + |u+-| + |u-+| + |u--|) / 4   (k != l)
+// summed over k, l in dimension order (k outer), E = 0 where the denominator vanishes.
+__device__ inline double
+lohner(const double * s, int dim, double eps)
+{
+  auto at = [&](int dx, int dy, int dz) { return s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)]; };
+  auto off = [](int d, int sg, int * o3) { o3[d] += sg; };
+  double num = 0, den = 0;
+  for (int k = 0; k < dim; ++k)
+    for (int l = 0; l < dim; ++l)
+    {
+      if (k == l)
+      {
+        int p[3] = { 0, 0, 0 }, q[3] = { 0, 0, 0 };
+        off(k, 1, p);
+        off(k, -1, q);
+        const double up = at(p[0], p[1], p[2]), u0 = at(0, 0, 0), um = at(q[0], q[1], q[2]);
+        const double n = (up - u0) - (u0 - um);
+        const double d = fabs(up - u0) + fabs(u0 - um) + eps * (fabs(up) + 2.0 * fabs(u0) + fabs(um));
+        num += n * n;
+        den += d * d;
+      }
+      else
+      {
+        int pp[3] = { 0, 0, 0 }, pm[3] = { 0, 0, 0 }, mp[3] = { 0, 0, 0 }, mm[3] = { 0, 0, 0 };
+        off(k, 1, pp), off(l, 1, pp);
+        off(k, 1, pm), off(l, -1, pm);
+        off(k, -1, mp), off(l, 1, mp);
+        off(k, -1, mm), off(l, -1, mm);
+        const double upp = at(pp[0], pp[1], pp[2]), upm = at(pm[0], pm[1], pm[2]);
+        const double ump = at(mp[0], mp[1], mp[2]), umm = at(mm[0], mm[1], mm[2]);
+        const double n = ((upp - upm) - (ump - umm)) * 0.25;
+        const double d = (fabs(upp - upm) + fabs(ump - umm)) * 0.25 +
+                         eps * (fabs(upp) + fabs(upm) + fabs(ump) + fabs(umm)) * 0.25;
+        num += n * n;
+        den += d * d;
+      }
+    }
+  return den > 0.0 ? sqrt(num / den) : 0.0;
+}
+
+// One thread per owned cell; the block maximum through the bits of the non-negative double.
+__global__ void __launch_bounds__(kThreads)
+k_refine_indicator(DevMesh m, int ivar, double eps, const double * __restrict__ U, unsigned long long * emax)
+{
+  const Idx  I{ m.dim, m.b };
+  const long ncell = I.cells();
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool live = t < long(m.nowned) * ncell;
+  double     e = 0.0;
+  int        o = 0;
+  if (live)
+  {
+    o = int(t / ncell);
+    const long ci = t - long(o) * ncell;
+    const int  b = m.b;
+    const int  i = int(ci % b), j = int((ci / b) % b), k = m.dim == 3 ? int(ci / (long(b) * b)) : 0;
+    double     s[27];
+    for (int dz = (m.dim == 3 ? -1 : 0); dz <= (m.dim == 3 ? 1 : 0); ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+        {
+          const int c[3] = { i + dx, j + dy, k + dz };
+          s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] = u_ghosted(m, I, U, ivar, o, c);
+        }
+    e = lohner(s, m.dim, eps);
+    if (!(e >= 0.0)) // NaN state: flag it as a maximal indicator
+      e = __builtin_huge_val();
+  }
+  // when every workgroup lies inside one block, reduce over the wavefront first
+  if ((ncell % kThreads) == 0)
+  {
+    for (int w = 32; w > 0; w >>= 1)
+      e = fmax(e, __shfl_xor(e, w));
+    if ((threadIdx.x & 63) == 0 && live)
+      atomicMax(emax + o, static_cast<unsigned long long>(__double_as_longlong(e)));
+  }
+  else if (live)
+    atomicMax(emax + o, static_cast<unsigned long long>(__double_as_longlong(e)));
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_refine_flags(DevMesh m, AmrParams p, const unsigned long long * __restrict__ emax, int8_t * __restrict__ flags,
+               double * __restrict__ ind)
+{
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= m.nowned)
+    return;
+  const double e = __longlong_as_double(static_cast<long long>(emax[o]));
+  const int    l = m.level[o];
+  int8_t       f = 0;
+  if (l < p.level_min)
+    f = 1;
+  else if (l > p.level_max)
+    f = -1;
+  else if (e > p.epsilon_refine && l < p.level_max)
+    f = 1;
+  else if (e < p.epsilon_coarsen && l > p.level_min)
+    f = -1;
+  flags[o] = f;
+  ind[o] = e;
+}
+
+// ---------------------------------------------------------------------------------------------
+// remap: one thread per (new block, variable, cell) and per (new block, face)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_remap_cells(int dim, int b, int nnew, const RemapEntry * __restrict__ E, const double * __restrict__ Uo,
+              double * __restrict__ Un)
+{
+  const Idx  I{ dim, b };
+  const long ncell = I.cells();
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(nnew) * 5 * ncell)
+    return;
+  const int      nb = int(t / (5 * ncell));
+  const long     r = t - long(nb) * 5 * ncell;
+  const int      v = int(r / ncell);
+  const long     ci = r - long(v) * ncell;
+  const int      i = int(ci % b), j = int((ci / b) % b), k = dim == 3 ? int(ci / (long(b) * b)) : 0;
+  const RemapEntry e = E[nb];
+  double           val;
+  if (e.kind == kRemapCopy)
+    val = Uo[(long(e.src[0]) * 5 + v) * ncell + ci];
+  else if (e.kind == kRemapProlong)
+  {
+    const int ox = e.oct & 1, oy = (e.oct >> 1) & 1, oz = (e.oct >> 2) & 1;
+    const int pi = (ox * b + i) >> 1, pj = (oy * b + j) >> 1, pk = dim == 3 ? (oz * b + k) >> 1 : 0;
+    val = Uo[(long(e.src[0]) * 5 + v) * ncell + I.cell(pi, pj, pk)];
+  }
+  else
+  {
+    // fine global index 2i + di within the new block's span: child octant (2i >= b), local 2i mod b
+    const int cx = (2 * i >= b), cy = (2 * j >= b), cz = dim == 3 ? (2 * k >= b) : 0;
+    const int child = cx | (cy << 1) | (cz << 2);
+    const int fi = 2 * i - cx * b, fj = 2 * j - cy * b, fk = dim == 3 ? 2 * k - cz * b : 0;
+    const double * src = Uo + (long(e.src[child]) * 5 + v) * ncell;
+    // pairwise sums, x pairs first: the mean of 2^dim equal values is that value exactly, so
+    // restricting a prolongation gives back the parent bit for bit
+    auto row = [&](int dj, int dk) { return src[I.cell(fi, fj + dj, fk + dk)] + src[I.cell(fi + 1, fj + dj, fk + dk)]; };
+    double s = row(0, 0) + row(1, 0);
+    if (dim == 3)
+      s = s + (row(0, 1) + row(1, 1));
+    val = s / double(1 << dim);
+  }
+  Un[t] = val;
+}
+
+// face B of copied and restricted leaves, one thread per face (prolonged leaves: k_prolong_faces)
+__global__ void __launch_bounds__(kThreads)
+k_remap_faces(int dim, int b, int nnew, const RemapEntry * __restrict__ E, const double * __restrict__ Bo,
+              double * __restrict__ Bn)
+{
+  const Idx  I{ dim, b };
+  const long fper = I.fper();
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(nnew) * fper)
+    return;
+  const int        nb = int(t / fper);
+  const RemapEntry e = E[nb];
+  if (e.kind == kRemapProlong)
+    return;
+  long r = t - long(nb) * fper;
+  if (e.kind == kRemapCopy)
+  {
+    Bn[t] = Bo[long(e.src[0]) * fper + r];
+    return;
+  }
+  int c = 0;
+  while (c < 2 && r >= I.fsize(c))
+    r -= I.fsize(c), ++c;
+  const int  nx = c == 0 ? b + 1 : b, ny = c == 1 ? b + 1 : b;
+  const int  f[3] = { int(r % nx), int((r / nx) % ny), dim == 3 ? int(r / (long(nx) * ny)) : 0 };
+  const bool cellc = dim == 2 && c == 2; // 2D B_z is cell centred
+  // a boundary face plane shared with an old leaf of this level: that leaf's faces
+  if (!cellc && (f[c] == 0 || f[c] == b))
+  {
+    const int side = f[c] == b;
+    const int o = e.nb[2 * c + side];
+    if (o >= 0)
+    {
+      int g[3] = { f[0], f[1], f[2] };
+      g[c] = side ? 0 : b;
+      Bn[t] = Bo[long(o) * fper + I.face(c, g[0], g[1], g[2])];
+      return;
+    }
+  }
+  // the children's faces covering this face: along c the plane 2 f_c of the fine span (the high
+  // child holds the middle plane and the last one), across c the 2 x 2 (2D: 2) faces, summed
+  // in pairs
+  int ch[3] = { 0, 0, 0 }, fl[3] = { 0, 0, 0 };
+  for (int d = 0; d < dim; ++d)
+  {
+    ch[d] = 2 * f[d] >= b;
+    fl[d] = 2 * f[d] - ch[d] * b;
+  }
+  const double * F = Bo + long(e.src[ch[0] | (ch[1] << 1) | (ch[2] << 2)]) * fper;
+  double         val;
+  if (cellc)
+  {
+    const double s = (F[I.face(2, fl[0], fl[1], 0)] + F[I.face(2, fl[0] + 1, fl[1], 0)]) +
+                     (F[I.face(2, fl[0], fl[1] + 1, 0)] + F[I.face(2, fl[0] + 1, fl[1] + 1, 0)]);
+    val = s / 4.0;
+  }
+  else
+  {
+    // transverse directions t0 < t1 of component c (2D: one); pairs along t0 first
+    const int t0 = c == 0 ? 1 : 0;
+    const int t1 = c == 2 ? 1 : 2;
+    auto      pair = [&](int d1) {
+      int q[3] = { fl[0], fl[1], fl[2] };
+      if (dim == 3)
+        q[t1] += d1;
+      const double a = F[I.face(c, q[0], q[1], q[2])];
+      q[t0] += 1;
+      return a + F[I.face(c, q[0], q[1], q[2])];
+    };
+    const double s = dim == 3 ? pair(0) + pair(1) : pair(0);
+    val = s / double(1 << (dim - 1));
+  }
+  Bn[t] = val;
+}
+
+// Face B of a prolonged leaf, one thread per parent cell inside it: the 2^dim fine cells of that
+// parent cell get
+//  * on the parent cell's face planes: the parent face (injection), except on the leaf's own
+//    boundary planes across which an old leaf of this level lies, which keep that leaf's faces;
+//  * inside the parent cell: the mean of the two opposite outer fine faces of the same row,
+//    plus the minimum-norm correction that makes every fine cell divergence free.  With fine
+//    cells as the nodes of the cube graph Q_dim and the interior faces as its edges, the
+//    correction of the face between cells n0 (low) and n1 (high) is phi[n0] - phi[n1], where
+//    L phi = -div and L is the graph Laplacian; its Walsh functions w_s(n) = (-1)^|s & n| are
+//    eigenvectors of eigenvalue 2 |s|, so phi = -sum_{s != 0} (w_s . div) w_s / (2^dim 2 |s|).
+// Each thread writes the low and interior planes of its parent cell, and the high planes on the
+// leaf's high boundary, so every face is written once.  2D B_z is cell centred: copied.
+__global__ void __launch_bounds__(kThreads)
+k_prolong_faces(int dim, int b, int nprol, const int * __restrict__ prolonged, const RemapEntry * __restrict__ E,
+                const double * __restrict__ Bo, double * __restrict__ Bn)
+{
+  const Idx  I{ dim, b };
+  const long fper = I.fper();
+  const int  h = b / 2; // parent cells per leaf edge
+  const long per = dim == 3 ? long(h) * h * h : long(h) * h;
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(nprol) * per)
+    return;
+  const int        nbk = prolonged[t / per];
+  const long       r = t % per;
+  const RemapEntry e = E[nbk];
+  const int        pl[3] = { int(r % h), int((r / h) % h), dim == 3 ? int(r / (long(h) * h)) : 0 }; // local
+  const int        o3[3] = { e.oct & 1, (e.oct >> 1) & 1, (e.oct >> 2) & 1 };
+  int              pc[3]; // parent cell in the parent's grid
+  for (int d = 0; d < 3; ++d)
+    pc[d] = d < dim ? o3[d] * h + pl[d] : 0;
+  const double * P = Bo + long(e.src[0]) * fper;
+  double *       out = Bn + long(nbk) * fper;
+  const int      nc = 1 << dim;
+  // outer fine faces: ofc[c][side][m], m the transverse fine index (t0 bit 0, t1 bit 1)
+  double ofc[3][2][4];
+  for (int c = 0; c < dim; ++c)
+  {
+    const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+    for (int side = 0; side < 2; ++side)
+    {
+      const bool edge = pl[c] == (side ? h - 1 : 0);
+      const int  o = edge ? e.nb[2 * c + side] : -1;
+      for (int m = 0; m < (1 << (dim - 1)); ++m)
+      {
+        double v;
+        if (o >= 0)
+        {
+          int g[3] = { 0, 0, 0 };
+          g[c] = side ? 0 : b;
+          g[t0] = 2 * pl[t0] + (m & 1);
+          if (dim == 3)
+            g[t1] = 2 * pl[t1] + ((m >> 1) & 1);
+          v = Bo[long(o) * fper + I.face(c, g[0], g[1], g[2])];
+        }
+        else
+        {
+          int q[3] = { pc[0], pc[1], pc[2] };
+          q[c] += side;
+          v = P[I.face(c, q[0], q[1], q[2])];
+        }
+        ofc[c][side][m] = v;
+      }
+    }
+  }
+  // interior guesses: ifc[c][m] between the two fine cells of row m along c
+  double ifc[3][4];
+  for (int c = 0; c < dim; ++c)
+    for (int m = 0; m < (1 << (dim - 1)); ++m)
+      ifc[c][m] = 0.5 * (ofc[c][0][m] + ofc[c][1][m]);
+  // transverse row index m of fine cell n (bits x, y, z) for component c
+  auto row = [&](int c, int n) {
+    const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+    return ((n >> t0) & 1) | (dim == 3 ? (((n >> t1) & 1) << 1) : 0);
+  };
+  double div[8];
+  for (int n = 0; n < nc; ++n)
+  {
+    double sdiv = 0;
+    for (int c = 0; c < dim; ++c)
+    {
+      const int    m = row(c, n);
+      const bool   hi = (n >> c) & 1;
+      const double lo_f = hi ? ifc[c][m] : ofc[c][0][m];
+      const double hi_f = hi ? ofc[c][1][m] : ifc[c][m];
+      sdiv = sdiv + (hi_f - lo_f);
+    }
+    div[n] = sdiv;
+  }
+  double phi[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+  for (int sw = 1; sw < nc; ++sw)
+  {
+    double w = 0;
+    for (int n = 0; n < nc; ++n)
+      w = (__popc(sw & n) & 1) ? w - div[n] : w + div[n];
+    const double coef = w / double(nc * 2 * __popc(sw));
+    for (int n = 0; n < nc; ++n)
+      phi[n] = (__popc(sw & n) & 1) ? phi[n] + coef : phi[n] - coef;
+  }
+  for (int c = 0; c < dim; ++c)
+    for (int m = 0; m < (1 << (dim - 1)); ++m)
+    {
+      // the two cells of row m along c
+      const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+      const int n0 = ((m & 1) << t0) | (dim == 3 ? (((m >> 1) & 1) << t1) : 0);
+      const int n1 = n0 | (1 << c);
+      ifc[c][m] = ifc[c][m] + (phi[n0] - phi[n1]);
+    }
+  // write: local fine face coordinates of this parent cell's fine cells start at 2 pl
+  for (int c = 0; c < dim; ++c)
+  {
+    const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+    for (int m = 0; m < (1 << (dim - 1)); ++m)
+    {
+      int g[3] = { 2 * pl[0], 2 * pl[1], 2 * pl[2] };
+      g[t0] += m & 1;
+      if (dim == 3)
+        g[t1] += (m >> 1) & 1;
+      out[I.face(c, g[0], g[1], g[2])] = ofc[c][0][m];
+      g[c] += 1;
+      out[I.face(c, g[0], g[1], g[2])] = ifc[c][m];
+      if (pl[c] == h - 1)
+      {
+        g[c] += 1;
+        out[I.face(c, g[0], g[1], g[2])] = ofc[c][1][m];
+      }
+    }
+  }
+  if (dim == 2)
+  {
+    const double bz = P[I.face(2, pc[0], pc[1], 0)];
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di)
+        out[I.face(2, 2 * pl[0] + di, 2 * pl[1] + dj, 0)] = bz;
+  }
+}
+
+unsigned
+grid_of(long n)
+{
+  return unsigned((n + kThreads - 1) / kThreads);
+}
+
+} // namespace
+
+#define AMR_HIP(errs, x)                                                                           \
+  do                                                                                               \
+  {                                                                                                \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess)                                                                          \
+    {                                                                                              \
+      errs = std::string(#x) + ": " + hipGetErrorString(e_);                                       \
+      return -1;                                                                                   \
+    }                                                                                              \
+  } while (0)
+
+int
+launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
+                    hipStream_t s)
+{
+  const DevMesh & m = c.dm;
+  const long      n = m.nowned;
+  if (n == 0)
+    return 0;
+  const size_t need = size_t(n) * (2 * sizeof(unsigned long long) + 1) + 256;
+  if (c.amr_bytes < need)
+  {
+    if (c.amr_buf)
+      (void)hipFree(c.amr_buf);
+    c.amr_buf = nullptr;
+    c.amr_bytes = 0;
+    AMR_HIP(c.err, hipMalloc(&c.amr_buf, need));
+    c.amr_bytes = need;
+  }
+  auto *   emax = static_cast<unsigned long long *>(c.amr_buf);
+  double * ind = reinterpret_cast<double *>(emax + n);
+  auto *   fl = reinterpret_cast<int8_t *>(ind + n);
+  AMR_HIP(c.err, hipMemsetAsync(emax, 0, size_t(n) * sizeof(unsigned long long), s));
+  const Shapes S{ c.dim, c.b };
+  hipLaunchKernelGGL(k_refine_indicator, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner, U,
+                     emax);
+  hipLaunchKernelGGL(k_refine_flags, grid_of(n), kThreads, 0, s, m, p, emax, fl, ind);
+  AMR_HIP(c.err, hipGetLastError());
+  AMR_HIP(c.err, hipMemcpyAsync(flags_host, fl, size_t(n), hipMemcpyDeviceToHost, s));
+  if (ind_host)
+    AMR_HIP(c.err, hipMemcpyAsync(ind_host, ind, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
+  AMR_HIP(c.err, hipStreamSynchronize(s));
+  return 0;
+}
+
+bool
+remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry> & plan,
+           std::vector<int> & prolonged, int counts[3], std::string & err)
+{
+  if (old_mesh.dim != new_mesh.dim || old_mesh.b != new_mesh.b)
+  {
+    err = "remap: the meshes differ in dimension or block size";
+    return false;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (old_mesh.nbrick[d] != new_mesh.nbrick[d])
+    {
+      err = "remap: the meshes differ in brick size";
+      return false;
+    }
+  const int dim = old_mesh.dim;
+  std::unordered_map<uint64_t, int> old_index;
+  old_index.reserve(old_mesh.leaves.size() * 2);
+  for (size_t i = 0; i < old_mesh.leaves.size(); ++i)
+  {
+    const Leaf & L = old_mesh.leaves[i];
+    old_index[Mesh::key(L.level, L.x, L.y, L.z)] = int(i);
+  }
+  auto find = [&](int l, int x, int y, int z) -> int {
+    if (l < 0)
+      return -1;
+    const auto it = old_index.find(Mesh::key(l, x, y, z));
+    return it == old_index.end() ? -1 : it->second;
+  };
+  for (int d = 0; d < 3; ++d)
+    if (old_mesh.periodic[d] != new_mesh.periodic[d])
+    {
+      err = "remap: the meshes differ in periodicity";
+      return false;
+    }
+  // the old leaf of level l across face plane (c, side) of block (l, x, y, z), or -1
+  auto across = [&](int l, int x, int y, int z, int c, int side) -> int {
+    long q[3] = { x, y, z };
+    q[c] += side ? 1 : -1;
+    const long n = old_mesh.nblocks(l, c);
+    if (q[c] < 0 || q[c] >= n)
+    {
+      if (!old_mesh.periodic[c])
+        return -1;
+      q[c] = (q[c] + n) % n;
+    }
+    return find(l, int(q[0]), int(q[1]), int(q[2]));
+  };
+  counts[0] = counts[1] = counts[2] = 0;
+  prolonged.clear();
+  plan.assign(new_mesh.leaves.size(), RemapEntry{});
+  for (size_t n = 0; n < new_mesh.leaves.size(); ++n)
+  {
+    const Leaf & L = new_mesh.leaves[n];
+    RemapEntry & e = plan[n];
+    std::fill(e.src, e.src + 8, -1);
+    const int same = find(L.level, L.x, L.y, L.z);
+    if (same >= 0)
+    {
+      e.kind = kRemapCopy;
+      e.src[0] = same;
+      ++counts[0];
+      continue;
+    }
+    const int parent = find(L.level - 1, L.x >> 1, L.y >> 1, dim == 3 ? (L.z >> 1) : 0);
+    if (parent >= 0)
+    {
+      e.kind = kRemapProlong;
+      e.src[0] = parent;
+      e.oct = (L.x & 1) | ((L.y & 1) << 1) | (dim == 3 ? ((L.z & 1) << 2) : 0);
+      for (int c = 0; c < dim; ++c)
+        for (int side = 0; side < 2; ++side)
+          e.nb[2 * c + side] = across(L.level, L.x, L.y, L.z, c, side);
+      prolonged.push_back(int(n));
+      ++counts[1];
+      continue;
+    }
+    bool all = true;
+    for (int ch = 0; ch < (1 << dim) && all; ++ch)
+    {
+      const int s = find(L.level + 1, 2 * L.x + (ch & 1), 2 * L.y + ((ch >> 1) & 1),
+                         dim == 3 ? 2 * L.z + ((ch >> 2) & 1) : 0);
+      all = s >= 0;
+      e.src[ch] = s;
+    }
+    if (!all)
+    {
+      err = "remap: a new leaf is neither an old leaf, nor a child of one, nor the parent of old leaves "
+            "(more than one level of change in one cycle)";
+      return false;
+    }
+    e.kind = kRemapRestrict;
+    for (int c = 0; c < dim; ++c)
+      for (int side = 0; side < 2; ++side)
+        e.nb[2 * c + side] = across(L.level, L.x, L.y, L.z, c, side);
+    ++counts[2];
+  }
+  return true;
+}
+
+int
+launch_remap(int dim, int b, int nnew, const RemapEntry * plan_dev, int nprolonged, const int * prolonged_dev,
+             const double * Uo, const double * Bo, double * Un, double * Bn, hipStream_t s, std::string & err)
+{
+  if (nnew == 0)
+    return 0;
+  const long ncell = dim == 3 ? long(b) * b * b : long(b) * b;
+  const long fper = dim == 3 ? 3 * long(b + 1) * b * b : 2 * long(b + 1) * b + long(b) * b;
+  hipLaunchKernelGGL(k_remap_cells, grid_of(long(nnew) * 5 * ncell), kThreads, 0, s, dim, b, nnew, plan_dev, Uo, Un);
+  hipLaunchKernelGGL(k_remap_faces, grid_of(long(nnew) * fper), kThreads, 0, s, dim, b, nnew, plan_dev, Bo, Bn);
+  if (nprolonged > 0)
+  {
+    const long per = dim == 3 ? long(b / 2) * (b / 2) * (b / 2) : long(b / 2) * (b / 2);
+    hipLaunchKernelGGL(k_prolong_faces, grid_of(long(nprolonged) * per), kThreads, 0, s, dim, b, nprolonged,
+                       prolonged_dev, plan_dev, Bo, Bn);
+  }
+  AMR_HIP(err, hipGetLastError());
+  return 0;
+}
+
+} // namespace mhdct

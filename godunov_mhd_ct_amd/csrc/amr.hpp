@@ -1,0 +1,59 @@
+// amr.hpp -- the device side of the AMR cycle: refinement flags and the remap of U and face B
+// between two leaf sets (amr.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace mhdct
+{
+
+// [amr] keys of the refinement criterion (SolverGodunovMHD.cpp:1107-1113)
+struct AmrParams
+{
+  int    level_min;
+  int    level_max;
+  double epsilon_refine;
+  double epsilon_coarsen;
+  double epsilon_lohner;
+  int    ivar; // conserved variable of U the indicator reads (0 = rho)
+};
+
+enum
+{
+  kRemapCopy = 0,
+  kRemapProlong = 1,
+  kRemapRestrict = 2
+};
+
+// how one new owned leaf gets its data: a copy of an old leaf, a prolongation from its old
+// parent (child octant `oct`, x bit 0, y bit 1, z bit 2), or a restriction of its 2^dim old
+// children (src in octant order).  nb[2c + side]: the old leaf at the new leaf's level across
+// its low (side 0) or high (side 1) face plane along c, or -1; its faces on that plane are
+// taken as they are, so that a face plane shared by two leaves of one level keeps one value.
+struct RemapEntry
+{
+  int kind = kRemapCopy;
+  int oct = 0;
+  int src[8] = { -1, -1, -1, -1, -1, -1, -1, -1 };
+  int nb[6] = { -1, -1, -1, -1, -1, -1 };
+};
+
+// Indicator and flags of every owned block into host arrays (flags: +1 refine, -1 coarsen,
+// 0 keep; ind optional).  Synchronises `s`.
+int launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
+                        hipStream_t s);
+// Host plan for the owned leaves of new_mesh from those of old_mesh; counts = copied,
+// prolonged, restricted.
+// prolonged lists the new leaves of kind kRemapProlong.
+bool remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry> & plan,
+                std::vector<int> & prolonged, int counts[3], std::string & err);
+int  launch_remap(int dim, int b, int nnew, const RemapEntry * plan_dev, int nprolonged, const int * prolonged_dev,
+                  const double * Uo, const double * Bo, double * Un, double * Bn, hipStream_t s, std::string & err);
+
+} // namespace mhdct

@@ -18,6 +18,14 @@ using mhdct::Mesh;
 namespace
 {
 
+// the message of the last failed mhdct_remap_create on this thread
+std::string &
+remap_create_error()
+{
+  thread_local std::string e;
+  return e;
+}
+
 template <typename T>
 int
 upload(Context & c, T *& dst, const std::vector<T> & src)
@@ -415,6 +423,144 @@ extern "C"
     return mhdct::launch_linear_combination(c, U, U_RK, B, B_RK, c0, c1, static_cast<hipStream_t>(stream));
   }
 
+  int
+  mhdct_refine_flags(mhdct_ctx * h, const double * U, const mhdct_amr_params * p, int8_t * flags, double * indicator,
+                     void * stream)
+  {
+    if (!h || !p || !flags)
+      return -1;
+    Context & c = h->c;
+    if (!c.has_mesh)
+    {
+      c.err = "no mesh set";
+      return -1;
+    }
+    if (c.dm.nowned > 0 && !U)
+    {
+      c.err = "null state pointer";
+      return -1;
+    }
+    if (p->ivar < 0 || p->ivar > 4 || p->level_min < 0 || p->level_max < p->level_min || !(p->epsilon_lohner >= 0.0))
+    {
+      c.err = "refine flags: invalid parameters (ivar 0..4, 0 <= level_min <= level_max, epsilon_lohner >= 0)";
+      return -1;
+    }
+    const mhdct::AmrParams ap{ p->level_min, p->level_max, p->epsilon_refine, p->epsilon_coarsen, p->epsilon_lohner,
+                               p->ivar };
+    return mhdct::launch_refine_flags(c, U, ap, flags, indicator, static_cast<hipStream_t>(stream));
+  }
+
+  int
+  mhdct_mesh_clone(const mhdct_mesh * mesh, mhdct_mesh ** out)
+  {
+    if (!mesh || !out)
+      return -1;
+    *out = new (std::nothrow) mhdct_mesh(*mesh);
+    return *out ? 0 : -1;
+  }
+
+  int
+  mhdct_remap_create(const mhdct_mesh * old_mesh, const mhdct_mesh * new_mesh, mhdct_remap ** out)
+  {
+    if (!out)
+      return -1;
+    *out = nullptr;
+    if (!old_mesh || !new_mesh)
+    {
+      remap_create_error() = "null mesh";
+      return -1;
+    }
+    auto * r = new (std::nothrow) mhdct_remap();
+    if (!r)
+      return -1;
+    r->dim = new_mesh->m.dim;
+    r->b = new_mesh->m.b;
+    if (!mhdct::remap_plan(old_mesh->m, new_mesh->m, r->plan, r->prolonged, r->counts, remap_create_error()))
+    {
+      delete r;
+      return -1;
+    }
+    *out = r;
+    return 0;
+  }
+
+  int
+  mhdct_remap_counts(const mhdct_remap * r, int counts[3])
+  {
+    if (!r || !counts)
+      return -1;
+    for (int i = 0; i < 3; ++i)
+      counts[i] = r->counts[i];
+    return 0;
+  }
+
+  int
+  mhdct_remap_apply(mhdct_remap * r, const double * U_old, const double * B_old, double * U_new, double * B_new,
+                    void * stream)
+  {
+    if (!r)
+      return -1;
+    const int nnew = int(r->plan.size());
+    if (nnew == 0)
+      return 0;
+    if (!U_old || !B_old || !U_new || !B_new)
+    {
+      r->err = "null state pointer";
+      return -1;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+    {
+      r->err = "no device";
+      return -1;
+    }
+    if (r->dev && r->device != dev)
+    {
+      r->err = "the remap was first applied on another device";
+      return -1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!r->dev)
+    {
+      const size_t bytes = r->plan.size() * sizeof(mhdct::RemapEntry);
+      const size_t pbytes = (r->prolonged.empty() ? 1 : r->prolonged.size()) * sizeof(int);
+      if (hipMalloc(&r->dev, bytes) != hipSuccess || hipMalloc(&r->dev_prolonged, pbytes) != hipSuccess)
+      {
+        r->err = "hipMalloc of the remap plan failed";
+        return -1;
+      }
+      r->device = dev;
+      // synchronous: the host plan is pageable
+      if (hipMemcpy(r->dev, r->plan.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+          (!r->prolonged.empty() && hipMemcpy(r->dev_prolonged, r->prolonged.data(), r->prolonged.size() * sizeof(int),
+                                              hipMemcpyHostToDevice) != hipSuccess))
+      {
+        r->err = "copy of the remap plan failed";
+        return -1;
+      }
+    }
+    return mhdct::launch_remap(r->dim, r->b, nnew, r->dev, int(r->prolonged.size()), r->dev_prolonged, U_old, B_old,
+                               U_new, B_new, s, r->err);
+  }
+
+  const char *
+  mhdct_remap_last_error(const mhdct_remap * r)
+  {
+    return r ? r->err.c_str() : remap_create_error().c_str();
+  }
+
+  void
+  mhdct_remap_destroy(mhdct_remap * r)
+  {
+    if (!r)
+      return;
+    if (r->dev)
+      (void)hipFree(r->dev);
+    if (r->dev_prolonged)
+      (void)hipFree(r->dev_prolonged);
+    delete r;
+  }
+
   const char *
   mhdct_last_error(const mhdct_ctx * h)
   {
@@ -428,6 +574,8 @@ extern "C"
       return;
     mhdct::plain_free(h->c);
     mhdct::fused_free(h->c);
+    if (h->c.amr_buf)
+      (void)hipFree(h->c.amr_buf);
     free_mesh(h->c);
     if (h->c.red)
       (void)hipFree(h->c.red);

@@ -111,6 +111,9 @@ struct Context
   std::string    err;
   // fused-form scratch (border EMFs etc.), owned by the fused implementation
   void *         fused = nullptr;
+  // refinement-indicator scratch (per owned block: indicator bits, indicator, flag)
+  void *         amr_buf = nullptr;
+  size_t         amr_bytes = 0;
 };
 
 Gas<double> gas_of(const Settings & s);

@@ -1,6 +1,10 @@
 // handles.hpp -- the opaque C-ABI handle types of include/mhdct.h.
 #pragma once
 
+#include <string>
+#include <vector>
+
+#include "amr.hpp"
 #include "ctx.hpp"
 
 struct mhdct_mesh
@@ -11,4 +15,17 @@ struct mhdct_mesh
 struct mhdct_ctx
 {
   mhdct::Context c;
+};
+
+struct mhdct_remap
+{
+  int                             dim = 3;
+  int                             b = 16;
+  int                             counts[3] = { 0, 0, 0 };
+  std::vector<mhdct::RemapEntry> plan;          // per new owned leaf
+  std::vector<int>                prolonged;     // the prolonged new leaves
+  mhdct::RemapEntry *             dev = nullptr; // the plan in device memory, on first apply
+  int *                           dev_prolonged = nullptr;
+  int                             device = -1;
+  std::string                     err;
 };

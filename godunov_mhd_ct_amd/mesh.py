@@ -50,6 +50,16 @@ class Mesh:
     def handle(self):
         return self._h
 
+    def clone(self) -> "Mesh":
+        """An independent copy (leaves, border conditions, tables)."""
+        h = C.c_void_p()
+        self._check(lib().mhdct_mesh_clone(self._h, C.byref(h)), "clone")
+        m = Mesh.__new__(Mesh)
+        m.dim, m.b, m.nbrick, m.periodic = self.dim, self.b, self.nbrick, self.periodic
+        m._h = h
+        m.finalized = self.finalized
+        return m
+
     def _check(self, rc, what):
         if rc != 0:
             msg = lib().mhdct_mesh_last_error(self._h)

@@ -11,11 +11,17 @@ One step follows SolverGodunovMHD::next_iteration_impl (SolverGodunovMHD.cpp:409
 
 The reference copies U2 back into U; here the buffers swap roles and the copy moves to the
 start of the next step (the same traffic, the same contract).
+
+`do_amr_cycle` is the reference's AMR cycle (:263-319) on this mesh: refinement flags on the
+device, adaptation and balance of a copy of the host octree, remap of U and face B on the
+device (amr.py); `next_iteration` runs it every `cycle_frequency` steps before the step, as
+next_iteration_impl does (:409-418).
 """
 from __future__ import annotations
 
 import numpy as np
 
+from .amr import AMRSettings, Remap, adapted, refine_flags
 from .implem import (IMPL_PLAIN, RK2_SSP_STAGE1_COEFS, RK2_SSP_STAGE2_COEFS, TIME_INTEGRATOR, GodunovImplemHIP,
                      hydro_settings)
 from .mesh import Mesh
@@ -46,6 +52,7 @@ class SolverGodunovMHD:
             self.B_RK = self.B.clone()
         self.t = 0.0
         self.iteration = 0
+        self.amr_cycles = 0
         self.dt = None
         self.total_num_cell_updates = 0
 
@@ -85,6 +92,44 @@ class SolverGodunovMHD:
         self.B2.copy_(self.B_RK)
         im.do_time_step(self.U_RK, self.U2, self.B_RK, self.B2, dt)
         im.linear_combination(self.U, self.U2, self.B, self.B2, RK2_SSP_STAGE2_COEFS)
+
+    def do_amr_cycle(self, amr: AMRSettings):
+        """One AMR cycle; returns the (copied, prolonged, restricted) leaf counts, or None when
+        no leaf is flagged (the mesh and the state are then unchanged)."""
+        torch = self.torch
+        if self.mesh.num_outside:
+            self.implem.fill_outside(self.U, self.B)
+        flags, _ = refine_flags(self.implem, self.U, amr)
+        self.amr_cycles += 1
+        if not flags.any():
+            return None
+        new_mesh = adapted(self.mesh, flags)
+        rm = Remap(self.mesh, new_mesh)
+        nt = new_mesh.num_total
+        Un = torch.zeros(nt * 5 * new_mesh.cells_per_block, dtype=torch.float64, device=self.U.device)
+        Bn = torch.zeros(nt * new_mesh.faces_per_block, dtype=torch.float64, device=self.U.device)
+        rm.apply(self.U, self.B, Un, Bn)
+        self.implem.resize_auxiliary_data(new_mesh)
+        self.mesh = new_mesh
+        self.U, self.B = Un, Bn
+        self.U2, self.B2 = Un.clone(), Bn.clone()
+        if self.rk2:
+            self.U_RK, self.B_RK = Un.clone(), Bn.clone()
+        return rm.counts()
+
+    def should_do_amr_cycle(self, amr: AMRSettings | None, disable_first: bool = False) -> bool:
+        # no cycle when level_min == level_max (SolverGodunovMHD.cpp:413)
+        if amr is None or amr.cycle_frequency <= 0 or amr.level_min >= amr.level_max:
+            return False
+        if self.iteration == 0 and disable_first:
+            return False
+        return self.iteration % amr.cycle_frequency == 0
+
+    def next_iteration(self, amr: AMRSettings | None = None, disable_first: bool = False) -> float:
+        """next_iteration_impl: the AMR cycle when due, then one step."""
+        if self.should_do_amr_cycle(amr, disable_first):
+            self.do_amr_cycle(amr)
+        return self.step()
 
     def state(self):
         nt = self.mesh.num_total

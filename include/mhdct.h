@@ -119,6 +119,49 @@ int mhdct_linear_combination(mhdct_ctx * ctx, const double * U, double * U_RK, c
 const char * mhdct_last_error(const mhdct_ctx * ctx);
 void         mhdct_destroy(mhdct_ctx * ctx);
 
+/* ---- AMR cycle (SolverGodunovMHD::do_amr_cycle, SolverGodunovMHD.cpp:263-319) -----------
+ * The reference computes the refinement flags on the device (ComputeRefineFlags), adapts and
+ * balances the forest on the host (p4est) and remaps U and Bface on the device
+ * (UserDataRemapper); the first and the last are kalypsso::core, absent from its tree, and are
+ * restated here (amr.hip): Loehner's unsplit second-derivative indicator with the noise filter
+ * epsilon_lohner, SIMPLE_COPY / mean cell prolongation and restriction, divergence-free face-B
+ * prolongation and flux-conserving face-B restriction. */
+typedef struct mhdct_amr_params
+{
+  int    level_min;       /* [amr] level_min */
+  int    level_max;       /* [amr] level_max */
+  double epsilon_refine;  /* [amr] epsilon_refine, default 0.01 */
+  double epsilon_coarsen; /* [amr] epsilon_coarsen, default 0.00125 */
+  double epsilon_lohner;  /* [amr] epsilon_lohner, default 0.02 */
+  int    ivar;            /* [amr] variable_used_to_compute_refine_criterion as a variable of U: 0 rho */
+} mhdct_amr_params;
+
+/* Replaces compute_refine_flags (SolverGodunovMHD.cpp:1094-1185) for a conserved variable: the
+ * indicator of every owned block (maximum over its cells, read with one ghost layer, so the
+ * outside blocks must be filled) and its flag, +1 refine (indicator > epsilon_refine and level
+ * < level_max, or level < level_min), -1 coarsen (indicator < epsilon_coarsen and level >
+ * level_min, or level > level_max), 0 keep, into host arrays of num_leaves entries
+ * (`indicator` may be NULL).  Synchronises `stream`. */
+int mhdct_refine_flags(mhdct_ctx * ctx, const double * U, const mhdct_amr_params * params, int8_t * flags,
+                       double * indicator, void * stream);
+/* A copy of a mesh (the leaf set before an adaptation, kept for the remap). */
+int mhdct_mesh_clone(const mhdct_mesh * mesh, mhdct_mesh ** out);
+
+/* Replaces UserDataRemapper (map_userdata_after_adapt, SolverGodunovMHD.cpp:1191-1269): the
+ * owned leaves of new_mesh from those of old_mesh, each a copy of an old leaf, a prolongation
+ * of its old parent, or a restriction of its old children (one level of change per cycle).
+ * The plan is built on the host; counts[3] = copied, prolonged, restricted leaves. */
+typedef struct mhdct_remap mhdct_remap;
+int mhdct_remap_create(const mhdct_mesh * old_mesh, const mhdct_mesh * new_mesh, mhdct_remap ** out);
+int mhdct_remap_counts(const mhdct_remap * remap, int counts[3]);
+/* U_new, B_new (the new mesh's layout) for its owned leaves from U_old, B_old (the old mesh's);
+ * the outside blocks of U_new, B_new are left for mhdct_fill_outside.  Asynchronous on `stream`. */
+int mhdct_remap_apply(mhdct_remap * remap, const double * U_old, const double * B_old, double * U_new,
+                      double * B_new, void * stream);
+/* the message of the last failed mhdct_remap_create (remap NULL) or mhdct_remap_apply */
+const char * mhdct_remap_last_error(const mhdct_remap * remap);
+void         mhdct_remap_destroy(mhdct_remap * remap);
+
 /* ---- batches ---------------------------------------------------------------------------
  * No counterpart in the reference, which advances one mesh per process
  * (SolverGodunovMHD::next_iteration_impl, SolverGodunovMHD.cpp:409-484): a batch advances

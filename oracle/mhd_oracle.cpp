@@ -2562,3 +2562,412 @@ extern "C"
     return long(v[which]->size());
   }
 }
+
+// ---------------------------------------------------------------------------
+// The AMR cycle's device work (SolverGodunovMHD.cpp:263-319): the refinement indicator and
+// flags (compute_refine_flags, :1094-1185) and the remap of U and face B to the adapted leaves
+// (map_userdata_after_adapt, :1191-1269).  The reference delegates both to kalypsso::core
+// (ComputeRefineFlags, UserDataRemapper; CMake target kalypsso::core, CMakeLists.txt:76, no
+// version pinned), which is not in its tree.  Restated from published methods:
+//  * Loehner's second-derivative error estimate (CMAME 61, 1987), unsplit (with the mixed
+//    derivatives), noise filter eps = [amr] epsilon_lohner, block indicator = max over cells;
+//  * cell data: SIMPLE_COPY prolongation, restriction by the mean of the 2^d fine cells summed
+//    in pairs (restricting a prolongation is then exact);
+//  * face B: injection on the parent's face planes and the mean of the two bracketing parent
+//    faces inside the parent cell (prolongation), the mean of the 2^(d-1) fine faces summed in
+//    pairs (restriction).
+// ---------------------------------------------------------------------------
+namespace oracle
+{
+
+// conserved variable v of owned block o at cell (i, j, k), one ghost layer at most
+static double
+u_cell(const Solver & S, const double * U, int v, int o, int i, int j, int k)
+{
+  const int  b = S.lay.b, dim = S.m.dim;
+  const long ncell = S.lay.cells();
+  if (i >= 0 && i < b && j >= 0 && j < b && (dim == 2 || (k >= 0 && k < b)))
+    return U[(long(o) * 5 + v) * ncell + S.lay.cidx(i, j, k)];
+  const Loc l = S.m.locate(o, i, j, k);
+  if (l.oct < 0)
+    throw std::runtime_error("refine indicator: missing neighbour");
+  if (l.rel == 1)
+  {
+    double sum = 0;
+    for (int dk = 0; dk < (dim == 3 ? 2 : 1); ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di)
+          sum += U[(long(l.oct) * 5 + v) * ncell + S.lay.cidx(l.ijk[0] + di, l.ijk[1] + dj, l.ijk[2] + dk)];
+    return sum / double(1 << dim);
+  }
+  return U[(long(l.oct) * 5 + v) * ncell + S.lay.cidx(l.ijk[0], l.ijk[1], l.ijk[2])];
+}
+
+static double
+lohner_at(const Solver & S, const double * U, int v, int o, int i, int j, int k, double eps)
+{
+  const int dim = S.m.dim;
+  auto u = [&](const int d[3]) { return u_cell(S, U, v, o, i + d[0], j + d[1], k + d[2]); };
+  double num = 0, den = 0;
+  for (int a = 0; a < dim; ++a)
+    for (int c = 0; c < dim; ++c)
+    {
+      if (a == c)
+      {
+        int p[3] = { 0, 0, 0 }, z[3] = { 0, 0, 0 }, q[3] = { 0, 0, 0 };
+        p[a] = 1;
+        q[a] = -1;
+        const double up = u(p), u0 = u(z), um = u(q);
+        const double n = (up - u0) - (u0 - um);
+        const double d = std::fabs(up - u0) + std::fabs(u0 - um) +
+                         eps * (std::fabs(up) + 2.0 * std::fabs(u0) + std::fabs(um));
+        num += n * n;
+        den += d * d;
+      }
+      else
+      {
+        int pp[3] = { 0, 0, 0 }, pm[3] = { 0, 0, 0 }, mp[3] = { 0, 0, 0 }, mm[3] = { 0, 0, 0 };
+        pp[a] = 1, pp[c] = 1;
+        pm[a] = 1, pm[c] = -1;
+        mp[a] = -1, mp[c] = 1;
+        mm[a] = -1, mm[c] = -1;
+        const double upp = u(pp), upm = u(pm), ump = u(mp), umm = u(mm);
+        const double n = ((upp - upm) - (ump - umm)) * 0.25;
+        const double d = (std::fabs(upp - upm) + std::fabs(ump - umm)) * 0.25 +
+                         eps * (std::fabs(upp) + std::fabs(upm) + std::fabs(ump) + std::fabs(umm)) * 0.25;
+        num += n * n;
+        den += d * d;
+      }
+    }
+  return den > 0.0 ? std::sqrt(num / den) : 0.0;
+}
+
+// one leaf's remap source: 0 copy, 1 prolongation (octant), 2 restriction (children)
+struct RemapSrc
+{
+  int kind = 0, oct = 0, src[8] = { -1, -1, -1, -1, -1, -1, -1, -1 };
+};
+
+static int
+parity(int v)
+{
+  return __builtin_popcount(unsigned(v)) & 1;
+}
+
+// Face B of a prolonged leaf.  For each parent cell inside the leaf, its 2^d fine cells take:
+// on the parent cell's face planes the parent's face (injection), or, on the leaf's boundary
+// plane across which an old leaf of this level lies (nbr), that leaf's faces; inside the parent
+// cell the mean of the two opposite outer fine faces of each row plus the minimum-norm
+// correction that makes every fine cell divergence free: with the fine cells as the nodes of
+// the cube graph and the interior faces as its edges, the face between cells n0 < n1 gains
+// phi[n0] - phi[n1] with L phi = -div, L the graph Laplacian, solved on its Walsh basis
+// (w_s(n) = (-1)^|s & n|, eigenvalue 2 |s|).  2D B_z (cell centred) is copied.
+static void
+prolong_faces(const Layout & lay, const double * Bo, const RemapSrc & r, const int nbr[6], const int o3[3], double * bn)
+{
+  const int  dim = lay.dim, b = lay.b, h = b / 2, nc = 1 << dim, nm = 1 << (dim - 1);
+  const long fper = lay.fper();
+  const double * P = Bo + long(r.src[0]) * fper;
+  for (int pk = 0; pk < (dim == 3 ? h : 1); ++pk)
+    for (int pj = 0; pj < h; ++pj)
+      for (int pi = 0; pi < h; ++pi)
+      {
+        const int pl[3] = { pi, pj, pk };
+        int       pc[3] = { 0, 0, 0 };
+        for (int d = 0; d < dim; ++d)
+          pc[d] = o3[d] * h + pl[d];
+        double outer[3][2][4], inner[3][4];
+        for (int c = 0; c < dim; ++c)
+        {
+          const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+          for (int side = 0; side < 2; ++side)
+          {
+            const int o = (pl[c] == (side ? h - 1 : 0)) ? nbr[2 * c + side] : -1;
+            for (int m = 0; m < nm; ++m)
+            {
+              if (o >= 0)
+              {
+                int g[3] = { 0, 0, 0 };
+                g[c] = side ? 0 : b;
+                g[t0] = 2 * pl[t0] + (m & 1);
+                if (dim == 3)
+                  g[t1] = 2 * pl[t1] + ((m >> 1) & 1);
+                outer[c][side][m] = Bo[long(o) * fper + lay.fidx(c, g[0], g[1], g[2])];
+              }
+              else
+              {
+                int q[3] = { pc[0], pc[1], pc[2] };
+                q[c] += side;
+                outer[c][side][m] = P[lay.fidx(c, q[0], q[1], q[2])];
+              }
+            }
+          }
+          for (int m = 0; m < nm; ++m)
+            inner[c][m] = 0.5 * (outer[c][0][m] + outer[c][1][m]);
+        }
+        // fine cell n = (x bit 0, y bit 1, z bit 2); its row m along c from its transverse bits
+        auto rowm = [&](int c, int n) {
+          const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+          return ((n >> t0) & 1) | (dim == 3 ? ((n >> t1) & 1) << 1 : 0);
+        };
+        double div[8];
+        for (int n = 0; n < nc; ++n)
+        {
+          double sdiv = 0;
+          for (int c = 0; c < dim; ++c)
+          {
+            const int    m = rowm(c, n);
+            const bool   high = (n >> c) & 1;
+            const double lo = high ? inner[c][m] : outer[c][0][m];
+            const double hi = high ? outer[c][1][m] : inner[c][m];
+            sdiv = sdiv + (hi - lo);
+          }
+          div[n] = sdiv;
+        }
+        double phi[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        for (int sw = 1; sw < nc; ++sw)
+        {
+          double w = 0;
+          for (int n = 0; n < nc; ++n)
+            w = parity(sw & n) ? w - div[n] : w + div[n];
+          const double coef = w / double(nc * 2 * __builtin_popcount(unsigned(sw)));
+          for (int n = 0; n < nc; ++n)
+            phi[n] = parity(sw & n) ? phi[n] + coef : phi[n] - coef;
+        }
+        for (int c = 0; c < dim; ++c)
+        {
+          const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+          for (int m = 0; m < nm; ++m)
+          {
+            const int n0 = ((m & 1) << t0) | (dim == 3 ? ((m >> 1) & 1) << t1 : 0);
+            inner[c][m] = inner[c][m] + (phi[n0] - phi[n0 | (1 << c)]);
+            int g[3] = { 2 * pl[0], 2 * pl[1], 2 * pl[2] };
+            g[t0] += m & 1;
+            if (dim == 3)
+              g[t1] += (m >> 1) & 1;
+            bn[lay.fidx(c, g[0], g[1], g[2])] = outer[c][0][m];
+            g[c] += 1;
+            bn[lay.fidx(c, g[0], g[1], g[2])] = inner[c][m];
+            g[c] += 1;
+            bn[lay.fidx(c, g[0], g[1], g[2])] = outer[c][1][m];
+          }
+        }
+        if (dim == 2)
+          for (int dj = 0; dj < 2; ++dj)
+            for (int di = 0; di < 2; ++di)
+              bn[lay.fidx(2, 2 * pl[0] + di, 2 * pl[1] + dj, 0)] = P[lay.fidx(2, pc[0], pc[1], 0)];
+      }
+}
+
+} // namespace oracle
+
+extern "C"
+{
+  // per owned block: the maximum over its cells of Loehner's estimate of U variable v (and,
+  // when ecell is not null, every cell's estimate, block by block)
+  int
+  oracle_refine_indicator(void * h, const double * U, int v, double eps, double * emax, double * ecell)
+  {
+    auto *     S = static_cast<oracle::Solver *>(h);
+    const int  b = S->lay.b, dim = S->m.dim;
+    const long ncell = S->lay.cells();
+    try
+    {
+      for (int o = 0; o < S->m.nowned; ++o)
+      {
+        double e = 0.0;
+        for (long ci = 0; ci < ncell; ++ci)
+        {
+          const int i = int(ci % b), j = int((ci / b) % b), k = dim == 3 ? int(ci / (long(b) * b)) : 0;
+          const double x = oracle::lohner_at(*S, U, v, o, i, j, k, eps);
+          e = std::fmax(e, (x >= 0.0) ? x : HUGE_VAL);
+          if (ecell)
+            ecell[long(o) * ncell + ci] = x;
+        }
+        emax[o] = e;
+      }
+    }
+    catch (std::exception & ex)
+    {
+      S->err = ex.what();
+      return -1;
+    }
+    return 0;
+  }
+
+  // flags from the indicator: +1 refine, -1 coarsen, 0 keep
+  void
+  oracle_refine_flags(int n, const int * levels, const double * emax, int level_min, int level_max, double eps_refine,
+                      double eps_coarsen, int8_t * flags)
+  {
+    for (int o = 0; o < n; ++o)
+    {
+      const int l = levels[o];
+      const double e = emax[o];
+      flags[o] = l < level_min ? 1
+                 : l > level_max ? -1
+                 : (e > eps_refine && l < level_max) ? 1
+                 : (e < eps_coarsen && l > level_min) ? -1
+                                                       : 0;
+    }
+  }
+
+  // U_new, B_new of the nnew leaves (levels, coords) from U_old, B_old of the nold leaves
+  int
+  oracle_remap(int dim, int b, const int * nbrick, const int * periodic, int nold, const int * old_levels,
+               const int * old_coords, int nnew, const int * new_levels, const int * new_coords, const double * Uo,
+               const double * Bo, double * Un, double * Bn)
+  {
+    oracle::Layout lay;
+    lay.dim = dim;
+    lay.b = b;
+    std::unordered_map<uint64_t, int> idx;
+    for (int o = 0; o < nold; ++o)
+      idx[oracle::Mesh::key(old_levels[o], old_coords[3 * o], old_coords[3 * o + 1], old_coords[3 * o + 2])] = o;
+    auto find = [&](int l, int x, int y, int z) {
+      if (l < 0)
+        return -1;
+      auto it = idx.find(oracle::Mesh::key(l, x, y, z));
+      return it == idx.end() ? -1 : it->second;
+    };
+    const long ncell = lay.cells(), fper = lay.fper();
+    const int  nch = 1 << dim;
+    for (int n = 0; n < nnew; ++n)
+    {
+      const int L = new_levels[n], x = new_coords[3 * n], y = new_coords[3 * n + 1], z = new_coords[3 * n + 2];
+      oracle::RemapSrc r;
+      if ((r.src[0] = find(L, x, y, z)) >= 0)
+        r.kind = 0;
+      else if ((r.src[0] = find(L - 1, x >> 1, y >> 1, dim == 3 ? z >> 1 : 0)) >= 0)
+      {
+        r.kind = 1;
+        r.oct = (x & 1) | ((y & 1) << 1) | (dim == 3 ? (z & 1) << 2 : 0);
+      }
+      else
+      {
+        r.kind = 2;
+        for (int c = 0; c < nch; ++c)
+          if ((r.src[c] = find(L + 1, 2 * x + (c & 1), 2 * y + ((c >> 1) & 1), dim == 3 ? 2 * z + ((c >> 2) & 1) : 0)) <
+              0)
+            return -1;
+      }
+      // the old leaf of this level across each face plane (periodic wrap), or -1
+      int nbr[6] = { -1, -1, -1, -1, -1, -1 };
+      for (int c = 0; c < dim; ++c)
+        for (int side = 0; side < 2; ++side)
+        {
+          long      q[3] = { x, y, z };
+          const long nb = long(nbrick[c]) << L;
+          q[c] += side ? 1 : -1;
+          if (q[c] < 0 || q[c] >= nb)
+          {
+            if (!periodic[c])
+              continue;
+            q[c] = (q[c] + nb) % nb;
+          }
+          nbr[2 * c + side] = find(L, int(q[0]), int(q[1]), int(q[2]));
+        }
+      double *       un = Un + long(n) * 5 * ncell;
+      double *       bn = Bn + long(n) * fper;
+      if (r.kind == 0)
+      {
+        std::memcpy(un, Uo + long(r.src[0]) * 5 * ncell, 5 * ncell * sizeof(double));
+        std::memcpy(bn, Bo + long(r.src[0]) * fper, fper * sizeof(double));
+        continue;
+      }
+      const int o3[3] = { r.oct & 1, (r.oct >> 1) & 1, (r.oct >> 2) & 1 };
+      // cells
+      for (int v = 0; v < 5; ++v)
+        for (int k = 0; k < (dim == 3 ? b : 1); ++k)
+          for (int j = 0; j < b; ++j)
+            for (int i = 0; i < b; ++i)
+            {
+              double val;
+              if (r.kind == 1)
+              {
+                const double * P = Uo + (long(r.src[0]) * 5 + v) * ncell;
+                val = P[lay.cidx((o3[0] * b + i) / 2, (o3[1] * b + j) / 2, dim == 3 ? (o3[2] * b + k) / 2 : 0)];
+              }
+              else
+              {
+                // coarse cell (i, j, k) covers fine cells 2i..2i+1 of the child holding them
+                const int cx = 2 * i / b, cy = 2 * j / b, cz = dim == 3 ? 2 * k / b : 0;
+                const double * F = Uo + (long(r.src[cx | (cy << 1) | (cz << 2)]) * 5 + v) * ncell;
+                // summed in pairs (x pairs, then y, then z): the mean of equal values is exact
+                double p[4];
+                for (int dk = 0; dk < (dim == 3 ? 2 : 1); ++dk)
+                  for (int dj = 0; dj < 2; ++dj)
+                  {
+                    const int fi = 2 * i - cx * b, fj = 2 * j - cy * b + dj, fk = dim == 3 ? 2 * k - cz * b + dk : 0;
+                    p[2 * dk + dj] = F[lay.cidx(fi, fj, fk)] + F[lay.cidx(fi + 1, fj, fk)];
+                  }
+                const double sum = dim == 3 ? (p[0] + p[1]) + (p[2] + p[3]) : p[0] + p[1];
+                val = sum / double(nch);
+              }
+              un[v * ncell + lay.cidx(i, j, k)] = val;
+            }
+      // faces
+      if (r.kind == 1)
+      {
+        prolong_faces(lay, Bo, r, nbr, o3, bn);
+        continue;
+      }
+      for (int c = 0; c < 3; ++c)
+      {
+        const bool cellc = dim == 2 && c == 2;
+        const int  ex[3] = { (!cellc && c == 0) ? b + 1 : b, (!cellc && c == 1) ? b + 1 : b,
+                             dim == 3 ? (c == 2 ? b + 1 : b) : 1 };
+        for (int k = 0; k < ex[2]; ++k)
+          for (int j = 0; j < ex[1]; ++j)
+            for (int i = 0; i < ex[0]; ++i)
+            {
+              const int f[3] = { i, j, k };
+              double    val;
+              const int side = (!cellc && f[c] == b) ? 1 : 0;
+              const int nbo = (!cellc && (f[c] == 0 || f[c] == b)) ? nbr[2 * c + side] : -1;
+              if (nbo >= 0)
+              {
+                // a boundary plane shared with an old leaf of this level keeps that leaf's faces
+                int g[3] = { i, j, k };
+                g[c] = side ? 0 : b;
+                val = Bo[long(nbo) * fper + lay.fidx(c, g[0], g[1], g[2])];
+              }
+              else
+              {
+                int ch[3] = { 0, 0, 0 }, fl[3] = { 0, 0, 0 };
+                for (int d = 0; d < dim; ++d)
+                {
+                  ch[d] = 2 * f[d] >= b ? 1 : 0;
+                  fl[d] = 2 * f[d] - ch[d] * b;
+                }
+                const double * F = Bo + long(r.src[ch[0] | (ch[1] << 1) | (ch[2] << 2)]) * fper;
+                if (cellc)
+                {
+                  const double r0 = F[lay.fidx(2, fl[0], fl[1], 0)] + F[lay.fidx(2, fl[0] + 1, fl[1], 0)];
+                  const double r1 = F[lay.fidx(2, fl[0], fl[1] + 1, 0)] + F[lay.fidx(2, fl[0] + 1, fl[1] + 1, 0)];
+                  val = (r0 + r1) / 4.0;
+                }
+                else
+                {
+                  // pairs along the first transverse direction, then the second
+                  const int t0 = c == 0 ? 1 : 0, t1 = c == 2 ? 1 : 2;
+                  double    p[2] = { 0, 0 };
+                  for (int d1 = 0; d1 < (dim == 3 ? 2 : 1); ++d1)
+                  {
+                    int q[3] = { fl[0], fl[1], fl[2] };
+                    if (dim == 3)
+                      q[t1] += d1;
+                    int q1[3] = { q[0], q[1], q[2] };
+                    q1[t0] += 1;
+                    p[d1] = F[lay.fidx(c, q[0], q[1], q[2])] + F[lay.fidx(c, q1[0], q1[1], q1[2])];
+                  }
+                  val = (dim == 3 ? p[0] + p[1] : p[0]) / double(1 << (dim - 1));
+                }
+              }
+              bn[lay.fidx(c, i, j, k)] = val;
+            }
+      }
+    }
+    return 0;
+  }
+}

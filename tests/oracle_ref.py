@@ -42,6 +42,15 @@ def lib():
         L.oracle_fill_outside.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.oracle_linear_combination.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
                                                 P(C.c_double), C.c_double, C.c_double]
+        L.oracle_refine_indicator.restype = C.c_int
+        L.oracle_refine_indicator.argtypes = [C.c_void_p, P(C.c_double), C.c_int, C.c_double, P(C.c_double),
+                                              P(C.c_double)]
+        L.oracle_refine_flags.argtypes = [C.c_int, P(C.c_int), P(C.c_double), C.c_int, C.c_int, C.c_double,
+                                          C.c_double, P(C.c_int8)]
+        L.oracle_remap.restype = C.c_int
+        L.oracle_remap.argtypes = [C.c_int, C.c_int, P(C.c_int), P(C.c_int), C.c_int, P(C.c_int), P(C.c_int),
+                                   C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), P(C.c_double), P(C.c_double),
+                                   P(C.c_double)]
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_last_error.argtypes = [C.c_void_p]
         L.oracle_riemann_hlld.argtypes = [P(C.c_double)] * 4
@@ -129,6 +138,16 @@ class OracleSolver:
         lib().oracle_linear_combination(self._h, dptr(U), dptr(U_RK), dptr(B), dptr(B_RK), float(coefs[0]),
                                         float(coefs[1]))
 
+    def refine_indicator(self, U, ivar=0, eps=0.02, per_cell=False):
+        """Per owned block the maximum of Loehner's estimate (and the per-cell values)."""
+        emax = np.zeros(max(self.nowned, 1))
+        ecell = np.zeros((max(self.nowned, 1), self.b ** self.dim)) if per_cell else None
+        rc = lib().oracle_refine_indicator(self._h, dptr(U), int(ivar), float(eps), dptr(emax),
+                                           dptr(ecell) if per_cell else None)
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error(self._h).decode())
+        return (emax[:self.nowned], ecell[:self.nowned]) if per_cell else emax[:self.nowned]
+
     @property
     def rk2(self):
         return int(self.settings[7]) == TIME_INTEGRATOR["RK2_SSP"]
@@ -159,3 +178,38 @@ class OracleSolver:
         self.do_time_step(U_RK, U2, B_RK, B2, dt)
         self.linear_combination(U, U2, B, B2, RK2_STAGE2)
         return U2, B2, dt
+
+
+def refine_flags(levels, emax, level_min, level_max, eps_refine, eps_coarsen):
+    n = len(emax)
+    lv = np.ascontiguousarray(levels[:n], dtype=np.int32)
+    e = np.ascontiguousarray(emax, dtype=np.float64)
+    out = np.zeros(max(n, 1), dtype=np.int8)
+    lib().oracle_refine_flags(n, lv.ctypes.data_as(C.POINTER(C.c_int)), dptr(e), int(level_min), int(level_max),
+                              float(eps_refine), float(eps_coarsen), out.ctypes.data_as(C.POINTER(C.c_int8)))
+    return out[:n]
+
+
+def remap(old_mesh, new_mesh, U_old, B_old):
+    """The oracle's remap of the owned leaves: (U_new, B_new) in the new mesh's layout (outside
+    blocks zero)."""
+    lo, co = old_mesh.leaves()
+    ln, cn = new_mesh.leaves()
+    no, nn = old_mesh.num_leaves, new_mesh.num_leaves
+    ncell = new_mesh.cells_per_block
+    fper = new_mesh.faces_per_block
+    Uo = np.ascontiguousarray(U_old, dtype=np.float64).reshape(-1)
+    Bo = np.ascontiguousarray(B_old, dtype=np.float64).reshape(-1)
+    Un = np.zeros(new_mesh.num_total * 5 * ncell)
+    Bn = np.zeros(new_mesh.num_total * fper)
+    lop, a1 = iptr(lo[:no])
+    cop, a2 = iptr(co[:no].reshape(-1))
+    lnp, a3 = iptr(ln[:nn])
+    cnp, a4 = iptr(cn[:nn].reshape(-1))
+    nbp, a5 = iptr(new_mesh.nbrick)
+    pep, a6 = iptr(new_mesh.periodic)
+    rc = lib().oracle_remap(new_mesh.dim, new_mesh.b, nbp, pep, no, lop, cop, nn, lnp, cnp, dptr(Uo), dptr(Bo),
+                            dptr(Un), dptr(Bn))
+    if rc != 0:
+        raise RuntimeError("oracle remap: a new leaf has no source one level away")
+    return Un.reshape(new_mesh.num_total, 5, ncell), Bn.reshape(new_mesh.num_total, fper)
