@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--impl", default="fused", choices=["fused", "fp32", "plain"])
     ap.add_argument("--workload", default="bl3d", choices=["bl3d", "mix", "kh3d", "ot2d", "srv"])
     ap.add_argument("--integrator", default="hancock", choices=["hancock", "rk2"])
+    ap.add_argument("--amr-cycle", action="store_true",
+                    help="time one AMR cycle (refinement flags, host adaptation, remap) on the workload after "
+                         "--steps steps, [amr] keys of app/test_mhd_blast_3d.ini")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
@@ -191,6 +194,8 @@ def main():
         return bench_serving(args)
 
     cases = cases_for(args.workload)
+    if args.amr_cycle:
+        return bench_amr(args, cases)
     if args.integrator == "rk2":
         return bench_rk2(args, cases)
     cells = sum(m.owned_cells() for (m, _, _, _) in cases)
@@ -264,6 +269,82 @@ def bench_rk2(args, cases):
         out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"], "speedup": r["value"] / p["value"]}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
+    print(json.dumps(out))
+
+
+def bench_amr(args, cases):
+    """One AMR cycle on the first mesh of the workload after args.steps steps: device
+    refinement flags, host adaptation + 2:1 balance + remap plan, device remap, new mesh tables.
+    The remap's algorithmic bytes are the owned state read once and the new owned state written
+    once."""
+    import torch
+    from godunov_mhd_ct_amd.amr import AMRSettings, Remap, adapted, refine_flags
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED
+    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+
+    m, U, B, st = cases[0]
+    lv, _ = m.leaves()
+    amr = AMRSettings(level_min=int(lv.min()), level_max=int(lv.max()), epsilon_refine=0.01,
+                      epsilon_coarsen=0.00125, epsilon_lohner=0.02)
+    sol = SolverGodunovMHD(m, U, B, st, impl_version=IMPL_FUSED)
+    for _ in range(args.steps):
+        sol.step()
+    torch.cuda.synchronize()
+    log(f"[bench] AMR cycle after {args.steps} steps: {m.num_leaves} leaves, levels {amr.level_min}-{amr.level_max}")
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    reps = 5
+    e0, e1 = ev(), ev()
+    e0.record()
+    for _ in range(reps):
+        flags, ind = refine_flags(sol.implem, sol.U, amr)
+    e1.record()
+    torch.cuda.synchronize()
+    flags_ms = e0.elapsed_time(e1) / reps
+    t0 = time.perf_counter()
+    new = adapted(sol.mesh, flags)
+    t1 = time.perf_counter()
+    rm = Remap(sol.mesh, new)
+    t2 = time.perf_counter()
+    nt = new.num_total
+    Un = torch.zeros(nt * 5 * new.cells_per_block, dtype=torch.float64, device="cuda")
+    Bn = torch.zeros(nt * new.faces_per_block, dtype=torch.float64, device="cuda")
+    rm.apply(sol.U, sol.B, Un, Bn)  # first call uploads the plan
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        rm.apply(sol.U, sol.B, Un, Bn)
+    e1.record()
+    torch.cuda.synchronize()
+    remap_ms = e0.elapsed_time(e1) / reps
+    t3 = time.perf_counter()
+    sol.implem.resize_auxiliary_data(new)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    per_block = 8 * (5 * m.cells_per_block + m.faces_per_block)
+    algo = per_block * (m.num_leaves + new.num_leaves)
+    counts = rm.counts()
+    step_ms = None
+    sol.mesh, sol.U, sol.B = new, Un, Bn
+    sol.U2, sol.B2 = Un.clone(), Bn.clone()
+    sol.step()
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(3):
+        sol.step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - s0) * 1e3 / 3
+    out = {"metric": "AMR-cycle remap throughput", "value": algo / (remap_ms * 1e-3) / 1e9, "unit": "GB/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": args.workload, "steps_before": args.steps, "leaves_before": m.num_leaves,
+                      "leaves_after": new.num_leaves, "copied": counts[0], "prolonged": counts[1],
+                      "restricted": counts[2], "refine": int((flags > 0).sum()), "coarsen": int((flags < 0).sum())},
+           "roofline": {"bound": "hbm", "achieved": algo / (remap_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": algo / (remap_ms * 1e-3) / HBM_PEAK, "traffic": None,
+                        "kernel": "k_remap_cells + k_remap_faces + k_prolong_faces",
+                        "algorithmic_bytes": f"{algo} B: old and new owned state, once each"},
+           "ms": {"refine_flags_device_and_readback": flags_ms, "host_adapt_balance": (t1 - t0) * 1e3,
+                  "host_remap_plan": (t2 - t1) * 1e3, "remap_device": remap_ms,
+                  "new_mesh_tables": (t4 - t3) * 1e3, "step_after": step_ms}}
     print(json.dumps(out))
 
 

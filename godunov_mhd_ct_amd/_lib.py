@@ -30,6 +30,8 @@ class Settings(C.Structure):
         ("prolongation", C.c_int),
         ("abort_when_negative_eint", C.c_int),
         ("time_integrator", C.c_int),
+        ("gravity_enabled", C.c_int),
+        ("gravity", C.c_double * 3),
     ]
 
 

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -172,70 +174,85 @@ k_refine_flags(DevMesh m, AmrParams p, const unsigned long long * __restrict__ e
 }
 
 // ---------------------------------------------------------------------------------------------
-// remap: one thread per (new block, variable, cell) and per (new block, face)
+// remap.  Copied leaves: whole-block copies (16-byte accesses).  Prolonged and restricted
+// leaves: one thread per cell (all five variables) and per face; 32-bit index arithmetic.
 // ---------------------------------------------------------------------------------------------
+// blocks of `per` doubles: dst block pairs[2n+1] <- src block pairs[2n]
 __global__ void __launch_bounds__(kThreads)
-k_remap_cells(int dim, int b, int nnew, const RemapEntry * __restrict__ E, const double * __restrict__ Uo,
-              double * __restrict__ Un)
+k_copy_blocks(int per, int npair, const int * __restrict__ pairs, const double * __restrict__ src,
+              double * __restrict__ dst, int vec)
 {
-  const Idx  I{ dim, b };
-  const long ncell = I.cells();
-  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= long(nnew) * 5 * ncell)
+  for (int n = blockIdx.y; n < npair; n += gridDim.y)
+  {
+    const double * a = src + long(pairs[2 * n]) * per;
+    double *       d = dst + long(pairs[2 * n + 1]) * per;
+    if (vec)
+    {
+      const int h = per / 2;
+      for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < h; t += gridDim.x * blockDim.x)
+        reinterpret_cast<double2 *>(d)[t] = reinterpret_cast<const double2 *>(a)[t];
+    }
+    else
+      for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < per; t += gridDim.x * blockDim.x)
+        d[t] = a[t];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_remap_cells(int dim, int b, int nmoved, const int * __restrict__ moved, const RemapEntry * __restrict__ E,
+              const double * __restrict__ Uo, double * __restrict__ Un)
+{
+  const int ncell = dim == 3 ? b * b * b : b * b;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nmoved * ncell)
     return;
-  const int      nb = int(t / (5 * ncell));
-  const long     r = t - long(nb) * 5 * ncell;
-  const int      v = int(r / ncell);
-  const long     ci = r - long(v) * ncell;
-  const int      i = int(ci % b), j = int((ci / b) % b), k = dim == 3 ? int(ci / (long(b) * b)) : 0;
+  const int        nb = moved[t / ncell];
+  const int        ci = t % ncell;
+  const int        i = ci % b, j = (ci / b) % b, k = dim == 3 ? ci / (b * b) : 0;
   const RemapEntry e = E[nb];
-  double           val;
-  if (e.kind == kRemapCopy)
-    val = Uo[(long(e.src[0]) * 5 + v) * ncell + ci];
-  else if (e.kind == kRemapProlong)
+  auto             cell = [b](int a, int c, int d) { return a + b * (c + b * d); };
+  double *         out = Un + long(nb) * 5 * ncell + ci;
+  if (e.kind == kRemapProlong)
   {
     const int ox = e.oct & 1, oy = (e.oct >> 1) & 1, oz = (e.oct >> 2) & 1;
-    const int pi = (ox * b + i) >> 1, pj = (oy * b + j) >> 1, pk = dim == 3 ? (oz * b + k) >> 1 : 0;
-    val = Uo[(long(e.src[0]) * 5 + v) * ncell + I.cell(pi, pj, pk)];
+    const int pc = cell((ox * b + i) >> 1, (oy * b + j) >> 1, dim == 3 ? (oz * b + k) >> 1 : 0);
+    const double * P = Uo + long(e.src[0]) * 5 * ncell + pc;
+    for (int v = 0; v < 5; ++v)
+      out[long(v) * ncell] = P[long(v) * ncell];
+    return;
   }
-  else
+  // restriction: fine global index 2i + di of the new block's span lies in child octant
+  // (2i >= b) at local 2i mod b
+  const int cx = (2 * i >= b), cy = (2 * j >= b), cz = dim == 3 ? (2 * k >= b) : 0;
+  const int fi = 2 * i - cx * b, fj = 2 * j - cy * b, fk = dim == 3 ? 2 * k - cz * b : 0;
+  const double * F = Uo + long(e.src[cx | (cy << 1) | (cz << 2)]) * 5 * ncell;
+  for (int v = 0; v < 5; ++v)
   {
-    // fine global index 2i + di within the new block's span: child octant (2i >= b), local 2i mod b
-    const int cx = (2 * i >= b), cy = (2 * j >= b), cz = dim == 3 ? (2 * k >= b) : 0;
-    const int child = cx | (cy << 1) | (cz << 2);
-    const int fi = 2 * i - cx * b, fj = 2 * j - cy * b, fk = dim == 3 ? 2 * k - cz * b : 0;
-    const double * src = Uo + (long(e.src[child]) * 5 + v) * ncell;
+    const double * src = F + long(v) * ncell;
     // pairwise sums, x pairs first: the mean of 2^dim equal values is that value exactly, so
     // restricting a prolongation gives back the parent bit for bit
-    auto row = [&](int dj, int dk) { return src[I.cell(fi, fj + dj, fk + dk)] + src[I.cell(fi + 1, fj + dj, fk + dk)]; };
+    auto row = [&](int dj, int dk) { return src[cell(fi, fj + dj, fk + dk)] + src[cell(fi + 1, fj + dj, fk + dk)]; };
     double s = row(0, 0) + row(1, 0);
     if (dim == 3)
       s = s + (row(0, 1) + row(1, 1));
-    val = s / double(1 << dim);
+    out[long(v) * ncell] = s / double(1 << dim);
   }
-  Un[t] = val;
 }
 
-// face B of copied and restricted leaves, one thread per face (prolonged leaves: k_prolong_faces)
+// face B of restricted leaves, one thread per face
 __global__ void __launch_bounds__(kThreads)
-k_remap_faces(int dim, int b, int nnew, const RemapEntry * __restrict__ E, const double * __restrict__ Bo,
-              double * __restrict__ Bn)
+k_restrict_faces(int dim, int b, int nrest, const int * __restrict__ restricted, const RemapEntry * __restrict__ E,
+                 const double * __restrict__ Bo, double * __restrict__ Bn)
 {
-  const Idx  I{ dim, b };
-  const long fper = I.fper();
-  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= long(nnew) * fper)
+  const Idx I{ dim, b };
+  const int fper = int(I.fper());
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrest * fper)
     return;
-  const int        nb = int(t / fper);
+  const int        nb = restricted[t / fper];
   const RemapEntry e = E[nb];
-  if (e.kind == kRemapProlong)
-    return;
-  long r = t - long(nb) * fper;
-  if (e.kind == kRemapCopy)
-  {
-    Bn[t] = Bo[long(e.src[0]) * fper + r];
-    return;
-  }
+  long             r = t % fper;
+  const long       tout = long(nb) * fper + r;
   int c = 0;
   while (c < 2 && r >= I.fsize(c))
     r -= I.fsize(c), ++c;
@@ -251,7 +268,7 @@ k_remap_faces(int dim, int b, int nnew, const RemapEntry * __restrict__ E, const
     {
       int g[3] = { f[0], f[1], f[2] };
       g[c] = side ? 0 : b;
-      Bn[t] = Bo[long(o) * fper + I.face(c, g[0], g[1], g[2])];
+      Bn[tout] = Bo[long(o) * fper + I.face(c, g[0], g[1], g[2])];
       return;
     }
   }
@@ -288,7 +305,7 @@ k_remap_faces(int dim, int b, int nnew, const RemapEntry * __restrict__ E, const
     const double s = dim == 3 ? pair(0) + pair(1) : pair(0);
     val = s / double(1 << (dim - 1));
   }
-  Bn[t] = val;
+  Bn[tout] = val;
 }
 
 // Face B of a prolonged leaf, one thread per parent cell inside it: the 2^dim fine cells of that
@@ -482,9 +499,10 @@ launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t *
 }
 
 bool
-remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry> & plan,
-           std::vector<int> & prolonged, int counts[3], std::string & err)
+remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry> & plan, RemapLists & lists,
+           int counts[3], std::string & err)
 {
+  std::vector<int> copies, moved, restricted, prolonged;
   if (old_mesh.dim != new_mesh.dim || old_mesh.b != new_mesh.b)
   {
     err = "remap: the meshes differ in dimension or block size";
@@ -530,7 +548,6 @@ remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry>
     return find(l, int(q[0]), int(q[1]), int(q[2]));
   };
   counts[0] = counts[1] = counts[2] = 0;
-  prolonged.clear();
   plan.assign(new_mesh.leaves.size(), RemapEntry{});
   for (size_t n = 0; n < new_mesh.leaves.size(); ++n)
   {
@@ -542,6 +559,8 @@ remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry>
     {
       e.kind = kRemapCopy;
       e.src[0] = same;
+      copies.push_back(same);
+      copies.push_back(int(n));
       ++counts[0];
       continue;
     }
@@ -555,6 +574,7 @@ remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry>
         for (int side = 0; side < 2; ++side)
           e.nb[2 * c + side] = across(L.level, L.x, L.y, L.z, c, side);
       prolonged.push_back(int(n));
+      moved.push_back(int(n));
       ++counts[1];
       continue;
     }
@@ -576,26 +596,53 @@ remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry>
     for (int c = 0; c < dim; ++c)
       for (int side = 0; side < 2; ++side)
         e.nb[2 * c + side] = across(L.level, L.x, L.y, L.z, c, side);
+    restricted.push_back(int(n));
+    moved.push_back(int(n));
     ++counts[2];
   }
+  lists.ncopy = int(copies.size() / 2);
+  lists.nmoved = int(moved.size());
+  lists.nrest = int(restricted.size());
+  lists.nprol = int(prolonged.size());
+  lists.ints = copies;
+  lists.ints.insert(lists.ints.end(), moved.begin(), moved.end());
+  lists.ints.insert(lists.ints.end(), restricted.begin(), restricted.end());
+  lists.ints.insert(lists.ints.end(), prolonged.begin(), prolonged.end());
   return true;
 }
 
 int
-launch_remap(int dim, int b, int nnew, const RemapEntry * plan_dev, int nprolonged, const int * prolonged_dev,
-             const double * Uo, const double * Bo, double * Un, double * Bn, hipStream_t s, std::string & err)
+launch_remap(int dim, int b, const RemapEntry * plan_dev, const RemapLists & L, const int * lists_dev, const double * Uo,
+             const double * Bo, double * Un, double * Bn, hipStream_t s, std::string & err)
 {
-  if (nnew == 0)
-    return 0;
-  const long ncell = dim == 3 ? long(b) * b * b : long(b) * b;
-  const long fper = dim == 3 ? 3 * long(b + 1) * b * b : 2 * long(b + 1) * b + long(b) * b;
-  hipLaunchKernelGGL(k_remap_cells, grid_of(long(nnew) * 5 * ncell), kThreads, 0, s, dim, b, nnew, plan_dev, Uo, Un);
-  hipLaunchKernelGGL(k_remap_faces, grid_of(long(nnew) * fper), kThreads, 0, s, dim, b, nnew, plan_dev, Bo, Bn);
-  if (nprolonged > 0)
+  const int ncell = dim == 3 ? b * b * b : b * b;
+  const int fper = dim == 3 ? 3 * (b + 1) * b * b : 2 * (b + 1) * b + b * b;
+  if (long(L.nmoved) * fper > long(INT32_MAX) || long(L.nmoved) * ncell > long(INT32_MAX))
+  {
+    err = "remap: too many prolonged or restricted leaves for one launch";
+    return -1;
+  }
+  if (L.ncopy > 0)
+  {
+    // whole blocks; 16-byte accesses when every block start is 16-byte aligned
+    const auto aligned = [](const void * p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const int  vec = aligned(Uo) && aligned(Un) && aligned(Bo) && aligned(Bn) ? 1 : 0; // per is even
+    const dim3 gu(std::min<unsigned>(grid_of(vec ? 5 * ncell / 2 : 5 * ncell), 64), std::min(L.ncopy, 65535));
+    const dim3 gb(std::min<unsigned>(grid_of(vec ? fper / 2 : fper), 64), std::min(L.ncopy, 65535));
+    hipLaunchKernelGGL(k_copy_blocks, gu, kThreads, 0, s, 5 * ncell, L.ncopy, L.copies(lists_dev), Uo, Un, vec);
+    hipLaunchKernelGGL(k_copy_blocks, gb, kThreads, 0, s, fper, L.ncopy, L.copies(lists_dev), Bo, Bn, vec);
+  }
+  if (L.nmoved > 0)
+    hipLaunchKernelGGL(k_remap_cells, grid_of(long(L.nmoved) * ncell), kThreads, 0, s, dim, b, L.nmoved,
+                       L.moved(lists_dev), plan_dev, Uo, Un);
+  if (L.nrest > 0)
+    hipLaunchKernelGGL(k_restrict_faces, grid_of(long(L.nrest) * fper), kThreads, 0, s, dim, b, L.nrest,
+                       L.restricted(lists_dev), plan_dev, Bo, Bn);
+  if (L.nprol > 0)
   {
     const long per = dim == 3 ? long(b / 2) * (b / 2) * (b / 2) : long(b / 2) * (b / 2);
-    hipLaunchKernelGGL(k_prolong_faces, grid_of(long(nprolonged) * per), kThreads, 0, s, dim, b, nprolonged,
-                       prolonged_dev, plan_dev, Bo, Bn);
+    hipLaunchKernelGGL(k_prolong_faces, grid_of(long(L.nprol) * per), kThreads, 0, s, dim, b, L.nprol,
+                       L.prolonged(lists_dev), plan_dev, Bo, Bn);
   }
   AMR_HIP(err, hipGetLastError());
   return 0;

@@ -48,12 +48,25 @@ struct RemapEntry
 // 0 keep; ind optional).  Synchronises `s`.
 int launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
                         hipStream_t s);
+// The work lists of a remap, in one int array: copy pairs (old leaf, new leaf), then the new
+// leaves that are prolonged or restricted ("moved"), then the restricted ones, then the
+// prolonged ones.
+struct RemapLists
+{
+  std::vector<int> ints;
+  int              ncopy = 0, nmoved = 0, nrest = 0, nprol = 0;
+  const int *      copies(const int * base) const { return base; }
+  const int *      moved(const int * base) const { return base + 2 * ncopy; }
+  const int *      restricted(const int * base) const { return base + 2 * ncopy + nmoved; }
+  const int *      prolonged(const int * base) const { return base + 2 * ncopy + nmoved + nrest; }
+};
+
 // Host plan for the owned leaves of new_mesh from those of old_mesh; counts = copied,
 // prolonged, restricted.
-// prolonged lists the new leaves of kind kRemapProlong.
-bool remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry> & plan,
-                std::vector<int> & prolonged, int counts[3], std::string & err);
-int  launch_remap(int dim, int b, int nnew, const RemapEntry * plan_dev, int nprolonged, const int * prolonged_dev,
-                  const double * Uo, const double * Bo, double * Un, double * Bn, hipStream_t s, std::string & err);
+bool remap_plan(const Mesh & old_mesh, const Mesh & new_mesh, std::vector<RemapEntry> & plan, RemapLists & lists,
+                int counts[3], std::string & err);
+// lists_dev: the device copy of lists.ints
+int launch_remap(int dim, int b, const RemapEntry * plan_dev, const RemapLists & lists, const int * lists_dev,
+                 const double * Uo, const double * Bo, double * Un, double * Bn, hipStream_t s, std::string & err);
 
 } // namespace mhdct

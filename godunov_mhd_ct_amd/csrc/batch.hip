@@ -109,8 +109,12 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
   bt.nslots = int(b->ent.size());
   bt.bsel = b->ent.empty() ? 0 : b->ent[0].ctx->c.b;
   for (const auto & e : b->ent)
+  {
     if (e.ctx->c.b != bt.bsel)
       bt.bsel = 0;
+    if (e.ctx->c.st.gravity_enabled)
+      bt.grav = 1;
+  }
   bt.tune = b->tune;
   if (bt.nslots == 0)
   {

@@ -254,6 +254,9 @@ extern "C"
     c.st.scaling = s->scaling_factor;
     c.st.abort_when_negative_eint = s->abort_when_negative_eint;
     c.st.time_integrator = s->time_integrator;
+    c.st.gravity_enabled = s->gravity_enabled != 0;
+    for (int d = 0; d < 3; ++d)
+      c.st.gravity[d] = d < dim ? s->gravity[d] : 0.0;
     if (hipGetDevice(&c.device) != hipSuccess)
     {
       delete h;
@@ -475,7 +478,7 @@ extern "C"
       return -1;
     r->dim = new_mesh->m.dim;
     r->b = new_mesh->m.b;
-    if (!mhdct::remap_plan(old_mesh->m, new_mesh->m, r->plan, r->prolonged, r->counts, remap_create_error()))
+    if (!mhdct::remap_plan(old_mesh->m, new_mesh->m, r->plan, r->lists, r->counts, remap_create_error()))
     {
       delete r;
       return -1;
@@ -523,8 +526,8 @@ extern "C"
     if (!r->dev)
     {
       const size_t bytes = r->plan.size() * sizeof(mhdct::RemapEntry);
-      const size_t pbytes = (r->prolonged.empty() ? 1 : r->prolonged.size()) * sizeof(int);
-      if (hipMalloc(&r->dev, bytes) != hipSuccess || hipMalloc(&r->dev_prolonged, pbytes) != hipSuccess)
+      const size_t lbytes = (r->lists.ints.empty() ? 1 : r->lists.ints.size()) * sizeof(int);
+      if (hipMalloc(&r->dev, bytes) != hipSuccess || hipMalloc(&r->dev_lists, lbytes) != hipSuccess)
       {
         r->err = "hipMalloc of the remap plan failed";
         return -1;
@@ -532,15 +535,15 @@ extern "C"
       r->device = dev;
       // synchronous: the host plan is pageable
       if (hipMemcpy(r->dev, r->plan.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-          (!r->prolonged.empty() && hipMemcpy(r->dev_prolonged, r->prolonged.data(), r->prolonged.size() * sizeof(int),
-                                              hipMemcpyHostToDevice) != hipSuccess))
+          (!r->lists.ints.empty() &&
+           hipMemcpy(r->dev_lists, r->lists.ints.data(), r->lists.ints.size() * sizeof(int), hipMemcpyHostToDevice) !=
+             hipSuccess))
       {
         r->err = "copy of the remap plan failed";
         return -1;
       }
     }
-    return mhdct::launch_remap(r->dim, r->b, nnew, r->dev, int(r->prolonged.size()), r->dev_prolonged, U_old, B_old,
-                               U_new, B_new, s, r->err);
+    return mhdct::launch_remap(r->dim, r->b, r->dev, r->lists, r->dev_lists, U_old, B_old, U_new, B_new, s, r->err);
   }
 
   const char *
@@ -556,8 +559,8 @@ extern "C"
       return;
     if (r->dev)
       (void)hipFree(r->dev);
-    if (r->dev_prolonged)
-      (void)hipFree(r->dev_prolonged);
+    if (r->dev_lists)
+      (void)hipFree(r->dev_lists);
     delete r;
   }
 

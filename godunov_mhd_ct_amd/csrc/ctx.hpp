@@ -25,6 +25,8 @@ struct Settings
   double scaling = 1.0;
   int    abort_when_negative_eint = 0;
   int    time_integrator = 0; // 0 HANCOCK, 1 RK2_SSP
+  int    gravity_enabled = 0;
+  double gravity[3] = { 0, 0, 0 };
 };
 
 // Mesh tables resident in HBM.

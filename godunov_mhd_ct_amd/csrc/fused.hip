@@ -529,7 +529,8 @@ cell_item(const FusedSlot<T> & S, long r0)
     const double dxo = dxd<BS>(m, S.scaling, o);
     const T      dtdx = T(S.dt / dxo), dtdy = dtdx, dtdz = dtdx;
     const T      gamma = S.gt.gamma0;
-    const T r = Q(0, i, j, k), p = Q(1, i, j, k), u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
+    const T r = Q(0, i, j, k), p = Q(1, i, j, k);
+    T       u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
     const T A = (Q(5, i, j, k) + Q(8, i, j, k)) * T(0.5);
     const T B = (Q(6, i, j, k) + Q(9, i, j, k)) * T(0.5);
     const T C = (Q(7, i, j, k) + Q(10, i, j, k)) * T(0.5);
@@ -579,6 +580,16 @@ cell_item(const FusedSlot<T> & S, long r0)
         sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
         sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
         sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+      }
+      // gravity, half time step, after the source terms
+      // (ComputeUpdatedPrimitiveVariablesFunctor.cpp:372-377, 505-511)
+      if (S.gt.grav)
+      {
+        const T dtT = T(S.dt);
+        u += S.gt.g[0] * T(0.5) * dtT;
+        v += S.gt.g[1] * T(0.5) * dtT;
+        if (DIM == 3)
+          w += S.gt.g[2] * T(0.5) * dtT;
       }
       bstore<T>(V.r_q2, 0 * G::G1 + c1, r + T(0.5) * sr0);
       bstore<T>(V.r_q2, 2 * G::G1 + c1, u + T(0.5) * su0);
@@ -1161,12 +1172,12 @@ inv_dt_item(const FusedSlot<T> & S, long r0)
   const double c2 = g.gamma0 * q[kP] / q[kD];
   const double d2 = 0.5 * (b2 / q[kD] + c2);
   const double dx = dxd<BS>(S.m, S.scaling, o);
-  const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
-  const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
-  if (DIM == 2)
-    return vx / dx + vy / dx;
-  const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
-  return vx / dx + vy / dx + vz / dx;
+  double       v[3];
+  v[0] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
+  v[1] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
+  if (DIM == 3)
+    v[2] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
+  return cell_rate<DIM>(v, dx, g);
 }
 
 template <typename T, int DIM, int BSEL>
@@ -1340,6 +1351,25 @@ grid_for(long n, int threads, int cap)
 }
 
 // Launch the fused stages of one step (which < 0), or only stage `which` of fused_stage_order.
+// Uniform gravity source on the owned cells of each slot that has gravity, over the dt stage's
+// work list (AddGravitySourceTerm.cpp:42-100); after the conservative update.
+template <typename T, int DIM, int BSEL>
+__global__ void __launch_bounds__(256)
+k_gravity(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  {
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = t - off[s];
+    if (!S.gd.grav || r0 >= S.cnt[kStageInvDt])
+      continue;
+    const long ncell = DIM == 3 ? long(S.m.b) * S.m.b * S.m.b : long(S.m.b) * S.m.b;
+    const long o = r0 / ncell, ci = r0 - o * ncell;
+    gravity_cell(DIM, S.gd, S.dt, S.U + o * 5 * ncell + ci, S.U2 + o * 5 * ncell + ci, ncell);
+  }
+}
+
 template <typename T, int DIM, int BSEL>
 int
 launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
@@ -1387,8 +1417,13 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
                        bt.tot[kStageFix]);
   }
   if (on(kStageBupdate))
+  {
     hipLaunchKernelGGL((k_bupdate<T, DIM, BSEL>), grid(kStageBupdate), 256, 0, st, sl, n, bt.d_off[kStageBupdate],
                        bt.tot[kStageBupdate]);
+    if (bt.grav)
+      hipLaunchKernelGGL((k_gravity<T, DIM, BSEL>), grid(kStageInvDt), 256, 0, st, sl, n, bt.d_off[kStageInvDt],
+                         bt.tot[kStageInvDt]);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -98,6 +98,7 @@ struct FusedBatch
   int            dim = 3;
   int            bsel = 0; // the block size of every slot, or 0 when they differ
   int            rk2 = 0;  // every slot is an RK2_SSP stage (per-context steps; batches are HANCOCK)
+  int            grav = 0; // some slot has gravity
   int            nslots = 0;
   FusedSlot<T> * d_slots[2] = { nullptr, nullptr };
   long *         d_off[kStages] = {};

@@ -67,6 +67,9 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.gt.smallc = T(c.st.smallc);
   sl.gt.smallp = T(sl.gd.smallp);
   sl.gt.slope_type = T(c.st.slope_type);
+  sl.gt.grav = c.st.gravity_enabled;
+  for (int d = 0; d < 3; ++d)
+    sl.gt.g[d] = T(c.st.gravity[d]);
   sl.cfl = c.st.cfl;
   sl.scaling = c.st.scaling;
   sl.abort_eint = c.st.abort_when_negative_eint;
@@ -125,6 +128,7 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   bt.dim = dim;
   bt.bsel = b;
   bt.rk2 = sl.rk2;
+  bt.grav = sl.gd.grav;
   bt.nslots = 1;
   for (int k = 0; k < kStages; ++k)
   {

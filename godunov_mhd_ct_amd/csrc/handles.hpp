@@ -23,9 +23,9 @@ struct mhdct_remap
   int                             b = 16;
   int                             counts[3] = { 0, 0, 0 };
   std::vector<mhdct::RemapEntry> plan;          // per new owned leaf
-  std::vector<int>                prolonged;     // the prolonged new leaves
+  mhdct::RemapLists               lists;         // copy pairs, moved, restricted, prolonged leaves
   mhdct::RemapEntry *             dev = nullptr; // the plan in device memory, on first apply
-  int *                           dev_prolonged = nullptr;
+  int *                           dev_lists = nullptr;
   int                             device = -1;
   std::string                     err;
 };

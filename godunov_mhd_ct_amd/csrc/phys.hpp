@@ -41,12 +41,68 @@ enum : int
 template <typename T>
 struct Gas
 {
-  T gamma0;
-  T smallr;
-  T smallc;
-  T smallp;
-  T slope_type;
+  T   gamma0;
+  T   smallr;
+  T   smallc;
+  T   smallp;
+  T   slope_type;
+  int grav;  // [gravity] enabled: uniform gravity field g
+  T   g[3];  // [gravity] uniform_gx, uniform_gy, uniform_gz
 };
+
+// Uniform gravity source of one cell (AddGravitySourceTerm.cpp:42-100); u and u2 point at the
+// cell's density in U and U2, variables `stride` apart.
+__host__ __device__ inline void
+gravity_cell(int dim, const Gas<double> & g, double dt, const double * u, double * u2, long stride)
+{
+  const double rho_old = u[0];
+  const double rho_new = u2[0];
+  double       rhou = u2[2 * stride], rhov = u2[3 * stride], rhow = 0.0;
+  if (dim == 3)
+    rhow = u2[4 * stride];
+  double ekin_old = 0.5 * (rhou * rhou + rhov * rhov) / rho_new;
+  if (dim == 3)
+    ekin_old += 0.5 * (rhow * rhow) / rho_new;
+  rhou += dt * g.g[0] * (rho_old + rho_new) / 2;
+  rhov += dt * g.g[1] * (rho_old + rho_new) / 2;
+  if (dim == 3)
+    rhow += dt * g.g[2] * (rho_old + rho_new) / 2;
+  u2[2 * stride] = rhou;
+  u2[3 * stride] = rhov;
+  if (dim == 3)
+    u2[4 * stride] = rhow;
+  double ekin_new = 0.5 * (rhou * rhou + rhov * rhov) / rho_new;
+  if (dim == 3)
+    ekin_new += 0.5 * (rhow * rhow) / rho_new;
+  u2[1 * stride] += (ekin_new - ekin_old);
+}
+
+// The CFL rate of a cell from its fastest signal speeds v[d] = c_f,d + |u_d|
+// (ComputeDtMHDFunctor.cpp:86-151), or, with gravity, the rate that also keeps
+// g dt^2 / (2 dx) + |v| dt / dx <= cfl: each v[d] / dx scaled by k / (sqrt(1 + 2k) - 1),
+// k = max(sum_d |g_d| dx / |v|^2, 1e-4) (compute_cfl_with_gravity, :108-249).
+template <int DIM>
+__host__ __device__ inline double
+cell_rate(const double * v, double dx, const Gas<double> & g)
+{
+  if (!g.grav)
+    return DIM == 2 ? v[0] / dx + v[1] / dx : v[0] / dx + v[1] / dx + v[2] / dx;
+  double v2 = v[0] * v[0] + v[1] * v[1];
+  if (DIM == 3)
+    v2 += v[2] * v[2];
+  double k = fabs(g.g[0]) + fabs(g.g[1]);
+  if (DIM == 3)
+    k += fabs(g.g[2]);
+  k *= dx / v2;
+  k = fmax(k, 1e-4);
+  const double factor = k / (sqrt(1.0 + 2.0 * k) - 1.0);
+  double       r = 0.0;
+  r += (v[0] * factor) / dx;
+  r += (v[1] * factor) / dx;
+  if (DIM == 3)
+    r += (v[2] * factor) / dx;
+  return r;
+}
 
 template <typename T>
 __host__ __device__ inline T

@@ -250,7 +250,7 @@ k_slopes(DevMesh m, double st, const double * __restrict__ Qp, double * Sx, doub
 // K6: Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_predictor(DevMesh m, double scaling, double gamma, double dt, int rk2, const double * __restrict__ Qp,
+k_predictor(DevMesh m, double scaling, Gas<double> gs, double dt, int rk2, const double * __restrict__ Qp,
             const double * __restrict__ Sx, const double * __restrict__ Sy, const double * __restrict__ Sz,
             double * __restrict__ Q2)
 {
@@ -270,8 +270,9 @@ k_predictor(DevMesh m, double scaling, double gamma, double dt, int rk2, const d
   auto        SZ = [&](int v) { return Sz[s0 + v * nc]; };
   const double dx = dx_of(m, scaling, o);
   const double dtdx = dt / dx, dtdy = dt / dx, dtdz = dt / dx;
-  const double r = q(i, j, k, 0, o), p = q(i, j, k, 1, o), u = q(i, j, k, 2, o), v = q(i, j, k, 3, o),
-               w = q(i, j, k, 4, o);
+  const double gamma = gs.gamma0;
+  const double r = q(i, j, k, 0, o), p = q(i, j, k, 1, o);
+  double       u = q(i, j, k, 2, o), v = q(i, j, k, 3, o), w = q(i, j, k, 4, o);
   const double A = (q(i, j, k, 5, o) + q(i, j, k, 8, o)) * 0.5;
   const double B = (q(i, j, k, 6, o) + q(i, j, k, 9, o)) * 0.5;
   const double C = (q(i, j, k, 7, o) + q(i, j, k, 10, o)) * 0.5;
@@ -319,6 +320,15 @@ k_predictor(DevMesh m, double scaling, double gamma, double dt, int rk2, const d
     sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
     sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
     sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+  }
+  // gravity, half time step, after the source terms (ComputeUpdatedPrimitiveVariablesFunctor.cpp:372-377,
+  // 505-511)
+  if (gs.grav)
+  {
+    u += gs.g[0] * 0.5 * dt;
+    v += gs.g[1] * 0.5 * dt;
+    if (dim == 3)
+      w += gs.g[2] * 0.5 * dt;
   }
   double * out = Q2 + s0;
   out[0 * nc] = r + 0.5 * sr0;
@@ -857,12 +867,13 @@ cell_inv_dt(const DevMesh & m, const IX & I, const Gas<double> & g, double scali
   const double c2 = g.gamma0 * q[kP] / q[kD];
   const double d2 = 0.5 * (b2 / q[kD] + c2);
   const double dx = 1.0 / double(long(b) << m.level[o]) * scaling;
-  const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
-  const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
+  double       v[3];
+  v[0] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
+  v[1] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
   if (dim == 2)
-    return vx / dx + vy / dx;
-  const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
-  return vx / dx + vy / dx + vz / dx;
+    return cell_rate<2>(v, dx, g);
+  v[2] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
+  return cell_rate<3>(v, dx, g);
 }
 
 // block max of the candidates, one atomic per workgroup; NaN candidates raise flag bit 2
@@ -1026,7 +1037,27 @@ gas_of(const Settings & s)
   g.smallc = s.smallc;
   g.smallp = s.smallc * s.smallc / s.gamma0;
   g.slope_type = s.slope_type;
+  g.grav = s.gravity_enabled;
+  for (int d = 0; d < 3; ++d)
+    g.g[d] = s.gravity[d];
   return g;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform gravity source (AddGravitySourceTerm.cpp:42-100): momentum += dt g (rho_old +
+// rho_new) / 2, total energy += the kinetic energy change, on the owned cells of U2.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_gravity_source(DevMesh m, Gas<double> g, double dt, const double * __restrict__ U, double * __restrict__ U2)
+{
+  const Idx  I{ m.dim, m.b };
+  const long ncell = I.cells();
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(m.nowned) * ncell)
+    return;
+  const int  o = int(t / ncell);
+  const long ci = t - long(o) * ncell;
+  gravity_cell(m.dim, g, dt, U + long(o) * 5 * ncell + ci, U2 + long(o) * 5 * ncell + ci, ncell);
 }
 
 #define MHDCT_HIP(x)                                                                               \
@@ -1105,7 +1136,7 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
   hipLaunchKernelGGL(k_slopes, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, g.slope_type, c.ps.Q, c.ps.S[0],
                      c.ps.S[1], c.ps.S[2]);
   const int rk2 = c.st.time_integrator == 1;
-  hipLaunchKernelGGL(k_predictor, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, g.gamma0, dt, rk2, c.ps.Q,
+  hipLaunchKernelGGL(k_predictor, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, g, dt, rk2, c.ps.Q,
                      c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.Q2);
   hipLaunchKernelGGL(k_elec_field, grid_of(nq * S.gcells(2)), kThreads, 0, s, m, c.ps.Q, c.ps.E);
   hipLaunchKernelGGL(k_sfacemag, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, dt, c.ps.E, c.ps.sF);
@@ -1122,6 +1153,8 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
   if (m.nfix > 0)
     hipLaunchKernelGGL(k_emf_fix, grid_of(m.nfix), kThreads, 0, s, m.nfix, m.fix, m.fix_src, c.ps.emf, c.ps.emf2);
   hipLaunchKernelGGL(k_emf_update, grid_of(nq * S.fper()), kThreads, 0, s, m, c.ps.emf2, B2);
+  if (g.grav)
+    hipLaunchKernelGGL(k_gravity_source, grid_of(nq * S.cells()), kThreads, 0, s, m, g, dt, U, U2);
   MHDCT_HIP(hipGetLastError());
   return 0;
 }

@@ -31,8 +31,9 @@ PROLONGATION = {"SIMPLE_COPY": 0, "EXTRAPOLATE_LINEAR_MINMOD": 1}
 
 def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling_factor=1.0,
                    riemann="HLLD", prolongation="SIMPLE_COPY", abort_when_negative_eint=False,
-                   time_integrator="HANCOCK") -> Settings:
-    """Defaults of HydroSettings (models/HydroSettings.cpp:19-30), plus [run] time_integrator."""
+                   time_integrator="HANCOCK", gravity=None) -> Settings:
+    """Defaults of HydroSettings (models/HydroSettings.cpp:19-30), plus [run] time_integrator and
+    [gravity] (gravity=(gx, gy, gz) enables a uniform field; None disables it)."""
     s = Settings()
     s.gamma0 = float(gamma0)
     s.cfl = float(cfl)
@@ -49,6 +50,11 @@ def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-
         s.time_integrator = TIME_INTEGRATOR[time_integrator]
     else:
         s.time_integrator = int(time_integrator)
+    if gravity is not None:
+        g = tuple(float(x) for x in gravity) + (0.0,) * (3 - len(gravity))
+        s.gravity_enabled = 1
+        for d in range(3):
+            s.gravity[d] = g[d]
     return s
 
 

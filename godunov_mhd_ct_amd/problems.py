@@ -167,6 +167,32 @@ def field_loop_2d(mesh: Mesh, gamma=5.0 / 3.0, scaling=1.0, xmin=(-1.0, -0.5, 0.
     return _assemble(mesh, rho, mom, p, [Bx.reshape(n, -1), By.reshape(n, -1), np.zeros((n, b * b))], gamma)
 
 
+def rayleigh_taylor(mesh: Mesh, gravity, gamma=1.4, rho_up=2.0, rho_down=1.0, P0=10.0, amplitude=0.01,
+                    nmodes=(1, 1, 1), bfield=(0.0, 0.5, 0.0), scaling=1.0, xmin=(0.0, 0.0, 0.0)):
+    """Magnetic Rayleigh-Taylor (init/InitRayleighTaylor.cpp, sine perturbation, not isothermal):
+    heavy fluid above the mid-plane normal to the last axis, hydrostatic P = P0 + rho g.x, a
+    velocity perturbation along that axis, uniform B."""
+    dim = mesh.dim
+    xs = _grid(mesh, scaling, xmin, "cell")
+    L = [mesh.nbrick[d] * scaling for d in range(3)]
+    up = xs[dim - 1] > xmin[dim - 1] + L[dim - 1] / 2
+    rho = np.where(up, rho_up, rho_down)
+    P = P0 + rho * sum(gravity[d] * xs[d] for d in range(dim))
+    pert = amplitude * rho
+    for d in range(dim):
+        pert = pert * (1 + np.cos(2 * math.pi * xs[d] * nmodes[d] / L[d]))
+    pert = pert / (4.0 if dim == 2 else 8.0)
+    mom = [np.zeros_like(rho) for _ in range(3)]
+    mom[dim - 1] = pert
+    n = mesh.num_total
+    b = mesh.b
+    parts = []
+    for c in range(3):
+        size = (b * b if c == 2 else (b + 1) * b) if dim == 2 else (b + 1) * b * b
+        parts.append(np.full((n, size), bfield[c]))
+    return _assemble(mesh, rho, mom, P, parts, gamma)
+
+
 # ---------------------------------------------------------------------------------------------
 # random turbulent-like state (the batching workload); evaluated with torch so that large
 # meshes are set up in seconds (on the GPU when there is one)

@@ -56,6 +56,8 @@ typedef struct mhdct_settings
   int    prolongation;   /* [amr] cell_centered_prolongation_type: 0 = SIMPLE_COPY, the only one supported */
   int    abort_when_negative_eint; /* [hydro] abort_when_negative_eint: report eint < 0 as an error */
   int    time_integrator; /* [run] time_integrator: MHDCT_TIME_HANCOCK or MHDCT_TIME_RK2_SSP */
+  int    gravity_enabled; /* [gravity] enabled: a uniform gravity field (AddGravitySourceTerm.cpp) */
+  double gravity[3];      /* [gravity] uniform_gx, uniform_gy, uniform_gz */
 } mhdct_settings;
 
 /* [run] time_integrator values (SolverGodunovMHD.cpp:494, 551; RK3_SSP is rejected, as the

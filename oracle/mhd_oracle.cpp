@@ -95,6 +95,9 @@ struct Settings
   // [run] time_integrator: 0 HANCOCK, 1 RK2_SSP (kalypsso::core's TimeIntegrator; read by the
   // functors through TimeIntegratorConfig::get_time_integrator)
   int    time_integrator = 0;
+  // [gravity] enabled, uniform_gx, uniform_gy, uniform_gz (get_uniform_gravity_vector, core)
+  int    gravity_enabled = 0;
+  double gravity[3] = { 0, 0, 0 };
 };
 
 // ---------------------------------------------------------------------------
@@ -1356,9 +1359,9 @@ public:
       const double dtdx = dt / dx, dtdy = dt / dx, dtdz = dt / dx;
       const double r = q(i, j, k, ID, o);
       const double p = q(i, j, k, IP, o);
-      const double u = q(i, j, k, IU, o);
-      const double v = q(i, j, k, IV, o);
-      const double w = q(i, j, k, IW, o);
+      double       u = q(i, j, k, IU, o);
+      double       v = q(i, j, k, IV, o);
+      double       w = q(i, j, k, IW, o);
       const double A = (q(i, j, k, IAL, o) + q(i, j, k, IAR, o)) * 0.5;
       const double B = (q(i, j, k, IBL, o) + q(i, j, k, IBR, o)) * 0.5;
       const double C = (q(i, j, k, ICL, o) + q(i, j, k, ICR, o)) * 0.5;
@@ -1412,6 +1415,14 @@ public:
         sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
         sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
         sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+      }
+      // gravity predictor, half time step (.cpp:372-377 2D: u, v; :505-511 3D: u, v, w)
+      if (s.gravity_enabled)
+      {
+        u += s.gravity[IX] * 0.5 * dt;
+        v += s.gravity[IY] * 0.5 * dt;
+        if (dim == 3)
+          w += s.gravity[IZ] * 0.5 * dt;
       }
       q2(i, j, k, ID, o) = r + 0.5 * sr0;
       q2(i, j, k, IU, o) = u + 0.5 * su0;
@@ -2206,6 +2217,39 @@ public:
     }
   }
 
+  // AddGravitySourceTerm::operator() (scheme/AddGravitySourceTerm.cpp:42-100), owned cells
+  void
+  add_gravity_source_term(const double * U, double * U2, double dt)
+  {
+    const int  dim = m.dim;
+    const long ncell = lay.cells();
+    const long total = long(m.nowned) * ncell;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long gi = 0; gi < total; ++gi)
+    {
+      const long   o = gi / ncell, ci = gi - o * ncell;
+      auto         at = [&](int var) -> double & { return U2[size_t(o * 5 + var) * ncell + ci]; };
+      const double rho_old = U[size_t(o * 5 + ID) * ncell + ci];
+      const double rho_new = at(ID);
+      double       rhou = at(IU), rhov = at(IV), rhow = dim == 3 ? at(IW) : 0.0;
+      double       ekin_old = 0.5 * (rhou * rhou + rhov * rhov) / rho_new;
+      if (dim == 3)
+        ekin_old += 0.5 * (rhow * rhow) / rho_new;
+      rhou += dt * s.gravity[IX] * (rho_old + rho_new) / 2;
+      rhov += dt * s.gravity[IY] * (rho_old + rho_new) / 2;
+      if (dim == 3)
+        rhow += dt * s.gravity[IZ] * (rho_old + rho_new) / 2;
+      at(IU) = rhou;
+      at(IV) = rhov;
+      if (dim == 3)
+        at(IW) = rhow;
+      double ekin_new = 0.5 * (rhou * rhou + rhov * rhov) / rho_new;
+      if (dim == 3)
+        ekin_new += 0.5 * (rhow * rhow) / rho_new;
+      at(IP) += (ekin_new - ekin_old);
+    }
+  }
+
   // GodunovImplemV0::do_time_step (scheme/GodunovImplemV0.cpp:101-207).  Contract: U2 == U
   // and B2 == B on entry (the solver copies them); on exit U2, B2 hold t+dt.
   int
@@ -2232,6 +2276,8 @@ public:
       emf2 = emf; // Kokkos::deep_copy(m_emf2, m_emf) (GodunovImplemV0.cpp:606)
       correct_emf();
       read_emf_and_update(B2);
+      if (s.gravity_enabled) // GodunovImplemV0.cpp:200-205
+        add_gravity_source_term(U, U2, dt);
     }
     catch (std::exception & ex)
     {
@@ -2277,10 +2323,32 @@ public:
       compute_primitives(u, qq, s);
       find_speed_info(qq, dim, s, v);
       double cand;
-      if (dim == 2)
-        cand = v[IX] / dx + v[IY] / dx;
+      if (!s.gravity_enabled)
+      {
+        if (dim == 2)
+          cand = v[IX] / dx + v[IY] / dx;
+        else
+          cand = v[IX] / dx + v[IY] / dx + v[IZ] / dx;
+      }
       else
-        cand = v[IX] / dx + v[IY] / dx + v[IZ] / dx;
+      {
+        // ComputeDtMHDFunctor::compute_cfl_with_gravity (.cpp:108-249): g dt^2/(2dx) + |v| dt/dx
+        // <= cfl, i.e. each v/dx scaled by k / (sqrt(1 + 2k) - 1), k = sum|g| dx / |v|^2 >= 1e-4
+        double v2 = v[IX] * v[IX] + v[IY] * v[IY];
+        if (dim == 3)
+          v2 += v[IZ] * v[IZ];
+        double kk = std::fabs(s.gravity[IX]) + std::fabs(s.gravity[IY]);
+        if (dim == 3)
+          kk += std::fabs(s.gravity[IZ]);
+        kk *= dx / v2;
+        kk = std::fmax(kk, 1e-4);
+        const double factor = kk / (std::sqrt(1.0 + 2.0 * kk) - 1.0);
+        cand = 0.0;
+        cand += (v[IX] * factor) / dx;
+        cand += (v[IY] * factor) / dx;
+        if (dim == 3)
+          cand += (v[IZ] * factor) / dx;
+      }
       invDt = std::fmax(invDt, cand);
     }
     return invDt;
@@ -2368,7 +2436,7 @@ extern "C"
 {
 
   // settings: gamma0, cfl, slope_type, smallr, smallc, scaling, abort_when_negative_eint,
-  // time_integrator (0 HANCOCK, 1 RK2_SSP)
+  // time_integrator (0 HANCOCK, 1 RK2_SSP), gravity_enabled, gx, gy, gz
   void *
   oracle_create(int dim, int b, const double * settings, int nowned, int noutside,
                 const int * levels, const int * coords, const int * nbrick, const int * periodic,
@@ -2383,6 +2451,9 @@ extern "C"
     S->s.scaling = settings[5];
     S->s.abort_when_negative_eint = settings[6] != 0.0;
     S->s.time_integrator = int(settings[7]);
+    S->s.gravity_enabled = settings[8] != 0.0;
+    for (int d = 0; d < 3; ++d)
+      S->s.gravity[d] = d < dim ? settings[9 + d] : 0.0;
     S->m.dim = dim;
     S->m.b = b;
     S->m.nowned = nowned;

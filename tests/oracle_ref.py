@@ -82,9 +82,11 @@ RK2_STAGE2 = (0.5, 0.5)
 
 
 def settings_array(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling=1.0,
-                   abort_eint=0, time_integrator="HANCOCK"):
+                   abort_eint=0, time_integrator="HANCOCK", gravity=None):
     ti = TIME_INTEGRATOR[time_integrator] if isinstance(time_integrator, str) else int(time_integrator)
-    return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint, ti], dtype=np.float64)
+    g = (0.0, 0.0, 0.0) if gravity is None else tuple(float(x) for x in gravity) + (0.0,) * (3 - len(gravity))
+    return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint, ti,
+                     0.0 if gravity is None else 1.0, *g], dtype=np.float64)
 
 
 class OracleSolver:

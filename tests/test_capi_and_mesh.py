@@ -123,7 +123,7 @@ def test_settings_struct_and_time_integrator():
     txt = HEADER.read_text()
     body = re.search(r"typedef struct mhdct_settings\s*\{(.*?)\}\s*mhdct_settings;", txt, re.S).group(1)
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    fields = re.findall(r"\b(?:double|int)\s+([a-z0-9_]+)\s*;", body)
+    fields = re.findall(r"\b(?:double|int)\s+([a-z0-9_]+)\s*(?:\[\d+\])?\s*;", body)
     assert fields == [f[0] for f in Settings._fields_]
     assert hydro_settings(time_integrator="RK2_SSP").time_integrator == 1
     assert hydro_settings().time_integrator == 0

@@ -34,16 +34,17 @@ def compare(Ug, Bg, Uo, Bo, nowned, tag):
     return same, errs
 
 
-def run_pair(mesh, U, B, gamma, cfl, slope_type, nsteps, impl, bc=None, time_integrator="HANCOCK"):
+def run_pair(mesh, U, B, gamma, cfl, slope_type, nsteps, impl, bc=None, time_integrator="HANCOCK", gravity=None):
     import torch
     from godunov_mhd_ct_amd.solver import SolverGodunovMHD
 
     print(f"[parity] impl={impl} {time_integrator} dim={mesh.dim} b={mesh.b} leaves={mesh.num_leaves} "
           f"outside={mesh.num_outside} steps={nsteps}", flush=True)
-    st = hydro_settings(gamma0=gamma, cfl=cfl, slope_type=slope_type, time_integrator=time_integrator)
+    st = hydro_settings(gamma0=gamma, cfl=cfl, slope_type=slope_type, time_integrator=time_integrator,
+                        gravity=gravity)
     sol = SolverGodunovMHD(mesh, U, B, st, impl_version=impl)
     ora = OracleSolver(mesh, settings_array(gamma0=gamma, cfl=cfl, slope_type=slope_type,
-                                            time_integrator=time_integrator), nthreads=4, bc=bc)
+                                            time_integrator=time_integrator, gravity=gravity), nthreads=4, bc=bc)
     Uo = U.reshape(mesh.num_total, 5, -1).copy()
     Bo = B.reshape(mesh.num_total, -1).copy()
     all_same = True
