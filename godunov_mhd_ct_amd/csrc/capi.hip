@@ -56,15 +56,17 @@ upload(Context & c, T *& dst, const std::vector<T> & src)
 void
 free_mesh(Context & c)
 {
-  void * ps[] = { c.dm.level, c.dm.nb, c.dm.conformal, c.dm.fix, c.dm.fix_src, c.dm.out_src, c.dm.out_dirs };
+  void * ps[] = { c.dm.level, c.dm.coord, c.dm.nb, c.dm.conformal, c.dm.fix, c.dm.fix_src, c.dm.out_src, c.dm.out_dirs };
   for (void * p : ps)
     if (p)
       (void)hipFree(p);
   c.dm.level = nullptr;
+  c.dm.coord = nullptr;
   c.dm.nb = nullptr;
   c.dm.conformal = nullptr;
   c.dm.fix = nullptr;
   c.dm.fix_src = nullptr;
+  c.dm.nfix = 0;
   c.dm.out_src = nullptr;
   c.dm.out_dirs = nullptr;
   c.mesh_bytes = 0;
@@ -296,17 +298,24 @@ extern "C"
     c.dm.nowned = int(c.mesh.leaves.size());
     c.dm.ntotal = c.mesh.ntotal();
     c.dm.noutside = int(c.mesh.outside.size());
-    c.dm.nfix = int(c.mesh.emf_fix.size());
     for (int f = 0; f < 6; ++f)
       c.dm.bc[f] = c.mesh.bc[f];
-    std::vector<int> lv;
-    for (auto & L : c.mesh.leaves)
-      lv.push_back(L.level);
-    for (auto & L : c.mesh.outside)
-      lv.push_back(L.level);
-    if (upload(c, c.dm.level, lv) || upload(c, c.dm.nb, c.mesh.nb) || upload(c, c.dm.conformal, c.mesh.conformal) ||
-        upload(c, c.dm.fix, c.mesh.emf_fix) || upload(c, c.dm.fix_src, c.mesh.emf_src) ||
-        upload(c, c.dm.out_src, c.mesh.outside_src) || upload(c, c.dm.out_dirs, c.mesh.outside_dirs))
+    for (int d = 0; d < 3; ++d)
+      c.dm.nbrick[d] = c.mesh.nbrick[d];
+    std::vector<int> lv, co;
+    lv.reserve(size_t(c.dm.ntotal));
+    co.reserve(size_t(c.dm.ntotal) * 3);
+    for (const auto * list : { &c.mesh.leaves, &c.mesh.outside })
+      for (auto & L : *list)
+      {
+        lv.push_back(L.level);
+        co.push_back(L.x);
+        co.push_back(L.y);
+        co.push_back(L.z);
+      }
+    if (upload(c, c.dm.level, lv) || upload(c, c.dm.coord, co) || upload(c, c.dm.nb, c.mesh.nb) ||
+        upload(c, c.dm.conformal, c.mesh.conformal) || upload(c, c.dm.out_src, c.mesh.outside_src) ||
+        upload(c, c.dm.out_dirs, c.mesh.outside_dirs) || mhdct::build_border_list(c))
       return -1;
     int rc = 0;
     if (c.impl == MHDCT_IMPL_PLAIN)

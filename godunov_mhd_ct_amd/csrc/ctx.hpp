@@ -39,11 +39,13 @@ struct DevMesh
   int              nfix = 0;
   int              noutside = 0;
   int              bc[6] = { 0, 0, 0, 0, 0, 0 };
+  int              nbrick[3] = { 1, 1, 1 };
   int *            level = nullptr;     // [ntotal]
+  int *            coord = nullptr;     // [ntotal][3] block coordinates at the block's level
   NbEntry *        nb = nullptr;        // [nowned][3^dim][2^dim]
   uint8_t *        conformal = nullptr; // [nowned][2*dim]
-  EmfFix *         fix = nullptr;       // [nfix]
-  int64_t *        fix_src = nullptr;
+  EmfFix *         fix = nullptr;       // [nfix] corrected border edges, built on the device
+  int64_t *        fix_src = nullptr;   // [nfix][8] their sources
   int *            out_src = nullptr;   // [noutside]
   int *            out_dirs = nullptr;  // [noutside*3]
 };
@@ -127,6 +129,8 @@ int  plain_step(Context & c, const double * U, double * U2, const double * B, do
                 hipStream_t s);
 int  launch_inv_dt(Context & c, const double * U, const double * B, hipStream_t s);
 int  launch_fill_outside(Context & c, double * U, double * B, hipStream_t s);
+// the border-edge gather list of the uploaded mesh, built on the device (border.hip)
+int  build_border_list(Context & c);
 int  launch_linear_combination(Context & c, const double * U, double * U_RK, const double * B, double * B_RK,
                                double c0, double c1, hipStream_t s);
 

@@ -340,7 +340,15 @@ Mesh::build_nb()
   const int nd = ndirs(), ns = 1 << dim;
   nb.assign(leaves.size() * size_t(nd) * ns, NbEntry{ -1, 0, 0, { 0, 0, 0 } });
   conformal.assign(leaves.size() * 2 * dim, 0);
-  for (size_t o = 0; o < leaves.size(); ++o)
+  // leaves in contiguous ranges on host threads (each writes its own entries); the first error
+  // in leaf order is reported
+  const int           nleaves = int(leaves.size());
+  const int           nthr = std::max(1, std::min<int>(int(std::thread::hardware_concurrency()), std::min(16, nleaves / 64)));
+  std::vector<std::string> errs(nthr);
+  auto one_range = [&](int t) -> bool {
+    const int lo = int(long(nleaves) * t / nthr), hi = int(long(nleaves) * (t + 1) / nthr);
+    std::string & err = errs[t];
+  for (size_t o = size_t(lo); o < size_t(hi); ++o)
   {
     const Leaf & C = leaves[o];
     const long   org[3] = { long(C.x) * b, long(C.y) * b, long(C.z) * b };
@@ -411,290 +419,25 @@ Mesh::build_nb()
         conformal[o * 2 * dim + 2 * d + side] = nb[(o * nd + di) * ns + 0].rel == 1 ? 1 : 0;
       }
   }
-  return true;
-}
-
-// Border-edge corrections (scheme/CorrectEmfAtBlockBorderFunctor.cpp:94-360) as an explicit
-// gather list.  The four cells around an edge are the quadrants (s1,s2) over the two
-// tangential axes (cell = edge index - 1 + s); the block's own quadrant rounds the edge index
-// down (clamped to b-1); the other three are the sibling slots, invalid when inside the block.
-bool
-Mesh::build_emf_fix()
-{
-  emf_fix.clear();
-  emf_src.clear();
-  const int  ncomp = dim == 3 ? 3 : 1;
-  const long n1 = b + 1;
-  const long ne = dim == 3 ? n1 * n1 * n1 : n1 * n1;
-  const double surface_ratio = double(1 << (dim - 1));
-  auto eidx = [&](long oct, int comp, long i, long j, long k) {
-    return (oct * ncomp + comp) * ne + i + n1 * (j + n1 * k);
+    return true;
   };
-  const int nowned = int(leaves.size());
-  const int nd = ndirs(), ns = 1 << dim;
-  // the gather list of one leaf, appended to (fx, sx); `first` indexes sx
-  auto one_leaf = [&](int o, std::vector<EmfFix> & fx, std::vector<int64_t> & sx) {
-    const Leaf & C = leaves[o];
-    const long   org[3] = { long(C.x) * b, long(C.y) * b, long(C.z) * b };
-    // level relation of the face neighbour across each face (uniform over the face in a 2:1
-    // balanced forest; from the neighbour table built before)
-    int face_rel[3][2];
-    for (int d = 0; d < dim; ++d)
-      for (int side = 0; side < 2; ++side)
-      {
-        int dl[3] = { 0, 0, 0 };
-        dl[d] = side ? 1 : -1;
-        const int di = (dl[0] + 1) + 3 * (dl[1] + 1) + (dim == 3 ? 9 * (dl[2] + 1) : 0);
-        const NbEntry & E = nb[(size_t(o) * nd + di) * ns + 0];
-        face_rel[d][side] = E.oct < 0 ? 99 : E.rel;
-      }
-    for (int comp = 0; comp < ncomp; ++comp)
-    {
-      const int a = dim == 3 ? comp : 2;
-      const int t1 = a == 0 ? 1 : 0;
-      const int t2 = a == 2 ? 1 : 2;
-      const int nal = dim == 3 ? b : 1;
-      // the border ring of the (e1, e2) square: rows e2 = 0 and e2 = b, then columns e1 = 0
-      // and e1 = b between them (4b edges per ea)
-      const int nring = 4 * b;
-      for (int ea = 0; ea < nal; ++ea)
-        for (int ri = 0; ri < nring; ++ri)
-          {
-            int e1, e2;
-            if (ri < 2 * (b + 1))
-            {
-              e2 = ri < b + 1 ? 0 : b;
-              e1 = ri % (b + 1);
-            }
-            else
-            {
-              const int rj = ri - 2 * (b + 1); // 0 .. 2(b-1)-1
-              e1 = rj < b - 1 ? 0 : b;
-              e2 = 1 + rj % (b - 1);
-            }
-            const bool b1 = (e1 == 0 || e1 == b), b2 = (e2 == 0 || e2 == b);
-            // an edge inside one face whose neighbour has this block's level: the two cells
-            // across are that neighbour's and the other one is this block's, so no correction
-            if (b1 != b2)
-            {
-              const int fd = b1 ? t1 : t2;
-              const int fs = (b1 ? e1 : e2) == b ? 1 : 0;
-              if (face_rel[fd][fs] == 0)
-                continue;
-            }
-            long e[3] = { 0, 0, 0 };
-            e[t1] = e1;
-            e[t2] = e2;
-            if (dim == 3)
-              e[a] = ea;
-            const int own1 = e1 < b ? 1 : 0, own2 = e2 < b ? 1 : 0;
-            const int ownq = own1 + 2 * own2;
-            struct S
-            {
-              bool valid = false, out = false;
-              int  oct = -1, rel = 0;
-              long e[3] = { 0, 0, 0 };
-            } sl[4];
-            for (int q = 0; q < 4; ++q)
-            {
-              if (q == ownq)
-                continue;
-              const int s1 = q & 1, s2 = q >> 1;
-              long      c[3] = { e[0], e[1], e[2] };
-              c[t1] = e[t1] - 1 + s1;
-              c[t2] = e[t2] - 1 + s2;
-              bool inside = true;
-              for (int d = 0; d < dim; ++d)
-                if (c[d] < 0 || c[d] >= b)
-                  inside = false;
-              if (inside)
-                continue;
-              // the leaf holding cell c, from the neighbour table (the cell's direction and,
-              // along the directions it does not leave, its half of the block)
-              int dir = 0, sub = 0, mul = 1;
-              for (int d = 0; d < dim; ++d)
-              {
-                const int dl = c[d] < 0 ? -1 : (c[d] >= b ? 1 : 0);
-                dir += (dl + 1) * mul;
-                mul *= 3;
-                if (dl == 0 && c[d] >= b / 2)
-                  sub |= 1 << d;
-              }
-              const NbEntry & E = nb[(size_t(o) * nd + dir) * ns + sub];
-              if (E.oct < 0)
-                continue;
-              const int oct = E.oct, rel = E.rel;
-              sl[q].valid = true;
-              sl[q].oct = oct;
-              sl[q].rel = rel;
-              sl[q].out = oct >= nowned;
-              const Leaf & F = oct < nowned ? leaves[oct] : outside[oct - nowned];
-              for (int d = 0; d < dim; ++d)
-              {
-                const int  Ls = C.level + rel;
-                const long n = long(nbrick[d]) * b << Ls; // ncells(Ls, d)
-                long       G = org[d] + e[d];
-                long       Gs = rel == 1 ? 2 * G : (rel == 0 ? G : (G >> 1)); // floor(G / 2)
-                long       lc = Gs - long(d == 0 ? F.x : (d == 1 ? F.y : F.z)) * b;
-                if (d == a)
-                {
-                  // into [0, n): lc lies within a few periods of it
-                  while (lc < 0)
-                    lc += n;
-                  while (lc >= n)
-                    lc -= n;
-                }
-                else
-                {
-                  if (lc < 0)
-                    lc += n;
-                  if (lc > b)
-                    lc -= n;
-                }
-                sl[q].e[d] = lc;
-              }
-            }
-            bool coarser = false, finer = false, all_same = true;
-            for (int q = 0; q < 4; ++q)
-            {
-              if (q == ownq)
-                continue;
-              if (sl[q].valid && sl[q].rel < 0)
-                coarser = true;
-              if (sl[q].valid && sl[q].rel > 0)
-                finer = true;
-              if (!(sl[q].valid && sl[q].rel == 0))
-                all_same = false;
-            }
-            int64_t srcv[8];
-            int     nsrc = 0;
-            auto    push = [&](int64_t v) { srcv[nsrc++] = v; };
-            int     cnt = 0;
-            double               ratio = 1.0;
-            bool                 write = false;
-            const int64_t own_idx = eidx(o, comp, e[0], e[1], e[2]);
-            if (coarser || all_same)
-            {
-              for (int q = 0; q < 4; ++q)
-              {
-                if (q == ownq)
-                {
-                  push(own_idx);
-                  ++cnt;
-                  continue;
-                }
-                const bool take = all_same ? (sl[q].valid && !sl[q].out)
-                                           : (sl[q].valid && sl[q].rel == 0 && !sl[q].out);
-                if (take)
-                {
-                  push(eidx(sl[q].oct, comp, sl[q].e[0], sl[q].e[1], sl[q].e[2]));
-                  ++cnt;
-                }
-              }
-              write = true;
-            }
-            else if (finer)
-            {
-              for (int q = 0; q < 4; ++q)
-              {
-                if (q == ownq)
-                  continue;
-                if (sl[q].valid && sl[q].rel > 0 && !sl[q].out)
-                {
-                  push(eidx(sl[q].oct, comp, sl[q].e[0], sl[q].e[1], sl[q].e[2]));
-                  if (dim == 3)
-                  {
-                    long e2v[3] = { sl[q].e[0], sl[q].e[1], sl[q].e[2] };
-                    e2v[a] += 1;
-                    push(eidx(sl[q].oct, comp, e2v[0], e2v[1], e2v[2]));
-                  }
-                  ++cnt;
-                }
-              }
-              if (cnt > 0)
-              {
-                ratio = surface_ratio;
-                write = true;
-              }
-            }
-            if (!write)
-              continue;
-            // a lone own value divided by one is the identity: nothing to correct
-            if (nsrc == 1 && srcv[0] == own_idx && cnt == 1)
-              continue;
-            EmfFix f;
-            f.dst = own_idx;
-            f.first = int32_t(sx.size());
-            f.nsrc = int16_t(nsrc);
-            f.cnt = int16_t(cnt);
-            f.ratio = ratio;
-            fx.push_back(f);
-            for (int q = 0; q < nsrc; ++q)
-              sx.push_back(srcv[q]);
-          }
-    }
-  };
-  // leaves in contiguous ranges on host threads, concatenated in leaf order (the list is the
-  // same for any thread count)
-  const int nthr = std::max(1, std::min<int>(int(std::thread::hardware_concurrency()), std::min(16, nowned / 64)));
-  std::vector<std::vector<EmfFix>>  fxs(nthr);
-  std::vector<std::vector<int64_t>> sxs(nthr);
-  auto run = [&](int t) {
-    const int lo = int(long(nowned) * t / nthr), hi = int(long(nowned) * (t + 1) / nthr);
-    // room for the block-edge lines of every leaf; faces beside another level add more
-    fxs[t].reserve(size_t(hi - lo) * ncomp * 4 * (b + 1));
-    sxs[t].reserve(fxs[t].capacity() * 4);
-    for (int o = lo; o < hi; ++o)
-      one_leaf(o, fxs[t], sxs[t]);
-  };
+  std::vector<char> ok(nthr, 1);
   if (nthr == 1)
-    run(0);
+    ok[0] = one_range(0);
   else
   {
     std::vector<std::thread> pool;
     for (int t = 0; t < nthr; ++t)
-      pool.emplace_back(run, t);
+      pool.emplace_back([&, t] { ok[t] = one_range(t); });
     for (auto & th : pool)
       th.join();
   }
-  size_t nf = 0, nsx = 0;
   for (int t = 0; t < nthr; ++t)
-    nf += fxs[t].size(), nsx += sxs[t].size();
-  if (nsx > size_t(INT32_MAX))
-  {
-    err = "border-edge gather list too large";
-    return false;
-  }
-  emf_fix.resize(nf);
-  emf_src.resize(nsx);
-  std::vector<size_t> f0(nthr + 1, 0), s0(nthr + 1, 0);
-  for (int t = 0; t < nthr; ++t)
-  {
-    f0[t + 1] = f0[t] + fxs[t].size();
-    s0[t + 1] = s0[t] + sxs[t].size();
-  }
-  auto gather = [&](int t) {
-    const int32_t base = int32_t(s0[t]);
-    EmfFix *      dst = emf_fix.data() + f0[t];
-    for (size_t i = 0; i < fxs[t].size(); ++i)
+    if (!ok[t])
     {
-      dst[i] = fxs[t][i];
-      dst[i].first += base;
+      err = errs[t];
+      return false;
     }
-    std::copy(sxs[t].begin(), sxs[t].end(), emf_src.begin() + long(s0[t]));
-    std::vector<EmfFix>().swap(fxs[t]);
-    std::vector<int64_t>().swap(sxs[t]);
-  };
-  if (nthr == 1)
-    gather(0);
-  else
-  {
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nthr; ++t)
-      pool.emplace_back(gather, t);
-    for (auto & th : pool)
-      th.join();
-  }
   return true;
 }
 
@@ -714,8 +457,6 @@ Mesh::finalize()
   if (!build_outside())
     return false;
   if (!build_nb())
-    return false;
-  if (!build_emf_fix())
     return false;
   return true;
 }

@@ -35,7 +35,8 @@ struct NbEntry
 };
 static_assert(sizeof(NbEntry) == 12, "NbEntry layout");
 
-// A corrected edge value: dst = (sum_k emf[src_k]) / cnt / ratio, summed in list order.
+// One corrected border edge of the gather list: dst = (sum of nsrc sources from `first`) /
+// cnt / ratio.
 struct EmfFix
 {
   int64_t dst;
@@ -45,6 +46,7 @@ struct EmfFix
   double  ratio;
 };
 static_assert(sizeof(EmfFix) == 24, "EmfFix layout");
+
 
 class Mesh
 {
@@ -63,8 +65,6 @@ public:
   // derived tables (rebuilt by finalize())
   std::vector<NbEntry> nb;           // [leaf][3^dim][2^dim]
   std::vector<uint8_t> conformal;    // [leaf][2*dim]: 1 if the face neighbour is finer
-  std::vector<EmfFix>  emf_fix;      // corrected border edges of owned leaves
-  std::vector<int64_t> emf_src;      // their sources (flat emf indices)
   std::string          err;
 
   static uint64_t key(int l, int x, int y, int z);
@@ -93,7 +93,6 @@ private:
   void rebuild_hash();
   bool build_outside();
   bool build_nb();
-  bool build_emf_fix();
 };
 
 } // namespace mhdct
