@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--amr-cycle", action="store_true",
                     help="time one AMR cycle (refinement flags, host adaptation, remap) on the workload after "
                          "--steps steps, [amr] keys of app/test_mhd_blast_3d.ini")
+    ap.add_argument("--checkpoint", action="store_true",
+                    help="time writing a checkpoint and the XDMF output of the workload's first mesh after "
+                         "--steps steps, and a restart from that checkpoint")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
@@ -196,6 +199,8 @@ def main():
     cases = cases_for(args.workload)
     if args.amr_cycle:
         return bench_amr(args, cases)
+    if args.checkpoint:
+        return bench_checkpoint(args, cases)
     if args.integrator == "rk2":
         return bench_rk2(args, cases)
     cells = sum(m.owned_cells() for (m, _, _, _) in cases)
@@ -365,6 +370,67 @@ def bench_amr(args, cases):
                           "leaves_at_end": sol.mesh.num_leaves, "cell_updates_per_s": cells_done / e2e_wall,
                           "ms_per_step": e2e_wall * 1e3 / e2e_steps,
                           "driver": "per-mesh solver (host dt each step), AMR cycles included"}}
+    print(json.dumps(out))
+
+
+def bench_checkpoint(args, cases):
+    """Checkpoint write, XDMF write and restart of the workload's first mesh (files under a
+    temporary directory). The checkpoint's bytes are the owned state (U and face B) plus the leaf
+    list; the restart reads them, rebuilds the mesh and its device tables, and uploads the state."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED
+    from godunov_mhd_ct_amd.io import restart_solver, save_checkpoint, write_xdmf
+    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+
+    m, U, B, st = cases[0]
+    sol = SolverGodunovMHD(m, U, B, st, impl_version=IMPL_FUSED)
+    for _ in range(args.steps):
+        sol.step()
+    torch.cuda.synchronize()
+    d = Path(tempfile.mkdtemp(prefix="mhdct_chk_"))
+    try:
+        log(f"[bench] checkpoint of {m.num_leaves} leaves after {args.steps} steps into {d}")
+        t0 = time.perf_counter()
+        Uh, Bh = sol.state()
+        t1 = time.perf_counter()
+        save_checkpoint(d / "chk", sol.mesh, Uh, Bh, sol.t, sol.iteration, settings=st)
+        t2 = time.perf_counter()
+        write_xdmf(d / "chk", sol.mesh, Uh, Bh, sol.t, st.gamma0, scaling=st.scaling_factor)
+        t3 = time.perf_counter()
+        log("[bench] checkpoint and XDMF written")
+        nbytes = (d / "chk.bin").stat().st_size
+        vbytes = (d / "chk_cells.bin").stat().st_size
+        t4 = time.perf_counter()
+        rst = restart_solver(d / "chk", impl_version=IMPL_FUSED)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        n = m.num_leaves
+        Ur, Br = rst.state()
+        same = bool(np.array_equal(Ur[:n], Uh[:n]) and np.array_equal(Br[:n], Bh[:n]))
+        rst.step()
+        sol.step()
+        torch.cuda.synchronize()
+        Ua, Ba = sol.state()
+        Ub, Bb = rst.state()
+        cont = bool(np.array_equal(Ua[:n], Ub[:n]) and np.array_equal(Ba[:n], Bb[:n]))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    write_s = t2 - t1
+    out = {"metric": "checkpoint write throughput", "value": nbytes / write_s / 1e9, "unit": "GB/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": args.workload, "leaves": n, "steps_before": args.steps,
+                      "checkpoint_bytes": nbytes, "xdmf_data_bytes": vbytes},
+           "roofline": None,
+           "ms": {"device_to_host": (t1 - t0) * 1e3, "checkpoint_write": write_s * 1e3,
+                  "xdmf_write": (t3 - t2) * 1e3, "restart_read_rebuild_upload": (t5 - t4) * 1e3},
+           "GB_per_s": {"device_to_host": nbytes / (t1 - t0) / 1e9, "xdmf_write": vbytes / (t3 - t2) / 1e9,
+                        "restart": nbytes / (t5 - t4) / 1e9},
+           "restart_state_bitwise": same, "next_step_bitwise": cont,
+           "note": "host file-system work: bounded by the box's page cache and PCIe, not by HBM"}
     print(json.dumps(out))
 
 

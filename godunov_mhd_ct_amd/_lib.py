@@ -55,6 +55,7 @@ SIGNATURES = {
     "mhdct_mesh_set_bc": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mhdct_mesh_adapt": (C.c_int, [C.c_void_p, C.POINTER(C.c_int8), C.c_int]),
     "mhdct_mesh_balance": (C.c_int, [C.c_void_p]),
+    "mhdct_mesh_set_leaves": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mhdct_mesh_finalize": (C.c_int, [C.c_void_p]),
     "mhdct_mesh_num_leaves": (C.c_int, [C.c_void_p]),
     "mhdct_mesh_num_outside": (C.c_int, [C.c_void_p]),

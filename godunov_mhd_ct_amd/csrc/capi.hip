@@ -135,6 +135,17 @@ extern "C"
   }
 
   int
+  mhdct_mesh_set_leaves(mhdct_mesh * mesh, int n, const int * levels, const int * coords)
+  {
+    if (!mesh || n < 0 || (n > 0 && (!levels || !coords)))
+      return -1;
+    std::vector<mhdct::Leaf> given(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+      given[i] = { levels[i], coords[3 * i], coords[3 * i + 1], coords[3 * i + 2] };
+    return mesh->m.set_leaves(given) ? 0 : -1;
+  }
+
+  int
   mhdct_mesh_balance(mhdct_mesh * mesh)
   {
     if (!mesh)

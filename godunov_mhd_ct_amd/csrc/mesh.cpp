@@ -2,6 +2,8 @@
 #include "mesh.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <unordered_map>
 #include <climits>
 #include <cstdint>
 #include <thread>
@@ -110,6 +112,64 @@ Mesh::adapt(const std::vector<int8_t> & flags)
   }
   leaves.swap(out);
   sort_leaves();
+  return true;
+}
+
+bool
+Mesh::set_leaves(const std::vector<Leaf> & given)
+{
+  // each leaf inside the brick at its level, no leaf inside another, and the leaves' volumes
+  // (in units of the finest level) summing to the brick's: together, an exact tiling
+  int lmax = 0;
+  for (const Leaf & L : given)
+  {
+    if (L.level < 0 || L.level > max_level)
+    {
+      err = "set_leaves: level out of range";
+      return false;
+    }
+    const int c[3] = { L.x, L.y, L.z };
+    for (int d = 0; d < 3; ++d)
+      if (c[d] < 0 || (d < dim && c[d] >= nblocks(L.level, d)) || (d >= dim && c[d] != 0))
+      {
+        err = "set_leaves: a leaf lies outside the brick";
+        return false;
+      }
+    lmax = std::max(lmax, L.level);
+  }
+  std::unordered_map<uint64_t, int> h;
+  h.reserve(given.size() * 2 + 16);
+  for (size_t i = 0; i < given.size(); ++i)
+    if (!h.emplace(key(given[i].level, given[i].x, given[i].y, given[i].z), int(i)).second)
+    {
+      err = "set_leaves: a leaf is given twice";
+      return false;
+    }
+  long double vol = 0, total = 1;
+  for (int d = 0; d < dim; ++d)
+    total *= (long double)nbrick[d];
+  for (const Leaf & L : given)
+  {
+    for (int l = L.level - 1; l >= 0; --l)
+    {
+      const int sh = L.level - l;
+      if (h.count(key(l, L.x >> sh, L.y >> sh, dim == 3 ? (L.z >> sh) : 0)))
+      {
+        err = "set_leaves: a leaf lies inside another";
+        return false;
+      }
+    }
+    vol += 1.0L / (long double)(1L << (dim * L.level));
+  }
+  if (std::fabs(double(vol - total)) > 1e-9 * double(total))
+  {
+    err = "set_leaves: the leaves do not cover the brick";
+    return false;
+  }
+  leaves = given;
+  outside.clear();
+  sort_leaves();
+  rebuild_hash();
   return true;
 }
 

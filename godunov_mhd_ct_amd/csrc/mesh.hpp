@@ -76,6 +76,8 @@ public:
   void uniform(int level);
   // flags per leaf: +1 refine, -1 coarsen (only applied when all siblings agree), 0 keep
   bool adapt(const std::vector<int8_t> & flags);
+  // replace the leaves by a given set that tiles the brick (no overlap, nothing missing)
+  bool set_leaves(const std::vector<Leaf> & given);
   // full (face+edge+corner) 2:1 balance
   void balance();
   void sort_leaves();

@@ -35,6 +35,7 @@ class Mesh:
         if rc != 0:
             raise MhdctError(f"mhdct_mesh_create failed ({rc})")
         self._h = h
+        self.bc = (0,) * 6
         self.finalized = False
 
     def __del__(self):
@@ -55,7 +56,7 @@ class Mesh:
         h = C.c_void_p()
         self._check(lib().mhdct_mesh_clone(self._h, C.byref(h)), "clone")
         m = Mesh.__new__(Mesh)
-        m.dim, m.b, m.nbrick, m.periodic = self.dim, self.b, self.nbrick, self.periodic
+        m.dim, m.b, m.nbrick, m.periodic, m.bc = self.dim, self.b, self.nbrick, self.periodic, self.bc
         m._h = h
         m.finalized = self.finalized
         return m
@@ -69,6 +70,7 @@ class Mesh:
         """bc: 6 BC_MHD values or names (XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX)."""
         vals = [BC[v] if isinstance(v, str) else int(v) for v in bc] + [0] * (6 - len(bc))
         self._check(lib().mhdct_mesh_set_bc(self._h, _ivec(vals)), "set_bc")
+        self.bc = tuple(vals[:6])
         self.finalized = False
 
     def adapt(self, flags):
@@ -76,6 +78,16 @@ class Mesh:
         if f.size != self.num_leaves:
             raise MhdctError("adapt: one flag per leaf")
         self._check(lib().mhdct_mesh_adapt(self._h, f.ctypes.data_as(C.POINTER(C.c_int8)), int(f.size)), "adapt")
+        self.finalized = False
+
+    def set_leaves(self, levels, coords):
+        """Replace the leaves by a given tiling (levels[n], coords[n, 3])."""
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        co = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
+        if co.shape[0] != lv.size:
+            raise MhdctError("set_leaves: one coordinate triple per level")
+        self._check(lib().mhdct_mesh_set_leaves(self._h, int(lv.size), lv.ctypes.data_as(C.POINTER(C.c_int)),
+                                                co.ctypes.data_as(C.POINTER(C.c_int))), "set_leaves")
         self.finalized = False
 
     def balance(self):

@@ -85,6 +85,10 @@ int mhdct_mesh_set_bc(mhdct_mesh * mesh, const int bc[6]);
 int mhdct_mesh_adapt(mhdct_mesh * mesh, const int8_t * flags, int nleaves);
 /* 2:1 balance across faces, edges and corners. */
 int mhdct_mesh_balance(mhdct_mesh * mesh);
+/* Replace the leaves by n given leaves (levels[n], coords[3n] in blocks at each leaf's level)
+ * that tile the brick exactly: the leaf set of a checkpoint, in place of reading the p4est file
+ * (init/init.cpp:123-127).  Call mhdct_mesh_finalize after it. */
+int mhdct_mesh_set_leaves(mhdct_mesh * mesh, int n, const int * levels, const int * coords);
 /* Outside blocks and neighbour tables; call after the last adapt/balance. */
 int mhdct_mesh_finalize(mhdct_mesh * mesh);
 int mhdct_mesh_num_leaves(const mhdct_mesh * mesh);
