@@ -97,6 +97,14 @@ SIGNATURES = {
     "mhdct_remap_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mhdct_remap_last_error": (C.c_char_p, [C.c_void_p]),
     "mhdct_remap_destroy": (None, [C.c_void_p]),
+    "mhdct_config_parse": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mhdct_config_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mhdct_config_get": (C.c_char_p, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "mhdct_config_settings": (C.c_int, [C.c_void_p, C.POINTER(Settings)]),
+    "mhdct_config_amr_params": (C.c_int, [C.c_void_p, C.POINTER(AmrParams)]),
+    "mhdct_config_mesh": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "mhdct_config_last_error": (C.c_char_p, [C.c_void_p]),
+    "mhdct_config_destroy": (None, [C.c_void_p]),
     "mhdct_batch_num_slots": (C.c_int, [C.c_void_p]),
     "mhdct_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "mhdct_batch_destroy": (None, [C.c_void_p]),

@@ -30,7 +30,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["mesh.cpp", "plain.hip", "fused.hip", "fused_host.hip", "batch.hip", "amr.hip", "border.hip", "capi.hip"]
+SOURCES = ["mesh.cpp", "config.cpp", "plain.hip", "fused.hip", "fused_host.hip", "batch.hip", "amr.hip", "border.hip", "capi.hip"]
 
 # Per-source extras.  fused.hip holds the FP32-bulk kernels (impl 12): their FP32 division and
 # square root use the hardware-approximate sequences (<= 2.5 ulp / 1 ulp) instead of the

@@ -2,7 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <sstream>
 #include <string>
 
 #include "../../include/mhdct.h"
@@ -603,5 +605,252 @@ extern "C"
     if (h->c.red)
       (void)hipFree(h->c.red);
     delete h;
+  }
+
+  // ---- .ini settings ---------------------------------------------------------------
+
+  int
+  mhdct_config_parse(const char * text, mhdct_config ** out)
+  {
+    if (!out)
+      return -1;
+    *out = nullptr;
+    if (!text)
+      return -1;
+    auto * h = new (std::nothrow) mhdct_config();
+    if (!h)
+      return -1;
+    const int bad = h->c.parse(text);
+    if (bad != 0)
+    {
+      delete h;
+      return bad;
+    }
+    *out = h;
+    return 0;
+  }
+
+  int
+  mhdct_config_load(const char * path, mhdct_config ** out)
+  {
+    if (!out)
+      return -1;
+    *out = nullptr;
+    if (!path)
+      return -1;
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+      return -2;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    if (f.bad())
+      return -2;
+    return mhdct_config_parse(ss.str().c_str(), out);
+  }
+
+  const char *
+  mhdct_config_get(const mhdct_config * cfg, const char * section, const char * key)
+  {
+    if (!cfg || !section || !key)
+      return nullptr;
+    const std::string * v = cfg->c.find(section, key);
+    return v ? v->c_str() : nullptr;
+  }
+
+  int
+  mhdct_config_settings(mhdct_config * cfg, mhdct_settings * out)
+  {
+    if (!cfg || !out)
+      return -1;
+    const mhdct::Config & c = cfg->c;
+    mhdct_settings        s{};
+    s.gamma0 = c.get_real("hydro", "gamma0", 1.4);
+    s.cfl = c.get_real("hydro", "cfl", 0.5);
+    s.slope_type = c.get_real("hydro", "slope_type", 1.0);
+    s.smallr = c.get_real("hydro", "smallr", 1e-10);
+    s.smallc = c.get_real("hydro", "smallc", 1e-10);
+    s.scaling_factor = c.get_real("mesh", "scaling_factor", 1.0);
+    // RiemannSolverType (models/riemann_solver_types.h:26-33), names matched as written
+    static const char * riemann_names[] = { "UNKNOWN", "APPROX", "LLF", "HLL", "HLLC", "HLLD" };
+    const std::string   rn = c.get_string("hydro", "riemann", "UNKNOWN");
+    s.riemann = 0;
+    for (int i = 0; i < 6; ++i)
+      if (rn == riemann_names[i])
+        s.riemann = i;
+    const std::string pro = c.get_string("amr", "cell_centered_prolongation_type", "SIMPLE_COPY");
+    if (pro == "SIMPLE_COPY")
+      s.prolongation = 0;
+    else if (pro == "EXTRAPOLATE_LINEAR_MINMOD")
+      s.prolongation = 1;
+    else
+    {
+      cfg->c.err = "[amr] cell_centered_prolongation_type " + pro + " is unknown";
+      return -1;
+    }
+    s.abort_when_negative_eint = c.get_bool("hydro", "abort_when_negative_eint", false) ? 1 : 0;
+    const std::string * ti = c.find("run", "time_integrator");
+    if (!ti)
+      ti = c.find("amr", "time_integrator");
+    const std::string tname = ti ? *ti : "HANCOCK";
+    if (tname == "HANCOCK")
+      s.time_integrator = MHDCT_TIME_HANCOCK;
+    else if (tname == "RK2_SSP")
+      s.time_integrator = MHDCT_TIME_RK2_SSP;
+    else
+    {
+      cfg->c.err = "time_integrator " + tname + " is not supported (HANCOCK or RK2_SSP)";
+      return -1;
+    }
+    const long dim = c.get_integer("run", "dimension", 3);
+    s.gravity_enabled = c.get_bool("gravity", "enabled", false) ? 1 : 0;
+    static const char * gnames[3] = { "uniform_gx", "uniform_gy", "uniform_gz" };
+    for (int d = 0; d < 3; ++d)
+      s.gravity[d] = d < dim ? c.get_real("gravity", gnames[d], 0.0) : 0.0;
+    *out = s;
+    return 0;
+  }
+
+  int
+  mhdct_config_amr_params(mhdct_config * cfg, mhdct_amr_params * out)
+  {
+    if (!cfg || !out)
+      return -1;
+    const mhdct::Config & c = cfg->c;
+    if (!c.find("amr", "level_min") || !c.find("amr", "level_max"))
+    {
+      cfg->c.err = "[amr] level_min and level_max are required";
+      return -1;
+    }
+    mhdct_amr_params p{};
+    p.level_min = int(c.get_integer("amr", "level_min", -1));
+    p.level_max = int(c.get_integer("amr", "level_max", -1));
+    if (p.level_min < 0 || p.level_max < p.level_min)
+    {
+      cfg->c.err = "[amr] needs 0 <= level_min <= level_max";
+      return -1;
+    }
+    p.epsilon_refine = c.get_real("amr", "epsilon_refine", 0.01);
+    p.epsilon_coarsen = c.get_real("amr", "epsilon_coarsen", 0.00125);
+    p.epsilon_lohner = c.get_real("amr", "epsilon_lohner", 0.02);
+    const std::string crit = c.get_string("amr", "refine_criterion", "LOHNER_UNSPLIT");
+    if (crit != "LOHNER_UNSPLIT")
+    {
+      cfg->c.err = "[amr] refine_criterion " + crit + " is not built (LOHNER_UNSPLIT only)";
+      return -1;
+    }
+    const auto names = c.get_string_vector("amr", "variable_used_to_compute_refine_criterion", { "rho" });
+    if (names.size() != 1)
+    {
+      cfg->c.err = "[amr] variable_used_to_compute_refine_criterion: one variable is supported";
+      return -1;
+    }
+    // DERIVED_QUANTITY (utils/ComputeDerivedQuantities.h:44-53), matched in any case
+    static const char * derived[] = { "thermal_pressure", "speed_of_sound", "specific_ekin", "magnetic_pressure",
+                                      "local_mach_number", "vx", "vy", "vz" };
+    for (const char * dq : derived)
+      if (mhdct::lower(names[0]) == dq)
+      {
+        cfg->c.err = "[amr] refining on the derived quantity " + names[0] + " is not built";
+        return -1;
+      }
+    // MHD::m_id2names_all (models/MHD.cpp:20-32): the cell variables of U; other names give rho
+    static const char * uvars[] = { "rho", "e_tot", "rho_vx", "rho_vy", "rho_vz" };
+    p.ivar = 0;
+    for (int v = 0; v < 5; ++v)
+      if (names[0] == uvars[v])
+        p.ivar = v;
+    *out = p;
+    return 0;
+  }
+
+  int
+  mhdct_config_mesh(mhdct_config * cfg, mhdct_mesh ** out)
+  {
+    if (!out)
+      return -1;
+    *out = nullptr;
+    if (!cfg)
+      return -1;
+    const mhdct::Config & c = cfg->c;
+    const long            dim = c.get_integer("run", "dimension", 0);
+    if (dim != 2 && dim != 3)
+    {
+      cfg->c.err = "[run] dimension must be 2 or 3";
+      return -1;
+    }
+    static const char * bnames[3] = { "bx", "by", "bz" };
+    long                b = -1;
+    for (int d = 0; d < dim; ++d)
+    {
+      const long bd = c.get_integer("amr", bnames[d], -1);
+      if (bd < 0)
+      {
+        cfg->c.err = std::string("[amr] ") + bnames[d] + " is required";
+        return -1;
+      }
+      if (d > 0 && bd != b)
+      {
+        cfg->c.err = "[amr] bx, by (and bz) must be equal: blocks are cubes here";
+        return -1;
+      }
+      b = bd;
+    }
+    if (b < 4 || (b % 2))
+    {
+      cfg->c.err = "[amr] block size must be even and at least 4";
+      return -1;
+    }
+    if (!c.find("amr", "level_min"))
+    {
+      cfg->c.err = "[amr] level_min is required";
+      return -1;
+    }
+    const long level = c.get_integer("amr", "level_min", -1);
+    static const char * ax = "xyz";
+    int                 nbrick[3] = { 1, 1, 1 }, periodic[3] = { 1, 1, 1 }, bc[6] = { 0, 0, 0, 0, 0, 0 };
+    for (int d = 0; d < dim; ++d)
+    {
+      nbrick[d] = int(c.get_integer("p4est_connectivity", std::string("nbrick_") + ax[d], 1));
+      periodic[d] = c.get_integer("p4est_connectivity", std::string("periodic_") + ax[d], 1) != 0 ? 1 : 0;
+    }
+    // BC_MHD (border_conditions/FillOutside.h:55-61)
+    static const char * bcn[4] = { "NONE", "PERIODIC", "ZERO_GRADIENT", "WALL" };
+    for (int f = 0; f < 2 * dim; ++f)
+    {
+      const std::string key = std::string("boundary_type_") + ax[f / 2] + (f % 2 ? "max" : "min");
+      const std::string v = c.get_string("mesh", key, "ZERO_GRADIENT");
+      int               k = -1;
+      for (int i = 0; i < 4; ++i)
+        if (v == bcn[i])
+          k = i;
+      if (k < 0)
+      {
+        cfg->c.err = "[mesh] " + key + " " + v + " is not built (NONE, PERIODIC, ZERO_GRADIENT or WALL)";
+        return -1;
+      }
+      bc[f] = k;
+    }
+    mhdct_mesh * m = nullptr;
+    const int    rc = mhdct_mesh_create(int(dim), int(b), nbrick, periodic, int(level), &m);
+    if (rc != 0)
+    {
+      cfg->c.err = "the mesh cannot be created (nbrick >= 1, 0 <= level_min <= 12)";
+      return rc;
+    }
+    mhdct_mesh_set_bc(m, bc);
+    *out = m;
+    return 0;
+  }
+
+  const char *
+  mhdct_config_last_error(const mhdct_config * cfg)
+  {
+    return cfg ? cfg->c.err.c_str() : "null config";
+  }
+
+  void
+  mhdct_config_destroy(mhdct_config * cfg)
+  {
+    delete cfg;
   }
 }

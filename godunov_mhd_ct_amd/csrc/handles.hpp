@@ -5,11 +5,17 @@
 #include <vector>
 
 #include "amr.hpp"
+#include "config.hpp"
 #include "ctx.hpp"
 
 struct mhdct_mesh
 {
   mhdct::Mesh m;
+};
+
+struct mhdct_config
+{
+  mhdct::Config c;
 };
 
 struct mhdct_ctx

@@ -272,18 +272,13 @@ Mesh::sort_leaves()
 bool
 Mesh::locate(int L, const long gin[3], int & oct, int & rel, long loc[3]) const
 {
+  // periodic directions wrap; beyond a bounded border the outside blocks answer
   long g[3] = { gin[0], gin[1], dim == 3 ? gin[2] : 0 };
-  bool outside_pos = false;
   for (int d = 0; d < dim; ++d)
   {
     const long n = ncells(L, d);
-    if (g[d] < 0 || g[d] >= n)
-    {
-      if (periodic[d])
-        g[d] = ((g[d] % n) + n) % n;
-      else
-        outside_pos = true;
-    }
+    if ((g[d] < 0 || g[d] >= n) && periodic[d])
+      g[d] = ((g[d] % n) + n) % n;
   }
   for (int r = 0; r >= -1; --r)
   {
@@ -291,8 +286,6 @@ Mesh::locate(int L, const long gin[3], int & oct, int & rel, long loc[3]) const
     long gg[3] = { g[0], g[1], g[2] };
     if (r == -1)
     {
-      if (outside_pos)
-        break;
       for (int d = 0; d < 3; ++d)
         gg[d] = floordiv(g[d], 2);
     }
@@ -310,8 +303,6 @@ Mesh::locate(int L, const long gin[3], int & oct, int & rel, long loc[3]) const
       return true;
     }
   }
-  if (outside_pos)
-    return false;
   long gf[3] = { 2 * g[0], 2 * g[1], 2 * g[2] };
   const int o = find(L + 1, int(floordiv(gf[0], b)), int(floordiv(gf[1], b)),
                      dim == 3 ? int(floordiv(gf[2], b)) : 0);
@@ -342,7 +333,13 @@ Mesh::build_outside()
     return true;
   }
   rebuild_hash();
-  std::unordered_map<uint64_t, int> made;
+  // Every leaf on a bounded border gets its outward images, at its own level: one per
+  // direction whose nonzero components all leave the brick (across a face, and across the
+  // edges and corners where borders meet). Each image's source is the leaf itself, the inside
+  // block FillOutside reads (border_conditions/FillOutside.cpp:133-152 looks it up at the
+  // outside block's level). The images of the border leaves tile the layer beyond the border,
+  // so a position there resolves, like one inside, to a block of the same, the next coarser or
+  // the next finer level.
   for (size_t i = 0; i < leaves.size(); ++i)
   {
     const Leaf & L = leaves[i];
@@ -351,43 +348,18 @@ Mesh::build_outside()
         for (int dx = -1; dx <= 1; ++dx)
         {
           const int dl[3] = { dx, dy, dz };
-          long      p[3] = { long(L.x) + dx, long(L.y) + dy, long(L.z) + dz };
-          int       od[3] = { 0, 0, 0 };
-          bool      out = false;
-          for (int d = 0; d < dim; ++d)
-          {
-            const long n = nblocks(L.level, d);
-            if (p[d] < 0 || p[d] >= n)
-            {
-              if (periodic[d])
-                p[d] = ((p[d] % n) + n) % n;
-              else
-              {
-                out = true;
-                od[d] = dl[d];
-              }
-            }
-          }
-          if (!out)
+          const long p[3] = { long(L.x) + dx, long(L.y) + dy, long(L.z) + dz };
+          bool       outward = dx || dy || dz;
+          for (int d = 0; d < dim && outward; ++d)
+            if (dl[d] != 0 && (periodic[d] || (p[d] >= 0 && p[d] < nblocks(L.level, d))))
+              outward = false;
+          if (!outward)
             continue;
-          const uint64_t k = key(L.level, int(p[0]), int(p[1]), int(p[2]));
-          if (made.count(k))
-            continue;
-          // source: the same-level leaf across the border (position minus outward part)
-          long s[3] = { p[0] - od[0], p[1] - od[1], p[2] - od[2] };
-          const int src = find(L.level, int(s[0]), int(s[1]), int(s[2]));
-          if (src < 0 || src >= int(leaves.size()))
-          {
-            err = "non-periodic border: the leaves along a bounded border must share one level "
-                  "next to each outside block (source leaf missing)";
-            return false;
-          }
-          made[k] = int(outside.size());
           outside.push_back({ L.level, int(p[0]), int(p[1]), int(p[2]) });
-          outside_src.push_back(src);
-          outside_dirs.push_back(od[0]);
-          outside_dirs.push_back(od[1]);
-          outside_dirs.push_back(od[2]);
+          outside_src.push_back(int(i));
+          outside_dirs.push_back(dl[0]);
+          outside_dirs.push_back(dl[1]);
+          outside_dirs.push_back(dl[2]);
         }
   }
   rebuild_hash();

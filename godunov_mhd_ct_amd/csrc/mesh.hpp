@@ -3,7 +3,8 @@
 // Restates the part of p4est / kalypsso::core the time step depends on
 // (SolverGodunovMHD.h:59-84, scheme/GodunovImplemV0.cpp:40-75): leaves in tree-then-Morton
 // order, 2:1 balance across faces, edges and corners, periodic or bounded bricks, the
-// same-level "outside" blocks beyond non-periodic borders (border_conditions/FillOutside.h),
+// "outside" blocks beyond non-periodic borders, each the outward image of a border leaf at
+// that leaf's level (border_conditions/FillOutside.h),
 // and the per-block neighbour tables the GPU kernels read instead of a hash map.
 #pragma once
 
@@ -58,7 +59,7 @@ public:
   int  bc[6] = { 0, 0, 0, 0, 0, 0 }; // BC_MHD per face when not periodic (2 ZERO_GRADIENT, 3 WALL)
   int  max_level = 20;
   std::vector<Leaf> leaves;          // owned leaves, in tree-then-Morton order
-  std::vector<Leaf> outside;         // outside blocks (same level as their source)
+  std::vector<Leaf> outside;         // outside blocks (each at its source leaf's level)
   std::vector<int>  outside_src;     // source leaf of each outside block
   std::vector<int>  outside_dirs;    // 3 per outside block: outward components
 

@@ -51,6 +51,15 @@ class Mesh:
     def handle(self):
         return self._h
 
+    @classmethod
+    def from_handle(cls, h, dim, b, nbrick, periodic, bc) -> "Mesh":
+        """Wrap a mesh handle the library created (it is destroyed with the wrapper)."""
+        m = cls.__new__(cls)
+        m.dim, m.b, m.nbrick, m.periodic, m.bc = int(dim), int(b), tuple(nbrick), tuple(periodic), tuple(bc)
+        m._h = h
+        m.finalized = False
+        return m
+
     def clone(self) -> "Mesh":
         """An independent copy (leaves, border conditions, tables)."""
         h = C.c_void_p()

@@ -14,14 +14,15 @@ from .mesh import Mesh
 
 
 def _grid(mesh: Mesh, scaling: float, xmin, kind: str, comp: int = 0):
-    """Physical coordinates of cell centres (kind 'cell') or of face centres of component
-    `comp` (kind 'face') for every owned leaf and outside block, in ABI order."""
+    """Physical coordinates of cell centres (kind 'cell'), of cells' lowest corners (kind
+    'vertex') or of face centres of component `comp` (kind 'face') for every owned leaf and
+    outside block, in ABI order."""
     lv, co = mesh.leaves()
     b, dim = mesh.b, mesh.dim
     n = len(lv)
-    if kind == "cell":
+    if kind in ("cell", "vertex"):
         shape = (b,) * dim
-        off = [0.5] * 3
+        off = [0.5 if kind == "cell" else 0.0] * 3
     else:
         if dim == 2 and comp == 2:
             shape = (b, b)
@@ -102,9 +103,15 @@ def orszag_tang_2d(mesh: Mesh, gamma=5.0 / 3.0, scaling=1.0, xmin=(0.0, 0.0, 0.0
 
 
 def blast(mesh: Mesh, gamma=5.0 / 3.0, radius=0.1, center=(0.5, 0.5, 0.5), rho_in=1.0, rho_out=1.0,
-          p_in=10.0, p_out=0.1, bfield=(1 / math.sqrt(3),) * 3, scaling=1.0, xmin=(0.0, 0.0, 0.0)):
-    """MHD blast (init/InitBlast.cpp:104-335) with the disc/sphere tested on cell centres."""
-    xs = _grid(mesh, scaling, xmin, "cell")
+          p_in=10.0, p_out=0.1, bfield=(1 / math.sqrt(3),) * 3, scaling=1.0, xmin=(0.0, 0.0, 0.0),
+          test_point="centre"):
+    """MHD blast (init/InitBlast.cpp:104-335): inside the disc/sphere when the cell's test point
+    lies strictly within the radius; the test point is the cell centre, or with
+    test_point="vertex" the cell's lowest corner, as the reference tests it
+    (init/InitBlast.cpp:141-146, 162-168)."""
+    if test_point not in ("centre", "vertex"):
+        raise ValueError("test_point is 'centre' or 'vertex'")
+    xs = _grid(mesh, scaling, xmin, "cell" if test_point == "centre" else "vertex")
     d2 = sum((xs[d] - center[d]) ** 2 for d in range(mesh.dim))
     inside = d2 < radius * radius
     rho = np.where(inside, rho_in, rho_out)

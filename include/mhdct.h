@@ -221,6 +221,43 @@ int          mhdct_batch_num_slots(const mhdct_batch * batch);
 const char * mhdct_batch_last_error(const mhdct_batch * batch);
 void         mhdct_batch_destroy(mhdct_batch * batch);
 
+/* ---- .ini settings ---------------------------------------------------------------- */
+
+/* The run's .ini file, read as the reference reads it through kalypsso::core's ConfigMap
+ * (absent from the reference tree; its calls are at models/HydroSettings.cpp:19-30,
+ * SolverGodunovMHD.cpp:76, 1107-1113, app/solver_godunov_mhd_ct.cpp:72, 87, init/init.cpp:40,
+ * 116-117).  Lines are `[section]`, `key = value`, blank, or comments starting with '#' or ';'.
+ * Section and key names are case-insensitive; a repeated key keeps its last value. */
+typedef struct mhdct_config mhdct_config;
+/* 0; a positive line number (the first line of none of those forms; *out stays NULL); -1. */
+int mhdct_config_parse(const char * text, mhdct_config ** out);
+/* As mhdct_config_parse on a file's text; -2 when the file cannot be read. */
+int mhdct_config_load(const char * path, mhdct_config ** out);
+/* The value of section/key as written (blanks around it dropped), or NULL when absent. */
+const char * mhdct_config_get(const mhdct_config * cfg, const char * section, const char * key);
+/* mhdct_settings from the keys the time step reads, with the reference's defaults:
+ * [hydro] gamma0 1.4, cfl 0.5, slope_type 1.0, smallr 1e-10, smallc 1e-10, riemann (no default:
+ * a missing or unknown name gives riemann 0, UNKNOWN, which mhdct_create refuses, as the
+ * reference fails on it, RiemannSolvers_MHD.h:586), abort_when_negative_eint false;
+ * [mesh] scaling_factor 1; [amr] cell_centered_prolongation_type SIMPLE_COPY;
+ * time_integrator HANCOCK, read from [run] (or from [amr], where the shipped .ini files write it;
+ * RK3_SSP is refused, SolverGodunovMHD.cpp:174-177); [gravity] enabled false, uniform_gx,
+ * uniform_gy, uniform_gz 0 (z in 3D only). */
+int mhdct_config_settings(mhdct_config * cfg, mhdct_settings * out);
+/* mhdct_amr_params from [amr] level_min and level_max (required), epsilon_refine 0.01,
+ * epsilon_coarsen 0.00125, epsilon_lohner 0.02 (SolverGodunovMHD.cpp:1107-1110),
+ * variable_used_to_compute_refine_criterion [rho] (:1112-1113; one variable of U; another name
+ * falls back to rho as at :1149-1160; derived quantities are refused) and refine_criterion
+ * LOHNER_UNSPLIT (the only one built). */
+int mhdct_config_amr_params(mhdct_config * cfg, mhdct_amr_params * out);
+/* The mesh the run starts from: [run] dimension (required), [amr] bx = by (= bz) (required),
+ * uniform at [amr] level_min, over [p4est_connectivity] nbrick_{x,y,z} (default 1) with
+ * periodic_{x,y,z} (default 1), [mesh] boundary_type_{x,y,z}{min,max} (NONE, PERIODIC,
+ * ZERO_GRADIENT or WALL; default ZERO_GRADIENT).  Not finalized. */
+int mhdct_config_mesh(mhdct_config * cfg, mhdct_mesh ** out);
+const char * mhdct_config_last_error(const mhdct_config * cfg);
+void         mhdct_config_destroy(mhdct_config * cfg);
+
 #ifdef __cplusplus
 }
 #endif

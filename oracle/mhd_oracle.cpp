@@ -678,8 +678,9 @@ struct Mesh
   // Restates kalypsso::core StencilHelper::getNeighLoc (used by
   // ConvertToPrimitivesVariablesFunctor.cpp:682-683 and friends): the leaf containing the
   // cell (i,j,k) given in the frame of block `oct` (possibly outside [0,b)), at the same,
-  // the next coarser or the next finer level.  Periodic directions wrap; in non-periodic
-  // directions a position outside the domain resolves to the outside block of the same level.
+  // the next coarser or the next finer level.  Periodic directions wrap; beyond a bounded
+  // border the outside blocks (the outward images of the border leaves, each at its leaf's
+  // level) answer in the same way.
   Loc
   locate(int oct, int i, int j, int k) const
   {
@@ -687,17 +688,11 @@ struct Mesh
     long      g[3] = { long(coord[3 * oct + 0]) * b + i,
                        long(coord[3 * oct + 1]) * b + j,
                        long(coord[3 * oct + 2]) * b + (dim == 3 ? k : 0) };
-    bool outside = false;
     for (int d = 0; d < dim; ++d)
     {
       const long n = ncells(L, d);
-      if (g[d] < 0 || g[d] >= n)
-      {
-        if (periodic[d])
-          g[d] = ((g[d] % n) + n) % n;
-        else
-          outside = true;
-      }
+      if ((g[d] < 0 || g[d] >= n) && periodic[d])
+        g[d] = ((g[d] % n) + n) % n;
     }
     Loc r;
     // same level
@@ -714,8 +709,6 @@ struct Mesh
         return r;
       }
     }
-    if (outside)
-      return r; // not found
     // coarser
     {
       long gc[3] = { fdiv(g[0], 2), fdiv(g[1], 2), fdiv(g[2], 2) };

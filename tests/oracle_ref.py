@@ -106,7 +106,7 @@ class OracleSolver:
         cop, a2 = iptr(co.reshape(-1))
         nbp, a3 = iptr(mesh.nbrick)
         pep, a4 = iptr(mesh.periodic)
-        bcv = bc if bc is not None else [0] * 6
+        bcv = list(bc) if bc is not None else list(mesh.bc)
         bcp, a5 = iptr(bcv)
         srcp, a6 = iptr(src if len(src) else np.zeros(1))
         dirp, a7 = iptr(dirs.reshape(-1) if len(dirs) else np.zeros(3))

@@ -174,3 +174,43 @@ def test_amr_cycle_conserves_on_device():
     t1 = (U1[:nm.num_leaves] * cell_volumes(nm)[:, None, None]).sum(axis=(0, 2))
     scale = np.abs(U0[:m.num_leaves] * cell_volumes(m)[:, None, None]).sum(axis=(0, 2)).max()
     assert np.all(np.abs(t1 - t0) <= 1e-13 * scale), t1 - t0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl", [IMPL_PLAIN, IMPL_FUSED])
+def test_amr_cycles_with_walls_and_gravity(impl):
+    # the Rayleigh-Taylor configuration (walls all around, a 1 x 3 brick, gravity): cycles
+    # refine leaves that touch the walls, so leaves of several levels meet a wall
+    import torch
+    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+
+    walls = [3] * 6
+    m = Mesh(2, 8, 1, nbrick=(1, 3, 1), periodic=(0, 0, 0))
+    m.set_bc(walls)
+    m.finalize()
+    g = (0.0, -0.1)
+    U, B = problems.rayleigh_taylor(m, g, gamma=1.4, amplitude=0.05, bfield=(0.0, 0.5, 0.0), xmin=(-0.5, -1.5, 0.0))
+    amr = AMRSettings(level_min=1, level_max=3, epsilon_refine=0.01, epsilon_coarsen=0.001, cycle_frequency=2)
+    print(f"[amr] impl={impl} Rayleigh-Taylor with walls, gravity and AMR cycles", flush=True)
+    st = hydro_settings(gamma0=1.4, cfl=0.7, slope_type=2.0, gravity=g)
+    sol = SolverGodunovMHD(m, U, B, st, impl_version=impl)
+    sa = settings_array(gamma0=1.4, cfl=0.7, slope_type=2.0, gravity=g)
+    ora = OracleSolver(m, sa, nthreads=4)
+    om, Uo, Bo = m, U.reshape(m.num_total, 5, -1).copy(), B.reshape(m.num_total, -1).copy()
+    wall_levels = set()
+    for it in range(6):
+        if sol.should_do_amr_cycle(amr):
+            sol.do_amr_cycle(amr)
+            om, Uo, Bo, ora = oracle_cycle(om, Uo, Bo, amr, sa, ora)
+            assert np.array_equal(sol.mesh.leaves()[0], om.leaves()[0])
+            src, _ = om.outside()
+            wall_levels |= set(om.leaves()[0][src].tolist())
+        dto = ora.compute_dt(Uo, Bo)
+        dtg = sol.compute_dt()
+        assert abs(dtg - dto) <= 1e-14 * dto, (dtg, dto)
+        Uo, Bo, _ = ora.step(Uo, Bo, dt=dto)
+        sol.step(dt=dto)
+        torch.cuda.synchronize()
+        Ug, Bg = sol.state()
+        compare(Ug, Bg, Uo, Bo, om.num_leaves, f"iteration {it}")
+    assert len(wall_levels) >= 2, "no refinement reached a wall"

@@ -128,3 +128,22 @@ def test_blast_2d_bounded(impl):
     m.finalize()
     U, B = problems.blast(m, radius=0.2, center=(0.3, 0.6, 0.0), bfield=(0.5, 0.2, 0.1))
     run_pair(m, U, B, 5.0 / 3.0, 0.8, 1.0, 5, impl, bc=bc)
+
+
+def walled_refined(dim, bc):
+    m = Mesh(dim, 8, 1, periodic=(0, 0, 0))
+    m.set_bc(bc)
+    return refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.3 and y0 < 0.3, 3 if dim == 2 else 2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_blast_refined_at_walls(impl, dim):
+    # leaves of two or three levels meet the walls: outside blocks of each level, diagonal
+    # neighbours across the border of a coarser or finer level
+    bc = [3, 3, 2, 3, 3, 2]  # WALL and ZERO_GRADIENT borders
+    m = walled_refined(dim, bc)
+    assert len(set(m.leaves()[0][:m.num_leaves].tolist())) >= 2
+    U, B = problems.blast(m, radius=0.25, center=(0.2, 0.25, 0.2), rho_in=2.0, bfield=(0.5, 0.3, 0.2))
+    run_pair(m, U, B, 5.0 / 3.0, 0.8, 1.0, 4, impl, bc=bc)
