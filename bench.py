@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--amr-cycle", action="store_true",
                     help="time one AMR cycle (refinement flags, host adaptation, remap) on the workload after "
                          "--steps steps, [amr] keys of app/test_mhd_blast_3d.ini")
+    ap.add_argument("--viscosity", type=float, default=None,
+                    help="plain form with viscous fluxes of this mu (and the viscous predictor term), next to "
+                         "the same steps without viscosity")
     ap.add_argument("--checkpoint", action="store_true",
                     help="time writing a checkpoint and the XDMF output of the workload's first mesh after "
                          "--steps steps, and a restart from that checkpoint")
@@ -201,6 +204,8 @@ def main():
         return bench_amr(args, cases)
     if args.checkpoint:
         return bench_checkpoint(args, cases)
+    if args.viscosity is not None:
+        return bench_viscosity(args, cases)
     if args.integrator == "rk2":
         return bench_rk2(args, cases)
     cells = sum(m.owned_cells() for (m, _, _, _) in cases)
@@ -274,6 +279,32 @@ def bench_rk2(args, cases):
         out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"], "speedup": r["value"] / p["value"]}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
+    print(json.dumps(out))
+
+
+def bench_viscosity(args, cases):
+    """The plain form (the only one with viscous fluxes) with [viscosity] enabled, mu, and the
+    predictor term, against the same steps without viscosity."""
+    cells = sum(m.owned_cells() for (m, _, _, _) in cases)
+    log(f"[bench] viscosity mu={args.viscosity} workload={args.workload} plain form, cells={cells}")
+    base = run_plain(cases, args.steps, args.warmup)
+    for (_, _, _, st) in cases:
+        st.viscosity_enabled = 1
+        st.viscosity_mu = args.viscosity
+        st.viscosity_hancock_predictor = 1
+    r = run_plain(cases, args.steps, args.warmup)
+    out = {"metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": args.workload, "impl": "plain", "viscosity_mu": args.viscosity,
+                      "viscous_predictor": True, "meshes": len(cases), "cells": cells,
+                      "driver": "per-mesh solver, host dt each step"},
+           "roofline": {"bound": "hbm", "achieved": r["value"] * ALGO_BYTES_PER_CELL / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": r["value"] * ALGO_BYTES_PER_CELL / HBM_PEAK, "traffic": None,
+                        "kernel": "whole step (plain form)",
+                        "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B per cell-update"},
+           "without_viscosity": {"value": base["value"], "ms_per_step": base["ms_per_step"]},
+           "viscous_cost_ms": r["ms_per_step"] - base["ms_per_step"]}
     print(json.dumps(out))
 
 

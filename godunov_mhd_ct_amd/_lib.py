@@ -32,6 +32,9 @@ class Settings(C.Structure):
         ("time_integrator", C.c_int),
         ("gravity_enabled", C.c_int),
         ("gravity", C.c_double * 3),
+        ("viscosity_enabled", C.c_int),
+        ("viscosity_mu", C.c_double),
+        ("viscosity_hancock_predictor", C.c_int),
     ]
 
 

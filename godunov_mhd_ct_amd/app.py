@@ -156,11 +156,11 @@ def run_simulation(ini, impl: str | None = None, log=print, max_steps: int | Non
     from .solver import SolverGodunovMHD
 
     cfg = ini if isinstance(ini, Config) else Config.load(ini)
-    if cfg.get_bool("viscosity", "enabled", False):
-        raise MhdctError("[viscosity] enabled: viscous fluxes are not built")
     st = cfg.settings()
     amr = cfg.amr()
     iv = impl_of(cfg, impl)
+    if st.viscosity_enabled and iv != IMPL_PLAIN:
+        raise MhdctError("[viscosity] enabled: viscous fluxes run in the plain form (implementation_version 0)")
     t_end = cfg.get_real("run", "tEnd", 0.0)
     nstep_max = cfg.get_integer("run", "nStepmax", 0)
     n_output = cfg.get_integer("run", "nOutput", 0)

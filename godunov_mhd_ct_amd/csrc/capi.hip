@@ -251,6 +251,9 @@ extern "C"
       return -5;
     if (s->time_integrator != MHDCT_TIME_HANCOCK && s->time_integrator != MHDCT_TIME_RK2_SSP)
       return -5;
+    // viscous fluxes are built in the plain form only
+    if (s->viscosity_enabled && (impl_version != MHDCT_IMPL_PLAIN || !(s->viscosity_mu >= 0.0)))
+      return -5;
     for (int d = 1; d < dim; ++d)
       if (block_size[d] != block_size[0])
         return -6; // dx == dy == dz (ComputeDtMHDFunctor.cpp:98)
@@ -272,6 +275,9 @@ extern "C"
     c.st.gravity_enabled = s->gravity_enabled != 0;
     for (int d = 0; d < 3; ++d)
       c.st.gravity[d] = d < dim ? s->gravity[d] : 0.0;
+    c.st.viscosity_enabled = s->viscosity_enabled != 0;
+    c.st.mu = s->viscosity_mu;
+    c.st.viscosity_predictor = s->viscosity_hancock_predictor != 0;
     if (hipGetDevice(&c.device) != hipSuccess)
     {
       delete h;
@@ -706,6 +712,9 @@ extern "C"
     static const char * gnames[3] = { "uniform_gx", "uniform_gy", "uniform_gz" };
     for (int d = 0; d < 3; ++d)
       s.gravity[d] = d < dim ? c.get_real("gravity", gnames[d], 0.0) : 0.0;
+    s.viscosity_enabled = c.get_bool("viscosity", "enabled", false) ? 1 : 0;
+    s.viscosity_mu = c.get_real("viscosity", "mu", 0.0);
+    s.viscosity_hancock_predictor = c.get_bool("viscosity", "hancock_predictor_enabled", false) ? 1 : 0;
     *out = s;
     return 0;
   }

@@ -27,6 +27,9 @@ struct Settings
   int    time_integrator = 0; // 0 HANCOCK, 1 RK2_SSP
   int    gravity_enabled = 0;
   double gravity[3] = { 0, 0, 0 };
+  int    viscosity_enabled = 0;   // [viscosity] enabled (plain form only)
+  double mu = 0.0;                // [viscosity] mu, the dynamic viscosity
+  int    viscosity_predictor = 0; // [viscosity] hancock_predictor_enabled
 };
 
 // Mesh tables resident in HBM.

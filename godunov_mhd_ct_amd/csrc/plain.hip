@@ -250,9 +250,9 @@ k_slopes(DevMesh m, double st, const double * __restrict__ Qp, double * Sx, doub
 // K6: Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_predictor(DevMesh m, double scaling, Gas<double> gs, double dt, int rk2, const double * __restrict__ Qp,
-            const double * __restrict__ Sx, const double * __restrict__ Sy, const double * __restrict__ Sz,
-            double * __restrict__ Q2)
+k_predictor(DevMesh m, double scaling, Gas<double> gs, double dt, int rk2, double visc_mu,
+            const double * __restrict__ Qp, const double * __restrict__ Sx, const double * __restrict__ Sy,
+            const double * __restrict__ Sz, double * __restrict__ Q2)
 {
   const Idx  I{ m.dim, m.b };
   const long nc = I.gcells(1);
@@ -329,6 +329,24 @@ k_predictor(DevMesh m, double scaling, Gas<double> gs, double dt, int rk2, const
     v += gs.g[1] * 0.5 * dt;
     if (dim == 3)
       w += gs.g[2] * 0.5 * dt;
+  }
+  // viscous predictor, after the gravity kick (add_viscous_predictor_{2d,3d},
+  // ComputeUpdatedPrimitiveVariablesFunctor.cpp:178-272, called at :380-383, 514-517): per
+  // velocity component (2D: u and v), per axis, 1/2 nu (second difference) / dx * dt / dx with
+  // nu = mu / rho; visc_mu < 0 leaves it out
+  if (visc_mu >= 0.0)
+  {
+    const double nu = visc_mu / r;
+    const double dtd[3] = { dtdx, dtdy, dtdz };
+    double *     vel[3] = { &u, &v, &w };
+    for (int c = 0; c < dim; ++c)
+      for (int a = 0; a < dim; ++a)
+      {
+        const int    e0 = a == 0, e1 = a == 1, e2 = a == 2;
+        const double d2 = q(i + e0, j + e1, k + e2, 2 + c, o) + q(i - e0, j - e1, k - e2, 2 + c, o) -
+                          2 * q(i, j, k, 2 + c, o);
+        *vel[c] += 0.5 * nu * d2 / dx * dtd[a];
+      }
   }
   double * out = Q2 + s0;
   out[0 * nc] = r + 0.5 * sr0;
@@ -488,6 +506,102 @@ k_fluxes(DevMesh m, Gas<double> g, double scaling, double dt, int dir, const dou
     if (dim == 2 && (v == kA || v == kB))
       continue;
     out[v * nf] = f[v] * dtdS;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K9v: viscous fluxes (ComputeViscousFluxesAndStoreFunctor.cpp:96-352) at the low face of every
+// flux cell (i,j,k) along dir, from Q's velocities: every gradient is the backward difference
+// at cell (i,j,k) along each axis; the stress is mu (grad v + grad v^T) minus its trace part
+// (1/2 in 2D, 1/3 in 3D); the energy flux uses cell (i,j,k)'s velocity.  Stored times dt/dx
+// where k_update reads fluxes; the density (and 2D Bz) flux is 0 (set_flux, .h:208-259).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_viscous_fluxes(DevMesh m, double scaling, double dt, double mu, int dir, const double * __restrict__ Qp,
+                 double * __restrict__ Fl)
+{
+  const Idx  I{ m.dim, m.b };
+  const int  b = m.b, dim = m.dim;
+  int        sh[3] = { b, b, dim == 3 ? b : 1 };
+  sh[dir] = b + 1;
+  const long nf = long(sh[0]) * sh[1] * sh[2];
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(m.nowned) * nf)
+    return;
+  const int  o = int(t / nf);
+  const long ci = t - long(o) * nf;
+  const int  i = int(ci % sh[0]), j = int((ci / sh[0]) % sh[1]), k = dim == 3 ? int(ci / (long(sh[0]) * sh[1])) : 0;
+  const QView  q{ Qp, I, I.gcells(2) };
+  const double dx = dx_of(m, scaling, o);
+  // g[c][a] = d v_c / d x_a
+  double g[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };
+  const double vel[3] = { q(i, j, k, kU, o), q(i, j, k, kV, o), dim == 3 ? q(i, j, k, kW, o) : 0.0 };
+  for (int a = 0; a < dim; ++a)
+  {
+    const int e0 = a == 0, e1 = a == 1, e2 = a == 2;
+    for (int c = 0; c < dim; ++c)
+      g[c][a] = (q(i, j, k, kU + c, o) - q(i - e0, j - e1, k - e2, kU + c, o)) / dx;
+  }
+  double fU, fV, fW = 0.0, fE;
+  if (dim == 2)
+  {
+    const double dv = g[0][0] + g[1][1];
+    const double txy = mu * (g[1][0] + g[0][1]);
+    if (dir == 0)
+    {
+      const double txx = 2 * mu * (g[0][0] - 0.5 * dv);
+      fU = -txx;
+      fV = -txy;
+      fE = -vel[0] * txx - vel[1] * txy;
+    }
+    else
+    {
+      const double tyy = 2 * mu * (g[1][1] - 0.5 * dv);
+      fU = -txy;
+      fV = -tyy;
+      fE = -vel[0] * txy - vel[1] * tyy;
+    }
+  }
+  else
+  {
+    const double third = 1.0 / 3.0;
+    const double dv = g[0][0] + g[1][1] + g[2][2];
+    double       t0, t1, t2;
+    if (dir == 0)
+    {
+      t0 = 2 * mu * (g[0][0] - third * dv);
+      t1 = mu * (g[1][0] + g[0][1]);
+      t2 = mu * (g[2][0] + g[0][2]);
+    }
+    else if (dir == 1)
+    {
+      t0 = mu * (g[0][1] + g[1][0]);
+      t1 = 2 * mu * (g[1][1] - third * dv);
+      t2 = mu * (g[2][1] + g[1][2]);
+    }
+    else
+    {
+      t0 = mu * (g[0][2] + g[2][0]);
+      t1 = mu * (g[1][2] + g[2][1]);
+      t2 = 2 * mu * (g[2][2] - third * dv);
+    }
+    fU = -t0;
+    fV = -t1;
+    fW = -t2;
+    fE = -vel[0] * t0 - vel[1] * t1 - vel[2] * t2;
+  }
+  const double dtdS = dt / dx;
+  double *     out = Fl + long(o) * 8 * nf + ci;
+  out[kD * nf] = 0.0;
+  out[kP * nf] = fE * dtdS;
+  out[kU * nf] = fU * dtdS;
+  out[kV * nf] = fV * dtdS;
+  out[kW * nf] = fW * dtdS;
+  out[kC * nf] = 0.0;
+  if (dim == 3)
+  {
+    out[kA * nf] = 0.0;
+    out[kB * nf] = 0.0;
   }
 }
 
@@ -1136,7 +1250,8 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
   hipLaunchKernelGGL(k_slopes, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, g.slope_type, c.ps.Q, c.ps.S[0],
                      c.ps.S[1], c.ps.S[2]);
   const int rk2 = c.st.time_integrator == 1;
-  hipLaunchKernelGGL(k_predictor, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, g, dt, rk2, c.ps.Q,
+  const double visc_pred = (c.st.viscosity_enabled && c.st.viscosity_predictor) ? c.st.mu : -1.0;
+  hipLaunchKernelGGL(k_predictor, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, g, dt, rk2, visc_pred, c.ps.Q,
                      c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.Q2);
   hipLaunchKernelGGL(k_elec_field, grid_of(nq * S.gcells(2)), kThreads, 0, s, m, c.ps.Q, c.ps.E);
   hipLaunchKernelGGL(k_sfacemag, grid_of(nq * S.gcells(1)), kThreads, 0, s, m, sc, dt, c.ps.E, c.ps.sF);
@@ -1146,6 +1261,13 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
                        c.ps.Q2, c.ps.S[dir], c.ps.sF, c.ps.Fl);
     hipLaunchKernelGGL(k_update, grid_of(nq * S.cells()), kThreads, 0, s, m, dir, c.ps.Fl, U2, B2);
   }
+  if (c.st.viscosity_enabled) // GodunovImplemV0.cpp:179-192
+    for (int dir = 0; dir < c.dim; ++dir)
+    {
+      hipLaunchKernelGGL(k_viscous_fluxes, grid_of(nq * S.flux_cells()), kThreads, 0, s, m, sc, dt, c.st.mu, dir,
+                         c.ps.Q, c.ps.Fl);
+      hipLaunchKernelGGL(k_update, grid_of(nq * S.cells()), kThreads, 0, s, m, dir, c.ps.Fl, U2, B2);
+    }
   hipLaunchKernelGGL(k_emf, grid_of(nq * S.edge_cells()), kThreads, 0, s, m, g, sc, dt, c.ps.Q, c.ps.Q2,
                      c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.sF, rk2 ? 0 : 1, c.ps.emf);
   MHDCT_HIP(hipMemcpyAsync(c.ps.emf2, c.ps.emf, size_t(nq) * S.ncomp_e() * S.edge_cells() * sizeof(double),

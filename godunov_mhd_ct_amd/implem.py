@@ -31,9 +31,11 @@ PROLONGATION = {"SIMPLE_COPY": 0, "EXTRAPOLATE_LINEAR_MINMOD": 1}
 
 def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling_factor=1.0,
                    riemann="HLLD", prolongation="SIMPLE_COPY", abort_when_negative_eint=False,
-                   time_integrator="HANCOCK", gravity=None) -> Settings:
-    """Defaults of HydroSettings (models/HydroSettings.cpp:19-30), plus [run] time_integrator and
-    [gravity] (gravity=(gx, gy, gz) enables a uniform field; None disables it)."""
+                   time_integrator="HANCOCK", gravity=None, viscosity=None, viscous_predictor=False) -> Settings:
+    """Defaults of HydroSettings (models/HydroSettings.cpp:19-30), plus [run] time_integrator,
+    [gravity] (gravity=(gx, gy, gz) enables a uniform field; None disables it) and [viscosity]
+    (viscosity=mu enables viscous fluxes, plain form only; viscous_predictor adds the viscous
+    term of the half-step predictor)."""
     s = Settings()
     s.gamma0 = float(gamma0)
     s.cfl = float(cfl)
@@ -55,6 +57,10 @@ def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-
         s.gravity_enabled = 1
         for d in range(3):
             s.gravity[d] = g[d]
+    if viscosity is not None:
+        s.viscosity_enabled = 1
+        s.viscosity_mu = float(viscosity)
+        s.viscosity_hancock_predictor = int(bool(viscous_predictor))
     return s
 
 
@@ -88,6 +94,9 @@ class GodunovImplemHIP:
         h = C.c_void_p()
         bs = (C.c_int * 3)(self.b, self.b, self.b)
         rc = lib().mhdct_create(C.byref(settings), self.dim, bs, self.impl_version, C.byref(h))
+        if rc == -5:
+            raise MhdctError("mhdct_create failed (-5): settings not run by this build (HLLD and SIMPLE_COPY only; "
+                             "HANCOCK or RK2_SSP; viscosity in the plain form only, mu >= 0)")
         if rc != 0:
             raise MhdctError(f"mhdct_create failed ({rc})")
         self._h = h

@@ -58,6 +58,11 @@ typedef struct mhdct_settings
   int    time_integrator; /* [run] time_integrator: MHDCT_TIME_HANCOCK or MHDCT_TIME_RK2_SSP */
   int    gravity_enabled; /* [gravity] enabled: a uniform gravity field (AddGravitySourceTerm.cpp) */
   double gravity[3];      /* [gravity] uniform_gx, uniform_gy, uniform_gz */
+  int    viscosity_enabled; /* [viscosity] enabled: viscous fluxes after the hydro ones
+                               (GodunovImplemV0.cpp:179-192); plain form only */
+  double viscosity_mu;      /* [viscosity] mu, the dynamic viscosity (>= 0) */
+  int    viscosity_hancock_predictor; /* [viscosity] hancock_predictor_enabled: the viscous term of
+                                         the half-step predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:380-383) */
 } mhdct_settings;
 
 /* [run] time_integrator values (SolverGodunovMHD.cpp:494, 551; RK3_SSP is rejected, as the
@@ -102,6 +107,9 @@ void         mhdct_mesh_destroy(mhdct_mesh * mesh);
 
 /* ---- GodunovImplemBase ---------------------------------------------------------------- */
 
+/* -4: unknown impl_version; -5: settings this build does not run (a Riemann solver other than
+ * HLLD, a prolongation other than SIMPLE_COPY, an unknown time integrator, viscosity outside the
+ * plain form or with mu < 0); -6: unequal block sizes. */
 int      mhdct_create(const mhdct_settings * settings, int dim, const int block_size[3], int impl_version,
                       mhdct_ctx ** out);
 int      mhdct_set_mesh(mhdct_ctx * ctx, const mhdct_mesh * mesh);

@@ -98,6 +98,10 @@ struct Settings
   // [gravity] enabled, uniform_gx, uniform_gy, uniform_gz (get_uniform_gravity_vector, core)
   int    gravity_enabled = 0;
   double gravity[3] = { 0, 0, 0 };
+  // [viscosity] enabled, mu, hancock_predictor_enabled (ViscosityParams, core)
+  int    viscosity_enabled = 0;
+  double mu = 0.0;
+  int    viscosity_predictor = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1329,9 +1333,9 @@ public:
     }
   }
 
-  // ComputeUpdatedPrimitiveVariablesFunctor::update_primitive_variables (.cpp:284-546), no
-  // gravity, no viscosity.  HANCOCK: the half-step predictor; RK2_SSP: q2 = the cell-centred
-  // state (.cpp:396-407, 531-542).
+  // ComputeUpdatedPrimitiveVariablesFunctor::update_primitive_variables (.cpp:284-546).
+  // HANCOCK: the half-step predictor, with the gravity kick and the viscous term after the
+  // source terms; RK2_SSP: q2 = the cell-centred state (.cpp:396-407, 531-542).
   void
   compute_primitives_predictor(double dt)
   {
@@ -1416,6 +1420,24 @@ public:
         v += s.gravity[IY] * 0.5 * dt;
         if (dim == 3)
           w += s.gravity[IZ] * 0.5 * dt;
+      }
+      // viscous predictor, add_viscous_predictor_{2d,3d} (.cpp:178-272, called at :380-383,
+      // 514-517): per velocity component (2D: u, v), per axis, half of nu times the
+      // second difference over dx, times dt/dx
+      if (s.viscosity_enabled && s.viscosity_predictor)
+      {
+        const double nu = s.mu / q(i, j, k, ID, o);
+        const double dtd[3] = { dtdx, dtdy, dtdz };
+        double *     vel[3] = { &u, &v, &w };
+        for (int c = 0; c < dim; ++c)
+          for (int a = 0; a < dim; ++a)
+          {
+            int e[3] = { 0, 0, 0 };
+            e[a] = 1;
+            const double d2 = q(i + e[0], j + e[1], k + e[2], IU + c, o) + q(i - e[0], j - e[1], k - e[2], IU + c, o) -
+                              2 * q(i, j, k, IU + c, o);
+            *vel[c] += 0.5 * nu * d2 / dx * dtd[a];
+          }
       }
       q2(i, j, k, ID, o) = r + 0.5 * sr0;
       q2(i, j, k, IU, o) = u + 0.5 * su0;
@@ -1612,6 +1634,102 @@ public:
         if (dim == 2 && (v == IA || v == IB))
           continue;
         fl(i, j, k, v, o) = flux[v];
+      }
+    }
+  }
+
+  // ComputeViscousFluxesAndStoreFunctor (scheme/ComputeViscousFluxesAndStoreFunctor.cpp:96-352):
+  // the viscous flux at the low face of flux cell (i,j,k) along dir, from the primitive
+  // velocities of Q (not the predicted Q2).  Every gradient is the backward difference at cell
+  // (i,j,k) along each axis (.cpp:96-163); the stress is mu (grad v + grad v^T) minus its trace
+  // part (1/2 in 2D, 1/3 in 3D); the energy flux uses the velocity of cell (i,j,k).  Stored
+  // times dt/dx in the flux array; density (and 2D Bz) flux 0 (set_flux, .h:208-259).
+  void
+  compute_viscous_fluxes_and_store(double dt, int dir)
+  {
+    const int b = lay.b, dim = m.dim;
+    flux_shape[0] = flux_shape[1] = flux_shape[2] = b;
+    if (dim == 2)
+      flux_shape[2] = 1;
+    flux_shape[dir] = b + 1;
+    const long   nc = fl_cells();
+    const long   total = long(m.nowned) * nc;
+    const double mu = s.mu;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long gi = 0; gi < total; ++gi)
+    {
+      const int  o = int(gi / nc);
+      const long ci = gi - long(o) * nc;
+      const int  i = int(ci % flux_shape[0]);
+      const int  j = int((ci / flux_shape[0]) % flux_shape[1]);
+      const int  k = dim == 3 ? int(ci / (long(flux_shape[0]) * flux_shape[1])) : 0;
+      const double dx = dx_of(o);
+      const double dtdS = dt / dx;
+      // g[c][a] = d v_c / d x_a
+      double g[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };
+      double vel[3] = { q(i, j, k, IU, o), q(i, j, k, IV, o), dim == 3 ? q(i, j, k, IW, o) : 0.0 };
+      for (int a = 0; a < dim; ++a)
+      {
+        int e[3] = { 0, 0, 0 };
+        e[a] = 1;
+        for (int c = 0; c < dim; ++c)
+          g[c][a] = (q(i, j, k, IU + c, o) - q(i - e[0], j - e[1], k - e[2], IU + c, o)) / dx;
+      }
+      double fU = 0, fV = 0, fW = 0, fE = 0;
+      if (dim == 2)
+      {
+        const double dv = g[0][0] + g[1][1];
+        if (dir == IX)
+        {
+          const double txx = 2 * mu * (g[0][0] - 0.5 * dv);
+          const double txy = mu * (g[1][0] + g[0][1]);
+          fU = -txx;
+          fV = -txy;
+          fE = -vel[0] * txx - vel[1] * txy;
+        }
+        else
+        {
+          const double tyy = 2 * mu * (g[1][1] - 0.5 * dv);
+          const double txy = mu * (g[1][0] + g[0][1]);
+          fU = -txy;
+          fV = -tyy;
+          fE = -vel[0] * txy - vel[1] * tyy;
+        }
+      }
+      else
+      {
+        const double third = 1.0 / 3.0;
+        const double dv = g[0][0] + g[1][1] + g[2][2];
+        double       t0, t1, t2;
+        if (dir == IX)
+        {
+          t0 = 2 * mu * (g[0][0] - third * dv);
+          t1 = mu * (g[1][0] + g[0][1]);
+          t2 = mu * (g[2][0] + g[0][2]);
+        }
+        else if (dir == IY)
+        {
+          t0 = mu * (g[0][1] + g[1][0]);
+          t1 = 2 * mu * (g[1][1] - third * dv);
+          t2 = mu * (g[2][1] + g[1][2]);
+        }
+        else
+        {
+          t0 = mu * (g[0][2] + g[2][0]);
+          t1 = mu * (g[1][2] + g[2][1]);
+          t2 = 2 * mu * (g[2][2] - third * dv);
+        }
+        fU = -t0;
+        fV = -t1;
+        fW = -t2;
+        fE = -vel[0] * t0 - vel[1] * t1 - vel[2] * t2;
+      }
+      double flux[8] = { 0.0, fE, fU, fV, fW, 0.0, 0.0, 0.0 };
+      for (int v = 0; v < 8; ++v)
+      {
+        if (dim == 2 && (v == IA || v == IB))
+          continue;
+        fl(i, j, k, v, o) = flux[v] * dtdS;
       }
     }
   }
@@ -2265,6 +2383,12 @@ public:
         compute_fluxes_and_store(dt, dir);
         read_fluxes_and_update(U2, B2, dir);
       }
+      if (s.viscosity_enabled) // GodunovImplemV0.cpp:179-192
+        for (int dir = 0; dir < m.dim; ++dir)
+        {
+          compute_viscous_fluxes_and_store(dt, dir);
+          read_fluxes_and_update(U2, B2, dir);
+        }
       compute_emf_and_store(dt);
       emf2 = emf; // Kokkos::deep_copy(m_emf2, m_emf) (GodunovImplemV0.cpp:606)
       correct_emf();
@@ -2429,7 +2553,8 @@ extern "C"
 {
 
   // settings: gamma0, cfl, slope_type, smallr, smallc, scaling, abort_when_negative_eint,
-  // time_integrator (0 HANCOCK, 1 RK2_SSP), gravity_enabled, gx, gy, gz
+  // time_integrator (0 HANCOCK, 1 RK2_SSP), gravity_enabled, gx, gy, gz, viscosity_enabled, mu,
+  // viscous predictor
   void *
   oracle_create(int dim, int b, const double * settings, int nowned, int noutside,
                 const int * levels, const int * coords, const int * nbrick, const int * periodic,
@@ -2447,6 +2572,9 @@ extern "C"
     S->s.gravity_enabled = settings[8] != 0.0;
     for (int d = 0; d < 3; ++d)
       S->s.gravity[d] = d < dim ? settings[9 + d] : 0.0;
+    S->s.viscosity_enabled = settings[12] != 0.0;
+    S->s.mu = settings[13];
+    S->s.viscosity_predictor = settings[14] != 0.0;
     S->m.dim = dim;
     S->m.b = b;
     S->m.nowned = nowned;

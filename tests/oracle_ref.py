@@ -60,6 +60,8 @@ def lib():
         L.oracle_slope.restype = C.c_double
         L.oracle_slope.argtypes = [C.c_double] * 4
         L.oracle_speed_info.argtypes = [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]
+        L.oracle_scratch.restype = C.c_long
+        L.oracle_scratch.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
         _lib = L
     return _lib
 
@@ -82,11 +84,13 @@ RK2_STAGE2 = (0.5, 0.5)
 
 
 def settings_array(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling=1.0,
-                   abort_eint=0, time_integrator="HANCOCK", gravity=None):
+                   abort_eint=0, time_integrator="HANCOCK", gravity=None, mu=None, viscous_predictor=False):
+    """The oracle's settings vector; mu (not None) enables viscosity."""
     ti = TIME_INTEGRATOR[time_integrator] if isinstance(time_integrator, str) else int(time_integrator)
     g = (0.0, 0.0, 0.0) if gravity is None else tuple(float(x) for x in gravity) + (0.0,) * (3 - len(gravity))
     return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint, ti,
-                     0.0 if gravity is None else 1.0, *g], dtype=np.float64)
+                     0.0 if gravity is None else 1.0, *g, 0.0 if mu is None else 1.0,
+                     0.0 if mu is None else float(mu), 1.0 if viscous_predictor else 0.0], dtype=np.float64)
 
 
 class OracleSolver:
@@ -129,6 +133,13 @@ class OracleSolver:
 
     def inv_dt(self, U, B):
         return lib().oracle_compute_inv_dt(self._h, dptr(U), dptr(B))
+
+    def scratch(self, which):
+        """A scratch array of the last step: 0 Bg, 1 Q, 2 Q2, 3-5 slopes, 6 E, 7 sF."""
+        n = lib().oracle_scratch(self._h, int(which), None)
+        out = np.empty(n)
+        lib().oracle_scratch(self._h, int(which), dptr(out))
+        return out
 
     def compute_dt(self, U, B):
         return self.settings[1] / self.inv_dt(U, B)

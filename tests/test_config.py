@@ -75,7 +75,12 @@ def test_restart_file_and_cavity():
     # the shipped cavity files write time_integrator in [amr]
     cav = Config.load(INI / "test_mhd_diamag_cavity_2d.ini")
     assert cav.get("amr", "time_integrator") == "HANCOCK"
-    assert Config.load(INI / "test_mhd_kelvin_helmholtz_3d.ini").get_bool("viscosity", "enabled", True) is False
+    kh = Config.load(INI / "test_mhd_kelvin_helmholtz_3d.ini")
+    assert kh.get_bool("viscosity", "enabled", True) is False
+    st = kh.settings()
+    assert st.viscosity_enabled == 0 and st.viscosity_mu == 1e-4 and st.viscosity_hancock_predictor == 0
+    st = Config.parse(text(*BASE, "[viscosity]", "enabled=yes", "mu=0.5", "hancock_predictor_enabled=true")).settings()
+    assert (st.viscosity_enabled, st.viscosity_mu, st.viscosity_hancock_predictor) == (1, 0.5, 1)
 
 
 def text(*sections):
