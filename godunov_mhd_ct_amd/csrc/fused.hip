@@ -521,9 +521,6 @@ cell_item(const FusedSlot<T> & S, long r0)
       // flux replaces the normal B by the face value; the edge reconstructions use the normal
       // component only as the quirk of x-edges in 3D, dBy/dy): 3D skips dBx/dx and dBz/dz, 2D
       // dBx/dx and dBy/dy
-      for (int v = 0; v < 8; ++v)
-        if (!(v == kA + d && (DIM == 2 || d != 1)))
-          bstore<T>(V.r_sl, (d * 8 + v) * G::G1 + c1, sl[d][v]);
     }
     // Hancock predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:284-546)
     const double dxo = dxd<BS>(m, S.scaling, o);
@@ -534,18 +531,19 @@ cell_item(const FusedSlot<T> & S, long r0)
     const T A = (Q(5, i, j, k) + Q(8, i, j, k)) * T(0.5);
     const T B = (Q(6, i, j, k) + Q(9, i, j, k)) * T(0.5);
     const T C = (Q(7, i, j, k) + Q(10, i, j, k)) * T(0.5);
+    T q2o[8];
     if (!HK)
     {
       // RK2_SSP: Q2 is the cell-centred state (ComputeUpdatedPrimitiveVariablesFunctor.cpp:396-407,
       // 531-542)
-      bstore<T>(V.r_q2, 0 * G::G1 + c1, r);
-      bstore<T>(V.r_q2, 2 * G::G1 + c1, u);
-      bstore<T>(V.r_q2, 3 * G::G1 + c1, v);
-      bstore<T>(V.r_q2, 4 * G::G1 + c1, w);
-      bstore<T>(V.r_q2, 1 * G::G1 + c1, p);
-      bstore<T>(V.r_q2, 5 * G::G1 + c1, A);
-      bstore<T>(V.r_q2, 6 * G::G1 + c1, B);
-      bstore<T>(V.r_q2, 7 * G::G1 + c1, C);
+      q2o[0] = r;
+      q2o[2] = u;
+      q2o[3] = v;
+      q2o[4] = w;
+      q2o[1] = p;
+      q2o[5] = A;
+      q2o[6] = B;
+      q2o[7] = C;
     }
     else
     {
@@ -591,14 +589,14 @@ cell_item(const FusedSlot<T> & S, long r0)
         if (DIM == 3)
           w += S.gt.g[2] * T(0.5) * dtT;
       }
-      bstore<T>(V.r_q2, 0 * G::G1 + c1, r + T(0.5) * sr0);
-      bstore<T>(V.r_q2, 2 * G::G1 + c1, u + T(0.5) * su0);
-      bstore<T>(V.r_q2, 3 * G::G1 + c1, v + T(0.5) * sv0);
-      bstore<T>(V.r_q2, 4 * G::G1 + c1, w + T(0.5) * sw0);
-      bstore<T>(V.r_q2, 1 * G::G1 + c1, p + T(0.5) * sp0);
-      bstore<T>(V.r_q2, 5 * G::G1 + c1, A + T(0.5) * sA0);
-      bstore<T>(V.r_q2, 6 * G::G1 + c1, B + T(0.5) * sB0);
-      bstore<T>(V.r_q2, 7 * G::G1 + c1, C + T(0.5) * sC0);
+      q2o[0] = r + T(0.5) * sr0;
+      q2o[2] = u + T(0.5) * su0;
+      q2o[3] = v + T(0.5) * sv0;
+      q2o[4] = w + T(0.5) * sw0;
+      q2o[1] = p + T(0.5) * sp0;
+      q2o[5] = A + T(0.5) * sA0;
+      q2o[6] = B + T(0.5) * sB0;
+      q2o[7] = C + T(0.5) * sC0;
     }
     // edge electric field v x B at t_n from k_efield
     const auto r_ed = rsrc(S.Ed + long(o) * G::NCE * G::NPE, G::NCE * G::NPE * int(sizeof(T)));
@@ -608,41 +606,53 @@ cell_item(const FusedSlot<T> & S, long r0)
     auto Ex = [&](int a, int c, int e) -> T { return bload<T>(r_ed, ep(a, c, e)); };
     auto Ey = [&](int a, int c, int e) -> T { return bload<T>(r_ed, G::NPE + ep(a, c, e)); };
     auto Ez = [&](int a, int c, int e) -> T { return bload<T>(r_ed, (DIM == 3 ? 2 * G::NPE : 0) + ep(a, c, e)); };
+    T sfo[3];
     if (DIM == 2)
     {
       const T e00 = Ez(i, j, 0);
-      bstore<T>(V.r_sf, c1, (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy);
-      bstore<T>(V.r_sf, G::G1 + c1, -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx);
+      sfo[0] = (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy;
+      sfo[1] = -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx;
     }
     else
     {
       const T ex = Ex(i, j, k), ey = Ey(i, j, k), ez = Ez(i, j, k);
-      bstore<T>(V.r_sf, c1, (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz);
-      bstore<T>(V.r_sf, G::G1 + c1,
-                -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz);
-      bstore<T>(V.r_sf, 2 * G::G1 + c1,
-                (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy);
+      sfo[0] = (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz;
+      sfo[1] = -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz;
+      sfo[2] = (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy;
     }
     // transverse slopes of the face field at this cell's low faces (dBx/dy, dBx/dz, dBy/dx,
     // dBy/dz, dBz/dx, dBz/dy; 2D: dBx/dy, dBy/dx)
+    T fso[G::NFS];
     if (DIM == 2)
     {
-      bstore<T>(V.r_fs, c1, limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st));
-      bstore<T>(V.r_fs, G::G1 + c1, limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st));
+      fso[0] = limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st);
+      fso[1] = limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st);
     }
     else
     {
-      bstore<T>(V.r_fs, c1, limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st));
-      bstore<T>(V.r_fs, 1 * G::G1 + c1, limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st));
-      bstore<T>(V.r_fs, 2 * G::G1 + c1, limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st));
-      bstore<T>(V.r_fs, 3 * G::G1 + c1, limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st));
-      bstore<T>(V.r_fs, 4 * G::G1 + c1, limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st));
-      bstore<T>(V.r_fs, 5 * G::G1 + c1, limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st));
+      fso[0] = limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st);
+      fso[1] = limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st);
+      fso[2] = limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st);
+      fso[3] = limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st);
+      fso[4] = limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st);
+      fso[5] = limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st);
     }
+    // every result is stored after the last load: buffer stores are not reordered across
+    // buffer loads, so stores issued early would split the loads into dependent phases
+    for (int d = 0; d < DIM; ++d)
+      for (int v = 0; v < 8; ++v)
+        if (!(v == kA + d && (DIM == 2 || d != 1)))
+          bstore<T>(V.r_sl, (d * 8 + v) * G::G1 + c1, sl[d][v]);
+    for (int v = 0; v < 8; ++v)
+      bstore<T>(V.r_q2, v * G::G1 + c1, q2o[v]);
+    for (int d = 0; d < DIM; ++d)
+      bstore<T>(V.r_sf, d * G::G1 + c1, sfo[d]);
+    for (int n2 = 0; n2 < G::NFS; ++n2)
+      bstore<T>(V.r_fs, n2 * G::G1 + c1, fso[n2]);
 }
 
 template <typename T, int DIM, int BSEL, bool HK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)

@@ -27,9 +27,21 @@ struct FusedState
   FusedSlot<float>    hf{};
   FusedBatch<double>  bd;
   FusedBatch<float>   bf;
-  std::vector<void *> allocs;
+  std::vector<void *> allocs;      // per-mesh buffers, rebuilt with every mesh
   size_t              bytes = 0;
+  void *              arena = nullptr; // the scratch records: grow-only across meshes
+  size_t              arena_bytes = 0;
 };
+
+// the per-mesh buffers of F (the arena stays)
+void
+free_mesh_buffers(FusedState & F)
+{
+  for (void * p : F.allocs)
+    (void)hipFree(p);
+  F.allocs.clear();
+  F.bytes = 0;
+}
 
 template <typename T>
 int
@@ -56,9 +68,33 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
     return 0;
   };
   const FusedSizes z(dim, b);
-  T *              arena = nullptr;
-  if (alloc(size_t(nq) * z.total() * sizeof(T), reinterpret_cast<void **>(&arena)))
-    return -1;
+  // The arena holds records that every step writes before it reads them, so a new mesh reuses
+  // the arena of the previous one when it fits (an AMR cycle then allocates nothing this
+  // large); it grows with a margin of 1/8 when it does not.
+  const size_t need = size_t(nq > 0 ? nq : 1) * z.total() * sizeof(T);
+  if (F.arena_bytes < need)
+  {
+    if (F.arena)
+      (void)hipFree(F.arena);
+    F.arena = nullptr;
+    F.arena_bytes = 0;
+    size_t want = need + need / 8;
+    if (hipMalloc(&F.arena, want) != hipSuccess)
+    {
+      (void)hipGetLastError(); // the margin did not fit: try the exact size
+      want = need;
+      if (hipMalloc(&F.arena, want) != hipSuccess)
+      {
+        F.arena = nullptr;
+        c.err = "hipMalloc failed for fused scratch";
+        return -1;
+      }
+    }
+    (void)hipMemset(F.arena, 0, want);
+    F.arena_bytes = want;
+  }
+  F.bytes += F.arena_bytes;
+  T * arena = static_cast<T *>(F.arena);
   std::memset(&sl, 0, sizeof(sl));
   sl.m = c.dm;
   sl.gd = gas_of(c.st);
@@ -175,10 +211,21 @@ step(Context & c, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, doubl
 int
 fused_alloc(Context & c)
 {
-  fused_free(c);
-  auto * F = new FusedState();
-  F->fp32 = c.impl == 12;
-  c.fused = F;
+  auto * F = static_cast<FusedState *>(c.fused);
+  const bool fp32 = c.impl == 12;
+  if (F && F->fp32 != fp32)
+  {
+    fused_free(c);
+    F = nullptr;
+  }
+  if (!F)
+  {
+    F = new FusedState();
+    F->fp32 = fp32;
+    c.fused = F;
+  }
+  else
+    free_mesh_buffers(*F);
   const int rc = F->fp32 ? build<float>(c, *F, F->bf, F->hf) : build<double>(c, *F, F->bd, F->hd);
   c.ps.bytes = F->bytes;
   return rc;
@@ -190,8 +237,9 @@ fused_free(Context & c)
   auto * F = static_cast<FusedState *>(c.fused);
   if (!F)
     return;
-  for (void * p : F->allocs)
-    (void)hipFree(p);
+  free_mesh_buffers(*F);
+  if (F->arena)
+    (void)hipFree(F->arena);
   delete F;
   c.fused = nullptr;
 }
