@@ -113,6 +113,21 @@ def load_traffic(kernel, workload):
         return None
 
 
+# wave64 FP64 VALU instructions per second: 1024 SIMDs x 2.4 GHz / 4 cycles per instruction
+VALU_PEAK = 1024 * 2.4e9 / 4
+
+
+def valu_utilisation(kernel, workload, kernel_ms):
+    """SQ_INSTS_VALU per launch of `kernel` (committed counter summary) over this run's kernel
+    time, as a fraction of the full-rate VALU issue peak, or None."""
+    p = ROOT / "profiles" / "hbm_traffic.json"
+    try:
+        d = json.loads(p.read_text())
+        return d[workload][kernel]["valu_insts"] / (kernel_ms * 1e-3) / VALU_PEAK
+    except Exception:
+        return None
+
+
 def run_plain(cases, steps, warmup, impl=None):
     """Steps through the per-mesh solver driver (SolverGodunovMHD: host dt read-back each step),
     the plain form unless `impl` names another."""
@@ -236,6 +251,7 @@ def main():
         out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK, "traffic": load_traffic(kern, args.workload),
                            "kernel": kern, "kernel_ms": kms,
+                           "valu_utilisation": valu_utilisation(kern, args.workload, kms),
                            "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B x {ncells} cell-updates per launch"}
         out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
         if tune:
