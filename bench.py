@@ -276,6 +276,9 @@ def bench_rk2(args, cases):
     impl = {"fused": IMPL_FUSED, "fp32": IMPL_FUSED_FP32, "plain": IMPL_PLAIN}[args.impl]
     cells = sum(m.owned_cells() for (m, _, _, _) in cases)
     log(f"[bench] RK2_SSP workload={args.workload} impl={args.impl} meshes={len(cases)} cells={cells}")
+    batch = None
+    if args.impl != "plain":
+        batch = bench_rk2_batch(args, cases, impl)
     r = run_plain(cases, args.steps, args.warmup, impl)
     out = {"metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -290,12 +293,44 @@ def bench_rk2(args, cases):
                        "frac": r["value"] * algo / HBM_PEAK, "traffic": None,
                        "kernel": "whole step (two stages, per-mesh driver: no per-kernel events)",
                        "algorithmic_bytes": f"{algo} B per cell-update"}
+    if batch is not None:
+        # the batch engine (dt on the device, HIP-graph replay) is the headline form; the
+        # per-mesh driver's figure stays beside it
+        out["per_mesh_driver"] = {"value": r["value"], "ms_per_step": r["ms_per_step"]}
+        out.update({"value": batch["value"], "ms_per_step": batch["ms_per_step"]})
+        out["config"]["driver"] = "batch engine, device dt, HIP-graph replay"
+        out["roofline"]["achieved"] = batch["value"] * algo / 1e9
+        out["roofline"]["frac"] = batch["value"] * algo / HBM_PEAK
+        out["roofline"]["kernel"] = "whole step (two stages; the batch times no single kernel under RK2_SSP)"
     if args.impl != "plain" and not args.no_plain:
         p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
-        out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"], "speedup": r["value"] / p["value"]}
+        out["plain"] = {"value": p["value"], "ms_per_step": p["ms_per_step"], "speedup": out["value"] / p["value"]}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
     print(json.dumps(out))
+
+
+def bench_rk2_batch(args, cases, impl):
+    """RK2_SSP steps through the batch engine with graph replay: the owned cells of every mesh
+    advanced per second."""
+    import torch
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+
+    bt = BatchGodunovMHD(cases[0][0].dim, impl)
+    for (m, U, B, st) in cases:
+        bt.add(m, U, B, st)
+    cells = bt.cells()
+    bt.set_graph(True)
+    bt.step(max(args.warmup, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bt.step(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    bt.check()
+    del bt
+    torch.cuda.empty_cache()
+    return {"value": cells * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps}
 
 
 def bench_viscosity(args, cases):

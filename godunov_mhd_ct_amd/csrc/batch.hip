@@ -7,6 +7,14 @@
 // the concatenated work list of every slot and block size.  The step can be captured in a HIP
 // graph (one per parity of the ping-pong).  Slots are added and removed between steps; a
 // slot's results do not depend on what else is in the batch.
+//
+// A batch holds HANCOCK contexts or RK2_SSP contexts, not both (the integrator is compiled into
+// the cell stage).  An RK2_SSP step is SolverGodunovMHD::godunov_unsplit's RK2 branch
+// (SolverGodunovMHD.cpp:551-647) for every slot at once: the outside fill, dt on the device,
+// the first stage into a stage buffer the batch owns per slot, the stage-1 combination, the
+// outside fill of the stage buffer, the second stage into the other state buffer, and the
+// stage-2 combination.  The fused stages write every owned cell and face of their output, so
+// the copies U_RK = U and U2 = U_RK of the per-mesh driver are not needed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,12 +41,15 @@ struct mhdct_batch
     mhdct_ctx * ctx;
     double *    X[2];  // the two state buffers of U; the initial state is in X[0]
     double *    XB[2]; // the same for B
+    double *    R;     // RK2_SSP: the stage buffer of U (owned by the batch), else null
+    double *    RB;    // ... and of B
     long long   base;  // batch step count when the slot joined
     double      time;  // simulated time and steps, carried across work-list rebuilds
     double      steps;
   };
   int                   dim = 3;
   int                   impl = MHDCT_IMPL_FUSED;
+  int                   integrator = -1; // MHDCT_TIME_* of the slots; -1 while the batch is empty
   std::vector<Entry>    ent;
   int                   next_id = 0;
   long long             gstep = 0; // batch steps enqueued; parity = gstep & 1
@@ -84,6 +95,22 @@ free_worklist(mhdct_batch * b)
   b->wl.clear();
 }
 
+void
+free_stage_buffers(mhdct_batch::Entry & e)
+{
+  if (e.R)
+    (void)hipFree(e.R);
+  if (e.RB)
+    (void)hipFree(e.RB);
+  e.R = nullptr;
+  e.RB = nullptr;
+}
+
+// the stage coefficients (c0, c1) of y = c0 x + c1 y after each stage of the two-stage SSP
+// Runge-Kutta scheme (Shu & Osher 1988; SolverGodunovMHD.cpp:583-588, 617-622)
+constexpr double kRk2Stage1[2] = { 0.0, 1.0 };
+constexpr double kRk2Stage2[2] = { 0.5, 0.5 };
+
 // the slot's current state buffers at batch parity p (slots step together from their base)
 int
 cur_index(const mhdct_batch::Entry & e, int p)
@@ -116,6 +143,7 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
       bt.grav = 1;
   }
   bt.tune = b->tune;
+  bt.rk2 = b->integrator == MHDCT_TIME_RK2_SSP ? 1 : 0;
   if (bt.nslots == 0)
   {
     b->dirty = false;
@@ -130,8 +158,9 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
     b->wl.push_back(*p);
     return 0;
   };
-  std::vector<FusedSlot<T>> hs[2];
-  for (int p = 0; p < 2; ++p)
+  const int narr = bt.rk2 ? 4 : 2;
+  std::vector<FusedSlot<T>> hs[4];
+  for (int p = 0; p < narr; ++p)
     hs[p].resize(b->ent.size());
   std::vector<long> off[mhdct::kStages];
   for (auto & o : off)
@@ -145,16 +174,25 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
       FusedSlot<T> v = *s;
       const int    c = cur_index(e, p);
       v.U = e.X[c];
-      v.U2 = e.X[1 - c];
+      v.U2 = bt.rk2 ? e.R : e.X[1 - c];
       v.B = e.XB[c];
-      v.B2 = e.XB[1 - c];
+      v.B2 = bt.rk2 ? e.RB : e.XB[1 - c];
       v.dt = 0.0;
       hs[p][i] = v;
+      if (bt.rk2)
+      {
+        // the second stage: from the stage buffers into the other state buffers
+        v.U = e.R;
+        v.U2 = e.X[1 - c];
+        v.B = e.RB;
+        v.B2 = e.XB[1 - c];
+        hs[2 + p][i] = v;
+      }
     }
     for (int k = 0; k < mhdct::kStages; ++k)
       off[k][i + 1] = off[k][i] + mhdct::fused_pad_slot(s->cnt[k]);
   }
-  for (int p = 0; p < 2; ++p)
+  for (int p = 0; p < narr; ++p)
   {
     if (alloc(hs[p].size() * sizeof(FusedSlot<T>), reinterpret_cast<void **>(&bt.d_slots[p])) ||
         hipMemcpy(bt.d_slots[p], hs[p].data(), hs[p].size() * sizeof(FusedSlot<T>), hipMemcpyHostToDevice) !=
@@ -218,7 +256,28 @@ enqueue(mhdct_batch * b, FusedBatch<T> & bt, int p, hipStream_t st)
       }
     }
   int rc = mhdct::fused_launch_dt<T>(bt, p, st);
-  if (rc == 0 && b->fork)
+  if (rc == 0 && bt.rk2)
+  {
+    rc = mhdct::fused_launch<T>(bt, p, st); // stage 1: X[c] -> R
+    for (auto & e : b->ent)
+    {
+      const int c = cur_index(e, p);
+      rc = rc ? rc
+              : mhdct::launch_linear_combination(e.ctx->c, e.X[c], e.R, e.XB[c], e.RB, kRk2Stage1[0], kRk2Stage1[1],
+                                                 st);
+      if (rc == 0 && e.ctx->c.dm.noutside > 0)
+        rc = mhdct::launch_fill_outside(e.ctx->c, e.R, e.RB, st);
+    }
+    rc = rc ? rc : mhdct::fused_launch<T>(bt, 2 + p, st); // stage 2: R -> X[1-c]
+    for (auto & e : b->ent)
+    {
+      const int c = cur_index(e, p);
+      rc = rc ? rc
+              : mhdct::launch_linear_combination(e.ctx->c, e.X[c], e.X[1 - c], e.XB[c], e.XB[1 - c], kRk2Stage2[0],
+                                                 kRk2Stage2[1], st);
+    }
+  }
+  else if (rc == 0 && b->fork)
   {
     // k_edge needs only k_cell's records: it runs on the side stream beside k_border_flux and
     // k_flux_tile, and the EMF correction waits for both branches
@@ -316,6 +375,11 @@ step_timed(mhdct_batch * b, FusedBatch<T> & bt, int nsteps, hipStream_t st, doub
     ms[k] = 0.0;
   if (bt.nslots == 0 || nsteps < 1)
     return 0;
+  if (bt.rk2)
+  {
+    b->err = "timed steps are for HANCOCK batches (an RK2_SSP step runs every stage twice)";
+    return -1;
+  }
   // per step: 2 events around each of the 10 stages, on the stream the stage runs on
   std::vector<hipEvent_t> ev(size_t(nsteps) * 2 * ns);
   for (auto & e : ev)
@@ -514,9 +578,15 @@ extern "C"
       b->err = "the context's dimension or implementation differs from the batch";
       return -1;
     }
-    if (ctx->c.st.time_integrator != MHDCT_TIME_HANCOCK)
+    const int integ = ctx->c.st.time_integrator;
+    if (integ != MHDCT_TIME_HANCOCK && integ != MHDCT_TIME_RK2_SSP)
     {
-      b->err = "a batch advances HANCOCK contexts only";
+      b->err = "unknown time integrator";
+      return -1;
+    }
+    if (!b->ent.empty() && integ != b->integrator)
+    {
+      b->err = "a batch holds contexts of one time integrator (HANCOCK or RK2_SSP)";
       return -1;
     }
     if ((fp32(b) ? static_cast<const void *>(mhdct::fused_slot<float>(ctx->c))
@@ -539,6 +609,21 @@ extern "C"
     if (fp32(b) ? save_clocks<float>(b, b->bf) : save_clocks<double>(b, b->bd))
       return -1;
     mhdct_batch::Entry e{};
+    if (integ == MHDCT_TIME_RK2_SSP)
+    {
+      // the stage buffers, zeroed so that the outside blocks (refilled before each use) hold
+      // finite values
+      const mhdct::Shapes S{ ctx->c.dim, ctx->c.b };
+      const size_t        nu = size_t(ctx->c.dm.ntotal > 0 ? ctx->c.dm.ntotal : 1) * 5 * S.cells() * sizeof(double);
+      const size_t        nb = size_t(ctx->c.dm.ntotal > 0 ? ctx->c.dm.ntotal : 1) * S.fper() * sizeof(double);
+      if (hipMalloc(&e.R, nu) != hipSuccess || hipMalloc(&e.RB, nb) != hipSuccess ||
+          hipMemset(e.R, 0, nu) != hipSuccess || hipMemset(e.RB, 0, nb) != hipSuccess)
+      {
+        free_stage_buffers(e);
+        b->err = "hipMalloc failed for the RK2_SSP stage buffers";
+        return -1;
+      }
+    }
     e.id = b->next_id++;
     e.ctx = ctx;
     e.X[0] = U;
@@ -549,6 +634,7 @@ extern "C"
     e.time = 0.0;
     e.steps = 0.0;
     b->ent.push_back(e);
+    b->integrator = integ;
     b->dirty = true;
     if (slot_id)
       *slot_id = e.id;
@@ -569,7 +655,15 @@ extern "C"
       b->err = "no such slot";
       return -1;
     }
+    if (it->R && hipDeviceSynchronize() != hipSuccess)
+    {
+      b->err = "device synchronisation failed";
+      return -1;
+    }
+    free_stage_buffers(*it);
     b->ent.erase(it);
+    if (b->ent.empty())
+      b->integrator = -1;
     b->dirty = true;
     return 0;
   }
@@ -725,6 +819,8 @@ extern "C"
     (void)hipDeviceSynchronize();
     drop_graphs(b);
     free_worklist(b);
+    for (auto & e : b->ent)
+      free_stage_buffers(e);
     if (b->cap)
       (void)hipStreamDestroy(b->cap);
     if (b->side)

@@ -1260,7 +1260,8 @@ k_dt_clear(FusedSlot<T> * __restrict__ slots, int nslots)
 
 template <typename T>
 __global__ void __launch_bounds__(64)
-k_dt_set(FusedSlot<T> * __restrict__ even, FusedSlot<T> * __restrict__ odd, int nslots, double * __restrict__ clock)
+k_dt_set(FusedSlot<T> * even, FusedSlot<T> * odd, FusedSlot<T> * even2, FusedSlot<T> * odd2, int nslots,
+         double * __restrict__ clock)
 {
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += gridDim.x * blockDim.x)
   {
@@ -1268,6 +1269,10 @@ k_dt_set(FusedSlot<T> * __restrict__ even, FusedSlot<T> * __restrict__ odd, int 
     const double dt = even[s].cfl / inv;
     even[s].dt = dt;
     odd[s].dt = dt;
+    if (even2)
+      even2[s].dt = dt; // the second-stage arrays of an RK2_SSP batch
+    if (odd2)
+      odd2[s].dt = dt;
     clock[2 * s] += dt;
     clock[2 * s + 1] += 1.0;
   }
@@ -1445,7 +1450,8 @@ launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
   hipLaunchKernelGGL((k_dt_clear<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[parity], n);
   hipLaunchKernelGGL((k_inv_dt<T, DIM, BSEL>), grid_for(bt.tot[kStageInvDt], 256, 2048), 256, 0, st,
                      bt.d_slots[parity], n, bt.d_off[kStageInvDt], bt.tot[kStageInvDt]);
-  hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], n, bt.clock);
+  hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], bt.d_slots[2],
+                     bt.d_slots[3], n, bt.clock);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

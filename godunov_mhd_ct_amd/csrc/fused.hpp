@@ -91,16 +91,17 @@ struct FusedTune
 
 // The device work list of a batch: per stage, the prefix offsets of every slot's work items
 // (each slot's range padded to 512), and the slot array in two parities: parity 0 advances
-// U -> U2, parity 1 advances U2 -> U.
+// U -> U2, parity 1 advances U2 -> U.  RK2_SSP batches use these two arrays for the first stage
+// (into the slots' stage buffers) and arrays 2 and 3 for the second (from them).
 template <typename T>
 struct FusedBatch
 {
   int            dim = 3;
   int            bsel = 0; // the block size of every slot, or 0 when they differ
-  int            rk2 = 0;  // every slot is an RK2_SSP stage (per-context steps; batches are HANCOCK)
+  int            rk2 = 0;  // every slot is an RK2_SSP stage (a batch holds one integrator)
   int            grav = 0; // some slot has gravity
   int            nslots = 0;
-  FusedSlot<T> * d_slots[2] = { nullptr, nullptr };
+  FusedSlot<T> * d_slots[4] = { nullptr, nullptr, nullptr, nullptr };
   long *         d_off[kStages] = {};
   long           tot[kStages] = {};
   double *       clock = nullptr; // [nslots][2]: simulated time, steps (device-side dt only)
@@ -131,7 +132,7 @@ constexpr int fused_stage_order[8] = { kStagePrim, kStageEfield, kStageCell, kSt
                                        kStageTile, kStageEdge, kStageFix,
                                        kStageBupdate };
 // dt on the device: clear the slots' reduction words, reduce over the owned cells of the state
-// of `parity`, then dt = cfl / inv_dt into both slot arrays and the clock advanced
+// of `parity`, then dt = cfl / inv_dt into every slot array and the clock advanced
 template <typename T>
 int fused_launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st);
 

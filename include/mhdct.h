@@ -191,7 +191,9 @@ typedef struct mhdct_batch mhdct_batch;
 
 int mhdct_batch_create(int dim, int impl_version, mhdct_batch ** out);
 /* Add a context (mesh set, same dim and impl) with its state buffers; the slot starts at the
- * next batch step.  Synchronises the device. */
+ * next batch step.  A batch holds contexts of one time integrator: HANCOCK, or RK2_SSP, whose
+ * step (SolverGodunovMHD.cpp:551-647) runs both stages through a stage buffer the batch
+ * allocates for the slot.  Synchronises the device. */
 int mhdct_batch_add(mhdct_batch * batch, mhdct_ctx * ctx, double * U, double * U2, double * B, double * B2,
                     int * slot_id);
 /* Remove a slot after the steps enqueued so far.  Synchronises the device. */
@@ -201,7 +203,8 @@ int mhdct_batch_step(mhdct_batch * batch, int nsteps, void * stream);
 /* The same as mhdct_batch_step without graphs, with HIP events recorded around each stage on
  * the stream it runs on: stage_ms[MHDCT_NUM_STAGES] receives the average milliseconds per step
  * of each stage (names from mhdct_stage_name; with the overlap on, k_edge runs beside
- * k_border_flux and k_flux_tile).  Synchronises `stream`. */
+ * k_border_flux and k_flux_tile).  HANCOCK batches only (-1 for an RK2_SSP batch).
+ * Synchronises `stream`. */
 #define MHDCT_NUM_STAGES 10
 int          mhdct_batch_step_timed(mhdct_batch * batch, int nsteps, void * stream, double * stage_ms);
 const char * mhdct_stage_name(int stage);

@@ -1,15 +1,18 @@
 """GPU parity of the RK2_SSP integrator (SolverGodunovMHD.cpp:551-647): the plain and fused FP64
 forms driven through two forward-Euler stages and the stage combinations, against the oracle's
 RK2_SSP step at the 1e-12 relative L-inf bar; the stage combination entry point against numpy
-bit for bit; the FP32-bulk form against the FP64 form; batches refuse RK2_SSP contexts.
+bit for bit; the FP32-bulk form against the FP64 form; RK2_SSP batches (device dt, three
+slots including a walled one) against the oracle per slot, alone and graph-replayed equal to
+the mix, and one integrator per batch.
 """
 import numpy as np
 import pytest
 
 from godunov_mhd_ct_amd import problems
 from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32, IMPL_PLAIN, GodunovImplemHIP, hydro_settings
-from godunov_mhd_ct_amd.mesh import Mesh, uniform_mesh
-from test_gpu_parity import amr_mesh_2d, amr_mesh_3d, run_pair
+from godunov_mhd_ct_amd.mesh import Mesh, refine_where, uniform_mesh
+from oracle_ref import OracleSolver, settings_array
+from test_gpu_parity import TOL, amr_mesh_2d, amr_mesh_3d, rel_linf, run_pair
 
 IMPLS = [IMPL_PLAIN, IMPL_FUSED]
 RK2 = "RK2_SSP"
@@ -105,15 +108,87 @@ def test_rk2_fp32_bulk_close_to_fp64():
     assert err <= 1e-5, err
 
 
+def rk2_batch_cases():
+    """Three 3D slots: a uniform KH (MC limiter), an AMR blast, a walled blast (the outside
+    fill of the stage buffer runs between the stages)."""
+    cases = []
+    m = uniform_mesh(3, 8, 1)
+    U, B = problems.kelvin_helmholtz_3d(m, seed=5)
+    cases.append((m, U, B, 2.0, None))
+    m = refine_where(Mesh(3, 8, 1), lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.5 and y0 < 0.5 and z0 < 0.5, 2)
+    U, B = problems.blast(m, radius=0.3, center=(0.3, 0.3, 0.3))
+    cases.append((m, U, B, 1.0, None))
+    bc = [3, 3, 2, 2, 3, 3]
+    m = Mesh(3, 8, 1, periodic=(0, 0, 0))
+    m.set_bc(bc)
+    m.finalize()
+    U, B = problems.blast(m, radius=0.3, center=(0.4, 0.5, 0.5), bfield=(0.3, 0.2, 0.4))
+    cases.append((m, U, B, 1.0, bc))
+    return cases
+
+
+def run_rk2_batch(cases, nsteps, graph=False, only=None):
+    import torch
+    from godunov_mhd_ct_amd.batch import BatchGodunovMHD
+
+    bt = BatchGodunovMHD(3, IMPL_FUSED)
+    if graph:
+        bt.set_graph(True)
+    ids = {}
+    for n, (m, U, B, slope, bc) in enumerate(cases):
+        if only is None or n == only:
+            ids[n] = bt.add(m, U, B, hydro_settings(gamma0=5.0 / 3.0, cfl=0.7, slope_type=slope,
+                                                    time_integrator=RK2))
+    bt.step(nsteps)
+    bt.check()
+    torch.cuda.synchronize()
+    out = {n: (bt.state(sid), bt.slot_state(sid)) for n, sid in ids.items()}
+    return out
+
+
 @pytest.mark.gpu
-def test_batch_refuses_rk2_context():
+def test_rk2_batch_matches_oracle_per_slot():
+    cases = rk2_batch_cases()
+    nsteps = 3
+    print(f"[rk2] batch of {len(cases)} RK2_SSP slots, {nsteps} steps, device dt", flush=True)
+    res = run_rk2_batch(cases, nsteps)
+    for n, (m, U, B, slope, bc) in enumerate(cases):
+        ora = OracleSolver(m, settings_array(gamma0=5.0 / 3.0, cfl=0.7, slope_type=slope, time_integrator=RK2),
+                           nthreads=4)
+        Uo = U.reshape(m.num_total, 5, -1).copy()
+        Bo = B.reshape(m.num_total, -1).copy()
+        t = 0.0
+        for _ in range(nsteps):
+            Uo, Bo, dt = ora.step(Uo, Bo)
+            t += dt
+        (Ug, Bg), (cur, tg, steps) = res[n]
+        assert steps == nsteps and abs(tg - t) <= 1e-14 * t, (n, tg, t)
+        no = m.num_leaves
+        for v in range(5):
+            assert rel_linf(Ug[:no, v], Uo[:no, v]) <= TOL, (n, v)
+        assert rel_linf(Bg[:no], Bo[:no]) <= TOL, n
+
+
+@pytest.mark.gpu
+def test_rk2_batch_alone_graph_and_refusals():
     import torch
     from godunov_mhd_ct_amd._lib import MhdctError
     from godunov_mhd_ct_amd.batch import BatchGodunovMHD
 
-    m = uniform_mesh(2, 8, 1)
-    U, B = problems.orszag_tang_2d(m)
-    bt = BatchGodunovMHD(2, IMPL_FUSED)
-    with pytest.raises(MhdctError, match="HANCOCK"):
-        bt.add(m, U, B, hydro_settings(time_integrator=RK2))
+    cases = rk2_batch_cases()
+    mix = run_rk2_batch(cases, 4)
+    graphed = run_rk2_batch(cases, 4, graph=True)
+    for n in range(len(cases)):
+        alone = run_rk2_batch(cases, 4, only=n)
+        no = cases[n][0].num_leaves
+        for other in (alone, graphed):
+            (Ua, Ba), _ = other[n]
+            (Um, Bm), _ = mix[n]
+            assert np.array_equal(Ua[:no], Um[:no]) and np.array_equal(Ba[:no], Bm[:no]), n
+    # one integrator per batch
+    m, U, B, slope, _ = cases[0]
+    bt = BatchGodunovMHD(3, IMPL_FUSED)
+    bt.add(m, U, B, hydro_settings(time_integrator=RK2))
+    with pytest.raises(MhdctError, match="one time integrator"):
+        bt.add(m, U.copy(), B.copy(), hydro_settings())
     torch.cuda.synchronize()
