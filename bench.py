@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--amr-cycle", action="store_true",
                     help="time one AMR cycle (refinement flags, host adaptation, remap) on the workload after "
                          "--steps steps, [amr] keys of app/test_mhd_blast_3d.ini")
+    ap.add_argument("--gravity", type=float, default=None,
+                    help="add a uniform gravity field of this magnitude along -z (-y in 2D) to the workload")
     ap.add_argument("--viscosity", type=float, default=None,
                     help="plain form with viscous fluxes of this mu (and the viscous predictor term), next to "
                          "the same steps without viscosity")
@@ -87,8 +89,12 @@ def cpu_baseline(cases, steps):
     cells = 0
     el = 0.0
     for mesh, U, B, st in cases:
+        grav = tuple(st.gravity) if st.gravity_enabled else None
+        mu = st.viscosity_mu if st.viscosity_enabled else None
         ora = OracleSolver(mesh, settings_array(gamma0=st.gamma0, cfl=st.cfl, slope_type=st.slope_type,
-                                                time_integrator=st.time_integrator), nthreads=threads)
+                                                time_integrator=st.time_integrator, gravity=grav, mu=mu,
+                                                viscous_predictor=bool(st.viscosity_hancock_predictor)),
+                           nthreads=threads)
         nt = mesh.num_total
         Uo = U.reshape(nt, 5, -1).copy()
         Bo = B.reshape(nt, -1).copy()
@@ -215,6 +221,11 @@ def main():
         return bench_serving(args)
 
     cases = cases_for(args.workload)
+    if args.gravity is not None:
+        # [gravity] enabled with a uniform field along the last axis (AddGravitySourceTerm.cpp)
+        for (m, _, _, st) in cases:
+            st.gravity_enabled = 1
+            st.gravity[m.dim - 1] = -abs(args.gravity)
     if args.amr_cycle:
         return bench_amr(args, cases)
     if args.checkpoint:
@@ -229,7 +240,8 @@ def main():
     out = {"metric": METRIC, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if args.impl != "fp32" else "f32 bulk / f64 accumulation", "data": "synthetic"}
-    config = {"workload": args.workload, "impl": args.impl, "meshes": len(cases),
+    config = {"workload": args.workload + ("+gravity" if args.gravity is not None else ""), "impl": args.impl,
+              "meshes": len(cases),
               "leaves": sum(m.num_leaves for (m, _, _, _) in cases), "cells": cells,
               "blocks": sorted({m.b for (m, _, _, _) in cases}), "dim": cases[0][0].dim}
     if args.impl == "plain":
