@@ -10,7 +10,9 @@ One step follows SolverGodunovMHD::next_iteration_impl (SolverGodunovMHD.cpp:409
   U2 = 1/2 U + 1/2 U2 (stage-2 coefficients).
 
 The reference copies U2 back into U; here the buffers swap roles and the copy moves to the
-start of the next step (the same traffic, the same contract).
+start of the next step (the same traffic, the same contract). The fused forms write every owned
+value of their output from the input alone, so they skip these copies (and RK2_SSP's U_RK = U and
+U2 = U_RK): the owned results are the same, and the outside blocks are refilled before use.
 
 `do_amr_cycle` is the reference's AMR cycle (:263-319) on this mesh: refinement flags on the
 device, adaptation and balance of a copy of the host octree, remap of U and face B on the
@@ -47,6 +49,9 @@ class SolverGodunovMHD:
         self.U2 = self.U.clone()
         self.B2 = self.B.clone()
         self.rk2 = self.settings.time_integrator == TIME_INTEGRATOR["RK2_SSP"]
+        # the plain form accumulates into U2 (U2 == U on entry, the reference's contract); the
+        # fused forms write every owned value of U2 and B2 from U and B, so they skip the copies
+        self.copies = impl_version == IMPL_PLAIN
         if self.rk2:
             self.U_RK = self.U.clone()
             self.B_RK = self.B.clone()
@@ -68,8 +73,9 @@ class SolverGodunovMHD:
         if self.rk2:
             self._rk2_stages(dt)
         else:
-            self.U2.copy_(self.U)
-            self.B2.copy_(self.B)
+            if self.copies:
+                self.U2.copy_(self.U)
+                self.B2.copy_(self.B)
             self.implem.do_time_step(self.U, self.U2, self.B, self.B2, dt)
         self.U, self.U2 = self.U2, self.U
         self.B, self.B2 = self.B2, self.B
@@ -82,14 +88,16 @@ class SolverGodunovMHD:
     def _rk2_stages(self, dt: float):
         # SolverGodunovMHD.cpp:555-622, after the outside fill of U
         im = self.implem
-        self.U_RK.copy_(self.U)
-        self.B_RK.copy_(self.B)
+        if self.copies:
+            self.U_RK.copy_(self.U)
+            self.B_RK.copy_(self.B)
         im.do_time_step(self.U, self.U_RK, self.B, self.B_RK, dt)
         im.linear_combination(self.U, self.U_RK, self.B, self.B_RK, RK2_SSP_STAGE1_COEFS)
         if self.mesh.num_outside:
             im.fill_outside(self.U_RK, self.B_RK)
-        self.U2.copy_(self.U_RK)
-        self.B2.copy_(self.B_RK)
+        if self.copies:
+            self.U2.copy_(self.U_RK)
+            self.B2.copy_(self.B_RK)
         im.do_time_step(self.U_RK, self.U2, self.B_RK, self.B2, dt)
         im.linear_combination(self.U, self.U2, self.B, self.B2, RK2_SSP_STAGE2_COEFS)
 
