@@ -23,13 +23,11 @@ Mesh::key(int l, int x, int y, int z)
 void
 Mesh::rebuild_hash()
 {
-  hash_.clear();
-  hash_.reserve(size_t(ntotal()) * 2 + 16);
+  hash_.clear(size_t(ntotal()));
   for (size_t i = 0; i < leaves.size(); ++i)
-    hash_[key(leaves[i].level, leaves[i].x, leaves[i].y, leaves[i].z)] = int(i);
+    hash_.put(key(leaves[i].level, leaves[i].x, leaves[i].y, leaves[i].z), int(i));
   for (size_t i = 0; i < outside.size(); ++i)
-    hash_[key(outside[i].level, outside[i].x, outside[i].y, outside[i].z)] =
-      int(leaves.size() + i);
+    hash_.put(key(outside[i].level, outside[i].x, outside[i].y, outside[i].z), int(leaves.size() + i));
 }
 
 int
@@ -37,8 +35,7 @@ Mesh::find(int l, int x, int y, int z) const
 {
   if (l < 0)
     return -1;
-  auto it = hash_.find(key(l, x, y, z));
-  return it == hash_.end() ? -1 : it->second;
+  return hash_.get(key(l, x, y, z));
 }
 
 void
@@ -176,55 +173,79 @@ Mesh::set_leaves(const std::vector<Leaf> & given)
 void
 Mesh::balance()
 {
-  // iterate until no leaf has a neighbour (face/edge/corner) more than one level coarser
+  // iterate until no leaf has a neighbour (face/edge/corner) more than one level coarser; each
+  // pass scans the leaves in contiguous ranges on host threads, each marking its own flags
   for (;;)
   {
     rebuild_hash();
-    std::vector<int8_t> flags(leaves.size(), 0);
-    bool                any = false;
-    for (size_t i = 0; i < leaves.size(); ++i)
-    {
-      const Leaf & L = leaves[i];
-      if (L.level < 2)
-        continue;
-      for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx)
-          {
-            if (!dx && !dy && !dz)
-              continue;
-            long p[3] = { long(L.x) + dx, long(L.y) + dy, long(L.z) + dz };
-            bool skip = false;
-            for (int d = 0; d < dim; ++d)
+    const int nleaves = int(leaves.size());
+    const int nthr = std::max(1, std::min<int>(int(std::thread::hardware_concurrency()), std::min(16, nleaves / 256)));
+    std::vector<std::vector<int8_t>> tflags(static_cast<size_t>(nthr));
+    auto scan = [&](int t) {
+      std::vector<int8_t> & flags = tflags[size_t(t)];
+      flags.assign(size_t(nleaves), 0);
+      const int lo = int(long(nleaves) * t / nthr), hi = int(long(nleaves) * (t + 1) / nthr);
+      for (int i = lo; i < hi; ++i)
+      {
+        const Leaf & L = leaves[size_t(i)];
+        if (L.level < 2)
+          continue;
+        for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx)
             {
-              const long n = nblocks(L.level, d);
-              if (p[d] < 0 || p[d] >= n)
+              if (!dx && !dy && !dz)
+                continue;
+              long p[3] = { long(L.x) + dx, long(L.y) + dy, long(L.z) + dz };
+              bool skip = false;
+              for (int d = 0; d < dim; ++d)
               {
-                if (periodic[d])
-                  p[d] = ((p[d] % n) + n) % n;
-                else
-                  skip = true;
-              }
-            }
-            if (skip)
-              continue;
-            // walk up: find the leaf holding position p
-            for (int l = L.level; l >= 0; --l)
-            {
-              const int sh = L.level - l;
-              const int o = find(l, int(p[0] >> sh), int(p[1] >> sh), int(p[2] >> sh));
-              if (o >= 0)
-              {
-                if (l < L.level - 1 && !flags[o])
+                const long n = nblocks(L.level, d);
+                if (p[d] < 0 || p[d] >= n)
                 {
-                  flags[o] = 1;
-                  any = true;
+                  if (periodic[d])
+                    p[d] = ((p[d] % n) + n) % n;
+                  else
+                    skip = true;
                 }
-                break;
+              }
+              if (skip)
+                continue;
+              // walk up: find the leaf holding position p; one more than a level coarser
+              // gets refined
+              for (int l = L.level; l >= 0; --l)
+              {
+                const int sh = L.level - l;
+                const int o = find(l, int(p[0] >> sh), int(p[1] >> sh), int(p[2] >> sh));
+                if (o >= 0)
+                {
+                  if (l < L.level - 1)
+                    flags[size_t(o)] = 1;
+                  break;
+                }
               }
             }
-          }
+      }
+    };
+    if (nthr == 1)
+      scan(0);
+    else
+    {
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nthr; ++t)
+        pool.emplace_back(scan, t);
+      for (auto & th : pool)
+        th.join();
     }
+    std::vector<int8_t> flags(size_t(nleaves), 0);
+    bool                any = false;
+    for (const auto & f : tflags)
+      for (int i = 0; i < nleaves; ++i)
+        if (f[size_t(i)])
+        {
+          flags[size_t(i)] = 1;
+          any = true;
+        }
     if (!any)
       break;
     adapt(flags);

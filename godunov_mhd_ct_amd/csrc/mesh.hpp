@@ -49,6 +49,56 @@ struct EmfFix
 static_assert(sizeof(EmfFix) == 24, "EmfFix layout");
 
 
+// Block key -> index map with open addressing and linear probing: the neighbour-table build
+// and the 2:1 balance look keys up millions of times per mesh.  Indices are >= 0, so an index
+// of -1 marks an empty bucket (every key value, 0 included, can be stored).
+class KeyIndex
+{
+public:
+  void clear(size_t n)
+  {
+    size_t cap = 16;
+    while (cap < 2 * n + 16)
+      cap <<= 1;
+    keys_.assign(cap, 0);
+    vals_.assign(cap, -1);
+    mask_ = cap - 1;
+  }
+  void put(uint64_t k, int v)
+  {
+    size_t h = mix(k) & mask_;
+    while (vals_[h] >= 0 && keys_[h] != k)
+      h = (h + 1) & mask_;
+    keys_[h] = k;
+    vals_[h] = v;
+  }
+  int get(uint64_t k) const
+  {
+    if (keys_.empty())
+      return -1;
+    size_t h = mix(k) & mask_;
+    while (vals_[h] >= 0)
+    {
+      if (keys_[h] == k)
+        return vals_[h];
+      h = (h + 1) & mask_;
+    }
+    return -1;
+  }
+
+private:
+  static size_t mix(uint64_t k)
+  {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    return size_t(k);
+  }
+  std::vector<uint64_t> keys_;
+  std::vector<int>      vals_;
+  size_t                mask_ = 0;
+};
+
 class Mesh
 {
 public:
@@ -92,7 +142,7 @@ public:
   int  ndirs() const { return dim == 3 ? 27 : 9; }
 
 private:
-  std::unordered_map<uint64_t, int> hash_; // owned + outside
+  KeyIndex hash_; // owned + outside
   void rebuild_hash();
   bool build_outside();
   bool build_nb();

@@ -129,3 +129,43 @@ def test_settings_struct_and_time_integrator():
     assert hydro_settings().time_integrator == 0
     with pytest.raises(MhdctError, match="RK3_SSP"):
         hydro_settings(time_integrator="RK3_SSP")  # the reference aborts on it (SolverGodunovMHD.cpp:174-177)
+
+
+def test_level0_origin_leaf_is_found():
+    # the key of the level-0 leaf at the origin is 0: a refined neighbour's tables must still
+    # find it (the coarser neighbour of every child beside it)
+    for dim in (2, 3):
+        m = Mesh(dim, 8, 0, nbrick=(2, 2, 2 if dim == 3 else 1), periodic=(1, 1, 1))
+        lv, co = m.leaves()
+        n = m.num_leaves
+        flags = np.zeros(n, dtype=np.int8)
+        flags[[i for i in range(n) if tuple(co[i][:dim]) == (1,) + (0,) * (dim - 1)]] = 1
+        m.adapt(flags)
+        m.balance()
+        m.finalize()
+        assert m.num_leaves == n - 1 + (1 << dim)
+
+
+def test_balance_large_mesh_is_two_to_one():
+    # the threaded balance on a mesh big enough to split over threads
+    m = Mesh(3, 8, 2)
+    def shell(L, x0, x1, y0, y1, z0, z1):
+        # blocks crossed by the sphere of radius 0.3 about (0.5, 0.45, 0.55)
+        c = ((x0 + x1) / 2 - 0.5, (y0 + y1) / 2 - 0.45, (z0 + z1) / 2 - 0.55)
+        return abs(np.sqrt(sum(v * v for v in c)) - 0.3) < 0.9 * (x1 - x0)
+
+    m = refine_where(m, shell, 4)
+    assert m.num_leaves > 1000  # split over several threads (256 leaves or more each)
+    lv, co = m.leaves()
+    n = m.num_leaves
+    h = 1.0 / (1 << lv[:n]).astype(np.float64)
+    lo = co[:n] * h[:, None]
+    hi = lo + h[:, None]
+    # every pair of touching leaves (faces, edges, corners, periodic images) differs by <= 1
+    for shift in itertools.product((-1.0, 0.0, 1.0), repeat=3):
+        s = np.array(shift)
+        lo2, hi2 = lo + s, hi + s
+        touch = np.all((lo[:, None, :] <= hi2[None, :, :] + 1e-12) & (lo2[None, :, :] <= hi[:, None, :] + 1e-12),
+                       axis=2)
+        a, b = np.nonzero(touch)
+        assert np.all(np.abs(lv[a].astype(int) - lv[b].astype(int)) <= 1)
