@@ -1,13 +1,12 @@
 // border.hip -- builds the CorrectEmfAtBlockBorder gather list (EmfFix records, 8 source slots
 // each) of the mesh in a context on the device, once per mesh: a count pass per workgroup of
-// border slots, an exclusive scan of the counts, and an emit pass that writes each workgroup's
-// records in slot order.
+// border slots, an exclusive scan of the counts on the device (only the total comes back to the
+// host, to size the list), and an emit pass that writes each workgroup's records in slot order.
 #include <hip/hip_runtime.h>
 
 #include <climits>
 #include <cstdint>
 #include <string>
-#include <vector>
 
 #include "border.hpp"
 #include "ctx.hpp"
@@ -79,8 +78,61 @@ k_border_emit(DevMesh m, long total, const int * __restrict__ offsets, EmfFix * 
   r.cnt = int16_t(cnt);
   r.ratio = ratio;
   fix[pos] = r;
-  for (int n = 0; n < nsrc; ++n)
-    fix_src[long(pos) * 8 + n] = src[n];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+    if (n < nsrc)
+      fix_src[long(pos) * 8 + n] = src[n];
+}
+
+constexpr int kScanThreads = 1024;
+
+// exclusive scan of counts[0..n) in place, the sum to counts[n] (one workgroup: each thread
+// sums a run of counts, the run sums are scanned in LDS, then each thread writes its run)
+__global__ void __launch_bounds__(kScanThreads)
+k_border_scan(int * __restrict__ counts, long n)
+{
+  __shared__ long long part[kScanThreads];
+  const long           per = (n + kScanThreads - 1) / kScanThreads;
+  const long           lo = min(n, long(threadIdx.x) * per), hi = min(n, lo + per);
+  long long            s = 0;
+  for (long i = lo; i < hi; ++i)
+    s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1)
+  {
+    const long long v = threadIdx.x >= unsigned(off) ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  long long run = part[threadIdx.x] - s;
+  for (long i = lo; i < hi; ++i)
+  {
+    const int x = counts[i];
+    counts[i] = int(run < INT_MAX ? run : INT_MAX);
+    run += x;
+  }
+  if (threadIdx.x == kScanThreads - 1)
+    counts[n] = int(run < INT_MAX ? run : INT_MAX);
+}
+
+// p holds at least `bytes`: reallocated (with half again as much room) only when too small
+template <class T>
+hipError_t
+grow(T *& p, size_t & cap, size_t bytes)
+{
+  if (p && bytes <= cap)
+    return hipSuccess;
+  if (p)
+    (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = bytes + bytes / 2;
+  const hipError_t e = hipMalloc(&p, want);
+  if (e == hipSuccess)
+    cap = want;
+  return e;
 }
 
 } // namespace
@@ -103,48 +155,34 @@ build_border_list(Context & c)
     c.err = "border list: mesh too large";
     return -1;
   }
-  int *      counts = nullptr;
-  hipError_t e = hipMalloc(&counts, size_t(nblk) * sizeof(int));
+  hipError_t e = grow(c.border_scan, c.table_cap[8], size_t(nblk + 1) * sizeof(int));
   if (e != hipSuccess)
     return fail("hipMalloc", e);
+  int * counts = c.border_scan;
   hipLaunchKernelGGL(k_border_count, dim3(unsigned(nblk)), kThreads, 0, nullptr, m, total, counts);
-  std::vector<int> host(static_cast<size_t>(nblk));
-  e = hipMemcpy(host.data(), counts, size_t(nblk) * sizeof(int), hipMemcpyDeviceToHost);
+  hipLaunchKernelGGL(k_border_scan, dim3(1), kScanThreads, 0, nullptr, counts, nblk);
+  int sum = 0;
+  e = hipMemcpy(&sum, counts + nblk, sizeof(int), hipMemcpyDeviceToHost);
   if (e != hipSuccess)
-  {
-    (void)hipFree(counts);
     return fail("count pass", e);
-  }
-  long sum = 0;
-  for (auto & v : host)
+  // the scan saturates at INT_MAX; the sources are addressed as int32 (8 per record)
+  if (long(sum) * 8 > long(INT32_MAX))
   {
-    const long x = v;
-    v = int(sum);
-    sum += x;
-  }
-  if (sum * 8 > long(INT32_MAX))
-  {
-    (void)hipFree(counts);
     c.err = "border list: too many corrected edges";
     return -1;
   }
-  e = hipMemcpy(counts, host.data(), size_t(nblk) * sizeof(int), hipMemcpyHostToDevice);
+  const size_t n = size_t(sum > 0 ? sum : 1);
+  e = grow(m.fix, c.table_cap[6], n * sizeof(EmfFix));
   if (e == hipSuccess)
-    e = hipMalloc(&m.fix, size_t(sum > 0 ? sum : 1) * sizeof(EmfFix));
-  if (e == hipSuccess)
-    e = hipMalloc(&m.fix_src, size_t(sum > 0 ? sum : 1) * 8 * sizeof(int64_t));
+    e = grow(m.fix_src, c.table_cap[7], n * 8 * sizeof(int64_t));
   if (e != hipSuccess)
-  {
-    (void)hipFree(counts);
     return fail("allocation", e);
-  }
-  c.mesh_bytes += size_t(sum) * (sizeof(EmfFix) + 8 * sizeof(int64_t));
+  c.mesh_bytes += c.table_cap[6] + c.table_cap[7] + c.table_cap[8];
   hipLaunchKernelGGL(k_border_emit, dim3(unsigned(nblk)), kThreads, 0, nullptr, m, total, counts, m.fix, m.fix_src);
   e = hipDeviceSynchronize();
-  (void)hipFree(counts);
   if (e != hipSuccess)
     return fail("emit pass", e);
-  m.nfix = int(sum);
+  m.nfix = sum;
   return 0;
 }
 

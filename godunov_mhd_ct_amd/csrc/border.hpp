@@ -34,6 +34,34 @@ struct BorderEdge
   long e[3];
 };
 
+// v[i] of a 3-vector and v[i] = x, written as selects over the three elements so that a
+// runtime index keeps the vector in registers (an indexed private array would live in scratch)
+template <class T>
+__device__ inline T
+get3(const T v[3], int i)
+{
+  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
+template <class T>
+__device__ inline void
+set3(T v[3], int i, T x)
+{
+  v[0] = i == 0 ? x : v[0];
+  v[1] = i == 1 ? x : v[1];
+  v[2] = i == 2 ? x : v[2];
+}
+
+// src[n++] = x, as a select over the eight slots (src stays in registers)
+__device__ inline void
+push8(long src[8], int & n, long x)
+{
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    src[k] = k == n ? x : src[k];
+  ++n;
+}
+
 __device__ inline BorderEdge
 border_edge(int dim, int b, int slot)
 {
@@ -64,11 +92,9 @@ border_edge(int dim, int b, int slot)
     e1 = side < 2 ? mid : (side == 2 ? 0 : b);
     e2 = side < 2 ? (side == 0 ? 0 : b) : mid;
   }
-  E.e[0] = E.e[1] = E.e[2] = 0;
-  E.e[E.t1] = e1;
-  E.e[E.t2] = e2;
-  if (dim == 3)
-    E.e[E.a] = ea;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    E.e[d] = d == E.t1 ? e1 : (d == E.t2 ? e2 : (dim == 3 && d == E.a ? ea : 0));
   return E;
 }
 
@@ -92,7 +118,7 @@ border_sources(const DevMesh & m, int o, int slot, long & dst, long src[8], int 
   const BorderEdge E = border_edge(dim, b, slot);
   const long *     e = E.e;
   const int        t1 = E.t1, t2 = E.t2, a = E.a;
-  const int        e1 = int(e[t1]), e2 = int(e[t2]);
+  const int        e1 = int(get3(e, t1)), e2 = int(get3(e, t2));
   const bool       b1 = (e1 == 0 || e1 == b), b2 = (e2 == 0 || e2 == b);
   const int        nd = dim == 3 ? 27 : 9, ns = 1 << dim;
   if (b1 != b2)
@@ -101,7 +127,7 @@ border_sources(const DevMesh & m, int o, int slot, long & dst, long src[8], int 
     const int fd = b1 ? t1 : t2;
     const int fs = (b1 ? e1 : e2) == b ? 1 : 0;
     int       dl[3] = { 0, 0, 0 };
-    dl[fd] = fs ? 1 : -1;
+    set3(dl, fd, fs ? 1 : -1);
     const int     di = (dl[0] + 1) + 3 * (dl[1] + 1) + (dim == 3 ? 9 * (dl[2] + 1) : 0);
     const NbEntry F = m.nb[(long(o) * nd + di) * ns];
     if (F.oct >= 0 && F.rel == 0)
@@ -119,8 +145,8 @@ border_sources(const DevMesh & m, int o, int slot, long & dst, long src[8], int 
     if (q == ownq)
       continue;
     long c[3] = { e[0], e[1], e[2] };
-    c[t1] = e[t1] - 1 + (q & 1);
-    c[t2] = e[t2] - 1 + (q >> 1);
+    set3(c, t1, get3(e, t1) - 1 + (q & 1));
+    set3(c, t2, get3(e, t2) - 1 + (q >> 1));
     bool inside = true;
     for (int d = 0; d < dim; ++d)
       if (c[d] < 0 || c[d] >= b)
@@ -189,14 +215,14 @@ border_sources(const DevMesh & m, int o, int slot, long & dst, long src[8], int 
     {
       if (q == ownq)
       {
-        src[nsrc++] = own;
+        push8(src, nsrc, own);
         ++cnt;
         continue;
       }
       const bool take = all_same ? (valid[q] && !out[q]) : (valid[q] && srel[q] == 0 && !out[q]);
       if (take)
       {
-        src[nsrc++] = edge_index(dim, b, soct[q], E.comp, se[q]);
+        push8(src, nsrc, edge_index(dim, b, soct[q], E.comp, se[q]));
         ++cnt;
       }
     }
@@ -210,12 +236,12 @@ border_sources(const DevMesh & m, int o, int slot, long & dst, long src[8], int 
     {
       if (q == ownq || !(valid[q] && srel[q] > 0 && !out[q]))
         continue;
-      src[nsrc++] = edge_index(dim, b, soct[q], E.comp, se[q]);
+      push8(src, nsrc, edge_index(dim, b, soct[q], E.comp, se[q]));
       if (dim == 3)
       {
         long e2v[3] = { se[q][0], se[q][1], se[q][2] };
-        e2v[a] += 1;
-        src[nsrc++] = edge_index(dim, b, soct[q], E.comp, e2v);
+        set3(e2v, a, get3(e2v, a) + 1);
+        push8(src, nsrc, edge_index(dim, b, soct[q], E.comp, e2v));
       }
       ++cnt;
     }

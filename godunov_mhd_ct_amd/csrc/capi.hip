@@ -28,19 +28,28 @@ remap_create_error()
   return e;
 }
 
+// copy a host table into its device buffer, reallocated (with a margin of 1/2) only when the
+// new table does not fit: an AMR cycle then mostly reuses the previous mesh's buffers
 template <typename T>
 int
-upload(Context & c, T *& dst, const std::vector<T> & src)
+upload(Context & c, T *& dst, size_t & cap, const std::vector<T> & src)
 {
-  if (dst)
-    (void)hipFree(dst);
-  dst = nullptr;
   const size_t bytes = (src.empty() ? 1 : src.size()) * sizeof(T);
-  hipError_t   e = hipMalloc(&dst, bytes);
-  if (e != hipSuccess)
+  hipError_t   e = hipSuccess;
+  if (!dst || cap < bytes)
   {
-    c.err = std::string("hipMalloc: ") + hipGetErrorString(e);
-    return -1;
+    if (dst)
+      (void)hipFree(dst);
+    dst = nullptr;
+    cap = 0;
+    e = hipMalloc(&dst, bytes + bytes / 2);
+    if (e != hipSuccess)
+    {
+      dst = nullptr;
+      c.err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      return -1;
+    }
+    cap = bytes + bytes / 2;
   }
   if (!src.empty())
   {
@@ -51,27 +60,33 @@ upload(Context & c, T *& dst, const std::vector<T> & src)
       return -1;
     }
   }
-  c.mesh_bytes += bytes;
+  c.mesh_bytes += cap;
   return 0;
 }
 
+// the mesh's device tables, the border-edge gather list included, and the scan buffer it is
+// built with
 void
 free_mesh(Context & c)
 {
-  void * ps[] = { c.dm.level, c.dm.coord, c.dm.nb, c.dm.conformal, c.dm.fix, c.dm.fix_src, c.dm.out_src, c.dm.out_dirs };
-  for (void * p : ps)
+  void * ts[] = { c.dm.level, c.dm.coord, c.dm.nb, c.dm.conformal, c.dm.out_src, c.dm.out_dirs, c.dm.fix,
+                  c.dm.fix_src, c.border_scan };
+  for (void * p : ts)
     if (p)
       (void)hipFree(p);
   c.dm.level = nullptr;
   c.dm.coord = nullptr;
   c.dm.nb = nullptr;
   c.dm.conformal = nullptr;
+  c.dm.out_src = nullptr;
+  c.dm.out_dirs = nullptr;
   c.dm.fix = nullptr;
   c.dm.fix_src = nullptr;
   c.dm.nfix = 0;
-  c.dm.out_src = nullptr;
-  c.dm.out_dirs = nullptr;
+  c.border_scan = nullptr;
   c.mesh_bytes = 0;
+  for (auto & cp : c.table_cap)
+    cp = 0;
 }
 
 } // namespace
@@ -309,22 +324,31 @@ extern "C"
       c.err = "mesh not finalized (call mhdct_mesh_finalize)";
       return -1;
     }
-    c.mesh = mesh->m;
+    const mhdct::Mesh & M = mesh->m;
+    // the device tables are reused when large enough: work still in flight on any stream may
+    // read the previous mesh's, so let it finish first
+    const hipError_t se = hipDeviceSynchronize();
+    if (se != hipSuccess)
+    {
+      c.err = std::string("set_mesh: ") + hipGetErrorString(se);
+      return -1;
+    }
     c.has_mesh = true;
-    free_mesh(c);
+    c.mesh_bytes = 0;
     c.dm.dim = c.dim;
     c.dm.b = c.b;
-    c.dm.nowned = int(c.mesh.leaves.size());
-    c.dm.ntotal = c.mesh.ntotal();
-    c.dm.noutside = int(c.mesh.outside.size());
+    c.dm.nowned = int(M.leaves.size());
+    c.dm.ntotal = M.ntotal();
+    c.dm.noutside = int(M.outside.size());
+    c.dm.nfix = 0;
     for (int f = 0; f < 6; ++f)
-      c.dm.bc[f] = c.mesh.bc[f];
+      c.dm.bc[f] = M.bc[f];
     for (int d = 0; d < 3; ++d)
-      c.dm.nbrick[d] = c.mesh.nbrick[d];
+      c.dm.nbrick[d] = M.nbrick[d];
     std::vector<int> lv, co;
     lv.reserve(size_t(c.dm.ntotal));
     co.reserve(size_t(c.dm.ntotal) * 3);
-    for (const auto * list : { &c.mesh.leaves, &c.mesh.outside })
+    for (const auto * list : { &M.leaves, &M.outside })
       for (auto & L : *list)
       {
         lv.push_back(L.level);
@@ -332,15 +356,16 @@ extern "C"
         co.push_back(L.y);
         co.push_back(L.z);
       }
-    if (upload(c, c.dm.level, lv) || upload(c, c.dm.coord, co) || upload(c, c.dm.nb, c.mesh.nb) ||
-        upload(c, c.dm.conformal, c.mesh.conformal) || upload(c, c.dm.out_src, c.mesh.outside_src) ||
-        upload(c, c.dm.out_dirs, c.mesh.outside_dirs) || mhdct::build_border_list(c))
+    if (upload(c, c.dm.level, c.table_cap[0], lv) || upload(c, c.dm.coord, c.table_cap[1], co) ||
+        upload(c, c.dm.nb, c.table_cap[2], M.nb) || upload(c, c.dm.conformal, c.table_cap[3], M.conformal) ||
+        upload(c, c.dm.out_src, c.table_cap[4], M.outside_src) ||
+        upload(c, c.dm.out_dirs, c.table_cap[5], M.outside_dirs) || mhdct::build_border_list(c))
       return -1;
     int rc = 0;
     if (c.impl == MHDCT_IMPL_PLAIN)
       rc = mhdct::plain_alloc(c);
     else
-      rc = mhdct::fused_alloc(c);
+      rc = mhdct::fused_alloc(c, M);
     return rc;
   }
 

@@ -108,10 +108,14 @@ struct Context
   int            b = 16;
   int            impl = 10;
   int            device = 0;
-  Mesh           mesh;
   bool           has_mesh = false;
   DevMesh        dm;
   size_t         mesh_bytes = 0;
+  // capacity in bytes of each mesh table of dm (level, coord, nb, conformal, out_src,
+  // out_dirs, fix, fix_src) and of the border-list count scan: each is reallocated only when
+  // a new mesh outgrows it
+  size_t         table_cap[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+  int *          border_scan = nullptr;
   PlainScratch   ps;
   // reduction scalars
   unsigned long long * red = nullptr; // [0] inv_dt bits, [1] error flags

@@ -27,25 +27,17 @@ struct FusedState
   FusedSlot<float>    hf{};
   FusedBatch<double>  bd;
   FusedBatch<float>   bf;
-  std::vector<void *> allocs;      // per-mesh buffers, rebuilt with every mesh
-  size_t              bytes = 0;
+  size_t              bytes = 0;   // device bytes held (arena and small block)
   void *              arena = nullptr; // the scratch records: grow-only across meshes
   size_t              arena_bytes = 0;
+  void *              small = nullptr; // the small per-mesh buffers in one block, grow-only too
+  size_t              small_bytes = 0;
 };
 
-// the per-mesh buffers of F (the arena stays)
-void
-free_mesh_buffers(FusedState & F)
-{
-  for (void * p : F.allocs)
-    (void)hipFree(p);
-  F.allocs.clear();
-  F.bytes = 0;
-}
 
 template <typename T>
 int
-build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
+build(Context & c, const Mesh & mesh, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
 {
   const int  dim = c.dim, b = c.b;
   const long nq = c.dm.nowned;
@@ -54,23 +46,12 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
     c.err = "the fused form is built for block sizes 8, 16 and 32";
     return -4;
   }
-  auto alloc = [&](size_t bytes, void ** p) -> int {
-    if (bytes == 0)
-      bytes = 8;
-    if (hipMalloc(p, bytes) != hipSuccess)
-    {
-      c.err = "hipMalloc failed for fused scratch";
-      return -1;
-    }
-    (void)hipMemset(*p, 0, bytes);
-    F.allocs.push_back(*p);
-    F.bytes += bytes;
-    return 0;
-  };
   const FusedSizes z(dim, b);
   // The arena holds records that every step writes before it reads them, so a new mesh reuses
   // the arena of the previous one when it fits (an AMR cycle then allocates nothing this
-  // large); it grows with a margin of 1/8 when it does not.
+  // large).  It is allocated with a margin of 1/2: an adaptive mesh may grow by half before the
+  // arena is reallocated (HBM is plentiful on this device; the margin falls back to nothing
+  // when it does not fit).
   const size_t need = size_t(nq > 0 ? nq : 1) * z.total() * sizeof(T);
   if (F.arena_bytes < need)
   {
@@ -78,7 +59,7 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
       (void)hipFree(F.arena);
     F.arena = nullptr;
     F.arena_bytes = 0;
-    size_t want = need + need / 8;
+    size_t want = need + need / 2;
     if (hipMalloc(&F.arena, want) != hipSuccess)
     {
       (void)hipGetLastError(); // the margin did not fit: try the exact size
@@ -121,12 +102,10 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   sl.BF = p;  p += nq * z.bf;
   sl.emf = p; p += nq * z.emf;
   sl.Ed = p;  p += nq * z.ed;
-  if (alloc(size_t(c.dm.nfix > 0 ? c.dm.nfix : 1) * sizeof(T), reinterpret_cast<void **>(&sl.fixval)))
-    return -1;
   // border fluxes are needed only where a coarser neighbour sums them
+  std::vector<int> sides;
   {
-    std::vector<int> sides;
-    const int        nd = dim == 3 ? 27 : 9, ns = 1 << dim;
+    const int nd = dim == 3 ? 27 : 9, ns = 1 << dim;
     for (long o = 0; o < nq; ++o)
       for (int d = 0; d < dim; ++d)
         for (int side = 0; side < 2; ++side)
@@ -134,18 +113,11 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
           int dl[3] = { 0, 0, 0 };
           dl[d] = side ? 1 : -1;
           const int di = (dl[0] + 1) + 3 * (dl[1] + 1) + (dim == 3 ? 9 * (dl[2] + 1) : 0);
-          if (c.mesh.nb[(o * nd + di) * ns].rel < 0)
+          if (mesh.nb[(o * nd + di) * ns].rel < 0)
             sides.push_back(int(o * 2 * dim + 2 * d + side));
         }
-    sl.nbside = int(sides.size());
-    int * d_sides = nullptr;
-    if (alloc(sides.size() * sizeof(int), reinterpret_cast<void **>(&d_sides)))
-      return -1;
-    if (!sides.empty() &&
-        hipMemcpy(d_sides, sides.data(), sides.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
-      return -1;
-    sl.bside = d_sides;
   }
+  sl.nbside = int(sides.size());
   for (int k = 0; k < kStages; ++k)
   {
     if (k == kStageBorder)
@@ -167,19 +139,54 @@ build(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl)
   bt.grav = sl.gd.grav;
   bt.nslots = 1;
   for (int k = 0; k < kStages; ++k)
-  {
     bt.tot[k] = fused_pad_slot(sl.cnt[k]);
-    const long off[2] = { 0, bt.tot[k] };
-    if (alloc(sizeof(off), reinterpret_cast<void **>(&bt.d_off[k])))
+  // The small per-mesh buffers share one device block, reused while it fits: the staged
+  // corrected border EMFs, the coarse-facing sides, the work-list offsets (one copy from a host
+  // staging area), then the slot and the clock (zeroed).
+  auto         al = [](size_t n) { return (n + 255) / 256 * 256; };
+  const size_t o_fix = 0;
+  const size_t o_sides = o_fix + al(size_t(c.dm.nfix > 0 ? c.dm.nfix : 1) * sizeof(T));
+  const size_t o_off = o_sides + al((sides.empty() ? 1 : sides.size()) * sizeof(int));
+  const size_t o_slot = o_off + al(size_t(kStages) * 2 * sizeof(long));
+  const size_t o_clock = o_slot + al(sizeof(FusedSlot<T>));
+  const size_t total = o_clock + al(2 * sizeof(double));
+  if (F.small_bytes < total)
+  {
+    if (F.small)
+      (void)hipFree(F.small);
+    F.small = nullptr;
+    F.small_bytes = 0;
+    if (hipMalloc(&F.small, total + total / 2) != hipSuccess)
+    {
+      F.small = nullptr;
+      c.err = "hipMalloc failed for the fused work list";
       return -1;
-    if (hipMemcpy(bt.d_off[k], off, sizeof(off), hipMemcpyHostToDevice) != hipSuccess)
-      return -1;
+    }
+    F.small_bytes = total + total / 2;
   }
-  if (alloc(sizeof(FusedSlot<T>), reinterpret_cast<void **>(&bt.d_slots[0])))
+  F.bytes += F.small_bytes;
+  char *            base = static_cast<char *>(F.small);
+  std::vector<char> host(o_slot, 0);
+  if (!sides.empty())
+    std::memcpy(host.data() + o_sides, sides.data(), sides.size() * sizeof(int));
+  for (int k = 0; k < kStages; ++k)
+  {
+    const long off[2] = { 0, bt.tot[k] };
+    std::memcpy(host.data() + o_off + size_t(k) * 2 * sizeof(long), off, sizeof(off));
+  }
+  if (hipMemcpy(base, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(base + o_slot, 0, total - o_slot) != hipSuccess)
+  {
+    c.err = "upload of the fused work list failed";
     return -1;
+  }
+  sl.fixval = reinterpret_cast<T *>(base + o_fix);
+  sl.bside = reinterpret_cast<const int *>(base + o_sides);
+  for (int k = 0; k < kStages; ++k)
+    bt.d_off[k] = reinterpret_cast<long *>(base + o_off) + 2 * k;
+  bt.d_slots[0] = reinterpret_cast<FusedSlot<T> *>(base + o_slot);
   bt.d_slots[1] = bt.d_slots[0];
-  if (alloc(2 * sizeof(double), reinterpret_cast<void **>(&bt.clock)))
-    return -1;
+  bt.clock = reinterpret_cast<double *>(base + o_clock);
   return 0;
 }
 
@@ -209,7 +216,7 @@ step(Context & c, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, doubl
 } // namespace
 
 int
-fused_alloc(Context & c)
+fused_alloc(Context & c, const Mesh & mesh)
 {
   auto * F = static_cast<FusedState *>(c.fused);
   const bool fp32 = c.impl == 12;
@@ -224,9 +231,8 @@ fused_alloc(Context & c)
     F->fp32 = fp32;
     c.fused = F;
   }
-  else
-    free_mesh_buffers(*F);
-  const int rc = F->fp32 ? build<float>(c, *F, F->bf, F->hf) : build<double>(c, *F, F->bd, F->hd);
+  F->bytes = 0;
+  const int rc = F->fp32 ? build<float>(c, mesh, *F, F->bf, F->hf) : build<double>(c, mesh, *F, F->bd, F->hd);
   c.ps.bytes = F->bytes;
   return rc;
 }
@@ -237,9 +243,10 @@ fused_free(Context & c)
   auto * F = static_cast<FusedState *>(c.fused);
   if (!F)
     return;
-  free_mesh_buffers(*F);
   if (F->arena)
     (void)hipFree(F->arena);
+  if (F->small)
+    (void)hipFree(F->small);
   delete F;
   c.fused = nullptr;
 }

@@ -9,8 +9,9 @@
 namespace mhdct
 {
 
-// allocate the context's slot (scratch, work list) for its current mesh; 0 or <0
-int  fused_alloc(Context & c);
+// allocate the context's slot (scratch, work list) for the mesh just uploaded (its host
+// neighbour table gives the coarse-facing sides); 0 or <0
+int  fused_alloc(Context & c, const Mesh & mesh);
 void fused_free(Context & c);
 // one step with the caller's dt: U2, B2 <- step(U, B)
 int  fused_step(Context & c, const double * U, double * U2, const double * B, double * B2, double dt,

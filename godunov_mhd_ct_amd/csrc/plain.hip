@@ -1190,8 +1190,8 @@ plain_alloc(Context & c)
 {
   plain_free(c);
   const Shapes S{ c.dim, c.b };
-  const long   nq = c.mesh.leaves.size();
-  const long   ntot = c.mesh.ntotal();
+  const long   nq = c.dm.nowned;
+  const long   ntot = c.dm.ntotal;
   struct A
   {
     double ** p;
