@@ -410,6 +410,12 @@ Mesh::build_nb()
       const int dl[3] = { di % 3 - 1, (di / 3) % 3 - 1, dim == 3 ? di / 9 - 1 : 0 };
       if (!dl[0] && !dl[1] && !dl[2])
         continue;
+      // A neighbour of this leaf's level or coarser covers the whole face, edge or corner, and
+      // its offset is the difference of the block origins (org - org_nb, or org - 2 org_nb),
+      // whatever the sub-position: sub-position 0 answers for all.  Only a finer neighbour
+      // differs per sub-position.
+      bool    share = false;
+      NbEntry first{ -1, 0, 0, { 0, 0, 0 } };
       for (int s = 0; s < ns; ++s)
       {
         // representative cell of this (direction, sub-position)
@@ -428,6 +434,11 @@ Mesh::build_nb()
         }
         if (!valid_sub)
           continue;
+        if (share)
+        {
+          nb[(o * nd + di) * ns + s] = first;
+          continue;
+        }
         const long g[3] = { org[0] + i[0], org[1] + i[1], org[2] + i[2] };
         int        oct, rel;
         long       loc[3];
@@ -461,6 +472,11 @@ Mesh::build_nb()
           e.off[d] = int16_t(off);
         }
         nb[(o * nd + di) * ns + s] = e;
+        if (s == 0 && rel <= 0)
+        {
+          first = e;
+          share = true;
+        }
       }
     }
     for (int d = 0; d < dim; ++d)
