@@ -112,6 +112,9 @@ void         mhdct_mesh_destroy(mhdct_mesh * mesh);
  * plain form or with mu < 0); -6: unequal block sizes. */
 int      mhdct_create(const mhdct_settings * settings, int dim, const int block_size[3], int impl_version,
                       mhdct_ctx ** out);
+/* Takes a finalized mesh: uploads its tables and sizes the scratch.  The context keeps no
+ * reference to `mesh` afterwards.  Synchronises the device first (work in flight may still
+ * read the previous mesh's tables, which are reused when the new mesh fits them). */
 int      mhdct_set_mesh(mhdct_ctx * ctx, const mhdct_mesh * mesh);
 uint64_t mhdct_mem_bytes(const mhdct_ctx * ctx);
 /* U2 == U and B2 == B on entry (the solver's contract); on exit U2, B2 hold t + dt for the
