@@ -159,8 +159,9 @@ def run_simulation(ini, impl: str | None = None, log=print, max_steps: int | Non
     st = cfg.settings()
     amr = cfg.amr()
     iv = impl_of(cfg, impl)
-    if st.viscosity_enabled and iv != IMPL_PLAIN:
-        raise MhdctError("[viscosity] enabled: viscous fluxes run in the plain form (implementation_version 0)")
+    if st.viscosity_enabled and iv == IMPL_FUSED_FP32:
+        raise MhdctError("[viscosity] enabled: viscous fluxes run in the FP64 forms (implementation_version 0, 10 "
+                         "or 11)")
     t_end = cfg.get_real("run", "tEnd", 0.0)
     nstep_max = cfg.get_integer("run", "nStepmax", 0)
     n_output = cfg.get_integer("run", "nOutput", 0)

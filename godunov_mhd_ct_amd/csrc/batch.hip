@@ -589,6 +589,12 @@ extern "C"
       b->err = "a batch holds contexts of one time integrator (HANCOCK or RK2_SSP)";
       return -1;
     }
+    if (ctx->c.st.viscosity_enabled)
+    {
+      // the viscous stage takes dt from the host, a batch keeps dt on the device
+      b->err = "a batch does not run viscosity: step the context with mhdct_do_time_step";
+      return -1;
+    }
     if ((fp32(b) ? static_cast<const void *>(mhdct::fused_slot<float>(ctx->c))
                  : static_cast<const void *>(mhdct::fused_slot<double>(ctx->c))) == nullptr)
     {

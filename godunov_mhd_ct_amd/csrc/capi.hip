@@ -267,7 +267,7 @@ extern "C"
     if (s->time_integrator != MHDCT_TIME_HANCOCK && s->time_integrator != MHDCT_TIME_RK2_SSP)
       return -5;
     // viscous fluxes are built in the plain form only
-    if (s->viscosity_enabled && (impl_version != MHDCT_IMPL_PLAIN || !(s->viscosity_mu >= 0.0)))
+    if (s->viscosity_enabled && (impl_version == MHDCT_IMPL_FUSED_FP32 || !(s->viscosity_mu >= 0.0)))
       return -5;
     for (int d = 1; d < dim; ++d)
       if (block_size[d] != block_size[0])

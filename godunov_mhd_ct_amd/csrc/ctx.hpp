@@ -135,6 +135,11 @@ void plain_free(Context & c);
 int  plain_step(Context & c, const double * U, double * U2, const double * B, double * B2, double dt,
                 hipStream_t s);
 int  launch_inv_dt(Context & c, const double * U, const double * B, hipStream_t s);
+// the viscous fluxes and their conservative update, direction by direction (GodunovImplemV0.cpp:
+// 179-192), from primitive records [block][qvars][(b+4)^d] whose velocities are variables 2-4;
+// Fl holds 8 x flux cells per owned block
+void launch_viscous(Context & c, const double * Qp, int qvars, double * Fl, double * U2, double * B2, double dt,
+                    hipStream_t s);
 int  launch_fill_outside(Context & c, double * U, double * B, hipStream_t s);
 // the border-edge gather list of the uploaded mesh, built on the device (border.hip)
 int  build_border_list(Context & c);

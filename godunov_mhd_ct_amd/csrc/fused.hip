@@ -589,6 +589,22 @@ cell_item(const FusedSlot<T> & S, long r0)
         if (DIM == 3)
           w += S.gt.g[2] * T(0.5) * dtT;
       }
+      // viscous predictor, after the gravity kick (ComputeUpdatedPrimitiveVariablesFunctor.cpp:
+      // 178-272, called at :380-383, 514-517): per velocity component, per axis,
+      // 1/2 nu (second difference) / dx * dt / dx with nu = mu / rho (FP64 form only: the
+      // FP32-bulk kernels carry none of it)
+      if (sizeof(T) == sizeof(double) && S.visc_mu >= 0.0)
+      {
+        const T nu = T(S.visc_mu) / r;
+        T *     vel[3] = { &u, &v, &w };
+        for (int c = 0; c < DIM; ++c)
+          for (int a = 0; a < DIM; ++a)
+          {
+            const int e0 = a == 0, e1 = a == 1, e2 = a == 2;
+            const T   d2 = Q(2 + c, i + e0, j + e1, k + e2) + Q(2 + c, i - e0, j - e1, k - e2) - T(2) * Q(2 + c, i, j, k);
+            *vel[c] += T(0.5) * nu * d2 / T(dxo) * dtdx;
+          }
+      }
       q2o[0] = r + T(0.5) * sr0;
       q2o[2] = u + T(0.5) * su0;
       q2o[3] = v + T(0.5) * sv0;

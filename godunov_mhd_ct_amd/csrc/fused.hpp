@@ -39,6 +39,7 @@ struct FusedSlot
   double               scaling;
   int                  abort_eint;
   int                  rk2; // RK2_SSP stage: no half-step predictor, no face-B source at the edges
+  double               visc_mu; // >= 0: the HANCOCK predictor's viscous term with this mu; < 0: none
   const double *       U;
   double *             U2;
   const double *       B;

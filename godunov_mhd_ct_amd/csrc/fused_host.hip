@@ -32,6 +32,8 @@ struct FusedState
   size_t              arena_bytes = 0;
   void *              small = nullptr; // the small per-mesh buffers in one block, grow-only too
   size_t              small_bytes = 0;
+  double *            visc_fl = nullptr; // viscous fluxes (FP64 form with viscosity), grow-only
+  size_t              visc_bytes = 0;
 };
 
 
@@ -91,7 +93,34 @@ build(Context & c, const Mesh & mesh, FusedState & F, FusedBatch<T> & bt, FusedS
   sl.scaling = c.st.scaling;
   sl.abort_eint = c.st.abort_when_negative_eint;
   sl.rk2 = c.st.time_integrator == 1;
+  sl.visc_mu = (c.st.viscosity_enabled && c.st.viscosity_predictor && !sl.rk2) ? c.st.mu : -1.0;
   sl.red = c.red;
+  // viscous fluxes, 8 variables per flux cell of one direction, as the plain form lays them out
+  if (c.st.viscosity_enabled)
+  {
+    if (sizeof(T) != sizeof(double))
+    {
+      c.err = "viscosity runs in the FP64 forms only";
+      return -5;
+    }
+    const long   fc = dim == 3 ? long(b + 1) * b * b : long(b + 1) * b;
+    const size_t vneed = size_t(nq > 0 ? nq : 1) * 8 * size_t(fc) * sizeof(double);
+    if (F.visc_bytes < vneed)
+    {
+      if (F.visc_fl)
+        (void)hipFree(F.visc_fl);
+      F.visc_fl = nullptr;
+      F.visc_bytes = 0;
+      if (hipMalloc(&F.visc_fl, vneed + vneed / 2) != hipSuccess)
+      {
+        F.visc_fl = nullptr;
+        c.err = "hipMalloc failed for the viscous fluxes";
+        return -1;
+      }
+      F.visc_bytes = vneed + vneed / 2;
+    }
+    F.bytes += F.visc_bytes;
+  }
   T * p = arena;
   sl.Qp = p;  p += nq * z.qp;
   sl.Bg = p;  p += nq * z.bg;
@@ -192,8 +221,8 @@ build(Context & c, const Mesh & mesh, FusedState & F, FusedBatch<T> & bt, FusedS
 
 template <typename T>
 int
-step(Context & c, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, double * U2, const double * B,
-     double * B2, double dt, hipStream_t s)
+step(Context & c, FusedState & F, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, double * U2,
+     const double * B, double * B2, double dt, hipStream_t s)
 {
   sl.U = U;
   sl.U2 = U2;
@@ -205,7 +234,25 @@ step(Context & c, FusedBatch<T> & bt, FusedSlot<T> & sl, const double * U, doubl
     c.err = "slot upload failed";
     return -1;
   }
-  if (fused_launch<T>(bt, 0, s))
+  int rc = 0;
+  if (!c.st.viscosity_enabled)
+    rc = fused_launch<T>(bt, 0, s);
+  else
+  {
+    // the stages one by one, with the viscous fluxes and their update after the hydro update
+    // (k_flux_tile) and before the face-B update and the gravity source, in the order of the
+    // plain form (GodunovImplemV0.cpp:161-192)
+    for (int k = 0; k < 8 && rc == 0; ++k)
+    {
+      rc = fused_launch_stage<T>(bt, 0, fused_stage_order[k], s);
+      if (rc == 0 && fused_stage_order[k] == kStageTile)
+      {
+        launch_viscous(c, reinterpret_cast<const double *>(sl.Qp), 5, F.visc_fl, U2, B2, dt, s);
+        rc = hipGetLastError() == hipSuccess ? 0 : -1;
+      }
+    }
+  }
+  if (rc)
   {
     c.err = std::string("fused launch failed: ") + hipGetErrorString(hipGetLastError());
     return -1;
@@ -247,6 +294,8 @@ fused_free(Context & c)
     (void)hipFree(F->arena);
   if (F->small)
     (void)hipFree(F->small);
+  if (F->visc_fl)
+    (void)hipFree(F->visc_fl);
   delete F;
   c.fused = nullptr;
 }
@@ -262,7 +311,8 @@ fused_step(Context & c, const double * U, double * U2, const double * B, double 
   }
   if (c.dm.nowned == 0)
     return 0;
-  return F->fp32 ? step<float>(c, F->bf, F->hf, U, U2, B, B2, dt, s) : step<double>(c, F->bd, F->hd, U, U2, B, B2, dt, s);
+  return F->fp32 ? step<float>(c, *F, F->bf, F->hf, U, U2, B, B2, dt, s)
+                 : step<double>(c, *F, F->bd, F->hd, U, U2, B, B2, dt, s);
 }
 
 template <>

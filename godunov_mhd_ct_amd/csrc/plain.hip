@@ -180,9 +180,10 @@ struct QView
   const double * p;
   Idx            I;
   long           nc;
+  int            nv = 11; // variables per block: 11 here, 5 in the fused form's primitive records
   __device__ double operator()(int i, int j, int k, int v, int o) const
   {
-    return p[(long(o) * 11 + v) * nc + I.gcell(2, i, j, k)];
+    return p[(long(o) * nv + v) * nc + I.gcell(2, i, j, k)];
   }
 };
 // ghost-1 arrays with nv variables
@@ -518,7 +519,7 @@ k_fluxes(DevMesh m, Gas<double> g, double scaling, double dt, int dir, const dou
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
 k_viscous_fluxes(DevMesh m, double scaling, double dt, double mu, int dir, const double * __restrict__ Qp,
-                 double * __restrict__ Fl)
+                 int qvars, double * __restrict__ Fl)
 {
   const Idx  I{ m.dim, m.b };
   const int  b = m.b, dim = m.dim;
@@ -531,7 +532,7 @@ k_viscous_fluxes(DevMesh m, double scaling, double dt, double mu, int dir, const
   const int  o = int(t / nf);
   const long ci = t - long(o) * nf;
   const int  i = int(ci % sh[0]), j = int((ci / sh[0]) % sh[1]), k = dim == 3 ? int(ci / (long(sh[0]) * sh[1])) : 0;
-  const QView  q{ Qp, I, I.gcells(2) };
+  const QView  q{ Qp, I, I.gcells(2), qvars };
   const double dx = dx_of(m, scaling, o);
   // g[c][a] = d v_c / d x_a
   double g[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };
@@ -1234,6 +1235,23 @@ plain_free(Context & c)
   c.ps.bytes = 0;
 }
 
+void
+launch_viscous(Context & c, const double * Qp, int qvars, double * Fl, double * U2, double * B2, double dt,
+               hipStream_t s)
+{
+  const DevMesh & m = c.dm;
+  const Shapes    S{ c.dim, c.b };
+  const long      nq = m.nowned;
+  if (nq == 0)
+    return;
+  for (int dir = 0; dir < c.dim; ++dir)
+  {
+    hipLaunchKernelGGL(k_viscous_fluxes, grid_of(nq * S.flux_cells()), kThreads, 0, s, m, c.st.scaling, dt, c.st.mu,
+                       dir, Qp, qvars, Fl);
+    hipLaunchKernelGGL(k_update, grid_of(nq * S.cells()), kThreads, 0, s, m, dir, Fl, U2, B2);
+  }
+}
+
 int
 plain_step(Context & c, const double * U, double * U2, const double * B, double * B2, double dt, hipStream_t s)
 {
@@ -1262,12 +1280,7 @@ plain_step(Context & c, const double * U, double * U2, const double * B, double 
     hipLaunchKernelGGL(k_update, grid_of(nq * S.cells()), kThreads, 0, s, m, dir, c.ps.Fl, U2, B2);
   }
   if (c.st.viscosity_enabled) // GodunovImplemV0.cpp:179-192
-    for (int dir = 0; dir < c.dim; ++dir)
-    {
-      hipLaunchKernelGGL(k_viscous_fluxes, grid_of(nq * S.flux_cells()), kThreads, 0, s, m, sc, dt, c.st.mu, dir,
-                         c.ps.Q, c.ps.Fl);
-      hipLaunchKernelGGL(k_update, grid_of(nq * S.cells()), kThreads, 0, s, m, dir, c.ps.Fl, U2, B2);
-    }
+    launch_viscous(c, c.ps.Q, 11, c.ps.Fl, U2, B2, dt, s);
   hipLaunchKernelGGL(k_emf, grid_of(nq * S.edge_cells()), kThreads, 0, s, m, g, sc, dt, c.ps.Q, c.ps.Q2,
                      c.ps.S[0], c.ps.S[1], c.ps.S[2], c.ps.sF, rk2 ? 0 : 1, c.ps.emf);
   MHDCT_HIP(hipMemcpyAsync(c.ps.emf2, c.ps.emf, size_t(nq) * S.ncomp_e() * S.edge_cells() * sizeof(double),

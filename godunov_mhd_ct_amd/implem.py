@@ -34,7 +34,7 @@ def hydro_settings(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-
                    time_integrator="HANCOCK", gravity=None, viscosity=None, viscous_predictor=False) -> Settings:
     """Defaults of HydroSettings (models/HydroSettings.cpp:19-30), plus [run] time_integrator,
     [gravity] (gravity=(gx, gy, gz) enables a uniform field; None disables it) and [viscosity]
-    (viscosity=mu enables viscous fluxes, plain form only; viscous_predictor adds the viscous
+    (viscosity=mu enables viscous fluxes, FP64 forms only; viscous_predictor adds the viscous
     term of the half-step predictor)."""
     s = Settings()
     s.gamma0 = float(gamma0)
@@ -96,7 +96,7 @@ class GodunovImplemHIP:
         rc = lib().mhdct_create(C.byref(settings), self.dim, bs, self.impl_version, C.byref(h))
         if rc == -5:
             raise MhdctError("mhdct_create failed (-5): settings not run by this build (HLLD and SIMPLE_COPY only; "
-                             "HANCOCK or RK2_SSP; viscosity in the plain form only, mu >= 0)")
+                             "HANCOCK or RK2_SSP; viscosity in the FP64 forms only, mu >= 0)")
         if rc != 0:
             raise MhdctError(f"mhdct_create failed ({rc})")
         self._h = h

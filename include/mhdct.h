@@ -59,7 +59,8 @@ typedef struct mhdct_settings
   int    gravity_enabled; /* [gravity] enabled: a uniform gravity field (AddGravitySourceTerm.cpp) */
   double gravity[3];      /* [gravity] uniform_gx, uniform_gy, uniform_gz */
   int    viscosity_enabled; /* [viscosity] enabled: viscous fluxes after the hydro ones
-                               (GodunovImplemV0.cpp:179-192); plain form only */
+                               (GodunovImplemV0.cpp:179-192); the FP64 forms (plain, fused),
+                               one context per step (not in a batch) */
   double viscosity_mu;      /* [viscosity] mu, the dynamic viscosity (>= 0) */
   int    viscosity_hancock_predictor; /* [viscosity] hancock_predictor_enabled: the viscous term of
                                          the half-step predictor (ComputeUpdatedPrimitiveVariablesFunctor.cpp:380-383) */
@@ -108,8 +109,8 @@ void         mhdct_mesh_destroy(mhdct_mesh * mesh);
 /* ---- GodunovImplemBase ---------------------------------------------------------------- */
 
 /* -4: unknown impl_version; -5: settings this build does not run (a Riemann solver other than
- * HLLD, a prolongation other than SIMPLE_COPY, an unknown time integrator, viscosity outside the
- * plain form or with mu < 0); -6: unequal block sizes. */
+ * HLLD, a prolongation other than SIMPLE_COPY, an unknown time integrator, viscosity in the
+ * FP32-bulk form or with mu < 0); -6: unequal block sizes. */
 int      mhdct_create(const mhdct_settings * settings, int dim, const int block_size[3], int impl_version,
                       mhdct_ctx ** out);
 /* Takes a finalized mesh: uploads its tables and sizes the scratch.  The context keeps no
