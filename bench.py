@@ -55,8 +55,8 @@ def parse():
     ap.add_argument("--gravity", type=float, default=None,
                     help="add a uniform gravity field of this magnitude along -z (-y in 2D) to the workload")
     ap.add_argument("--viscosity", type=float, default=None,
-                    help="plain form with viscous fluxes of this mu (and the viscous predictor term), next to "
-                         "the same steps without viscosity")
+                    help="the FP64 forms (plain and fused) with viscous fluxes of this mu (and the viscous "
+                         "predictor term), next to the same steps without viscosity")
     ap.add_argument("--checkpoint", action="store_true",
                     help="time writing a checkpoint and the XDMF output of the workload's first mesh after "
                          "--steps steps, and a restart from that checkpoint")
@@ -346,28 +346,35 @@ def bench_rk2_batch(args, cases, impl):
 
 
 def bench_viscosity(args, cases):
-    """The plain form (the only one with viscous fluxes) with [viscosity] enabled, mu, and the
-    predictor term, against the same steps without viscosity."""
+    """The FP64 forms (plain and fused, the per-mesh solver) with [viscosity] enabled, mu, and the
+    predictor term, against the same steps without viscosity.  The line's value is the fused
+    form's."""
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_PLAIN
+
     cells = sum(m.owned_cells() for (m, _, _, _) in cases)
-    log(f"[bench] viscosity mu={args.viscosity} workload={args.workload} plain form, cells={cells}")
-    base = run_plain(cases, args.steps, args.warmup)
+    log(f"[bench] viscosity mu={args.viscosity} workload={args.workload} plain and fused forms, cells={cells}")
+    base = {f: run_plain(cases, args.steps, args.warmup, impl) for f, impl in (("plain", IMPL_PLAIN),
+                                                                               ("fused", IMPL_FUSED))}
     for (_, _, _, st) in cases:
         st.viscosity_enabled = 1
         st.viscosity_mu = args.viscosity
         st.viscosity_hancock_predictor = 1
-    r = run_plain(cases, args.steps, args.warmup)
-    out = {"metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": args.workload, "impl": "plain", "viscosity_mu": args.viscosity,
+    r = {f: run_plain(cases, args.steps, args.warmup, impl) for f, impl in (("plain", IMPL_PLAIN),
+                                                                            ("fused", IMPL_FUSED))}
+    v = r["fused"]["value"]
+    out = {"metric": METRIC, "value": v, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": r["fused"]["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": args.workload, "impl": "fused", "viscosity_mu": args.viscosity,
                       "viscous_predictor": True, "meshes": len(cases), "cells": cells,
                       "driver": "per-mesh solver, host dt each step"},
-           "roofline": {"bound": "hbm", "achieved": r["value"] * ALGO_BYTES_PER_CELL / 1e9, "peak": HBM_PEAK / 1e9,
-                        "unit": "GB/s", "frac": r["value"] * ALGO_BYTES_PER_CELL / HBM_PEAK, "traffic": None,
-                        "kernel": "whole step (plain form)",
+           "roofline": {"bound": "hbm", "achieved": v * ALGO_BYTES_PER_CELL / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": v * ALGO_BYTES_PER_CELL / HBM_PEAK, "traffic": None,
+                        "kernel": "whole step (fused form)",
                         "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B per cell-update"},
-           "without_viscosity": {"value": base["value"], "ms_per_step": base["ms_per_step"]},
-           "viscous_cost_ms": r["ms_per_step"] - base["ms_per_step"]}
+           "forms": {f: {"value": r[f]["value"], "ms_per_step": r[f]["ms_per_step"],
+                         "without_viscosity": base[f], "viscous_cost_ms": r[f]["ms_per_step"] - base[f]["ms_per_step"]}
+                     for f in r}}
     print(json.dumps(out))
 
 
