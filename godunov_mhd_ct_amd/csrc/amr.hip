@@ -71,14 +71,19 @@ u_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__ U, int v
 //   N_kk = (u+ - u0) - (u0 - u-),  D_kk = |u+ - u0| + |u0 - u-| + eps (|u+| + 2|u0| + |u-|)
 //   N_kl = ((u++ - u+-) - (u-+ - u--)) / 4
 //   D_kl = (|u++ - u+-| + |u-+ - u--|) / 4 + eps (|u++| + |u+-| + |u-+| + |u--|) / 4   (k != l)
-// summed over k, l in dimension order (k outer), E = 0 where the denominator vanishes.
+// summed over k, l in dimension order (k outer), E = 0 where the denominator vanishes.  The
+// dimension is a template parameter: every stencil index is then a constant and s stays in
+// registers.
+template <int dim>
 __device__ inline double
-lohner(const double * s, int dim, double eps)
+lohner(const double * s, double eps)
 {
   auto at = [&](int dx, int dy, int dz) { return s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)]; };
   auto off = [](int d, int sg, int * o3) { o3[d] += sg; };
   double num = 0, den = 0;
+#pragma unroll
   for (int k = 0; k < dim; ++k)
+#pragma unroll
     for (int l = 0; l < dim; ++l)
     {
       if (k == l)
@@ -112,10 +117,11 @@ lohner(const double * s, int dim, double eps)
 }
 
 // One thread per owned cell; the block maximum through the bits of the non-negative double.
+template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_refine_indicator(DevMesh m, int ivar, double eps, const double * __restrict__ U, unsigned long long * emax)
 {
-  const Idx  I{ m.dim, m.b };
+  const Idx  I{ DIM, m.b }; // m.dim == DIM: the launch picks the instantiation
   const long ncell = I.cells();
   const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool live = t < long(m.nowned) * ncell;
@@ -126,16 +132,19 @@ k_refine_indicator(DevMesh m, int ivar, double eps, const double * __restrict__ 
     o = int(t / ncell);
     const long ci = t - long(o) * ncell;
     const int  b = m.b;
-    const int  i = int(ci % b), j = int((ci / b) % b), k = m.dim == 3 ? int(ci / (long(b) * b)) : 0;
+    const int  i = int(ci % b), j = int((ci / b) % b), k = DIM == 3 ? int(ci / (long(b) * b)) : 0;
     double     s[27];
-    for (int dz = (m.dim == 3 ? -1 : 0); dz <= (m.dim == 3 ? 1 : 0); ++dz)
+#pragma unroll
+    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
       for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
         for (int dx = -1; dx <= 1; ++dx)
         {
           const int c[3] = { i + dx, j + dy, k + dz };
           s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] = u_ghosted(m, I, U, ivar, o, c);
         }
-    e = lohner(s, m.dim, eps);
+    e = lohner<DIM>(s, eps);
     if (!(e >= 0.0)) // NaN state: flag it as a maximal indicator
       e = __builtin_huge_val();
   }
@@ -487,8 +496,12 @@ launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t *
   auto *   fl = reinterpret_cast<int8_t *>(ind + n);
   AMR_HIP(c.err, hipMemsetAsync(emax, 0, size_t(n) * sizeof(unsigned long long), s));
   const Shapes S{ c.dim, c.b };
-  hipLaunchKernelGGL(k_refine_indicator, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner, U,
-                     emax);
+  if (c.dim == 3)
+    hipLaunchKernelGGL(k_refine_indicator<3>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
+                       U, emax);
+  else
+    hipLaunchKernelGGL(k_refine_indicator<2>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
+                       U, emax);
   hipLaunchKernelGGL(k_refine_flags, grid_of(n), kThreads, 0, s, m, p, emax, fl, ind);
   AMR_HIP(c.err, hipGetLastError());
   AMR_HIP(c.err, hipMemcpyAsync(flags_host, fl, size_t(n), hipMemcpyDeviceToHost, s));
