@@ -467,9 +467,11 @@ k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __res
 // ---------------------------------------------------------------------------------------------
 // k_cell: per cell of the 1-ghost tile: slopes, predictor, face source, face-B slopes.  HK:
 // HANCOCK (half-step predictor), else RK2_SSP (Q2 = the cell-centred state): a kernel template
-// parameter, so that the HANCOCK kernel carries no trace of the other integrator.
+// parameter, so that the HANCOCK kernel carries no trace of the other integrator.  VISC: the
+// predictor's viscous term is compiled in (FP64 kernels of one block size: viscosity runs per
+// context, never in a batch of mixed block sizes, and the FP32 form refuses it).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM, int BS, bool HK>
+template <typename T, int DIM, int BS, bool HK, bool VISC>
 __device__ __forceinline__ void
 cell_item(const FusedSlot<T> & S, long r0)
 {
@@ -591,9 +593,8 @@ cell_item(const FusedSlot<T> & S, long r0)
       }
       // viscous predictor, after the gravity kick (ComputeUpdatedPrimitiveVariablesFunctor.cpp:
       // 178-272, called at :380-383, 514-517): per velocity component, per axis,
-      // 1/2 nu (second difference) / dx * dt / dx with nu = mu / rho (FP64 form only: the
-      // FP32-bulk kernels carry none of it)
-      if (sizeof(T) == sizeof(double) && S.visc_mu >= 0.0)
+      // 1/2 nu (second difference) / dx * dt / dx with nu = mu / rho
+      if (VISC && S.visc_mu >= 0.0)
       {
         const T nu = T(S.visc_mu) / r;
         T *     vel[3] = { &u, &v, &w };
@@ -678,7 +679,7 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[1])
       continue;
-    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS, HK>(S, r0)));
+    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS, HK, (BSEL != 0 && sizeof(T) == sizeof(double))>(S, r0)));
   }
 }
 
