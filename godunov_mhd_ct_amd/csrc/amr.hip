@@ -329,10 +329,14 @@ k_restrict_faces(int dim, int b, int nrest, const int * __restrict__ restricted,
 //    eigenvectors of eigenvalue 2 |s|, so phi = -sum_{s != 0} (w_s . div) w_s / (2^dim 2 |s|).
 // Each thread writes the low and interior planes of its parent cell, and the high planes on the
 // leaf's high boundary, so every face is written once.  2D B_z is cell centred: copied.
+template <int DIM>
 __global__ void __launch_bounds__(kThreads)
-k_prolong_faces(int dim, int b, int nprol, const int * __restrict__ prolonged, const RemapEntry * __restrict__ E,
+k_prolong_faces(int b, int nprol, const int * __restrict__ prolonged, const RemapEntry * __restrict__ E,
                 const double * __restrict__ Bo, double * __restrict__ Bn)
 {
+  // the dimension is a template parameter: the face arrays below are then indexed by constants
+  // and stay in registers
+  constexpr int dim = DIM;
   const Idx  I{ dim, b };
   const long fper = I.fper();
   const int  h = b / 2; // parent cells per leaf edge
@@ -654,8 +658,12 @@ launch_remap(int dim, int b, const RemapEntry * plan_dev, const RemapLists & L, 
   if (L.nprol > 0)
   {
     const long per = dim == 3 ? long(b / 2) * (b / 2) * (b / 2) : long(b / 2) * (b / 2);
-    hipLaunchKernelGGL(k_prolong_faces, grid_of(long(L.nprol) * per), kThreads, 0, s, dim, b, L.nprol,
-                       L.prolonged(lists_dev), plan_dev, Bo, Bn);
+    if (dim == 3)
+      hipLaunchKernelGGL(k_prolong_faces<3>, grid_of(long(L.nprol) * per), kThreads, 0, s, b, L.nprol,
+                         L.prolonged(lists_dev), plan_dev, Bo, Bn);
+    else
+      hipLaunchKernelGGL(k_prolong_faces<2>, grid_of(long(L.nprol) * per), kThreads, 0, s, b, L.nprol,
+                         L.prolonged(lists_dev), plan_dev, Bo, Bn);
   }
   AMR_HIP(err, hipGetLastError());
   return 0;
