@@ -159,3 +159,15 @@ def test_implementation_version():
     assert impl_of(Config.parse(text(*BASE)), "fused") == 11
     with pytest.raises(MhdctError):
         impl_of(Config.parse(text(*BASE, "implementation_version=2")))
+
+
+def test_viscosity_refused_for_the_fp32_form_before_any_device_work():
+    from godunov_mhd_ct_amd.app import IMPLS, impl_of, run_simulation
+
+    cfg = Config.parse(text(*BASE, "[viscosity]", "enabled=true", "mu=1e-4", "hancock_predictor_enabled=yes"))
+    st = cfg.settings()
+    assert st.viscosity_enabled == 1 and st.viscosity_mu == 1e-4 and st.viscosity_hancock_predictor == 1
+    # both FP64 forms take viscosity; the FP32-bulk form is refused by the driver itself
+    assert impl_of(cfg) == IMPL_PLAIN and impl_of(cfg, "fused") == IMPLS["fused"]
+    with pytest.raises(MhdctError, match="FP64 forms"):
+        run_simulation(cfg, impl="fp32", log=lambda *a: None, max_steps=1)
