@@ -51,7 +51,7 @@ class AmrParams(C.Structure):
     ]
 
 
-# exported symbols and their signatures (also checked by tests/test_capi_symbols.py)
+# exported symbols and their signatures (also checked by tests/test_capi_and_mesh.py)
 SIGNATURES = {
     "mhdct_mesh_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                     C.POINTER(C.c_void_p)]),

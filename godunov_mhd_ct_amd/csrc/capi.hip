@@ -266,7 +266,7 @@ extern "C"
       return -5;
     if (s->time_integrator != MHDCT_TIME_HANCOCK && s->time_integrator != MHDCT_TIME_RK2_SSP)
       return -5;
-    // viscous fluxes are built in the plain form only
+    // viscous fluxes run in the FP64 forms (plain and fused), not in the FP32-bulk form
     if (s->viscosity_enabled && (impl_version == MHDCT_IMPL_FUSED_FP32 || !(s->viscosity_mu >= 0.0)))
       return -5;
     for (int d = 1; d < dim; ++d)
@@ -333,7 +333,9 @@ extern "C"
       c.err = std::string("set_mesh: ") + hipGetErrorString(se);
       return -1;
     }
-    c.has_mesh = true;
+    // the context holds a mesh again only once every table and the scratch are in place: a
+    // failure below leaves it without one, so the step entry points refuse to run
+    c.has_mesh = false;
     c.mesh_bytes = 0;
     c.dm.dim = c.dim;
     c.dm.b = c.b;
@@ -366,6 +368,8 @@ extern "C"
       rc = mhdct::plain_alloc(c);
     else
       rc = mhdct::fused_alloc(c, M);
+    if (rc == 0)
+      c.has_mesh = true;
     return rc;
   }
 

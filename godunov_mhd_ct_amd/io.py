@@ -66,6 +66,12 @@ def settings_from_dict(d: dict) -> Settings:
     return st
 
 
+def _named(prefix: Path, suffix: str) -> Path:
+    """``<prefix><suffix>``: the suffix is appended, so a prefix with dots in its name (e.g.
+    ``kh.mu1e-4_iter0000010``) keeps every character of it."""
+    return prefix.parent / (prefix.name + suffix)
+
+
 def save_checkpoint(prefix, mesh: Mesh, U, B, time: float, iteration: int, settings=None, extra=None) -> Path:
     """Write ``<prefix>.json`` and ``<prefix>.bin`` (owned leaves only); returns the header path."""
     prefix = Path(prefix)
@@ -88,13 +94,13 @@ def save_checkpoint(prefix, mesh: Mesh, U, B, time: float, iteration: int, setti
             "bc": list(mesh.bc), "time": float(time), "iteration": int(iteration),
             "num_leaves": n, "cell_variables": CELL_VARS, "arrays": layout,
             "settings": settings_dict(settings) if settings is not None else None, "extra": extra or {}}
-    with open(prefix.with_suffix(".bin"), "wb") as f:
+    with open(_named(prefix, ".bin"), "wb") as f:
         pos = 0
         for (name, a), ent in zip(arrays, layout):
             f.write(b"\0" * (ent["offset"] - pos))
             f.write(a.tobytes())
             pos = ent["offset"] + a.nbytes
-    path = prefix.with_suffix(".json")
+    path = _named(prefix, ".json")
     path.write_text(json.dumps(head, indent=1))
     return path
 
@@ -103,10 +109,10 @@ def load_checkpoint(prefix):
     """(mesh, U, B, header) from a checkpoint; U and B in the ABI layout of the rebuilt mesh,
     outside blocks zero (the step's outside fill sets them)."""
     prefix = Path(prefix)
-    head = json.loads(prefix.with_suffix(".json").read_text())
+    head = json.loads(_named(prefix, ".json").read_text())
     if head.get("format") != FORMAT or head.get("version") != VERSION:
         raise MhdctError(f"{prefix}.json is not a version {VERSION} checkpoint of this package")
-    raw = np.fromfile(prefix.with_suffix(".bin"), dtype=np.uint8)
+    raw = np.fromfile(_named(prefix, ".bin"), dtype=np.uint8)
     arrs = {}
     for ent in head["arrays"]:
         dt = np.dtype(ent["dtype"])
@@ -207,7 +213,7 @@ def write_xdmf(prefix, mesh: Mesh, U, B, time: float, gamma: float, variables=No
             lines.append("    </Attribute>")
         lines.append("   </Grid>")
     lines += ["  </Grid>", " </Domain>", "</Xdmf>"]
-    path = prefix.with_suffix(".xmf")
+    path = _named(prefix, ".xmf")
     path.write_text("\n".join(lines) + "\n")
     return path
 

@@ -80,14 +80,21 @@ def kh3d(level=4, b=16, seed=0):
     return mesh, U, B, st
 
 
-def random_amr_mesh(b, target, seed, level_min=2, level_max=6) -> Mesh:
-    """Periodic 2:1-balanced octree of about `target` leaves of b^3 cells over levels
-    level_min..level_max: leaves are refined, highest score first, where a smooth random
-    indicator (seeded) is large, until the leaf count reaches the target."""
+def random_amr_mesh(b, target, seed, level_min=2, level_max=6, nbrick=(1, 1, 1)) -> Mesh:
+    """Periodic 2:1-balanced octree of exactly `target` leaves of b^3 cells over levels
+    level_min..level_max on an nbrick brick of trees: leaves are refined, highest score first,
+    where a smooth random indicator (seeded) is large.  Far from the target many leaves are
+    refined per pass; near it one leaf at a time, skipping any whose refinement (with the 2:1
+    balance it triggers) would pass the target.  Refining a leaf adds 7 leaves, so the target
+    must equal the uniform level_min leaf count plus a multiple of 7."""
+    ntrees = nbrick[0] * nbrick[1] * nbrick[2]
+    base = ntrees * 8 ** level_min
+    if target < base or (target - base) % 7:
+        raise ValueError(f"{target} leaves are not reachable from {base} by refinement (steps of 7)")
     rng = np.random.default_rng(seed)
     ks = rng.integers(-3, 4, size=(6, 3)).astype(np.float64)
     ph = rng.uniform(0, 2 * math.pi, size=6)
-    mesh = Mesh(3, b, level_min)
+    mesh = Mesh(3, b, level_min, nbrick=nbrick)
     while mesh.num_leaves < target:
         lv, co = mesh.leaves()
         n = mesh.num_leaves
@@ -95,24 +102,50 @@ def random_amr_mesh(b, target, seed, level_min=2, level_max=6) -> Mesh:
         score = sum(np.cos(2 * math.pi * (c @ k) + p) for k, p in zip(ks, ph))
         score = score + 1e-3 * rng.random(n)
         score[lv[:n] >= level_max] = -np.inf
-        if not np.isfinite(score).any():
-            break
-        nref = max(1, min(int((target - n) / 7 / 2), int(np.isfinite(score).sum())))
-        flags = np.zeros(n, dtype=np.int8)
-        flags[np.argsort(-score)[:nref]] = 1
-        mesh.adapt(flags)
-        mesh.balance()
+        order = [int(i) for i in np.argsort(-score) if np.isfinite(score[i])]
+        if not order:
+            raise ValueError(f"no leaf below level {level_max} left to refine at {n} leaves")
+        rem = (target - n) // 7
+        nref = rem // 4
+        if nref >= 2:
+            flags = np.zeros(n, dtype=np.int8)
+            flags[order[:nref]] = 1
+            trial = mesh.clone()
+            trial.adapt(flags)
+            trial.balance()
+            if trial.num_leaves <= target:
+                mesh = trial
+                continue
+        # close to the target: the best leaf whose refinement stays within it
+        for i in order:
+            flags = np.zeros(n, dtype=np.int8)
+            flags[i] = 1
+            trial = mesh.clone()
+            trial.adapt(flags)
+            trial.balance()
+            if trial.num_leaves <= target:
+                mesh = trial
+                break
+        else:
+            raise ValueError(f"no single refinement fits between {n} and {target} leaves")
     return mesh.finalize()
 
 
-def mix3d(seed=0, counts=((8, 2048), (16, 1434), (32, 614)), level_min=2, level_max=6):
+# The batching configuration: 4096 blocks, 2048 x 8^3 + 1434 x 16^3 + 614 x 32^3 = 27,041,792
+# cells, levels 2-6.  A single tree's leaf count stays 1 modulo 7 under refinement, so each
+# mesh lies on a periodic brick of trees whose count matches its target modulo 7 (4, 6 and 5
+# trees); the random fields are periodic on the unit cube, hence on every brick of whole units.
+MIX_MESHES = ((8, 2048, (2, 2, 1)), (16, 1434, (3, 2, 1)), (32, 614, (5, 1, 1)))
+
+
+def mix3d(seed=0, meshes=MIX_MESHES, level_min=2, level_max=6):
     """The batching workload: three independent periodic meshes of 8^3, 16^3 and 32^3 blocks
-    (about 2048, 1434 and 614 leaves, 27 M cells), levels 2-6, each with a smooth random MHD
+    (2048, 1434 and 614 leaves, 27,041,792 cells), levels 2-6, each with a smooth random MHD
     state (problems.random_mhd_3d), gamma 5/3, minmod, HLLD, CFL 0.8.  Returns a list of
     (mesh, U, B, settings)."""
     out = []
-    for n, (b, target) in enumerate(counts):
-        mesh = random_amr_mesh(b, target, seed + 101 * n, level_min, level_max)
+    for n, (b, target, nbrick) in enumerate(meshes):
+        mesh = random_amr_mesh(b, target, seed + 101 * n, level_min, level_max, nbrick)
         U, B = problems.random_mhd_3d(mesh, seed=seed + 7 * n)
         st = hydro_settings(gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0)
         out.append((mesh, U, B, st))

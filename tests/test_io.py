@@ -98,6 +98,30 @@ def test_checkpoint_round_trip(tmp_path, dim):
     assert settings_dict(st2) == settings_dict(st)
 
 
+def test_dotted_prefix_keeps_every_file_apart(tmp_path):
+    """A prefix with dots in its name ('kh.mu1e-4_iterNNN') names its files by appending: two
+    checkpoints of one run do not overwrite each other, and the XDMF data file sits beside its
+    index under the same name."""
+    m = refined(2, b=4)
+    U, B = random_state(m, seed=1)
+    U2, B2 = random_state(m, seed=2)
+    save_checkpoint(tmp_path / "kh.mu1e-4_iter0000010", m, U, B, 0.1, 10)
+    save_checkpoint(tmp_path / "kh.mu1e-4_iter0000020", m, U2, B2, 0.2, 20)
+    for it in (10, 20):
+        assert (tmp_path / f"kh.mu1e-4_iter00000{it}.json").exists()
+        assert (tmp_path / f"kh.mu1e-4_iter00000{it}.bin").exists()
+    assert not (tmp_path / "kh.json").exists() and not (tmp_path / "kh.bin").exists()
+    n = m.num_leaves
+    _, Ua, _, ha = load_checkpoint(tmp_path / "kh.mu1e-4_iter0000010")
+    _, Ub, _, hb = load_checkpoint(tmp_path / "kh.mu1e-4_iter0000020")
+    assert ha["iteration"] == 10 and hb["iteration"] == 20
+    assert np.array_equal(Ua[:n], U[:n]) and np.array_equal(Ub[:n], U2[:n])
+    path = write_xdmf(tmp_path / "kh.mu1e-4_iter0000020", m, U2, B2, 0.2, 1.4)
+    assert path == tmp_path / "kh.mu1e-4_iter0000020.xmf"
+    assert (tmp_path / "kh.mu1e-4_iter0000020_cells.bin").exists()
+    assert "kh.mu1e-4_iter0000020_cells.bin" in path.read_text()
+
+
 def test_checkpoint_with_leaves_in_another_order(tmp_path):
     m = refined(3)
     U, B = random_state(m, 4)
