@@ -5,6 +5,7 @@ Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this mo
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -12,7 +13,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE_DIR = ROOT / "oracle"
-ORACLE_LIB = ORACLE_DIR / "_build" / "libmhd_oracle.so"
+# MHDCT_ORACLE_LIB names another build of the oracle's sources (e.g. one with a slip planted on
+# purpose, to show that the pin tests fail on it); by default the one oracle/Makefile builds
+ORACLE_LIB = Path(os.environ.get("MHDCT_ORACLE_LIB", ORACLE_DIR / "_build" / "libmhd_oracle.so"))
 
 _lib = None
 
