@@ -94,6 +94,8 @@ SIGNATURES = {
                                        C.POINTER(C.c_int)]),
     "mhdct_refine_flags": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AmrParams), C.POINTER(C.c_int8),
                                      C.POINTER(C.c_double), C.c_void_p]),
+    "mhdct_refine_flags_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AmrParams), C.POINTER(C.c_int8),
+                                             C.POINTER(C.c_double), C.c_void_p]),
     "mhdct_mesh_clone": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mhdct_remap_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "mhdct_remap_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

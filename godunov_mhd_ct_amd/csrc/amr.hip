@@ -28,8 +28,12 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/mhdct.h"
 #include "amr.hpp"
 #include "common.hpp"
+#include "handles.hpp"
+
+#include <mutex>
 
 namespace mhdct
 {
@@ -180,6 +184,15 @@ k_refine_flags(DevMesh m, AmrParams p, const unsigned long long * __restrict__ e
     f = -1;
   flags[o] = f;
   ind[o] = e;
+}
+
+// words[0, n) = 0 (the per-block maxima start from zero)
+__global__ void __launch_bounds__(kThreads)
+k_clear_words(unsigned long long * __restrict__ words, long n)
+{
+  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n)
+    words[i] = 0ull;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -477,12 +490,40 @@ grid_of(long n)
     }                                                                                              \
   } while (0)
 
+// The indicator and flags of every owned block into the device buffers emax (n words), ind
+// (n doubles) and fl (n bytes), then read back into the host arrays; emax is cleared first, by
+// hipMemsetAsync (clear_by_kernel false) or by k_clear_words.
+int
+refine_flags_into(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
+                  hipStream_t s, unsigned long long * emax, double * ind, int8_t * fl, bool clear_by_kernel)
+{
+  const DevMesh & m = c.dm;
+  const long      n = m.nowned;
+  if (clear_by_kernel)
+    hipLaunchKernelGGL(k_clear_words, grid_of(n), kThreads, 0, s, emax, n);
+  else
+    AMR_HIP(c.err, hipMemsetAsync(emax, 0, size_t(n) * sizeof(unsigned long long), s));
+  const Shapes S{ c.dim, c.b };
+  if (c.dim == 3)
+    hipLaunchKernelGGL(k_refine_indicator<3>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
+                       U, emax);
+  else
+    hipLaunchKernelGGL(k_refine_indicator<2>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
+                       U, emax);
+  hipLaunchKernelGGL(k_refine_flags, grid_of(n), kThreads, 0, s, m, p, emax, fl, ind);
+  AMR_HIP(c.err, hipGetLastError());
+  AMR_HIP(c.err, hipMemcpyAsync(flags_host, fl, size_t(n), hipMemcpyDeviceToHost, s));
+  if (ind_host)
+    AMR_HIP(c.err, hipMemcpyAsync(ind_host, ind, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
+  AMR_HIP(c.err, hipStreamSynchronize(s));
+  return 0;
+}
+
 int
 launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
                     hipStream_t s)
 {
-  const DevMesh & m = c.dm;
-  const long      n = m.nowned;
+  const long n = c.dm.nowned;
   if (n == 0)
     return 0;
   const size_t need = size_t(n) * (2 * sizeof(unsigned long long) + 1) + 256;
@@ -498,21 +539,7 @@ launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t *
   auto *   emax = static_cast<unsigned long long *>(c.amr_buf);
   double * ind = reinterpret_cast<double *>(emax + n);
   auto *   fl = reinterpret_cast<int8_t *>(ind + n);
-  AMR_HIP(c.err, hipMemsetAsync(emax, 0, size_t(n) * sizeof(unsigned long long), s));
-  const Shapes S{ c.dim, c.b };
-  if (c.dim == 3)
-    hipLaunchKernelGGL(k_refine_indicator<3>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
-                       U, emax);
-  else
-    hipLaunchKernelGGL(k_refine_indicator<2>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
-                       U, emax);
-  hipLaunchKernelGGL(k_refine_flags, grid_of(n), kThreads, 0, s, m, p, emax, fl, ind);
-  AMR_HIP(c.err, hipGetLastError());
-  AMR_HIP(c.err, hipMemcpyAsync(flags_host, fl, size_t(n), hipMemcpyDeviceToHost, s));
-  if (ind_host)
-    AMR_HIP(c.err, hipMemcpyAsync(ind_host, ind, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
-  AMR_HIP(c.err, hipStreamSynchronize(s));
-  return 0;
+  return refine_flags_into(c, U, p, flags_host, ind_host, s, emax, ind, fl, false);
 }
 
 bool
@@ -669,4 +696,106 @@ launch_remap(int dim, int b, const RemapEntry * plan_dev, const RemapLists & L, 
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Refinement flags without allocation: the per-block maxima, indicators and flags live in a
+// file-scope device array ("scratch") sized for MHDCT_NOALLOC_MAX_BLOCKS owned blocks, one copy
+// per GPU for the life of the process.  Its device address is looked up once per device with
+// hipGetSymbolAddress (the array's name is only a host-side stand-in) and kept per device.
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+
+constexpr long kScratchWords = 2L * MHDCT_NOALLOC_MAX_BLOCKS + MHDCT_NOALLOC_MAX_BLOCKS / 8 + 1;
+
+__device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mutex;
+void *        g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of this device's copy of scratch (the current device), or nullptr
+void *
+scratch_address(int device, std::string & err)
+{
+  if (device < 0 || device >= kMaxDevices)
+  {
+    err = "refine flags (no allocation): device index out of range";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device])
+  {
+    void *           ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+    {
+      err = std::string("refine flags (no allocation): hipGetSymbolAddress: ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
+} // namespace
+
+int
+launch_refine_flags_noalloc(Context & c, const double * U, const AmrParams & p, int8_t * flags_host,
+                            double * ind_host, hipStream_t s)
+{
+  const long n = c.dm.nowned;
+  if (n > MHDCT_NOALLOC_MAX_BLOCKS)
+  {
+    c.err = "refine flags (no allocation): " + std::to_string(n) + " owned blocks, more than the " +
+            std::to_string(long(MHDCT_NOALLOC_MAX_BLOCKS)) + " its scratch holds";
+    return -8;
+  }
+  if (n == 0)
+    return 0;
+  int dev = -1;
+  AMR_HIP(c.err, hipGetDevice(&dev));
+  if (dev != c.device)
+  {
+    c.err = "refine flags (no allocation): the context's device is not the current device";
+    return -1;
+  }
+  void * base = scratch_address(dev, c.err);
+  if (!base)
+    return -1;
+  auto *   emax = static_cast<unsigned long long *>(base);
+  double * ind = reinterpret_cast<double *>(emax + n);
+  auto *   fl = reinterpret_cast<int8_t *>(ind + n);
+  return refine_flags_into(c, U, p, flags_host, ind_host, s, emax, ind, fl, true);
+}
+
 } // namespace mhdct
+
+extern "C"
+{
+  int
+  mhdct_refine_flags_noalloc(mhdct_ctx * h, const double * U, const mhdct_amr_params * p, int8_t * flags,
+                             double * indicator, void * stream)
+  {
+    if (!h || !p || !flags)
+      return -1;
+    mhdct::Context & c = h->c;
+    if (!c.has_mesh)
+    {
+      c.err = "no mesh set";
+      return -1;
+    }
+    if (c.dm.nowned > 0 && !U)
+    {
+      c.err = "null state pointer";
+      return -1;
+    }
+    if (p->ivar < 0 || p->ivar > 4 || p->level_min < 0 || p->level_max < p->level_min || !(p->epsilon_lohner >= 0.0))
+    {
+      c.err = "refine flags: invalid parameters (ivar 0..4, 0 <= level_min <= level_max, epsilon_lohner >= 0)";
+      return -1;
+    }
+    const mhdct::AmrParams ap{ p->level_min, p->level_max, p->epsilon_refine, p->epsilon_coarsen, p->epsilon_lohner,
+                               p->ivar };
+    return mhdct::launch_refine_flags_noalloc(c, U, ap, flags, indicator, static_cast<hipStream_t>(stream));
+  }
+}

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/mhdct.h"
 #include "ctx.hpp"
 
 namespace mhdct
@@ -48,6 +49,10 @@ struct RemapEntry
 // 0 keep; ind optional).  Synchronises `s`.
 int launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
                         hipStream_t s);
+// The same flags with no device allocation: the buffers are a file-scope device array for at
+// most MHDCT_NOALLOC_MAX_BLOCKS owned blocks (-8 beyond it); calls that overlap on one GPU race.
+int launch_refine_flags_noalloc(Context & c, const double * U, const AmrParams & p, int8_t * flags_host,
+                                double * ind_host, hipStream_t s);
 // The work lists of a remap, in one int array: copy pairs (old leaf, new leaf), then the new
 // leaves that are prolonged or restricted ("moved"), then the restricted ones, then the
 // prolonged ones.

@@ -162,6 +162,17 @@ typedef struct mhdct_amr_params
  * (`indicator` may be NULL).  Synchronises `stream`. */
 int mhdct_refine_flags(mhdct_ctx * ctx, const double * U, const mhdct_amr_params * params, int8_t * flags,
                        double * indicator, void * stream);
+/* No counterpart in the reference.  mhdct_refine_flags for callers that cannot allocate device
+ * memory (an AMR cycle inside a serving loop or another allocation-free host loop): the same
+ * results as mhdct_refine_flags, with its device buffers (per-block maxima, indicators, flags)
+ * in a file-scope device array of the library sized for at most MHDCT_NOALLOC_MAX_BLOCKS owned
+ * blocks (4.25 MiB, one copy per GPU, held until the process ends); a larger mesh is refused
+ * with -8 before any device work.  The array keeps the last call's contents; calls that overlap
+ * on one GPU (several contexts or streams) race on it.  The context's device must be current.
+ * Synchronises `stream`. */
+#define MHDCT_NOALLOC_MAX_BLOCKS 262144
+int mhdct_refine_flags_noalloc(mhdct_ctx * ctx, const double * U, const mhdct_amr_params * params, int8_t * flags,
+                               double * indicator, void * stream);
 /* A copy of a mesh (the leaf set before an adaptation, kept for the remap). */
 int mhdct_mesh_clone(const mhdct_mesh * mesh, mhdct_mesh ** out);
 
