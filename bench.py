@@ -1,14 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark: cell-updates/s of the MHD-CT time step on one MI355X.  Prints one JSON line.
 
-  python bench.py [--workload bl3d|mix|kh3d|ot2d|srv] [--steps K] [--warmup W] [--impl fused|fp32|plain]
+  python bench.py [--workload mix|bl3d|kh3d|ot2d|srv] [--steps K] [--warmup W] [--impl fused|fp32|plain]
                   [--integrator hancock|rk2]
 
 A step advances every owned cell of the workload once: the outside-block fill, the CFL dt
 reduction (dt = cfl / inv_dt, on the device) and the time step, inputs resident in HBM.
 
-  bl3d (default)  3D MHD blast, 16^3 blocks, levels 3-6 (2808 leaves, 11.5 M cells), FP64
-  mix             three periodic AMR meshes of 8^3, 16^3 and 32^3 blocks (27 M cells) in one batch
+  mix (default)   the mixed-block 3D AMR mesh: 4096 blocks, 2048 x 8^3 + 1434 x 16^3 + 614 x 32^3
+                  (27,041,792 cells, levels 2-6) as three periodic meshes in one batch, FP64
+  bl3d            3D MHD blast, 16^3 blocks, levels 3-6 (2808 leaves, 11.5 M cells), FP64
   kh3d            3D Kelvin-Helmholtz, 256^3 as 16^3 blocks
   ot2d            2D Orszag-Tang, 128^2 as 8^2 blocks
   srv             the serving loop: a seeded Poisson request stream, p50/p99 latency
@@ -47,7 +48,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--impl", default="fused", choices=["fused", "fp32", "plain"])
-    ap.add_argument("--workload", default="bl3d", choices=["bl3d", "mix", "kh3d", "ot2d", "srv"])
+    ap.add_argument("--workload", default="mix", choices=["bl3d", "mix", "kh3d", "ot2d", "srv"])
     ap.add_argument("--integrator", default="hancock", choices=["hancock", "rk2"])
     ap.add_argument("--amr-cycle", action="store_true",
                     help="time one AMR cycle (refinement flags, host adaptation, remap) on the workload after "
@@ -62,6 +63,8 @@ def parse():
                          "--steps steps, and a restart from that checkpoint")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the serving and BL3D sub-lines of the default (mix) run")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
     ap.add_argument("--precision-steps", type=int, default=0,
                     help="with --impl fp32: run the FP32-bulk and the FP64 form this many steps from the same "
@@ -79,12 +82,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cases, steps):
-    """The CPU restatement (oracle) timed on the host cores over `steps` steps of the workload's
-    meshes; reported beside the GPU figure, not a target."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_ref import OracleSolver, settings_array  # test infrastructure: the CPU baseline leg
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
+
+def cpu_baseline(cases, steps):
+    """The CPU restatement (oracle/, its -O3 OpenMP build) timed on the host cores over `steps`
+    steps of the workload's meshes; reported beside the GPU figure, not a target."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ref  # test infrastructure: the CPU baseline leg
+    from oracle_ref import OracleSolver, settings_array
+
+    oracle_ref.use_library(oracle_ref.ORACLE_LIB_O3)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cells = 0
     el = 0.0
@@ -103,20 +118,52 @@ def cpu_baseline(cases, steps):
             Uo, Bo, _ = ora.step(Uo, Bo)
         el += time.perf_counter() - t0
         cells += mesh.owned_cells() * steps
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
     return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "build": "g++ -O3 -fopenmp (oracle/Makefile, libmhd_oracle_o3.so), OpenMP schedule(static)",
             "sample": f"{steps} full step(s) of every mesh of the workload ({cells // steps} cells), "
                       f"C++ FP64 restatement with OpenMP over {threads} threads, {el:.2f} s"}
+
+
+# the kernel sources whose counters profiles/hbm_traffic.json records
+COUNTED_SOURCES = ["fused.hip", "fused.hpp", "phys.hpp", "common.hpp", "batch.hip"]
+
+
+def sources_digest():
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in COUNTED_SOURCES:
+        h.update((ROOT / "godunov_mhd_ct_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()
+
+
+def _counters(workload, kernel):
+    """The committed counter record of `kernel` on `workload` and its provenance, or (None, why):
+    the record counts only if it was taken from the kernel sources of this tree (their SHA-256 is
+    stored beside the counters)."""
+    p = ROOT / "profiles" / "hbm_traffic.json"
+    try:
+        d = json.loads(p.read_text())
+    except Exception:
+        return None, "no counter summary"
+    if d.get("sources_sha256") != sources_digest():
+        return None, f"counters of other kernel sources (taken at {d.get('commit', '?')})"
+    rec = d.get(workload, {}).get(kernel)
+    if rec is None:
+        return None, f"no counters for {kernel} on {workload}"
+    return rec, f"profiles/hbm_traffic.json, taken at {d.get('commit', '?')} from these kernel sources"
 
 
 def load_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, scaled by a copy of known size), or None."""
-    p = ROOT / "profiles" / "hbm_traffic.json"
-    try:
-        d = json.loads(p.read_text())
-        return d[workload][kernel]["bytes_per_launch"]
-    except Exception:
-        return None
+    rec, _ = _counters(workload, kernel)
+    return rec["bytes_per_launch"] if rec else None
 
 
 # wave64 FP64 VALU instructions per second: 1024 SIMDs x 2.4 GHz / 4 cycles per instruction
@@ -126,12 +173,8 @@ VALU_PEAK = 1024 * 2.4e9 / 4
 def valu_utilisation(kernel, workload, kernel_ms):
     """SQ_INSTS_VALU per launch of `kernel` (committed counter summary) over this run's kernel
     time, as a fraction of the full-rate VALU issue peak, or None."""
-    p = ROOT / "profiles" / "hbm_traffic.json"
-    try:
-        d = json.loads(p.read_text())
-        return d[workload][kernel]["valu_insts"] / (kernel_ms * 1e-3) / VALU_PEAK
-    except Exception:
-        return None
+    rec, _ = _counters(workload, kernel)
+    return rec["valu_insts"] / (kernel_ms * 1e-3) / VALU_PEAK if rec else None
 
 
 def run_plain(cases, steps, warmup, impl=None):
@@ -264,7 +307,10 @@ def main():
                            "frac": achieved / HBM_PEAK, "traffic": load_traffic(kern, args.workload),
                            "kernel": kern, "kernel_ms": kms,
                            "valu_utilisation": valu_utilisation(kern, args.workload, kms),
-                           "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B x {ncells} cell-updates per launch"}
+                           "counters": _counters(args.workload, kern)[1],
+                           "algorithmic_bytes": f"{ALGO_BYTES_PER_CELL} B x {ncells} cell-updates per launch",
+                           "end_to_end": {"achieved": value * ALGO_BYTES_PER_CELL / 1e9,
+                                          "frac": value * ALGO_BYTES_PER_CELL / HBM_PEAK}}
         out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
         if tune:
             config["tune"] = {"grid_cap": tune[0], "tile_cap": tune[1], "tile_z": tune[2]}
@@ -277,7 +323,57 @@ def main():
     out["config"] = config
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_steps)
+    if args.workload == "mix" and args.impl == "fused" and not args.no_extras:
+        del cases
+        out["serving"] = serving_lines(args)
+        out["bl3d"] = bl3d_line(args)
     print(json.dumps(out))
+
+
+def serving_lines(args):
+    """The serving workload at the stated rate (50 requests/s) and at a rate the GPU keeps up
+    with (8 requests/s): p50/p99 latency from arrival to the result on the host."""
+    import torch
+    from godunov_mhd_ct_amd.serving import Server, request_stream
+
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.5 * free)
+    Server(hbm_budget=budget).run(request_stream(2, rate=1000.0, seed=99, nsteps=5, sizes2d=(32,), sizes3d=(32,)))
+    res = {}
+    for rate in (50.0, 8.0):
+        log(f"[bench] serving {args.requests} requests at {rate}/s")
+        r = Server(hbm_budget=budget, max_tick=20).run(request_stream(args.requests, rate=rate, seed=0, nsteps=200))
+        res[f"rate_{int(rate)}"] = {"rate_per_s": rate, "requests": r["requests"],
+                                    "latency_ms_p50": r["latency_ms_p50"], "latency_ms_p99": r["latency_ms_p99"],
+                                    "requests_per_s": r["requests"] / r["wall_s"],
+                                    "cell_updates_per_s": r["cell_updates_per_s"]}
+    res["workload"] = ("Poisson arrivals (seed 0); 2D field loop or 2D/3D blast (beta log-uniform in [0.1, 10]); "
+                       "2D 64^2-512^2, 3D 32^3-128^3; 8^3 or 16^3 blocks; 200 steps each; FP64")
+    torch.cuda.empty_cache()
+    return res
+
+
+def bl3d_line(args):
+    """The 3D blast (16^3 blocks, levels 3-6, 11.5 M cells) through the same batch engine, with
+    the plain form beside it."""
+    import torch
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED
+
+    log("[bench] bl3d sub-line")
+    cases = cases_for("bl3d")
+    ncells, wall, stage_ms, gwall, tune = bench_batch(args, cases, IMPL_FUSED)
+    value = ncells * args.steps / wall
+    kern = max((k for k in stage_ms if k.startswith("k_")), key=lambda k: stage_ms[k])
+    out = {"value": value, "ms_per_step": wall * 1e3 / args.steps, "cells": ncells,
+           "leaves": sum(m.num_leaves for (m, _, _, _) in cases),
+           "roofline_frac": value * ALGO_BYTES_PER_CELL / HBM_PEAK,
+           "dominant_kernel": kern, "dominant_kernel_ms": stage_ms[kern],
+           "dominant_kernel_frac": ALGO_BYTES_PER_CELL * ncells / (stage_ms[kern] * 1e-3) / HBM_PEAK}
+    if not args.no_plain:
+        p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
+        out["plain"] = {"value": p["value"], "speedup": value / p["value"]}
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_rk2(args, cases):

@@ -17,7 +17,17 @@ ORACLE_DIR = ROOT / "oracle"
 # purpose, to show that the pin tests fail on it); by default the one oracle/Makefile builds
 ORACLE_LIB = Path(os.environ.get("MHDCT_ORACLE_LIB", ORACLE_DIR / "_build" / "libmhd_oracle.so"))
 
+# the -O3 build of the same source, timed as the CPU baseline (not used for parity)
+ORACLE_LIB_O3 = ORACLE_DIR / "_build" / "libmhd_oracle_o3.so"
+
 _lib = None
+
+
+def use_library(path):
+    """Load the oracle from `path` from now on (the -O3 timing build for the CPU baseline)."""
+    global ORACLE_LIB, _lib
+    ORACLE_LIB = Path(path)
+    _lib = None
 
 
 def build_oracle():
