@@ -108,7 +108,8 @@ def cpu_baseline(cases, steps):
         mu = st.viscosity_mu if st.viscosity_enabled else None
         ora = OracleSolver(mesh, settings_array(gamma0=st.gamma0, cfl=st.cfl, slope_type=st.slope_type,
                                                 time_integrator=st.time_integrator, gravity=grav, mu=mu,
-                                                viscous_predictor=bool(st.viscosity_hancock_predictor)),
+                                                viscous_predictor=bool(st.viscosity_hancock_predictor),
+                                                prolongation=int(st.prolongation)),
                            nthreads=threads)
         nt = mesh.num_total
         Uo = U.reshape(nt, 5, -1).copy()

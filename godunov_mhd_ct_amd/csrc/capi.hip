@@ -262,7 +262,7 @@ extern "C"
       return -1;
     if (impl_version != MHDCT_IMPL_PLAIN && impl_version != MHDCT_IMPL_FUSED && impl_version != MHDCT_IMPL_FUSED_FP32)
       return -4;
-    if (s->riemann != 5 || s->prolongation != 0)
+    if (s->riemann != 5 || (s->prolongation != 0 && s->prolongation != 1))
       return -5;
     if (s->time_integrator != MHDCT_TIME_HANCOCK && s->time_integrator != MHDCT_TIME_RK2_SSP)
       return -5;
@@ -293,6 +293,7 @@ extern "C"
     c.st.viscosity_enabled = s->viscosity_enabled != 0;
     c.st.mu = s->viscosity_mu;
     c.st.viscosity_predictor = s->viscosity_hancock_predictor != 0;
+    c.st.prolongation = s->prolongation;
     if (hipGetDevice(&c.device) != hipSuccess)
     {
       delete h;
@@ -342,6 +343,7 @@ extern "C"
     c.dm.nowned = int(M.leaves.size());
     c.dm.ntotal = M.ntotal();
     c.dm.noutside = int(M.outside.size());
+    c.dm.prolong = c.st.prolongation;
     c.dm.nfix = 0;
     for (int f = 0; f < 6; ++f)
       c.dm.bc[f] = M.bc[f];

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ctx.hpp"
+#include "phys.hpp"
 
 namespace mhdct
 {
@@ -176,6 +177,89 @@ __device__ inline Hit
 neighbour(const DevMesh & m, int oct, const int c[3])
 {
   return neighbour(m, Idx{ m.dim, m.b }, oct, c);
+}
+
+// ---------------------------------------------------------------------------
+// Ghost cells beside a coarser block, EXTRAPOLATE_LINEAR_MINMOD
+// (linear_extrapolate_hydro_vars_using_limited_slopes, ConvertToPrimitivesVariablesFunctor.cpp:
+// 471-635).  The five conserved variables at coarse-level cell c of block oct as the slopes read
+// them: the cell itself inside the block; beyond it the neighbour's cell (same level), the mean
+// of its 2^dim cells (finer) or its containing cell (coarser); false beyond an outside block,
+// which has no neighbour table.
+// ---------------------------------------------------------------------------
+template <class IX>
+__device__ inline bool
+stencil_u(const DevMesh & m, const IX & I, const double * __restrict__ U, int oct, const int c[3], double * u)
+{
+  const int b = I.b;
+  bool      inside = true;
+  for (int d = 0; d < I.dim; ++d)
+    inside = inside && c[d] >= 0 && c[d] < b;
+  const long ncell = I.cells();
+  if (inside)
+  {
+    const long ci = I.cell(c[0], c[1], I.dim == 3 ? c[2] : 0);
+    for (int v = 0; v < 5; ++v)
+      u[v] = U[(long(oct) * 5 + v) * ncell + ci];
+    return true;
+  }
+  if (oct >= m.nowned)
+    return false;
+  const Hit h = neighbour(m, I, oct, c);
+  if (h.oct < 0)
+    return false;
+  if (h.rel > 0)
+  {
+    for (int v = 0; v < 5; ++v)
+    {
+      const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+      double         sum = 0;
+      for (int dk = 0; dk < (I.dim == 3 ? 2 : 1); ++dk)
+        for (int dj = 0; dj < 2; ++dj)
+          for (int di = 0; di < 2; ++di)
+            sum += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+      u[v] = sum / double(1 << I.dim);
+    }
+  }
+  else
+  {
+    const long ci = I.cell(h.l[0], h.l[1], h.l[2]);
+    for (int v = 0; v < 5; ++v)
+      u[v] = U[(long(h.oct) * 5 + v) * ncell + ci];
+  }
+  return true;
+}
+
+// The conserved variables of ghost cell (i, j, k) from the coarse cell h that contains it:
+// u = u_c + 1/4 s_x du/dx + 1/4 s_y du/dy (+ 1/4 s_z du/dz), summed in that order, du/dx the
+// minmod slope (slope_type 1) over the coarse cell's neighbours along x, zero when one of them
+// lies beyond an outside block; s = -1 / +1 in the low / high half of the coarse cell.
+template <class IX>
+__device__ inline void
+extrapolate_linear_minmod(const DevMesh & m, const IX & I, const double * __restrict__ U, const Hit & h, int i,
+                          int j, int k, double * u)
+{
+  const int dim = I.dim;
+  const int cell[3] = { i, j, k };
+  const int c0[3] = { h.l[0], h.l[1], h.l[2] };
+  double    u0[5];
+  stencil_u(m, I, U, h.oct, c0, u0);
+  for (int v = 0; v < 5; ++v)
+    u[v] = u0[v];
+  for (int d = 0; d < dim; ++d)
+  {
+    int cp[3] = { c0[0], c0[1], c0[2] }, cm[3] = { c0[0], c0[1], c0[2] };
+    cp[d] += 1;
+    cm[d] -= 1;
+    double     up[5], um[5];
+    const bool ok = stencil_u(m, I, U, h.oct, cp, up) & stencil_u(m, I, U, h.oct, cm, um);
+    const double sgn = (cell[d] & 1) ? 1.0 : -1.0;
+    for (int v = 0; v < 5; ++v)
+    {
+      const double slope = ok ? limited_slope(u0[v], up[v], um[v], 1.0) : 0.0;
+      u[v] = u[v] + 0.25 * sgn * slope;
+    }
+  }
 }
 
 __device__ inline double

@@ -30,6 +30,7 @@ struct Settings
   int    viscosity_enabled = 0;   // [viscosity] enabled (plain form only)
   double mu = 0.0;                // [viscosity] mu, the dynamic viscosity
   int    viscosity_predictor = 0; // [viscosity] hancock_predictor_enabled
+  int    prolongation = 0;        // [amr] cell_centered_prolongation_type: 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
 };
 
 // Mesh tables resident in HBM.
@@ -41,6 +42,7 @@ struct DevMesh
   int              ntotal = 0;
   int              nfix = 0;
   int              noutside = 0;
+  int              prolong = 0; // ghost cells beside a coarser block: 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
   int              bc[6] = { 0, 0, 0, 0, 0, 0 };
   int              nbrick[3] = { 1, 1, 1 };
   int *            level = nullptr;     // [ntotal]

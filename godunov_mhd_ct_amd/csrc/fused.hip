@@ -300,18 +300,24 @@ prim_item(const FusedSlot<T> & S, long r0)
     {
       const int c[3] = { i, j, k };
       const Hit h = neighbour(m, I, o, c);
-      if (h.rel > 0)
+      const bool lin = h.rel < 0 && m.prolong == 1;
+      if (h.rel > 0 || lin)
       {
-        for (int v = 0; v < 5; ++v)
-        {
-          const double * Un = U + (long(h.oct) * 5 + v) * ncell;
-          double         sum = 0;
-          for (int dk = 0; dk < (DIM == 3 ? 2 : 1); ++dk)
-            for (int dj = 0; dj < 2; ++dj)
-              for (int di = 0; di < 2; ++di)
-                sum += Un[IX::cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
-          u[v] = sum / double(1 << DIM);
-        }
+        if (lin)
+          extrapolate_linear_minmod(m, I, U, h, i, j, k, u);
+        else
+          for (int v = 0; v < 5; ++v)
+          {
+            const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+            double         sum = 0;
+            for (int dk = 0; dk < (DIM == 3 ? 2 : 1); ++dk)
+              for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di)
+                  sum += Un[IX::cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+            u[v] = sum / double(1 << DIM);
+          }
+        // the cell-centred field from this block's ghosted faces (finer neighbour, and the
+        // linear prolongation: ConvertToPrimitivesVariablesFunctor.cpp:536-541, 623-628)
         if (DIM == 2)
         {
           u[kA] = 0.5 * (gfv(0, i, j, 0) + gfv(0, i + 1, j, 0));
@@ -380,7 +386,7 @@ prim_item(const FusedSlot<T> & S, long r0)
 }
 
 template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)

@@ -98,18 +98,24 @@ k_convert_to_primitives(DevMesh m, Gas<double> g, int abort_eint, const double *
   {
     const int c[3] = { i, j, k };
     const Hit h = neighbour(m, o, c);
-    if (h.rel > 0)
+    const bool lin = h.rel < 0 && m.prolong == 1;
+    if (h.rel > 0 || lin)
     {
-      for (int v = 0; v < 5; ++v)
-      {
-        const double * Un = U + (long(h.oct) * 5 + v) * ncell;
-        double         s = 0;
-        for (int dk = 0; dk < (dim == 3 ? 2 : 1); ++dk)
-          for (int dj = 0; dj < 2; ++dj)
-            for (int di = 0; di < 2; ++di)
-              s += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
-        u[v] = s / double(1 << dim);
-      }
+      if (lin)
+        extrapolate_linear_minmod(m, I, U, h, i, j, k, u);
+      else
+        for (int v = 0; v < 5; ++v)
+        {
+          const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+          double         s = 0;
+          for (int dk = 0; dk < (dim == 3 ? 2 : 1); ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+              for (int di = 0; di < 2; ++di)
+                s += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+          u[v] = s / double(1 << dim);
+        }
+      // the cell-centred field from this block's ghosted faces (finer neighbour, and the linear
+      // prolongation: ConvertToPrimitivesVariablesFunctor.cpp:536-541, 623-628)
       if (dim == 2)
       {
         u[kA] = 0.5 * (bgv(0, i, j, 0) + bgv(0, i + 1, j, 0));

@@ -53,7 +53,9 @@ typedef struct mhdct_settings
   double smallc;         /* [hydro] smallc, default 1e-10 (smallp = smallc^2/gamma0) */
   double scaling_factor; /* [mesh] scaling_factor, default 1 */
   int    riemann;        /* RiemannSolverType (models/riemann_solver_types.h): 5 = HLLD, the only one supported */
-  int    prolongation;   /* [amr] cell_centered_prolongation_type: 0 = SIMPLE_COPY, the only one supported */
+  int    prolongation;   /* [amr] cell_centered_prolongation_type of the primitive conversion's ghost cells
+                            beside a coarser block (ConvertToPrimitivesVariablesFunctor.cpp:697-710):
+                            0 = SIMPLE_COPY, 1 = EXTRAPOLATE_LINEAR_MINMOD */
   int    abort_when_negative_eint; /* [hydro] abort_when_negative_eint: report eint < 0 as an error */
   int    time_integrator; /* [run] time_integrator: MHDCT_TIME_HANCOCK or MHDCT_TIME_RK2_SSP */
   int    gravity_enabled; /* [gravity] enabled: a uniform gravity field (AddGravitySourceTerm.cpp) */
@@ -109,7 +111,7 @@ void         mhdct_mesh_destroy(mhdct_mesh * mesh);
 /* ---- GodunovImplemBase ---------------------------------------------------------------- */
 
 /* -4: unknown impl_version; -5: settings this build does not run (a Riemann solver other than
- * HLLD, a prolongation other than SIMPLE_COPY, an unknown time integrator, viscosity in the
+ * HLLD, a prolongation other than SIMPLE_COPY or EXTRAPOLATE_LINEAR_MINMOD, an unknown time integrator, viscosity in the
  * FP32-bulk form or with mu < 0); -6: unequal block sizes. */
 int      mhdct_create(const mhdct_settings * settings, int dim, const int block_size[3], int impl_version,
                       mhdct_ctx ** out);

@@ -102,6 +102,9 @@ struct Settings
   int    viscosity_enabled = 0;
   double mu = 0.0;
   int    viscosity_predictor = 0;
+  // [amr] cell_centered_prolongation_type of the ghost cells beside a coarser block
+  // (ConvertToPrimitivesVariablesFunctor.cpp:697-710): 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+  int    prolongation = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1148,8 +1151,92 @@ public:
   // -------------------------------------------------------------------
   // ConvertToPrimitivesVariablesFunctor::fill_ghosts
   // (scheme/ConvertToPrimitivesVariablesFunctor.cpp:639-719), cell_centered_prolongation_type
-  // SIMPLE_COPY.
+  // SIMPLE_COPY or EXTRAPOLATE_LINEAR_MINMOD.
   // -------------------------------------------------------------------
+
+  // The five conserved variables at coarse-level cell c of block oct, as the minmod slopes of
+  // the linear ghost prolongation read them: the cell itself inside the block; beyond it, the
+  // neighbour's cell (same level), the mean of its 2^dim cells (finer, x fastest, then y, then
+  // z) or its containing cell (coarser).  Restates kalypsso::core
+  // StencilHelper::getNeighLoc + compute_minmod_slopes (called at
+  // ConvertToPrimitivesVariablesFunctor.cpp:485-508, 571-597; not in the reference tree): false
+  // when the cell lies beyond an outside block, whose neighbours are not part of the mesh.
+  bool
+  stencil_u(const double * U, int oct, const int c[3], double * u) const
+  {
+    const int  b = lay.b, dim = m.dim;
+    const long ncell = lay.cells();
+    bool       inside = true;
+    for (int d = 0; d < dim; ++d)
+      inside = inside && c[d] >= 0 && c[d] < b;
+    if (inside)
+    {
+      for (int v = 0; v < 5; ++v)
+        u[v] = U[size_t(oct * 5 + v) * ncell + lay.cidx(c[0], c[1], dim == 3 ? c[2] : 0)];
+      return true;
+    }
+    if (oct >= m.nowned)
+      return false;
+    const Loc l = m.locate(oct, c[0], c[1], dim == 3 ? c[2] : 0);
+    if (l.oct < 0)
+      return false;
+    if (l.rel == 1)
+    {
+      for (int v = 0; v < 5; ++v)
+      {
+        double sum = 0;
+        for (int dk = 0; dk < (dim == 3 ? 2 : 1); ++dk)
+          for (int dj = 0; dj < 2; ++dj)
+            for (int di = 0; di < 2; ++di)
+              sum += U[size_t(l.oct * 5 + v) * ncell + lay.cidx(l.ijk[0] + di, l.ijk[1] + dj, l.ijk[2] + dk)];
+        u[v] = sum / double(1 << dim);
+      }
+    }
+    else
+    {
+      for (int v = 0; v < 5; ++v)
+        u[v] = U[size_t(l.oct * 5 + v) * ncell + lay.cidx(l.ijk[0], l.ijk[1], l.ijk[2])];
+    }
+    return true;
+  }
+
+  // linear_extrapolate_hydro_vars_using_limited_slopes (ConvertToPrimitivesVariablesFunctor.cpp:
+  // 471-635): the conserved variables of ghost cell (i, j, k) from the coarse cell l that
+  // contains it, u = u_c + 1/4 s_x du/dx + 1/4 s_y du/dy (+ 1/4 s_z du/dz), du/dx the minmod
+  // slope (slope_type 1) over the coarse cell's neighbours along x (zero when one of them lies
+  // beyond an outside block), s = -1 / +1 for the low / high half of the coarse cell.
+  void
+  extrapolate_linear_minmod(const double * U, const Loc & l, int i, int j, int k, double * u) const
+  {
+    const int dim = m.dim;
+    const int cell[3] = { i, j, k };
+    double    u0[5];
+    const int c0[3] = { l.ijk[0], l.ijk[1], l.ijk[2] };
+    stencil_u(U, l.oct, c0, u0);
+    double slope[3][5] = {};
+    for (int d = 0; d < dim; ++d)
+    {
+      int cp[3] = { c0[0], c0[1], c0[2] }, cm[3] = { c0[0], c0[1], c0[2] };
+      cp[d] += 1;
+      cm[d] -= 1;
+      double up[5], um[5];
+      const bool okp = stencil_u(U, l.oct, cp, up);
+      const bool okm = stencil_u(U, l.oct, cm, um);
+      for (int v = 0; v < 5; ++v)
+        slope[d][v] = (okp && okm) ? slope_unsplit_scalar(u0[v], up[v], um[v], 1.0) : 0.0;
+    }
+    for (int v = 0; v < 5; ++v)
+    {
+      double x = u0[v];
+      for (int d = 0; d < dim; ++d)
+      {
+        const int sgn = 2 * (((cell[d] % 2) + 2) % 2) - 1;
+        x = x + 0.25 * double(sgn) * slope[d][v];
+      }
+      u[v] = x;
+    }
+  }
+
   bool
   convert_to_primitives(const double * U, const double * B)
   {
@@ -1196,7 +1283,25 @@ public:
           ok = false;
           continue;
         }
-        if (l.rel == 1)
+        if (l.rel == -1 && s.prolongation == 1)
+        {
+          // coarser neighbour, EXTRAPOLATE_LINEAR_MINMOD: the extrapolated conserved variables
+          // and the cell-centred field of this block's ghosted faces (:536-541, 623-628)
+          extrapolate_linear_minmod(U, l, i, j, k, u);
+          if (dim == 2)
+          {
+            u[IA] = 0.5 * (bg(IX, i, j, 0, o) + bg(IX, i + 1, j, 0, o));
+            u[IB] = 0.5 * (bg(IY, i, j, 0, o) + bg(IY, i, j + 1, 0, o));
+            u[IC] = bg(IZ, i, j, 0, o);
+          }
+          else
+          {
+            u[IA] = 0.5 * (bg(IX, i, j, k, o) + bg(IX, i + 1, j, k, o));
+            u[IB] = 0.5 * (bg(IY, i, j, k, o) + bg(IY, i, j + 1, k, o));
+            u[IC] = 0.5 * (bg(IZ, i, j, k, o) + bg(IZ, i, j, k + 1, o));
+          }
+        }
+        else if (l.rel == 1)
         {
           // restriction: siblings average (x fastest, then y, then z)
           for (int v = 0; v < 5; ++v)
@@ -2575,6 +2680,7 @@ extern "C"
     S->s.viscosity_enabled = settings[12] != 0.0;
     S->s.mu = settings[13];
     S->s.viscosity_predictor = settings[14] != 0.0;
+    S->s.prolongation = int(settings[15]);
     S->m.dim = dim;
     S->m.b = b;
     S->m.nowned = nowned;

@@ -96,14 +96,20 @@ RK2_STAGE1 = (0.0, 1.0)
 RK2_STAGE2 = (0.5, 0.5)
 
 
+PROLONGATION = {"SIMPLE_COPY": 0, "EXTRAPOLATE_LINEAR_MINMOD": 1}
+
+
 def settings_array(gamma0=1.4, cfl=0.5, slope_type=1.0, smallr=1e-10, smallc=1e-10, scaling=1.0,
-                   abort_eint=0, time_integrator="HANCOCK", gravity=None, mu=None, viscous_predictor=False):
+                   abort_eint=0, time_integrator="HANCOCK", gravity=None, mu=None, viscous_predictor=False,
+                   prolongation="SIMPLE_COPY"):
     """The oracle's settings vector; mu (not None) enables viscosity."""
+    pr = PROLONGATION[prolongation] if isinstance(prolongation, str) else int(prolongation)
     ti = TIME_INTEGRATOR[time_integrator] if isinstance(time_integrator, str) else int(time_integrator)
     g = (0.0, 0.0, 0.0) if gravity is None else tuple(float(x) for x in gravity) + (0.0,) * (3 - len(gravity))
     return np.array([gamma0, cfl, slope_type, smallr, smallc, scaling, abort_eint, ti,
                      0.0 if gravity is None else 1.0, *g, 0.0 if mu is None else 1.0,
-                     0.0 if mu is None else float(mu), 1.0 if viscous_predictor else 0.0], dtype=np.float64)
+                     0.0 if mu is None else float(mu), 1.0 if viscous_predictor else 0.0, float(pr)],
+                    dtype=np.float64)
 
 
 class OracleSolver:

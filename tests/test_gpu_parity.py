@@ -35,18 +35,20 @@ def compare(Ug, Bg, Uo, Bo, nowned, tag):
 
 
 def run_pair(mesh, U, B, gamma, cfl, slope_type, nsteps, impl, bc=None, time_integrator="HANCOCK", gravity=None,
-             viscosity=None, viscous_predictor=False):
+             viscosity=None, viscous_predictor=False, prolongation="SIMPLE_COPY"):
     import torch
     from godunov_mhd_ct_amd.solver import SolverGodunovMHD
 
-    print(f"[parity] impl={impl} {time_integrator} dim={mesh.dim} b={mesh.b} leaves={mesh.num_leaves} "
-          f"outside={mesh.num_outside} steps={nsteps}", flush=True)
+    print(f"[parity] impl={impl} {time_integrator} {prolongation} dim={mesh.dim} b={mesh.b} "
+          f"leaves={mesh.num_leaves} outside={mesh.num_outside} steps={nsteps}", flush=True)
     st = hydro_settings(gamma0=gamma, cfl=cfl, slope_type=slope_type, time_integrator=time_integrator,
-                        gravity=gravity, viscosity=viscosity, viscous_predictor=viscous_predictor)
+                        gravity=gravity, viscosity=viscosity, viscous_predictor=viscous_predictor,
+                        prolongation=prolongation)
     sol = SolverGodunovMHD(mesh, U, B, st, impl_version=impl)
     ora = OracleSolver(mesh, settings_array(gamma0=gamma, cfl=cfl, slope_type=slope_type,
                                             time_integrator=time_integrator, gravity=gravity, mu=viscosity,
-                                            viscous_predictor=viscous_predictor), nthreads=4, bc=bc)
+                                            viscous_predictor=viscous_predictor, prolongation=prolongation),
+                       nthreads=4, bc=bc)
     Uo = U.reshape(mesh.num_total, 5, -1).copy()
     Bo = B.reshape(mesh.num_total, -1).copy()
     all_same = True
