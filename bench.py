@@ -354,6 +354,40 @@ def serving_lines(args):
     return res
 
 
+def amr_end_to_end(case, amr, steps, impl=None):
+    """The per-mesh solver (host dt each step, as the reference) with an AMR cycle when due
+    (next_iteration_impl, SolverGodunovMHD.cpp:409-484), counted as the reference counts: the
+    owned cells of each step's mesh (:482), over the whole run including the cycles."""
+    import torch
+    from godunov_mhd_ct_amd.implem import IMPL_FUSED
+    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+
+    m, U, B, st = case
+    sol = SolverGodunovMHD(m, U, B, st, impl_version=IMPL_FUSED if impl is None else impl)
+    leaves0 = m.num_leaves
+    sol.step()  # warm-up (the first cycle is disabled, as in the blast configuration)
+    torch.cuda.synchronize()
+    sol.iteration = 1
+    cells = cycles = 0
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        if sol.should_do_amr_cycle(amr, disable_first=True):
+            sol.do_amr_cycle(amr)
+            cycles += 1
+        cells += sol.mesh.owned_cells()
+        sol.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    out = {"steps": steps, "amr_cycles": cycles, "cycle_frequency": amr.cycle_frequency,
+           "criterion": f"{amr.criterion} on {', '.join(amr.names())} > {amr.epsilon_refine}",
+           "levels": [amr.level_min, amr.level_max], "leaves_at_start": leaves0,
+           "leaves_at_end": sol.mesh.num_leaves, "cell_updates_per_s": cells / wall,
+           "ms_per_step": wall * 1e3 / steps, "driver": "per-mesh solver, host dt each step, AMR cycles included"}
+    del sol
+    torch.cuda.empty_cache()
+    return out
+
+
 def bl3d_line(args):
     """The 3D blast (16^3 blocks, levels 3-6, 11.5 M cells) through the same batch engine, with
     the plain form beside it."""
@@ -374,6 +408,10 @@ def bl3d_line(args):
         p = run_plain(cases, min(args.steps, 10), min(args.warmup, 2))
         out["plain"] = {"value": p["value"], "speedup": value / p["value"]}
     torch.cuda.empty_cache()
+    from godunov_mhd_ct_amd.workloads import bl3d_amr_settings
+
+    log("[bench] bl3d end to end with AMR cycles (200 steps)")
+    out["end_to_end_amr"] = amr_end_to_end(cases[0], bl3d_amr_settings(), 200)
     return out
 
 
@@ -487,8 +525,13 @@ def bench_amr(args, cases):
 
     m, U, B, st = cases[0]
     lv, _ = m.leaves()
-    amr = AMRSettings(level_min=int(lv.min()), level_max=int(lv.max()), epsilon_refine=0.01,
-                      epsilon_coarsen=0.00125, epsilon_lohner=0.02)
+    if args.workload == "bl3d":
+        from godunov_mhd_ct_amd.workloads import bl3d_amr_settings
+
+        amr = bl3d_amr_settings(int(lv.min()), int(lv.max()))
+    else:
+        amr = AMRSettings(level_min=int(lv.min()), level_max=int(lv.max()), epsilon_refine=0.01,
+                          epsilon_coarsen=0.00125, epsilon_lohner=0.02)
     sol = SolverGodunovMHD(m, U, B, st, impl_version=IMPL_FUSED)
     for _ in range(args.steps):
         sol.step()
@@ -499,7 +542,7 @@ def bench_amr(args, cases):
     e0, e1 = ev(), ev()
     e0.record()
     for _ in range(reps):
-        flags, ind = refine_flags(sol.implem, sol.U, amr)
+        flags, ind = refine_flags(sol.implem, sol.U, amr, B=sol.B)
     e1.record()
     torch.cuda.synchronize()
     flags_ms = e0.elapsed_time(e1) / reps

@@ -48,6 +48,9 @@ class AmrParams(C.Structure):
         ("epsilon_coarsen", C.c_double),
         ("epsilon_lohner", C.c_double),
         ("ivar", C.c_int),
+        ("criterion", C.c_int),
+        ("nvars", C.c_int),
+        ("vars", C.c_int * 4),
     ]
 
 
@@ -92,10 +95,10 @@ SIGNATURES = {
     "mhdct_batch_set_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "mhdct_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
-    "mhdct_refine_flags": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AmrParams), C.POINTER(C.c_int8),
-                                     C.POINTER(C.c_double), C.c_void_p]),
-    "mhdct_refine_flags_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AmrParams), C.POINTER(C.c_int8),
-                                             C.POINTER(C.c_double), C.c_void_p]),
+    "mhdct_refine_flags": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(AmrParams),
+                                     C.POINTER(C.c_int8), C.POINTER(C.c_double), C.c_void_p]),
+    "mhdct_refine_flags_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(AmrParams),
+                                             C.POINTER(C.c_int8), C.POINTER(C.c_double), C.c_void_p]),
     "mhdct_mesh_clone": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mhdct_remap_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "mhdct_remap_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

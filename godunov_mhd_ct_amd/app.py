@@ -111,7 +111,7 @@ def initial_mesh(cfg: Config, st, amr, impl: int, log=print):
             Bd = torch.from_numpy(np.ascontiguousarray(B).reshape(-1)).cuda()
             if mesh.num_outside:
                 im.fill_outside(Ud, Bd)
-            flags, _ = refine_flags(im, Ud, amr)
+            flags, _ = refine_flags(im, Ud, amr, B=Bd)
             flags = np.where(flags > 0, 1, 0).astype(np.int8)
         if not flags.any():
             break

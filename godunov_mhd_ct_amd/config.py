@@ -15,7 +15,7 @@ from __future__ import annotations
 import ctypes as C
 
 from ._lib import AmrParams, MhdctError, Settings, lib
-from .amr import VARIABLES, AMRSettings
+from .amr import CRITERIA, DERIVED, VARIABLES, AMRSettings
 from .mesh import BC, Mesh
 
 _TRUE = {"true", "yes", "on", "1"}
@@ -111,10 +111,13 @@ class Config:
         rc = lib().mhdct_config_amr_params(self._h, C.byref(p))
         if rc != 0:
             self._err("amr params", rc)
-        names = {v: k for k, v in VARIABLES.items()}
+        names = {v: k for k, v in list(VARIABLES.items()) + list(DERIVED.items())}
+        crit = {v: k for k, v in CRITERIA.items()}[p.criterion]
+        variables = [names[p.vars[i]] for i in range(p.nvars)] if p.nvars > 0 else [names[p.ivar]]
         return AMRSettings(level_min=p.level_min, level_max=p.level_max, epsilon_refine=p.epsilon_refine,
                            epsilon_coarsen=p.epsilon_coarsen, epsilon_lohner=p.epsilon_lohner,
-                           variable=names[p.ivar], cycle_frequency=self.get_integer("amr", "cycle_frequency", 1))
+                           variable=variables[0] if len(variables) == 1 else variables, criterion=crit,
+                           cycle_frequency=self.get_integer("amr", "cycle_frequency", 1))
 
     def mesh(self) -> Mesh:
         h = C.c_void_p()

@@ -43,31 +43,111 @@ namespace
 
 constexpr int kThreads = 256;
 
-// conserved variable v of block o at cell c (at most one cell outside [0, b)^dim): the ghost
-// cell rule of the primitive conversion -- same level: copy; finer: mean of the 2^dim fine
-// cells (x fastest, then y, then z); coarser: the containing cell (SIMPLE_COPY)
+// Refinement variable v of block o at its cell c inside [0, b)^dim: 0-4 a conserved variable of U;
+// 10-17 a derived quantity (ComputeDerivedQuantities, utils/ComputeDerivedQuantities.h:120-228)
+// from the cell's primitives: thermal pressure, speed of sound, specific kinetic energy, magnetic
+// pressure, local Mach number, vx, vy, vz.  The reference computes the derived array for the
+// owned blocks only into a zero-initialised view (:104, 109-111): outside blocks read as zero.
 __device__ inline double
-u_ghosted(const DevMesh & m, const Idx & I, const double * __restrict__ U, int v, int o, const int c[3])
+field_cell(const DevMesh & m, const Idx & I, const Gas<double> & g, const double * __restrict__ U,
+           const double * __restrict__ B, int v, int o, const int c[3])
 {
-  const int  b = I.b;
   const long ncell = I.cells();
-  bool       inside = true;
+  const long ci = I.cell(c[0], c[1], I.dim == 3 ? c[2] : 0);
+  if (v < 10)
+    return U[(long(o) * 5 + v) * ncell + ci];
+  if (o >= m.nowned)
+    return 0.0;
+  double u[8], q[8];
+  for (int n = 0; n < 5; ++n)
+    u[n] = U[(long(o) * 5 + n) * ncell + ci];
+  const double * Bo = B + long(o) * I.fper();
+  const int      i = c[0], j = c[1], k = c[2];
+  if (I.dim == 2)
+  {
+    u[kA] = 0.5 * (Bo[I.face(0, i, j, 0)] + Bo[I.face(0, i + 1, j, 0)]);
+    u[kB] = 0.5 * (Bo[I.face(1, i, j, 0)] + Bo[I.face(1, i, j + 1, 0)]);
+    u[kC] = 0.5 * (Bo[I.face(2, i, j, 0)] + Bo[I.face(2, i, j, 0)]);
+  }
+  else
+  {
+    u[kA] = 0.5 * (Bo[I.face(0, i, j, k)] + Bo[I.face(0, i + 1, j, k)]);
+    u[kB] = 0.5 * (Bo[I.face(1, i, j, k)] + Bo[I.face(1, i, j + 1, k)]);
+    u[kC] = 0.5 * (Bo[I.face(2, i, j, k)] + Bo[I.face(2, i, j, k + 1)]);
+  }
+  to_primitive(u, q, g);
+  switch (v)
+  {
+  case 10: return q[kP];
+  case 11: return sqrt(g.gamma0 * q[kP] / q[kD]);
+  case 12:
+    return I.dim == 2 ? 0.5 * (q[kU] * q[kU] + q[kV] * q[kV]) : 0.5 * (q[kU] * q[kU] + q[kV] * q[kV] + q[kW] * q[kW]);
+  case 13: return 0.5 * (q[kA] * q[kA] + q[kB] * q[kB] + q[kC] * q[kC]);
+  case 14:
+  {
+    double un = q[kU] * q[kU] + q[kV] * q[kV];
+    if (I.dim == 3)
+      un += q[kW] * q[kW];
+    un = sqrt(un);
+    const double cs = sqrt(g.gamma0 * q[kP] / q[kD]);
+    return un / cs;
+  }
+  case 15: return q[kU];
+  case 16: return q[kV];
+  default: return I.dim == 3 ? q[kW] : 0.0;
+  }
+}
+
+// variable v of block o at cell c (at most one cell outside [0, b)^dim): the ghost cell rule
+// of the primitive conversion -- same level: copy; finer: mean of the 2^dim fine cells (x
+// fastest, then y, then z); coarser: the containing cell (SIMPLE_COPY)
+__device__ inline double
+u_ghosted(const DevMesh & m, const Idx & I, const Gas<double> & g, const double * __restrict__ U,
+          const double * __restrict__ B, int v, int o, const int c[3])
+{
+  const int b = I.b;
+  bool      inside = true;
   for (int d = 0; d < I.dim; ++d)
     inside = inside && c[d] >= 0 && c[d] < b;
   if (inside)
-    return U[(long(o) * 5 + v) * ncell + I.cell(c[0], c[1], I.dim == 3 ? c[2] : 0)];
-  const Hit      h = neighbour(m, I, o, c);
-  const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+    return field_cell(m, I, g, U, B, v, o, c);
+  const Hit h = neighbour(m, I, o, c);
   if (h.rel > 0)
   {
     double s = 0;
     for (int dk = 0; dk < (I.dim == 3 ? 2 : 1); ++dk)
       for (int dj = 0; dj < 2; ++dj)
         for (int di = 0; di < 2; ++di)
-          s += Un[I.cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+        {
+          const int f[3] = { h.l[0] + di, h.l[1] + dj, h.l[2] + dk };
+          s += field_cell(m, I, g, U, B, v, h.oct, f);
+        }
     return s / double(1 << I.dim);
   }
-  return Un[I.cell(h.l[0], h.l[1], h.l[2])];
+  return field_cell(m, I, g, U, B, v, h.oct, h.l);
+}
+
+// SIMPLE_GRADIENT (kalypsso::core's ComputeRefineFlags, absent from the reference tree; restated):
+// the largest normalised jump |q_n - q_c| / max(|q_n|, |q_c|) between the cell and its 2 dim face
+// neighbours (zero where both vanish), directions x, y, z, the low neighbour first
+template <int dim>
+__device__ inline double
+simple_gradient(const double * s)
+{
+  auto at = [&](int dx, int dy, int dz) { return s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)]; };
+  const double u0 = at(0, 0, 0);
+  double       e = 0.0;
+#pragma unroll
+  for (int a = 0; a < dim; ++a)
+#pragma unroll
+    for (int sg = -1; sg <= 1; sg += 2)
+    {
+      const double un = at(a == 0 ? sg : 0, a == 1 ? sg : 0, a == 2 ? sg : 0);
+      const double mx = fmax(fabs(un), fabs(u0));
+      const double x = mx > 0.0 ? fabs(un - u0) / mx : 0.0;
+      e = fmax(e, x);
+    }
+  return e;
 }
 
 // Loehner's estimate at cell (i, j, k) of block o from the 3^dim stencil s[(dz+1)*9+(dy+1)*3+dx+1]:
@@ -120,10 +200,13 @@ lohner(const double * s, double eps)
   return den > 0.0 ? sqrt(num / den) : 0.0;
 }
 
-// One thread per owned cell; the block maximum through the bits of the non-negative double.
-template <int DIM>
+// One thread per owned cell; the block maximum (over its cells and the refinement variables)
+// through the bits of the non-negative double.  Only the face neighbours are read for
+// SIMPLE_GRADIENT (CRIT 1), the whole 3^dim stencil for LOHNER_UNSPLIT (CRIT 0).
+template <int DIM, int CRIT>
 __global__ void __launch_bounds__(kThreads)
-k_refine_indicator(DevMesh m, int ivar, double eps, const double * __restrict__ U, unsigned long long * emax)
+k_refine_indicator(DevMesh m, AmrParams p, Gas<double> g, const double * __restrict__ U,
+                   const double * __restrict__ B, unsigned long long * emax)
 {
   const Idx  I{ DIM, m.b }; // m.dim == DIM: the launch picks the instantiation
   const long ncell = I.cells();
@@ -137,20 +220,28 @@ k_refine_indicator(DevMesh m, int ivar, double eps, const double * __restrict__ 
     const long ci = t - long(o) * ncell;
     const int  b = m.b;
     const int  i = int(ci % b), j = int((ci / b) % b), k = DIM == 3 ? int(ci / (long(b) * b)) : 0;
-    double     s[27];
+    const int  nv = p.nvars > 0 ? p.nvars : 1;
+    for (int n = 0; n < nv; ++n)
+    {
+      const int v = p.nvars > 0 ? p.vars[n] : p.ivar;
+      double    s[27];
 #pragma unroll
-    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
 #pragma unroll
-      for (int dy = -1; dy <= 1; ++dy)
+        for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx)
-        {
-          const int c[3] = { i + dx, j + dy, k + dz };
-          s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] = u_ghosted(m, I, U, ivar, o, c);
-        }
-    e = lohner<DIM>(s, eps);
-    if (!(e >= 0.0)) // NaN state: flag it as a maximal indicator
-      e = __builtin_huge_val();
+          for (int dx = -1; dx <= 1; ++dx)
+          {
+            if (CRIT == 1 && (dx != 0) + (dy != 0) + (dz != 0) > 1)
+              continue;
+            const int c[3] = { i + dx, j + dy, k + dz };
+            s[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] = u_ghosted(m, I, g, U, B, v, o, c);
+          }
+      double x = CRIT == 1 ? simple_gradient<DIM>(s) : lohner<DIM>(s, p.epsilon_lohner);
+      if (!(x >= 0.0)) // NaN state: flag it as a maximal indicator
+        x = __builtin_huge_val();
+      e = fmax(e, x);
+    }
   }
   // when every workgroup lies inside one block, reduce over the wavefront first
   if ((ncell % kThreads) == 0)
@@ -494,8 +585,9 @@ grid_of(long n)
 // (n doubles) and fl (n bytes), then read back into the host arrays; emax is cleared first, by
 // hipMemsetAsync (clear_by_kernel false) or by k_clear_words.
 int
-refine_flags_into(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
-                  hipStream_t s, unsigned long long * emax, double * ind, int8_t * fl, bool clear_by_kernel)
+refine_flags_into(Context & c, const double * U, const double * B, const AmrParams & p, int8_t * flags_host,
+                  double * ind_host, hipStream_t s, unsigned long long * emax, double * ind, int8_t * fl,
+                  bool clear_by_kernel)
 {
   const DevMesh & m = c.dm;
   const long      n = m.nowned;
@@ -503,13 +595,17 @@ refine_flags_into(Context & c, const double * U, const AmrParams & p, int8_t * f
     hipLaunchKernelGGL(k_clear_words, grid_of(n), kThreads, 0, s, emax, n);
   else
     AMR_HIP(c.err, hipMemsetAsync(emax, 0, size_t(n) * sizeof(unsigned long long), s));
-  const Shapes S{ c.dim, c.b };
-  if (c.dim == 3)
-    hipLaunchKernelGGL(k_refine_indicator<3>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
-                       U, emax);
+  const Shapes      S{ c.dim, c.b };
+  const Gas<double> g = gas_of(c.st);
+  const unsigned    grid = grid_of(n * S.cells());
+  if (c.dim == 3 && p.criterion == 1)
+    hipLaunchKernelGGL((k_refine_indicator<3, 1>), grid, kThreads, 0, s, m, p, g, U, B, emax);
+  else if (c.dim == 3)
+    hipLaunchKernelGGL((k_refine_indicator<3, 0>), grid, kThreads, 0, s, m, p, g, U, B, emax);
+  else if (p.criterion == 1)
+    hipLaunchKernelGGL((k_refine_indicator<2, 1>), grid, kThreads, 0, s, m, p, g, U, B, emax);
   else
-    hipLaunchKernelGGL(k_refine_indicator<2>, grid_of(n * S.cells()), kThreads, 0, s, m, p.ivar, p.epsilon_lohner,
-                       U, emax);
+    hipLaunchKernelGGL((k_refine_indicator<2, 0>), grid, kThreads, 0, s, m, p, g, U, B, emax);
   hipLaunchKernelGGL(k_refine_flags, grid_of(n), kThreads, 0, s, m, p, emax, fl, ind);
   AMR_HIP(c.err, hipGetLastError());
   AMR_HIP(c.err, hipMemcpyAsync(flags_host, fl, size_t(n), hipMemcpyDeviceToHost, s));
@@ -520,8 +616,8 @@ refine_flags_into(Context & c, const double * U, const AmrParams & p, int8_t * f
 }
 
 int
-launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
-                    hipStream_t s)
+launch_refine_flags(Context & c, const double * U, const double * B, const AmrParams & p, int8_t * flags_host,
+                    double * ind_host, hipStream_t s)
 {
   const long n = c.dm.nowned;
   if (n == 0)
@@ -539,7 +635,7 @@ launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t *
   auto *   emax = static_cast<unsigned long long *>(c.amr_buf);
   double * ind = reinterpret_cast<double *>(emax + n);
   auto *   fl = reinterpret_cast<int8_t *>(ind + n);
-  return refine_flags_into(c, U, p, flags_host, ind_host, s, emax, ind, fl, false);
+  return refine_flags_into(c, U, B, p, flags_host, ind_host, s, emax, ind, fl, false);
 }
 
 bool
@@ -696,6 +792,54 @@ launch_remap(int dim, int b, const RemapEntry * plan_dev, const RemapLists & L, 
   return 0;
 }
 
+// The checks of the refinement entry points, and their parameters in device form
+int
+check_amr_params(Context & c, const double * U, const double * B, const mhdct_amr_params * p, AmrParams & out)
+{
+  if (!c.has_mesh)
+  {
+    c.err = "no mesh set";
+    return -1;
+  }
+  if (c.dm.nowned > 0 && !U)
+  {
+    c.err = "null state pointer";
+    return -1;
+  }
+  const int nv = p->nvars;
+  bool      ok = p->level_min >= 0 && p->level_max >= p->level_min && p->epsilon_lohner >= 0.0 && nv >= 0 &&
+          nv <= 4 && (p->criterion == MHDCT_REFINE_LOHNER_UNSPLIT || p->criterion == MHDCT_REFINE_SIMPLE_GRADIENT);
+  bool derived = false;
+  for (int n = 0; ok && n < (nv > 0 ? nv : 1); ++n)
+  {
+    const int v = nv > 0 ? p->vars[n] : p->ivar;
+    ok = (v >= 0 && v <= 4) || (nv > 0 && v >= MHDCT_RVAR_THERMAL_PRESSURE && v <= MHDCT_RVAR_VZ);
+    derived = derived || v >= MHDCT_RVAR_THERMAL_PRESSURE;
+  }
+  if (!ok)
+  {
+    c.err = "refine flags: invalid parameters (0 <= level_min <= level_max, epsilon_lohner >= 0, criterion "
+            "LOHNER_UNSPLIT or SIMPLE_GRADIENT, 0 to 4 variables 0-4 or MHDCT_RVAR_*)";
+    return -1;
+  }
+  if (derived && c.dm.nowned > 0 && !B)
+  {
+    c.err = "refine flags: a derived quantity needs the face field B";
+    return -1;
+  }
+  out.level_min = p->level_min;
+  out.level_max = p->level_max;
+  out.epsilon_refine = p->epsilon_refine;
+  out.epsilon_coarsen = p->epsilon_coarsen;
+  out.epsilon_lohner = p->epsilon_lohner;
+  out.ivar = p->ivar;
+  out.criterion = p->criterion;
+  out.nvars = nv;
+  for (int n = 0; n < 4; ++n)
+    out.vars[n] = n < nv ? p->vars[n] : 0;
+  return 0;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Refinement flags without allocation: the per-block maxima, indicators and flags live in a
 // file-scope device array ("scratch") sized for MHDCT_NOALLOC_MAX_BLOCKS owned blocks, one copy
@@ -740,8 +884,8 @@ scratch_address(int device, std::string & err)
 } // namespace
 
 int
-launch_refine_flags_noalloc(Context & c, const double * U, const AmrParams & p, int8_t * flags_host,
-                            double * ind_host, hipStream_t s)
+launch_refine_flags_noalloc(Context & c, const double * U, const double * B, const AmrParams & p,
+                            int8_t * flags_host, double * ind_host, hipStream_t s)
 {
   const long n = c.dm.nowned;
   if (n > MHDCT_NOALLOC_MAX_BLOCKS)
@@ -765,7 +909,7 @@ launch_refine_flags_noalloc(Context & c, const double * U, const AmrParams & p, 
   auto *   emax = static_cast<unsigned long long *>(base);
   double * ind = reinterpret_cast<double *>(emax + n);
   auto *   fl = reinterpret_cast<int8_t *>(ind + n);
-  return refine_flags_into(c, U, p, flags_host, ind_host, s, emax, ind, fl, true);
+  return refine_flags_into(c, U, B, p, flags_host, ind_host, s, emax, ind, fl, true);
 }
 
 } // namespace mhdct
@@ -773,29 +917,15 @@ launch_refine_flags_noalloc(Context & c, const double * U, const AmrParams & p, 
 extern "C"
 {
   int
-  mhdct_refine_flags_noalloc(mhdct_ctx * h, const double * U, const mhdct_amr_params * p, int8_t * flags,
-                             double * indicator, void * stream)
+  mhdct_refine_flags_noalloc(mhdct_ctx * h, const double * U, const double * B, const mhdct_amr_params * p,
+                             int8_t * flags, double * indicator, void * stream)
   {
     if (!h || !p || !flags)
       return -1;
     mhdct::Context & c = h->c;
-    if (!c.has_mesh)
-    {
-      c.err = "no mesh set";
+    mhdct::AmrParams ap;
+    if (mhdct::check_amr_params(c, U, B, p, ap))
       return -1;
-    }
-    if (c.dm.nowned > 0 && !U)
-    {
-      c.err = "null state pointer";
-      return -1;
-    }
-    if (p->ivar < 0 || p->ivar > 4 || p->level_min < 0 || p->level_max < p->level_min || !(p->epsilon_lohner >= 0.0))
-    {
-      c.err = "refine flags: invalid parameters (ivar 0..4, 0 <= level_min <= level_max, epsilon_lohner >= 0)";
-      return -1;
-    }
-    const mhdct::AmrParams ap{ p->level_min, p->level_max, p->epsilon_refine, p->epsilon_coarsen, p->epsilon_lohner,
-                               p->ivar };
-    return mhdct::launch_refine_flags_noalloc(c, U, ap, flags, indicator, static_cast<hipStream_t>(stream));
+    return mhdct::launch_refine_flags_noalloc(c, U, B, ap, flags, indicator, static_cast<hipStream_t>(stream));
   }
 }

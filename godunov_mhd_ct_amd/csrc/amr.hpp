@@ -22,7 +22,10 @@ struct AmrParams
   double epsilon_refine;
   double epsilon_coarsen;
   double epsilon_lohner;
-  int    ivar; // conserved variable of U the indicator reads (0 = rho)
+  int    ivar;      // the single refinement variable when nvars == 0
+  int    criterion; // 0 LOHNER_UNSPLIT, 1 SIMPLE_GRADIENT
+  int    nvars;     // refinement variables in vars (0: ivar alone)
+  int    vars[4];   // 0-4 a conserved variable of U, 10-17 a derived quantity (mhdct.h MHDCT_RVAR_*)
 };
 
 enum
@@ -45,14 +48,16 @@ struct RemapEntry
   int nb[6] = { -1, -1, -1, -1, -1, -1 };
 };
 
+// The checks of the refinement entry points (-1 with c.err set), and the parameters in device form
+int check_amr_params(Context & c, const double * U, const double * B, const mhdct_amr_params * p, AmrParams & out);
 // Indicator and flags of every owned block into host arrays (flags: +1 refine, -1 coarsen,
 // 0 keep; ind optional).  Synchronises `s`.
-int launch_refine_flags(Context & c, const double * U, const AmrParams & p, int8_t * flags_host, double * ind_host,
-                        hipStream_t s);
+int launch_refine_flags(Context & c, const double * U, const double * B, const AmrParams & p, int8_t * flags_host,
+                        double * ind_host, hipStream_t s);
 // The same flags with no device allocation: the buffers are a file-scope device array for at
 // most MHDCT_NOALLOC_MAX_BLOCKS owned blocks (-8 beyond it); calls that overlap on one GPU race.
-int launch_refine_flags_noalloc(Context & c, const double * U, const AmrParams & p, int8_t * flags_host,
-                                double * ind_host, hipStream_t s);
+int launch_refine_flags_noalloc(Context & c, const double * U, const double * B, const AmrParams & p,
+                                int8_t * flags_host, double * ind_host, hipStream_t s);
 // The work lists of a remap, in one int array: copy pairs (old leaf, new leaf), then the new
 // leaves that are prolonged or restricted ("moved"), then the restricted ones, then the
 // prolonged ones.

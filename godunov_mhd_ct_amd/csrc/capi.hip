@@ -486,30 +486,16 @@ extern "C"
   }
 
   int
-  mhdct_refine_flags(mhdct_ctx * h, const double * U, const mhdct_amr_params * p, int8_t * flags, double * indicator,
-                     void * stream)
+  mhdct_refine_flags(mhdct_ctx * h, const double * U, const double * B, const mhdct_amr_params * p, int8_t * flags,
+                     double * indicator, void * stream)
   {
     if (!h || !p || !flags)
       return -1;
     Context & c = h->c;
-    if (!c.has_mesh)
-    {
-      c.err = "no mesh set";
+    mhdct::AmrParams ap;
+    if (mhdct::check_amr_params(c, U, B, p, ap))
       return -1;
-    }
-    if (c.dm.nowned > 0 && !U)
-    {
-      c.err = "null state pointer";
-      return -1;
-    }
-    if (p->ivar < 0 || p->ivar > 4 || p->level_min < 0 || p->level_max < p->level_min || !(p->epsilon_lohner >= 0.0))
-    {
-      c.err = "refine flags: invalid parameters (ivar 0..4, 0 <= level_min <= level_max, epsilon_lohner >= 0)";
-      return -1;
-    }
-    const mhdct::AmrParams ap{ p->level_min, p->level_max, p->epsilon_refine, p->epsilon_coarsen, p->epsilon_lohner,
-                               p->ivar };
-    return mhdct::launch_refine_flags(c, U, ap, flags, indicator, static_cast<hipStream_t>(stream));
+    return mhdct::launch_refine_flags(c, U, B, ap, flags, indicator, static_cast<hipStream_t>(stream));
   }
 
   int
@@ -773,32 +759,44 @@ extern "C"
     p.epsilon_coarsen = c.get_real("amr", "epsilon_coarsen", 0.00125);
     p.epsilon_lohner = c.get_real("amr", "epsilon_lohner", 0.02);
     const std::string crit = c.get_string("amr", "refine_criterion", "LOHNER_UNSPLIT");
-    if (crit != "LOHNER_UNSPLIT")
+    if (crit == "LOHNER_UNSPLIT")
+      p.criterion = MHDCT_REFINE_LOHNER_UNSPLIT;
+    else if (crit == "SIMPLE_GRADIENT")
+      p.criterion = MHDCT_REFINE_SIMPLE_GRADIENT;
+    else
     {
-      cfg->c.err = "[amr] refine_criterion " + crit + " is not built (LOHNER_UNSPLIT only)";
+      cfg->c.err = "[amr] refine_criterion " + crit + " is unknown (LOHNER_UNSPLIT or SIMPLE_GRADIENT)";
       return -1;
     }
     const auto names = c.get_string_vector("amr", "variable_used_to_compute_refine_criterion", { "rho" });
-    if (names.size() != 1)
+    if (names.empty() || names.size() > 4)
     {
-      cfg->c.err = "[amr] variable_used_to_compute_refine_criterion: one variable is supported";
+      cfg->c.err = "[amr] variable_used_to_compute_refine_criterion: one to four variables";
       return -1;
     }
-    // DERIVED_QUANTITY (utils/ComputeDerivedQuantities.h:44-53), matched in any case
+    // SolverGodunovMHD.cpp:1117-1167: a DERIVED_QUANTITY name (utils/ComputeDerivedQuantities.h:44-53,
+    // matched in any case) first, else a cell variable of U by MHD::m_id2names_all
+    // (models/MHD.cpp:20-32), else rho
     static const char * derived[] = { "thermal_pressure", "speed_of_sound", "specific_ekin", "magnetic_pressure",
                                       "local_mach_number", "vx", "vy", "vz" };
-    for (const char * dq : derived)
-      if (mhdct::lower(names[0]) == dq)
-      {
-        cfg->c.err = "[amr] refining on the derived quantity " + names[0] + " is not built";
-        return -1;
-      }
-    // MHD::m_id2names_all (models/MHD.cpp:20-32): the cell variables of U; other names give rho
     static const char * uvars[] = { "rho", "e_tot", "rho_vx", "rho_vy", "rho_vz" };
-    p.ivar = 0;
-    for (int v = 0; v < 5; ++v)
-      if (names[0] == uvars[v])
-        p.ivar = v;
+    p.nvars = int(names.size());
+    for (int n = 0; n < p.nvars; ++n)
+    {
+      int code = 0;
+      bool found = false;
+      for (int d = 0; d < 8; ++d)
+        if (mhdct::lower(names[n]) == derived[d])
+        {
+          code = MHDCT_RVAR_THERMAL_PRESSURE + d;
+          found = true;
+        }
+      for (int v = 0; v < 5 && !found; ++v)
+        if (names[n] == uvars[v])
+          code = v;
+      p.vars[n] = code;
+    }
+    p.ivar = p.vars[0] < 5 ? p.vars[0] : 0;
     *out = p;
     return 0;
   }

@@ -107,7 +107,7 @@ class SolverGodunovMHD:
         torch = self.torch
         if self.mesh.num_outside:
             self.implem.fill_outside(self.U, self.B)
-        flags, _ = refine_flags(self.implem, self.U, amr)
+        flags, _ = refine_flags(self.implem, self.U, amr, B=self.B)
         self.amr_cycles += 1
         if not flags.any():
             return None

@@ -73,6 +73,17 @@ def bl3d(b=16, level_min=3, level_max=6, threshold=0.1):
     return mesh, U, B, st
 
 
+def bl3d_amr_settings(level_min=3, level_max=6):
+    """The blast configuration's AMR cycle: refine where the normalised jump of the density or of
+    the thermal pressure exceeds 0.1 (SIMPLE_GRADIENT on rho and thermal_pressure), coarsen below
+    0.01, a cycle every 10 steps after the first (cycle_frequency and disable_first_amr_cycle of
+    app/test_mhd_blast_3d.ini)."""
+    from .amr import AMRSettings
+
+    return AMRSettings(level_min=level_min, level_max=level_max, epsilon_refine=0.1, epsilon_coarsen=0.01,
+                       variable=["rho", "thermal_pressure"], criterion="SIMPLE_GRADIENT", cycle_frequency=10)
+
+
 def kh3d(level=4, b=16, seed=0):
     mesh = uniform_mesh(3, b, level)
     U, B = problems.kelvin_helmholtz_3d(mesh, gamma=1.4, seed=seed)

@@ -153,17 +153,39 @@ typedef struct mhdct_amr_params
   double epsilon_refine;  /* [amr] epsilon_refine, default 0.01 */
   double epsilon_coarsen; /* [amr] epsilon_coarsen, default 0.00125 */
   double epsilon_lohner;  /* [amr] epsilon_lohner, default 0.02 */
-  int    ivar;            /* [amr] variable_used_to_compute_refine_criterion as a variable of U: 0 rho */
+  int    ivar;            /* the refinement variable when nvars == 0: a variable of U, 0 rho */
+  int    criterion;       /* [amr] refine_criterion: MHDCT_REFINE_LOHNER_UNSPLIT or MHDCT_REFINE_SIMPLE_GRADIENT */
+  int    nvars;           /* [amr] variable_used_to_compute_refine_criterion: 1-4 entries of vars (0: ivar) */
+  int    vars[4];         /* 0-4 a variable of U (rho, e_tot, rho_vx, rho_vy, rho_vz) or MHDCT_RVAR_* */
 } mhdct_amr_params;
 
-/* Replaces compute_refine_flags (SolverGodunovMHD.cpp:1094-1185) for a conserved variable: the
- * indicator of every owned block (maximum over its cells, read with one ghost layer, so the
- * outside blocks must be filled) and its flag, +1 refine (indicator > epsilon_refine and level
- * < level_max, or level < level_min), -1 coarsen (indicator < epsilon_coarsen and level >
- * level_min, or level > level_max), 0 keep, into host arrays of num_leaves entries
- * (`indicator` may be NULL).  Synchronises `stream`. */
-int mhdct_refine_flags(mhdct_ctx * ctx, const double * U, const mhdct_amr_params * params, int8_t * flags,
-                       double * indicator, void * stream);
+/* [amr] refine_criterion values (kalypsso::core's RefineIndicatorData, absent from the reference
+ * tree): Loehner's unsplit second-derivative estimate with the noise filter epsilon_lohner, or the
+ * largest normalised jump |q_n - q_c| / max(|q_n|, |q_c|) to a face neighbour. */
+#define MHDCT_REFINE_LOHNER_UNSPLIT 0
+#define MHDCT_REFINE_SIMPLE_GRADIENT 1
+/* derived refinement variables (DERIVED_QUANTITY, utils/ComputeDerivedQuantities.h:44-53), from the
+ * cell's primitives; zero in outside blocks, as the reference's derived array is only computed
+ * for the owned blocks (:104, 109-111) */
+#define MHDCT_RVAR_THERMAL_PRESSURE 10
+#define MHDCT_RVAR_SPEED_OF_SOUND 11
+#define MHDCT_RVAR_SPECIFIC_EKIN 12
+#define MHDCT_RVAR_MAGNETIC_PRESSURE 13
+#define MHDCT_RVAR_LOCAL_MACH_NUMBER 14
+#define MHDCT_RVAR_VX 15
+#define MHDCT_RVAR_VY 16
+#define MHDCT_RVAR_VZ 17
+
+/* Replaces compute_refine_flags (SolverGodunovMHD.cpp:1094-1185): the indicator of every owned
+ * block (the maximum over its cells and over the refinement variables, each read with one ghost
+ * layer, so the outside blocks must be filled) and its flag, +1 refine (indicator >
+ * epsilon_refine and level < level_max, or level < level_min), -1 coarsen (indicator <
+ * epsilon_coarsen and level > level_min, or level > level_max), 0 keep, into host arrays of
+ * num_leaves entries (`indicator` may be NULL).  With several variables a block is refined when
+ * any asks it and coarsened when all do.  B (the face field) is read for derived variables only
+ * and may be NULL otherwise.  Synchronises `stream`. */
+int mhdct_refine_flags(mhdct_ctx * ctx, const double * U, const double * B, const mhdct_amr_params * params,
+                       int8_t * flags, double * indicator, void * stream);
 /* No counterpart in the reference.  mhdct_refine_flags for callers that cannot allocate device
  * memory (an AMR cycle inside a serving loop or another allocation-free host loop): the same
  * results as mhdct_refine_flags, with its device buffers (per-block maxima, indicators, flags)
@@ -173,8 +195,8 @@ int mhdct_refine_flags(mhdct_ctx * ctx, const double * U, const mhdct_amr_params
  * on one GPU (several contexts or streams) race on it.  The context's device must be current.
  * Synchronises `stream`. */
 #define MHDCT_NOALLOC_MAX_BLOCKS 262144
-int mhdct_refine_flags_noalloc(mhdct_ctx * ctx, const double * U, const mhdct_amr_params * params, int8_t * flags,
-                               double * indicator, void * stream);
+int mhdct_refine_flags_noalloc(mhdct_ctx * ctx, const double * U, const double * B, const mhdct_amr_params * params,
+                               int8_t * flags, double * indicator, void * stream);
 /* A copy of a mesh (the leaf set before an adaptation, kept for the remap). */
 int mhdct_mesh_clone(const mhdct_mesh * mesh, mhdct_mesh ** out);
 
@@ -274,9 +296,9 @@ const char * mhdct_config_get(const mhdct_config * cfg, const char * section, co
 int mhdct_config_settings(mhdct_config * cfg, mhdct_settings * out);
 /* mhdct_amr_params from [amr] level_min and level_max (required), epsilon_refine 0.01,
  * epsilon_coarsen 0.00125, epsilon_lohner 0.02 (SolverGodunovMHD.cpp:1107-1110),
- * variable_used_to_compute_refine_criterion [rho] (:1112-1113; one variable of U; another name
- * falls back to rho as at :1149-1160; derived quantities are refused) and refine_criterion
- * LOHNER_UNSPLIT (the only one built). */
+ * variable_used_to_compute_refine_criterion [rho] (:1112-1113; one to four comma-separated names:
+ * a derived quantity in any case, else a variable of U, else rho, as at :1117-1160) and
+ * refine_criterion LOHNER_UNSPLIT (default) or SIMPLE_GRADIENT. */
 int mhdct_config_amr_params(mhdct_config * cfg, mhdct_amr_params * out);
 /* The mesh the run starts from: [run] dimension (required), [amr] bx = by (= bz) (required),
  * uniform at [amr] level_min, over [p4est_connectivity] nbrick_{x,y,z} (default 1) with

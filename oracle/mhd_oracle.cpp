@@ -2878,14 +2878,84 @@ extern "C"
 namespace oracle
 {
 
-// conserved variable v of owned block o at cell (i, j, k), one ghost layer at most
-static double
-u_cell(const Solver & S, const double * U, int v, int o, int i, int j, int k)
+// Refinement variables: 0-4 a conserved variable of U (rho, E, rho u, rho v, rho w); 10-17 a
+// derived quantity of ComputeDerivedQuantities (utils/ComputeDerivedQuantities.h:42-51, 120-228):
+// thermal pressure, speed of sound, specific kinetic energy, magnetic pressure, local Mach
+// number, vx, vy, vz, from the cell's primitives (compute_primitives of U and the cell-centred
+// face field).  The derived array is computed for the owned blocks only and is zero elsewhere
+// (a zero-initialised view, :104, 109-111), so outside blocks read as zero.
+enum
 {
-  const int  b = S.lay.b, dim = S.m.dim;
+  kDerivedFirst = 10,
+  kThermalPressure = 10,
+  kSpeedOfSound = 11,
+  kSpecificEkin = 12,
+  kMagneticPressure = 13,
+  kLocalMach = 14,
+  kVx = 15,
+  kVy = 16,
+  kVz = 17
+};
+
+// variable v of block o at its cell (i, j, k) inside [0, b)
+static double
+field_cell(const Solver & S, const double * U, const double * B, int v, int o, int i, int j, int k)
+{
+  const int  dim = S.m.dim;
   const long ncell = S.lay.cells();
+  const long ci = S.lay.cidx(i, j, k);
+  if (v < kDerivedFirst)
+    return U[(long(o) * 5 + v) * ncell + ci];
+  if (o >= S.m.nowned)
+    return 0.0;
+  double u[8], q[8];
+  for (int n = 0; n < 5; ++n)
+    u[n] = U[(long(o) * 5 + n) * ncell + ci];
+  if (dim == 2)
+  {
+    u[IA] = 0.5 * (S.face_value(B, o, IX, i, j, 0) + S.face_value(B, o, IX, i + 1, j, 0));
+    u[IB] = 0.5 * (S.face_value(B, o, IY, i, j, 0) + S.face_value(B, o, IY, i, j + 1, 0));
+    u[IC] = 0.5 * (S.face_value(B, o, IZ, i, j, 0) + S.face_value(B, o, IZ, i, j, 0));
+  }
+  else
+  {
+    u[IA] = 0.5 * (S.face_value(B, o, IX, i, j, k) + S.face_value(B, o, IX, i + 1, j, k));
+    u[IB] = 0.5 * (S.face_value(B, o, IY, i, j, k) + S.face_value(B, o, IY, i, j + 1, k));
+    u[IC] = 0.5 * (S.face_value(B, o, IZ, i, j, k) + S.face_value(B, o, IZ, i, j, k + 1));
+  }
+  compute_primitives(u, q, S.s);
+  switch (v)
+  {
+  case kThermalPressure: return q[IP];
+  case kSpeedOfSound: return std::sqrt(S.s.gamma0 * q[IP] / q[ID]);
+  case kSpecificEkin:
+    return dim == 2 ? 0.5 * (q[IU] * q[IU] + q[IV] * q[IV]) : 0.5 * (q[IU] * q[IU] + q[IV] * q[IV] + q[IW] * q[IW]);
+  case kMagneticPressure: return 0.5 * (q[IA] * q[IA] + q[IB] * q[IB] + q[IC] * q[IC]);
+  case kLocalMach:
+  {
+    double un = q[IU] * q[IU] + q[IV] * q[IV];
+    if (dim == 3)
+      un += q[IW] * q[IW];
+    un = std::sqrt(un);
+    const double cs = std::sqrt(S.s.gamma0 * q[IP] / q[ID]);
+    return un / cs;
+  }
+  case kVx: return q[IU];
+  case kVy: return q[IV];
+  case kVz: return dim == 3 ? q[IW] : 0.0;
+  default: throw std::runtime_error("refine indicator: unknown variable");
+  }
+}
+
+// variable v of owned block o at cell (i, j, k), one ghost layer at most: a neighbour's cell
+// (same level), the mean of its 2^dim cells (finer; x fastest, then y, then z) or its
+// containing cell (coarser)
+static double
+u_cell(const Solver & S, const double * U, const double * B, int v, int o, int i, int j, int k)
+{
+  const int b = S.lay.b, dim = S.m.dim;
   if (i >= 0 && i < b && j >= 0 && j < b && (dim == 2 || (k >= 0 && k < b)))
-    return U[(long(o) * 5 + v) * ncell + S.lay.cidx(i, j, k)];
+    return field_cell(S, U, B, v, o, i, j, k);
   const Loc l = S.m.locate(o, i, j, k);
   if (l.oct < 0)
     throw std::runtime_error("refine indicator: missing neighbour");
@@ -2895,17 +2965,39 @@ u_cell(const Solver & S, const double * U, int v, int o, int i, int j, int k)
     for (int dk = 0; dk < (dim == 3 ? 2 : 1); ++dk)
       for (int dj = 0; dj < 2; ++dj)
         for (int di = 0; di < 2; ++di)
-          sum += U[(long(l.oct) * 5 + v) * ncell + S.lay.cidx(l.ijk[0] + di, l.ijk[1] + dj, l.ijk[2] + dk)];
+          sum += field_cell(S, U, B, v, l.oct, l.ijk[0] + di, l.ijk[1] + dj, l.ijk[2] + dk);
     return sum / double(1 << dim);
   }
-  return U[(long(l.oct) * 5 + v) * ncell + S.lay.cidx(l.ijk[0], l.ijk[1], l.ijk[2])];
+  return field_cell(S, U, B, v, l.oct, l.ijk[0], l.ijk[1], l.ijk[2]);
+}
+
+// SIMPLE_GRADIENT (the [amr] refine_criterion of kalypsso::core's ComputeRefineFlags, absent from
+// the reference tree; restated): the largest normalised jump between the cell and its 2 dim face
+// neighbours, |q_n - q_c| / max(|q_n|, |q_c|), zero where both vanish; directions in order x, y,
+// z, the low neighbour before the high one
+static double
+gradient_at(const Solver & S, const double * U, const double * B, int v, int o, int i, int j, int k)
+{
+  const double u0 = u_cell(S, U, B, v, o, i, j, k);
+  double       e = 0.0;
+  for (int a = 0; a < S.m.dim; ++a)
+    for (int sg = -1; sg <= 1; sg += 2)
+    {
+      int d[3] = { 0, 0, 0 };
+      d[a] = sg;
+      const double un = u_cell(S, U, B, v, o, i + d[0], j + d[1], k + d[2]);
+      const double m = std::fmax(std::fabs(un), std::fabs(u0));
+      const double x = m > 0.0 ? std::fabs(un - u0) / m : 0.0;
+      e = std::fmax(e, x);
+    }
+  return e;
 }
 
 static double
-lohner_at(const Solver & S, const double * U, int v, int o, int i, int j, int k, double eps)
+lohner_at(const Solver & S, const double * U, const double * B, int v, int o, int i, int j, int k, double eps)
 {
   const int dim = S.m.dim;
-  auto u = [&](const int d[3]) { return u_cell(S, U, v, o, i + d[0], j + d[1], k + d[2]); };
+  auto u = [&](const int d[3]) { return u_cell(S, U, B, v, o, i + d[0], j + d[1], k + d[2]); };
   double num = 0, den = 0;
   for (int a = 0; a < dim; ++a)
     for (int c = 0; c < dim; ++c)
@@ -3063,8 +3155,14 @@ extern "C"
 {
   // per owned block: the maximum over its cells of Loehner's estimate of U variable v (and,
   // when ecell is not null, every cell's estimate, block by block)
+  // Block indicators of compute_refine_flags (SolverGodunovMHD.cpp:1094-1185) over nvar refinement
+  // variables (codes above): criterion 0 LOHNER_UNSPLIT (noise filter eps), 1 SIMPLE_GRADIENT; the
+  // maximum over the block's cells and the variables (the flags of several variables then follow
+  // from that maximum: refine when any variable asks it, coarsen when all do).  ecell (optional):
+  // the per-cell maximum over the variables.
   int
-  oracle_refine_indicator(void * h, const double * U, int v, double eps, double * emax, double * ecell)
+  oracle_refine_indicator_vars(void * h, const double * U, const double * B, int nvar, const int * vars,
+                               int criterion, double eps, double * emax, double * ecell)
   {
     auto *     S = static_cast<oracle::Solver *>(h);
     const int  b = S->lay.b, dim = S->m.dim;
@@ -3077,10 +3175,16 @@ extern "C"
         for (long ci = 0; ci < ncell; ++ci)
         {
           const int i = int(ci % b), j = int((ci / b) % b), k = dim == 3 ? int(ci / (long(b) * b)) : 0;
-          const double x = oracle::lohner_at(*S, U, v, o, i, j, k, eps);
-          e = std::fmax(e, (x >= 0.0) ? x : HUGE_VAL);
+          double    xc = 0.0;
+          for (int n = 0; n < nvar; ++n)
+          {
+            const double x = criterion == 1 ? oracle::gradient_at(*S, U, B, vars[n], o, i, j, k)
+                                            : oracle::lohner_at(*S, U, B, vars[n], o, i, j, k, eps);
+            xc = std::fmax(xc, (x >= 0.0) ? x : HUGE_VAL);
+          }
+          e = std::fmax(e, xc);
           if (ecell)
-            ecell[long(o) * ncell + ci] = x;
+            ecell[long(o) * ncell + ci] = xc;
         }
         emax[o] = e;
       }
@@ -3091,6 +3195,12 @@ extern "C"
       return -1;
     }
     return 0;
+  }
+
+  int
+  oracle_refine_indicator(void * h, const double * U, int v, double eps, double * emax, double * ecell)
+  {
+    return oracle_refine_indicator_vars(h, U, nullptr, 1, &v, 0, eps, emax, ecell);
   }
 
   // flags from the indicator: +1 refine, -1 coarsen, 0 keep

@@ -58,6 +58,9 @@ def lib():
         L.oracle_refine_indicator.restype = C.c_int
         L.oracle_refine_indicator.argtypes = [C.c_void_p, P(C.c_double), C.c_int, C.c_double, P(C.c_double),
                                               P(C.c_double)]
+        L.oracle_refine_indicator_vars.restype = C.c_int
+        L.oracle_refine_indicator_vars.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_int, P(C.c_int),
+                                                   C.c_int, C.c_double, P(C.c_double), P(C.c_double)]
         L.oracle_refine_flags.argtypes = [C.c_int, P(C.c_int), P(C.c_double), C.c_int, C.c_int, C.c_double,
                                           C.c_double, P(C.c_int8)]
         L.oracle_remap.restype = C.c_int
@@ -176,6 +179,18 @@ class OracleSolver:
         ecell = np.zeros((max(self.nowned, 1), self.b ** self.dim)) if per_cell else None
         rc = lib().oracle_refine_indicator(self._h, dptr(U), int(ivar), float(eps), dptr(emax),
                                            dptr(ecell) if per_cell else None)
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error(self._h).decode())
+        return (emax[:self.nowned], ecell[:self.nowned]) if per_cell else emax[:self.nowned]
+
+    def refine_indicator_vars(self, U, B, variables, criterion=0, eps=0.02, per_cell=False):
+        """Per owned block the maximum over its cells and the refinement variables (codes 0-4 of
+        U, 10-17 derived) of the criterion (0 LOHNER_UNSPLIT, 1 SIMPLE_GRADIENT)."""
+        emax = np.zeros(max(self.nowned, 1))
+        ecell = np.zeros((max(self.nowned, 1), self.b ** self.dim)) if per_cell else None
+        vp, keep = iptr(list(variables))
+        rc = lib().oracle_refine_indicator_vars(self._h, dptr(U), dptr(B), len(variables), vp, int(criterion),
+                                                float(eps), dptr(emax), dptr(ecell) if per_cell else None)
         if rc != 0:
             raise RuntimeError(lib().oracle_last_error(self._h).decode())
         return (emax[:self.nowned], ecell[:self.nowned]) if per_cell else emax[:self.nowned]

@@ -124,14 +124,26 @@ def test_malformed_lines_name_their_line():
     (["[run]", "time_integrator=RK3_SSP"], "settings", "time_integrator"),
     (["[amr]", "by=16"], "mesh", "equal"),
     (["[mesh]", "boundary_type_xmin=CUSTOM"], "mesh", "boundary_type_xmin"),
-    (["[amr]", "variable_used_to_compute_refine_criterion=specific_ekin"], "amr", "derived"),
-    (["[amr]", "variable_used_to_compute_refine_criterion=rho,e_tot"], "amr", "one variable"),
-    (["[amr]", "refine_criterion=SIMPLE_GRADIENT"], "amr", "LOHNER_UNSPLIT"),
+    (["[amr]", "variable_used_to_compute_refine_criterion=rho,e_tot,vx,vy,vz"], "amr", "one to four"),
+    (["[amr]", "refine_criterion=QUADTREE_MAGIC"], "amr", "SIMPLE_GRADIENT"),
 ])
 def test_refusals(extra, what, match):
     cfg = Config.parse(text(*BASE, *extra))
     with pytest.raises(MhdctError, match=match):
         getattr(cfg, what)()
+
+
+def test_refine_criterion_and_derived_variables():
+    """refine_criterion SIMPLE_GRADIENT; several variables, derived quantities matched in any case
+    (SolverGodunovMHD.cpp:1117-1160), a name that is neither falls back to rho."""
+    cfg = Config.parse(text(*BASE, "[amr]", "refine_criterion=SIMPLE_GRADIENT",
+                            "variable_used_to_compute_refine_criterion=rho, Thermal_Pressure,magnetic_pressure, bogus"))
+    amr = cfg.amr()
+    assert amr.criterion == "SIMPLE_GRADIENT"
+    assert amr.variable == ["rho", "thermal_pressure", "magnetic_pressure", "rho"]
+    p = amr.params()
+    assert p.criterion == 1 and p.nvars == 4 and list(p.vars) == [0, 10, 13, 0]
+    assert Config.parse(text(*BASE)).amr().criterion == "LOHNER_UNSPLIT"
 
 
 def test_refine_variable_names():

@@ -104,7 +104,8 @@ def test_remap_matches_oracle(name, mk):
 def oracle_cycle(m, Uo, Bo, amr, st, ora):
     if m.num_outside:
         ora.fill_outside(Uo, Bo)
-    emax = ora.refine_indicator(Uo, ivar=0, eps=amr.epsilon_lohner)
+    p = amr.params()
+    emax = ora.refine_indicator_vars(Uo, Bo, list(p.vars)[:p.nvars], criterion=p.criterion, eps=amr.epsilon_lohner)
     lv, _ = m.leaves()
     flags = oracle_flags(lv, emax, amr.level_min, amr.level_max, amr.epsilon_refine, amr.epsilon_coarsen)
     if not flags.any():
@@ -214,3 +215,113 @@ def test_amr_cycles_with_walls_and_gravity(impl):
         Ug, Bg = sol.state()
         compare(Ug, Bg, Uo, Bo, om.num_leaves, f"iteration {it}")
     assert len(wall_levels) >= 2, "no refinement reached a wall"
+
+
+def varied_state(m):
+    """A state in which every refinement variable varies: 3D the random one; 2D Orszag-Tang (flow
+    and a varying field) with a density bump (momenta scaled with it, so velocities and pressure
+    are kept)."""
+    if m.dim == 3:
+        return state(m, 4)
+    U, B = problems.orszag_tang_2d(m)
+    U, B = U.reshape(m.num_total, 5, -1).copy(), B.reshape(m.num_total, -1).copy()
+    xs = problems._grid(m, 1.0, (0.0, 0.0, 0.0), "cell")
+    f = 1.0 + 0.5 * np.exp(-((xs[0] - 0.45) ** 2 + (xs[1] - 0.5) ** 2) / 0.02)
+    ek = 0.5 * (U[:, 2] ** 2 + U[:, 3] ** 2 + U[:, 4] ** 2) / U[:, 0]
+    U[:, 0] *= f
+    U[:, 2:5] *= f[:, None, :]
+    U[:, 1] += (f - 1.0) * ek
+    return U, B
+
+
+CRITERIA_CASES = [
+    ("SIMPLE_GRADIENT", ["rho"]),
+    ("SIMPLE_GRADIENT", ["rho", "thermal_pressure"]),
+    ("LOHNER_UNSPLIT", ["magnetic_pressure", "local_mach_number"]),
+    ("SIMPLE_GRADIENT", ["speed_of_sound", "specific_ekin", "vx", "vz"]),
+    ("LOHNER_UNSPLIT", ["e_tot", "vy"]),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("crit,variables", CRITERIA_CASES)
+@pytest.mark.parametrize("name,mk", CASES)
+def test_refine_criteria_and_derived_variables_match_oracle(name, mk, crit, variables):
+    """SIMPLE_GRADIENT and LOHNER_UNSPLIT over conserved and derived variables (several at once):
+    the device indicators equal the oracle's bit for bit, and so do the flags."""
+    print(f"[amr] {crit} {variables} {name}", flush=True)
+    m = mk()
+    U, B = varied_state(m)
+    amr = AMRSettings(level_min=1, level_max=3, epsilon_refine=0.1, epsilon_coarsen=0.01, epsilon_lohner=0.02,
+                      variable=variables, criterion=crit)
+    im = GodunovImplemHIP(hydro_settings(gamma0=GAMMA), m.dim, m.b, IMPL_PLAIN)
+    im.resize_auxiliary_data(m)
+    flags, ind = refine_flags(im, dev(U), amr, B=dev(B))
+    p = amr.params()
+    ora = OracleSolver(m, settings_array(gamma0=GAMMA))
+    emax = ora.refine_indicator_vars(U, B, list(p.vars)[:p.nvars], criterion=p.criterion, eps=amr.epsilon_lohner)
+    assert np.array_equal(ind, emax), np.abs(ind - emax).max()
+    lv, _ = m.leaves()
+    want = oracle_flags(lv, emax, amr.level_min, amr.level_max, amr.epsilon_refine, amr.epsilon_coarsen)
+    assert np.array_equal(flags, want)
+    assert ind.max() > ind.min()
+
+
+@pytest.mark.gpu
+def test_derived_variable_with_walls_matches_oracle():
+    """Outside blocks read as zero for a derived quantity (the reference fills its derived array
+    for the owned blocks only), as in the oracle."""
+    bc = [2, 3, 2, 2, 0, 0]
+    m = Mesh(2, 8, 2, periodic=(0, 0, 1))
+    m.set_bc(bc)
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: x0 < 0.3 and y0 < 0.3, 3)
+    U, B = state(m, 5)
+    amr = AMRSettings(level_min=1, level_max=4, epsilon_refine=0.2, variable=["thermal_pressure", "rho"],
+                      criterion="SIMPLE_GRADIENT")
+    im = GodunovImplemHIP(hydro_settings(gamma0=GAMMA), m.dim, m.b, IMPL_PLAIN)
+    im.resize_auxiliary_data(m)
+    Ud, Bd = dev(U), dev(B)
+    im.fill_outside(Ud, Bd)
+    flags, ind = refine_flags(im, Ud, amr, B=Bd)
+    ora = OracleSolver(m, settings_array(gamma0=GAMMA), bc=bc)
+    ora.fill_outside(U, B)
+    emax = ora.refine_indicator_vars(U, B, [10, 0], criterion=1)
+    assert np.array_equal(ind, emax)
+    assert (ind == 1.0).any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl", [IMPL_PLAIN, IMPL_FUSED])
+def test_solver_cycles_on_density_and_pressure_gradient(impl):
+    """The blast configuration's criterion: SIMPLE_GRADIENT on rho and the thermal pressure,
+    normalised jump > 0.1, cycles every two steps, against the oracle driven through the same
+    cycles (1e-12 per field, equal dt)."""
+    import torch
+    from godunov_mhd_ct_amd.solver import SolverGodunovMHD
+
+    m = Mesh(3, 8, 1).finalize()
+    U, B = problems.blast(m, radius=0.3, center=(0.5, 0.5, 0.5), p_in=10.0, p_out=0.1, bfield=(0.4, 0.3, 0.2))
+    amr = AMRSettings(level_min=1, level_max=2, epsilon_refine=0.1, epsilon_coarsen=0.01, cycle_frequency=2,
+                      variable=["rho", "thermal_pressure"], criterion="SIMPLE_GRADIENT")
+    print(f"[amr] solver impl={impl} SIMPLE_GRADIENT on rho and p", flush=True)
+    st = hydro_settings(gamma0=GAMMA, cfl=0.8, slope_type=1.0)
+    sol = SolverGodunovMHD(m, U, B, st, impl_version=impl)
+    sa = settings_array(gamma0=GAMMA, cfl=0.8, slope_type=1.0)
+    ora = OracleSolver(m, sa, nthreads=4)
+    om, Uo, Bo = m, U.reshape(m.num_total, 5, -1).copy(), B.reshape(m.num_total, -1).copy()
+    changed = 0
+    for it in range(4):
+        if sol.should_do_amr_cycle(amr):
+            counts = sol.do_amr_cycle(amr)
+            om, Uo, Bo, ora = oracle_cycle(om, Uo, Bo, amr, sa, ora)
+            assert np.array_equal(sol.mesh.leaves()[0], om.leaves()[0])
+            changed += counts is not None
+        dto = ora.compute_dt(Uo, Bo)
+        dtg = sol.compute_dt()
+        assert abs(dtg - dto) <= 1e-14 * dto, (dtg, dto)
+        Uo, Bo, _ = ora.step(Uo, Bo, dt=dto)
+        sol.step(dt=dto)
+        torch.cuda.synchronize()
+        Ug, Bg = sol.state()
+        compare(Ug, Bg, Uo, Bo, om.num_leaves, f"iteration {it}")
+    assert changed >= 1, "the cycles did not change the mesh"

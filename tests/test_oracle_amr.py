@@ -280,3 +280,104 @@ def test_remap_plan_counts():
     twice = adapted(fine, np.where(lv[:fine.num_leaves] == 3, 1, 0).astype(np.int8))
     with pytest.raises(Exception, match="one level"):
         Remap(m, twice)
+
+
+# ---------------------------------------------------------------------------------------------
+# SIMPLE_GRADIENT and derived refinement variables (SolverGodunovMHD.cpp:1094-1185,
+# utils/ComputeDerivedQuantities.h:63-265)
+# ---------------------------------------------------------------------------------------------
+def _uniform_field_state(m, rho, E=None, B=None):
+    n = m.num_total
+    U = np.zeros((n, 5, m.cells_per_block))
+    U[:, 0] = rho
+    U[:, 1] = 2.0 if E is None else E
+    Bf = np.zeros((n, m.faces_per_block)) if B is None else B
+    return U, Bf
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_simple_gradient_brute_force_on_uniform_mesh(dim):
+    """On a uniform periodic mesh the per-cell indicator is the largest |q_n - q_c| /
+    max(|q_n|, |q_c|) over the 2 dim face neighbours of the global grid (np.roll wraps)."""
+    m = uniform_mesh(dim, 8, 1)
+    U, B = problems.blast(m, radius=0.3, center=(0.4, 0.55, 0.45), rho_in=2.0)
+    U = U.reshape(m.num_total, 5, -1)
+    B = B.reshape(m.num_total, -1)
+    ora = OracleSolver(m, settings_array(gamma0=1.4))
+    _, ec = ora.refine_indicator_vars(U, B, [0], criterion=1, per_cell=True)
+    # the global grid of rho (16^dim), from the blocks' coordinates
+    b, n = m.b, 2 * m.b
+    lv, co = m.leaves()
+    g = np.zeros((n,) * dim)
+    for o in range(m.num_leaves):
+        blk = U[o, 0].reshape((b,) * dim)  # x fastest: [z][y][x]
+        sl = tuple(slice(co[o, d] * b, co[o, d] * b + b) for d in reversed(range(dim)))
+        g[sl] = blk
+    want = np.zeros_like(g)
+    for ax in range(dim):
+        for sh in (1, -1):
+            nb = np.roll(g, sh, axis=ax)
+            mx = np.maximum(np.abs(nb), np.abs(g))
+            want = np.maximum(want, np.where(mx > 0, np.abs(nb - g) / np.where(mx > 0, mx, 1), 0))
+    for o in range(m.num_leaves):
+        sl = tuple(slice(co[o, d] * b, co[o, d] * b + b) for d in reversed(range(dim)))
+        assert np.array_equal(ec[o], want[sl].reshape(-1))
+    assert ec.max() == 0.5  # the step from 1 to 2: (2 - 1) / 2
+
+
+def test_simple_gradient_of_a_step_across_levels():
+    """A density step of 1 -> 4 on the refinement boundary: the indicator is 3/4 on both sides
+    (ghost values by restriction and by the containing coarse cell), zero away from it."""
+    m = amr2d(8)
+    lv, co = m.leaves()
+    xs = problems._grid(m, 1.0, (0.0, 0.0, 0.0), "cell")
+    level = np.repeat(lv[:, None], m.cells_per_block, axis=1)
+    rho = np.where(level == lv.max(), 4.0, 1.0)
+    U, B = _uniform_field_state(m, rho)
+    ora = OracleSolver(m, settings_array(gamma0=1.4))
+    emax = ora.refine_indicator_vars(U, B, [0], criterion=1)
+    n = m.num_leaves
+    fine = lv[:n] == lv.max()
+    assert set(np.round(emax, 15)) <= {0.0, 0.75}
+    assert (emax[fine] == 0.75).any() and (emax[~fine] == 0.75).any() and (emax == 0.0).any()
+
+
+@pytest.mark.parametrize("criterion", [0, 1])
+def test_derived_pressure_follows_total_energy_at_rest(criterion):
+    """At rest and without field, the thermal pressure is (gamma - 1) E: both criteria are
+    invariant under scaling, so the indicator on thermal_pressure equals the one on e_tot; the
+    speed of sound sqrt(gamma p / rho) at uniform rho is a monotone function of it; velocities
+    are zero everywhere."""
+    m = amr3d(8)
+    xs = problems._grid(m, 1.0, (0.0, 0.0, 0.0), "cell")
+    E = 2.0 + np.exp(-20 * ((xs[0] - 0.4) ** 2 + (xs[1] - 0.5) ** 2 + (xs[2] - 0.45) ** 2))
+    U, B = _uniform_field_state(m, 1.0, E=E)
+    ora = OracleSolver(m, settings_array(gamma0=1.4))
+    e_tot = ora.refine_indicator_vars(U, B, [1], criterion=criterion)
+    p = ora.refine_indicator_vars(U, B, [10], criterion=criterion)
+    np.testing.assert_allclose(p, e_tot, rtol=1e-12, atol=1e-15)
+    for v in (15, 16, 17, 12, 13):  # vx, vy, vz, specific kinetic energy, magnetic pressure: all zero
+        assert not ora.refine_indicator_vars(U, B, [v], criterion=criterion).any()
+    both = ora.refine_indicator_vars(U, B, [0, 10], criterion=criterion)
+    rho_only = ora.refine_indicator_vars(U, B, [0], criterion=criterion)
+    assert np.array_equal(both, np.maximum(p, rho_only))
+
+
+def test_derived_quantity_reads_zero_in_outside_blocks():
+    """The reference computes derived quantities for the owned blocks only, into a zero-filled
+    array: with walls, a uniform state's thermal pressure jumps to zero across the border (the
+    indicator of every border block is 1), while rho, read from the filled outside blocks, stays
+    flat."""
+    m = Mesh(2, 8, 2, periodic=(0, 0, 1))
+    m.set_bc([2, 2, 2, 2, 0, 0])
+    m.finalize()
+    U, B = _uniform_field_state(m, 1.0)
+    ora = OracleSolver(m, settings_array(gamma0=1.4), bc=[2, 2, 2, 2, 0, 0])
+    ora.fill_outside(U, B)
+    e_p = ora.refine_indicator_vars(U, B, [10], criterion=1)
+    e_rho = ora.refine_indicator_vars(U, B, [0], criterion=1)
+    lv, co = m.leaves()
+    n = m.num_leaves
+    border = (co[:n, 0] == 0) | (co[:n, 0] == 3) | (co[:n, 1] == 0) | (co[:n, 1] == 3)
+    assert np.all(e_p[border] == 1.0) and np.all(e_p[~border] == 0.0)
+    assert not e_rho.any()

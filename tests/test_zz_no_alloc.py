@@ -20,7 +20,7 @@ def _flags(fn, implem, U, amr):
     flags = np.full(n, 7, dtype=np.int8)
     ind = np.full(n, np.nan)
     p = amr.params()
-    rc = fn(implem._h, _ptr(U), C.byref(p), flags.ctypes.data_as(C.POINTER(C.c_int8)),
+    rc = fn(implem._h, _ptr(U), None, C.byref(p), flags.ctypes.data_as(C.POINTER(C.c_int8)),
             ind.ctypes.data_as(C.POINTER(C.c_double)), C.c_void_p(0))
     assert rc == 0, lib().mhdct_last_error(implem._h)
     return flags, ind
