@@ -33,6 +33,7 @@
 #include "common.hpp"
 #include "ctx.hpp"
 #include "fused.hpp"
+#include "phys_fast.hpp"
 
 namespace mhdct
 {
@@ -722,7 +723,7 @@ face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
     x = qL[kA]; qL[kA] = qL[bt]; qL[bt] = x;
     x = qR[kA]; qR[kA] = qR[bt]; qR[bt] = x;
   }
-  hlld(qL, qR, f, g);
+  hlld_fast(qL, qR, f, g);
   if (DIR != 0)
   {
     constexpr int vt = kU + DIR, bt = kA + DIR;
@@ -1075,7 +1076,7 @@ edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx
     edge_state<T, DIM, BS, 2, 1, 2, HK>(V, i, j - 1, k, qe[kRB]);
     edge_state<T, DIM, BS, 3, 1, 2, HK>(V, i, j, k - 1, qe[kLT]);
   }
-  const T       E = edge_emf_ax<AX>(qe, S.gt) * dtdx;
+  const T       E = edge_emf_fast<AX>(qe, S.gt) * dtdx;
   constexpr int comp = DIM == 3 ? AX : 0;
   S.emf[(long(o) * G::NCE + comp) * G::NEC + IdxC<DIM, BS>::edge(i, j, k)] = E;
 }

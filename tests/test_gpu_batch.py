@@ -13,6 +13,7 @@ from godunov_mhd_ct_amd import problems
 from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32, hydro_settings
 from godunov_mhd_ct_amd.mesh import Mesh, refine_where, uniform_mesh
 from oracle_ref import OracleSolver, settings_array
+from parity import assert_parity
 
 TOL = 1e-12
 
@@ -88,12 +89,7 @@ def test_batch_matches_oracle_per_slot():
         (Ug, Bg), tg, steps = res[n]
         assert steps == nsteps
         assert abs(tg - t) <= 1e-14 * t, f"slot {n}: time {tg!r} vs oracle {t!r}"
-        no = m.num_leaves
-        for v in range(5):
-            e = rel_linf(Ug[:no, v], Uo[:no, v])
-            assert e <= TOL, f"slot {n} var {v}: {e}"
-        e = rel_linf(Bg[:no], Bo[:no])
-        assert e <= TOL, f"slot {n} B: {e}"
+        assert_parity(Ug, Bg, Uo, Bo, m.num_leaves, f"slot {n}")
 
 
 @pytest.mark.gpu
@@ -154,9 +150,7 @@ def test_batch_2d_mixed_block_sizes():
         for _ in range(nsteps):
             Uo, Bo, _ = ora.step(Uo, Bo)
         Ug, Bg = bt.state(sid)
-        no = m.num_leaves
-        assert rel_linf(Ug[:no], Uo[:no]) <= TOL
-        assert rel_linf(Bg[:no], Bo[:no]) <= TOL
+        assert_parity(Ug, Bg, Uo, Bo, m.num_leaves, f"b={m.b}")
 
 
 # FP32 bulk: slopes, predictor, HLLD and EMF in FP32; primitive recovery, accumulation and dt in
@@ -241,6 +235,4 @@ def test_batch_bounded_meshes_vs_oracle():
         for _ in range(nsteps):
             Uo, Bo, _ = ora.step(Uo, Bo)
         Ug, Bg = bt.state(sid)
-        no = m.num_leaves
-        assert rel_linf(Ug[:no], Uo[:no]) <= TOL, f"bc {bc}"
-        assert rel_linf(Bg[:no], Bo[:no]) <= TOL, f"bc {bc}"
+        assert_parity(Ug, Bg, Uo, Bo, m.num_leaves, f"bc {bc}")

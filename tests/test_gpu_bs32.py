@@ -7,8 +7,8 @@ mixed-block benchmark mesh.  These tests run 32-cell blocks on two-level meshes:
 * alone, through the per-mesh solver, in the plain and fused FP64 forms, 2D and 3D;
 * inside batches of 8/16/32 blocks (3D) and 8/32 blocks (2D), per slot, with device dt.
 
-Bar: relative L-inf <= 1e-12 per field (rho, E, each momentum component, face B) after every
-step of the per-mesh runs and after the last step of the batch runs.
+Bar (tests/parity.py): relative L-inf <= 1e-12 on rho, E, momentum and face B after every step
+of the per-mesh runs and after the last step of the batch runs.
 """
 import numpy as np
 import pytest
@@ -17,6 +17,7 @@ from godunov_mhd_ct_amd import problems
 from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_PLAIN, hydro_settings
 from godunov_mhd_ct_amd.mesh import Mesh, refine_where, uniform_mesh
 from oracle_ref import OracleSolver, settings_array
+from parity import assert_parity
 from test_gpu_parity import run_pair
 
 TOL = 1e-12
@@ -87,11 +88,7 @@ def _batch_vs_oracle(dim, cases, nsteps):
         Ug, Bg = bt.state(sid)
         _, tg, steps = bt.slot_state(sid)
         assert steps == nsteps and abs(tg - t) <= 1e-14 * t, f"slot {n} (b={m.b}): time {tg!r} vs {t!r}"
-        no = m.num_leaves
-        errs = {f"U{v}": rel_linf(Ug[:no, v], Uo[:no, v]) for v in range(5)}
-        errs["B"] = rel_linf(Bg[:no], Bo[:no])
-        bad = {k: e for k, e in errs.items() if not e <= TOL}
-        assert not bad, f"slot {n} (b={m.b}): {bad}"
+        assert_parity(Ug, Bg, Uo, Bo, m.num_leaves, f"slot {n} (b={m.b})")
 
 
 @pytest.mark.gpu

@@ -11,8 +11,10 @@ import pytest
 from godunov_mhd_ct_amd import problems
 from godunov_mhd_ct_amd.implem import IMPL_FUSED, IMPL_FUSED_FP32, IMPL_PLAIN, GodunovImplemHIP, hydro_settings
 from godunov_mhd_ct_amd.mesh import Mesh, refine_where, uniform_mesh
+from parity import assert_parity
 from oracle_ref import OracleSolver, settings_array
-from test_gpu_parity import TOL, amr_mesh_2d, amr_mesh_3d, rel_linf, run_pair
+from parity import TOL, rel_linf
+from test_gpu_parity import amr_mesh_2d, amr_mesh_3d, run_pair
 
 IMPLS = [IMPL_PLAIN, IMPL_FUSED]
 RK2 = "RK2_SSP"
@@ -163,10 +165,7 @@ def test_rk2_batch_matches_oracle_per_slot():
             t += dt
         (Ug, Bg), (cur, tg, steps) = res[n]
         assert steps == nsteps and abs(tg - t) <= 1e-14 * t, (n, tg, t)
-        no = m.num_leaves
-        for v in range(5):
-            assert rel_linf(Ug[:no, v], Uo[:no, v]) <= TOL, (n, v)
-        assert rel_linf(Bg[:no], Bo[:no]) <= TOL, n
+        assert_parity(Ug, Bg, Uo, Bo, m.num_leaves, f"slot {n}")
 
 
 @pytest.mark.gpu

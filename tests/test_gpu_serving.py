@@ -46,6 +46,7 @@ def test_served_requests_match_oracle():
     after its full 200 steps (each with its own CFL dt, reduced on the device) against the
     oracle's 200 steps of the same request, relative L-inf <= 1e-12 per field."""
     from godunov_mhd_ct_amd.serving import Request, Server, prepare
+    from parity import assert_parity
 
     reqs = [Request(0, 0.000, "loop2d", 64, 8, 1.0, 200, seed=1),
             Request(1, 0.002, "blast2d", 64, 16, 0.3, 200, seed=2),
@@ -58,10 +59,5 @@ def test_served_requests_match_oracle():
         mesh, U, B, _ = prepare(r)
         Uo, Bo = _oracle_final(mesh, U, B, r.nsteps)
         Us, Bs = r.result
-        no = mesh.num_leaves
-        errs = {f"U{v}": float(np.abs(Us[:no, v] - Uo[:no, v]).max() / np.abs(Uo[:no, v]).max())
-                for v in range(5) if np.abs(Uo[:no, v]).max() > 0}
-        errs["B"] = float(np.abs(Bs[:no] - Bo[:no]).max() / np.abs(Bo[:no]).max())
+        errs, _, _ = assert_parity(Us, Bs, Uo, Bo, mesh.num_leaves, f"request {r.rid} ({r.kind})")
         print(f"  request {r.rid} {r.kind}: max rel L-inf {max(errs.values()):.3e}", flush=True)
-        bad = {k: e for k, e in errs.items() if not e <= 1e-12}
-        assert not bad, f"request {r.rid} ({r.kind}): {bad}"
