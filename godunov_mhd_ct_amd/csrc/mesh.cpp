@@ -253,15 +253,69 @@ Mesh::balance()
   rebuild_hash();
 }
 
+// Tree numbers of a brick of nbrick[0] x nbrick[1] (x nbrick[2]) trees, indexed by the tree's
+// position tx + nbrick[0] (ty + nbrick[1] tz): the order in which p4est's brick connectivity
+// numbers its trees (p4est_connectivity_new_brick / p8est_connectivity_new_brick of p4est, the
+// reference's mesh library, CMakeLists.txt:77, version not pinned; not in the reference tree).
+// Linear indices ti = 0, 1, ... of the enclosing power-of-two brick are mapped to positions by
+// interleaving bits: the axes are ranked by their bit count log_d = ceil(log2 nbrick[d]) (fewest
+// first, ties in axis order); while every remaining axis still has bits, one bit per axis is
+// taken in turn (lowest-ranked remaining axes first in axis order), then the axis with fewer
+// bits drops out; positions outside the brick are skipped and the rest numbered in order.
+std::vector<int>
+brick_tree_numbers(int dim, const int nbrick[3])
+{
+  int logx[3] = { 0, 0, 0 };
+  for (int d = 0; d < dim; ++d)
+    while ((1 << logx[d]) < nbrick[d])
+      ++logx[d];
+  int rank[3] = { 0, 1, 2 };
+  std::stable_sort(rank, rank + dim, [&](int a, int b) { return logx[a] < logx[b]; });
+  int total = 0;
+  for (int d = 0; d < dim; ++d)
+    total += logx[d];
+  const long       n = long(nbrick[0]) * nbrick[1] * (dim == 3 ? nbrick[2] : 1);
+  std::vector<int> num(size_t(n), -1);
+  int              next = 0;
+  for (long ti = 0; ti < (long(1) << total); ++ti)
+  {
+    long pos[3] = { 0, 0, 0 };
+    long t = ti;
+    int  last = 0;
+    for (int i = 0; i < dim - 1; ++i)
+    {
+      const int rounds = logx[rank[i]] - last;
+      // the axes still taking bits in this phase, in axis order
+      int active[3], na = 0;
+      for (int d = 0; d < dim; ++d)
+        for (int k = i; k < dim; ++k)
+          if (rank[k] == d)
+            active[na++] = d;
+      for (int j = 0; j < rounds; ++j)
+        for (int a = 0; a < na; ++a)
+        {
+          pos[active[a]] |= ((t >> (na * j + a)) & 1) << (last + j);
+        }
+      t >>= na * rounds;
+      last += rounds;
+    }
+    pos[rank[dim - 1]] += t << last;
+    if (pos[0] < nbrick[0] && pos[1] < nbrick[1] && (dim == 2 || pos[2] < nbrick[2]))
+      num[size_t(pos[0] + nbrick[0] * (pos[1] + long(nbrick[1]) * pos[2]))] = next++;
+  }
+  return num;
+}
+
 void
 Mesh::sort_leaves()
 {
   int M = 0;
   for (auto & L : leaves)
     M = std::max(M, L.level);
+  const std::vector<int> treeno = brick_tree_numbers(dim, nbrick);
   auto morton = [&](const Leaf & L) {
     const long tx = L.x >> L.level, ty = L.y >> L.level, tz = dim == 3 ? (L.z >> L.level) : 0;
-    const uint64_t tree = uint64_t(tx + nbrick[0] * (ty + long(nbrick[1]) * tz));
+    const uint64_t tree = uint64_t(treeno[size_t(tx + nbrick[0] * (ty + long(nbrick[1]) * tz))]);
     const uint64_t lx = uint64_t(L.x - (tx << L.level)) << (M - L.level);
     const uint64_t ly = uint64_t(L.y - (ty << L.level)) << (M - L.level);
     const uint64_t lz = dim == 3 ? uint64_t(L.z - (tz << L.level)) << (M - L.level) : 0;

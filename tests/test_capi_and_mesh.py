@@ -169,3 +169,44 @@ def test_balance_large_mesh_is_two_to_one():
                        axis=2)
         a, b = np.nonzero(touch)
         assert np.all(np.abs(lv[a].astype(int) - lv[b].astype(int)) <= 1)
+
+
+def _order(nb, level, dim):
+    m = Mesh(dim, 8, level, nbrick=nb).finalize()
+    lv, co = m.leaves()
+    return [tuple(int(x) for x in c[:dim]) for c in co[:m.num_leaves]]
+
+
+def test_leaf_order_follows_p4est_brick_trees():
+    """Owned leaves come in p4est's order: trees as p4est_connectivity_new_brick numbers them
+    (bits of the tree position interleaved, the axis with fewer bits dropping out first, positions
+    outside the brick skipped), then the leaves of each tree along the Morton curve (x bit
+    fastest).  So U and B in the reference's DataArrayBlock order pass the ABI as they are."""
+    # 3 x 2: 2 bits in x, 1 in y -> (0,0) (1,0) (0,1) (1,1) then the x >= 2 pair (2,0) (2,1)
+    assert _order((3, 2, 1), 0, 2) == [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (2, 1)]
+    # 3 x 3 inside the 4 x 4 Morton square
+    assert _order((3, 3, 1), 0, 2) == [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (2, 1), (0, 2), (1, 2), (2, 2)]
+    # a line of trees, and a power-of-two cube (pure Morton)
+    assert _order((5, 1, 1), 0, 3) == [(0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0), (4, 0, 0)]
+    assert _order((2, 2, 2), 0, 3) == [(x, y, z) for z in range(2) for y in range(2) for x in range(2)]
+    # 1 x 3 (the Rayleigh-Taylor brick): y only
+    assert _order((1, 3, 1), 0, 2) == [(0, 0), (0, 1), (0, 2)]
+    # 2 x 1 at level 1: tree 0's four leaves in Morton order, then tree 1's
+    assert _order((2, 1, 1), 1, 2) == [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (3, 0), (2, 1), (3, 1)]
+    # 3 x 2 x 2: 2 bits in x, 1 in y and z: one interleaved round of x, y, z, then x's last bit
+    got = _order((3, 2, 2), 0, 3)
+    assert got[:8] == [(x, y, z) for z in range(2) for y in range(2) for x in range(2)]
+    assert got[8:] == [(2, y, z) for z in range(2) for y in range(2)]
+
+
+def test_set_leaves_puts_a_shuffled_list_in_p4est_order():
+    rng = np.random.default_rng(3)
+    m = refine_where(Mesh(2, 8, 1, nbrick=(3, 2, 1)), lambda L, x0, x1, y0, y1, z0, z1: x0 < 1.5 and y0 < 0.5, 2)
+    lv, co = m.leaves()
+    n = m.num_leaves
+    perm = rng.permutation(n)
+    m2 = Mesh(2, 8, 1, nbrick=(3, 2, 1))
+    m2.set_leaves(lv[:n][perm], co[:n][perm])
+    m2.finalize()
+    lv2, co2 = m2.leaves()
+    assert np.array_equal(lv2[:n], lv[:n]) and np.array_equal(co2[:n], co[:n])

@@ -139,3 +139,22 @@ def test_blast_refined_at_walls(impl, dim):
     assert len(set(m.leaves()[0][:m.num_leaves].tolist())) >= 2
     U, B = problems.blast(m, radius=0.25, center=(0.2, 0.25, 0.2), rho_in=2.0, bfield=(0.5, 0.3, 0.2))
     run_pair(m, U, B, 5.0 / 3.0, 0.8, 1.0, 4, impl, bc=bc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_blast_on_a_brick_in_p4est_order(impl, dim):
+    """A 3 x 2 (x 2) brick of trees, refined across trees: the leaves (and U, B) are in p4est's
+    brick order, which differs from the lexicographic tree order here, and the step matches the
+    oracle in that order."""
+    nb = (3, 2, 1) if dim == 2 else (3, 2, 2)
+    m = Mesh(dim, 8, 1, nbrick=nb)
+    m = refine_where(m, lambda L, x0, x1, y0, y1, z0, z1: 0.9 < x0 < 2.1 and y0 < 1.0, 2)
+    lv, co = m.leaves()
+    trees = [tuple(int(c) >> int(l) for c in cc[:dim]) for l, cc in zip(lv[:m.num_leaves], co[:m.num_leaves])]
+    firsts = list(dict.fromkeys(trees))
+    assert firsts[:5] == ([(0, 0), (1, 0), (0, 1), (1, 1), (2, 0)] if dim == 2 else
+                          [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1)])
+    U, B = problems.blast(m, radius=0.4, center=(1.5, 1.0, 1.0), bfield=(0.4, 0.3, 0.2))
+    run_pair(m, U, B, 5.0 / 3.0, 0.8, 1.0, 3, impl)
