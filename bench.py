@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the serving and BL3D sub-lines of the default (mix) run")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
+    ap.add_argument("--front", default="off", choices=["on", "off"],
+                    help="primitive, electric-field and cell stages as one LDS-tiled kernel (on) or three (off)")
     ap.add_argument("--precision-steps", type=int, default=0,
                     help="with --impl fp32: run the FP32-bulk and the FP64 form this many steps from the same "
                          "state and report the per-field relative L1 error (the precision config: 500 on kh3d)")
@@ -217,6 +219,7 @@ def bench_batch(args, cases, impl):
 
     dim = cases[0][0].dim
     bt = BatchGodunovMHD(dim, impl)
+    bt.set_front(args.front == "on")
     for (m, U, B, st) in cases:
         bt.add(m, U, B, st)
     cells = bt.cells()
@@ -288,6 +291,8 @@ def main():
               "meshes": len(cases),
               "leaves": sum(m.num_leaves for (m, _, _, _) in cases), "cells": cells,
               "blocks": sorted({m.b for (m, _, _, _) in cases}), "dim": cases[0][0].dim}
+    if args.impl != "plain":
+        config["front"] = args.front == "on"
     if args.impl == "plain":
         p = run_plain(cases, args.steps, args.warmup)
         out.update({"value": p["value"], "ms_per_step": p["ms_per_step"]})

@@ -36,6 +36,7 @@ class BatchGodunovMHD:
             raise MhdctError("mhdct_batch_create failed (dim 2 or 3, impl 11 or 12)")
         self._h = h
         self.slots: dict[int, _Slot] = {}
+        self.front = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -102,7 +103,11 @@ class BatchGodunovMHD:
         ms = (C.c_double * n)()
         self._check(lib().mhdct_batch_step_timed(self._h, int(nsteps), C.c_void_p(_stream(stream)), ms),
                     "batch_step_timed")
-        return {lib().mhdct_stage_name(k).decode(): ms[k] for k in range(n)}
+        names = [lib().mhdct_stage_name(k).decode() for k in range(n)]
+        if self.front:
+            # k_front runs the primitive, electric-field and cell stages in the first of them
+            names[2:5] = ["k_front", "k_efield (in k_front)", "k_cell (in k_front)"]
+        return {name: ms[k] for k, name in enumerate(names)}
 
     def slot_state(self, slot: int):
         cur, t, n = C.c_int(), C.c_double(), C.c_int64()
@@ -137,6 +142,18 @@ class BatchGodunovMHD:
     def set_tune(self, grid_cap: int, tile_cap: int, tile_z: int = 8):
         self._check(lib().mhdct_batch_set_tune(self._h, int(grid_cap), int(tile_cap), int(tile_z)),
                     "batch_set_tune")
+
+    def set_class_launch(self, enable: bool = True):
+        """Slots of several block sizes: each stage launched once per block-size class with the
+        kernels specialised to it (the default), or once over all slots; same results."""
+        self._check(lib().mhdct_batch_set_class_launch(self._h, int(bool(enable))), "batch_set_class_launch")
+
+    def set_front(self, enable: bool = True):
+        """k_front (one LDS-tiled workgroup per sub-tile) for the primitive, electric-field and
+        cell stages, or the three kernels through HBM scratch (the default); same results bit for
+        bit, and on MI355X the three kernels are the faster."""
+        self._check(lib().mhdct_batch_set_front(self._h, int(bool(enable))), "batch_set_front")
+        self.front = bool(enable)
 
     def autotune(self, reps: int = 2, stream=None):
         """Pick the launch caps and the flux-tile depth by timing; returns (grid_cap, tile_cap, tile_z)."""

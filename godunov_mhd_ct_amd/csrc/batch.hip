@@ -131,6 +131,10 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
   // (the entries hold the clocks: save_clocks ran before the entry list changed)
   drop_graphs(b);
   free_worklist(b);
+  // slots ordered by block size, so that each block-size class is a range of slots
+  std::stable_sort(b->ent.begin(), b->ent.end(), [](const mhdct_batch::Entry & x, const mhdct_batch::Entry & y) {
+    return x.ctx->c.b < y.ctx->c.b;
+  });
   bt = FusedBatch<T>();
   bt.dim = b->dim;
   bt.nslots = int(b->ent.size());
@@ -205,6 +209,36 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
     if (alloc(off[k].size() * sizeof(long), reinterpret_cast<void **>(&bt.d_off[k])) ||
         hipMemcpy(bt.d_off[k], off[k].data(), off[k].size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
       return -1;
+  }
+  // the block-size classes and their work lists (each class's offsets from its first slot)
+  if (bt.bsel == 0)
+  {
+    for (int i = 0; i < bt.nslots;)
+    {
+      int j = i;
+      while (j < bt.nslots && b->ent[j].ctx->c.b == b->ent[i].ctx->c.b)
+        ++j;
+      if (bt.ncls == 3)
+      {
+        b->err = "more than three block sizes in one batch";
+        return -1;
+      }
+      mhdct::FusedClass & c = bt.cls[bt.ncls++];
+      c.bs = b->ent[i].ctx->c.b;
+      c.s0 = i;
+      c.n = j - i;
+      for (int k = 0; k < mhdct::kStages; ++k)
+      {
+        std::vector<long> o(size_t(c.n) + 1);
+        for (int q = 0; q <= c.n; ++q)
+          o[q] = off[k][i + q] - off[k][i];
+        c.tot[k] = o.back();
+        if (alloc(o.size() * sizeof(long), reinterpret_cast<void **>(&c.d_off[k])) ||
+            hipMemcpy(c.d_off[k], o.data(), o.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+          return -1;
+      }
+      i = j;
+    }
   }
   std::vector<double> clk(size_t(2) * bt.nslots);
   for (size_t i = 0; i < b->ent.size(); ++i)
@@ -788,6 +822,30 @@ extern "C"
     b->tune.grid_cap = grid_cap;
     b->tune.tile_cap = tile_cap;
     b->tune.tile_z = tile_z;
+    b->bd.tune = b->tune;
+    b->bf.tune = b->tune;
+    drop_graphs(b);
+    return 0;
+  }
+
+  int
+  mhdct_batch_set_class_launch(mhdct_batch * b, int enable)
+  {
+    if (!b)
+      return -1;
+    b->tune.per_class = enable != 0;
+    b->bd.tune = b->tune;
+    b->bf.tune = b->tune;
+    drop_graphs(b);
+    return 0;
+  }
+
+  int
+  mhdct_batch_set_front(mhdct_batch * b, int enable)
+  {
+    if (!b)
+      return -1;
+    b->tune.front = enable != 0;
     b->bd.tune = b->tune;
     b->bf.tune = b->tune;
     drop_graphs(b);

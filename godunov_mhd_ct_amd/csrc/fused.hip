@@ -185,6 +185,29 @@ struct View
   __device__ T fs(int n, int c) const { return bload<T>(r_fs, n * G::G1 + c); }
 };
 
+// Where the cell and edge stages read the primitive state, the ghosted face field and the edge
+// electric field: the HBM scratch of k_prim / k_efield (GlobalSrc) or a front tile in LDS
+// (LdsSrc, k_front).  Both give the same values, so the records are the same bit for bit.
+template <typename T, int DIM, int BS>
+struct GlobalSrc
+{
+  using G = Geo<DIM, BS>;
+  const View<T, DIM, BS> & V;
+  __amdgpu_buffer_rsrc_t   r_ed;
+  __device__ GlobalSrc(const View<T, DIM, BS> & v, const FusedSlot<T> & S, int o)
+    : V(v)
+    , r_ed(rsrc(S.Ed + long(o) * G::NCE * G::NPE, G::NCE * G::NPE * int(sizeof(T))))
+  {
+  }
+  __device__ T qp(int v, int a, int c, int e) const { return V.qp(v, a, c, e); }
+  __device__ T bf(int fc, int a, int c, int e) const { return V.bf(fc, a, c, e); }
+  // component comp (0 x, 1 y, 2 z; 2D: 0 = z) of E at edge point (a, c, e), a in [-1, b+1]
+  __device__ T ed(int comp, int a, int c, int e) const
+  {
+    return bload<T>(r_ed, comp * G::NPE + (a + 1) + G::NP * ((c + 1) + (DIM == 3 ? G::NP * (e + 1) : 0)));
+  }
+};
+
 // Cell (i,j,k) of the g-th ghost cell of the 2-ghost tile, g < (b+4)^dim - b^dim.  3D: two
 // full planes below the block, two above, then for each interior plane two full rows below,
 // two above and the four side cells of each interior row; 2D: the same for rows and cells.
@@ -241,6 +264,100 @@ dxd(const DevMesh & m, double scaling, int o)
 // k_prim: U, B -> primitive (rho, p, u, v, w) on the 2-ghost tile and the ghosted face field.
 // The cell owns its low faces (and its high faces on the last ghost layer).
 // ---------------------------------------------------------------------------------------------
+// The conserved state u[8] (5 of U, cell-centred B) of cell (i, j, k) of the 2-ghost tile of block
+// o: the cell itself inside the block (inside_state); a ghost cell from the neighbour (ghost_state:
+// same level: copy; finer: mean of the 2^dim cells and this block's ghosted faces; coarser:
+// SIMPLE_COPY of the coarse cell, or EXTRAPOLATE_LINEAR_MINMOD and this block's ghosted faces).
+// gf(c, a0, a1, a2) is the ghosted face value face_value_ghosted gives.
+template <typename T, int DIM, int BS>
+__device__ __forceinline__ void
+inside_state(const FusedSlot<T> & S, int o, int i, int j, int k, double * u)
+{
+  using IX = IdxC<DIM, BS>;
+  constexpr int  ncell = IX::cells();
+  const int      cidx = IX::cell(i, j, k);
+  const double * Bo = S.B + long(o) * IX::fper();
+  for (int v = 0; v < 5; ++v)
+    u[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
+  if (DIM == 2)
+  {
+    u[kA] = 0.5 * (Bo[IX::face(0, i, j, 0)] + Bo[IX::face(0, i + 1, j, 0)]);
+    u[kB] = 0.5 * (Bo[IX::face(1, i, j, 0)] + Bo[IX::face(1, i, j + 1, 0)]);
+    u[kC] = 0.5 * (Bo[IX::face(2, i, j, 0)] + Bo[IX::face(2, i, j, 0)]);
+  }
+  else
+  {
+    u[kA] = 0.5 * (Bo[IX::face(0, i, j, k)] + Bo[IX::face(0, i + 1, j, k)]);
+    u[kB] = 0.5 * (Bo[IX::face(1, i, j, k)] + Bo[IX::face(1, i, j + 1, k)]);
+    u[kC] = 0.5 * (Bo[IX::face(2, i, j, k)] + Bo[IX::face(2, i, j, k + 1)]);
+  }
+}
+
+template <typename T, int DIM, int BS, class GF>
+__device__ __forceinline__ void
+ghost_state(const FusedSlot<T> & S, int o, int i, int j, int k, double * u, const GF & gf)
+{
+  using IX = IdxC<DIM, BS>;
+  constexpr IX    I{};
+  constexpr int   ncell = IX::cells();
+  const DevMesh & m = S.m;
+  const double * __restrict__ U = S.U;
+  const double * __restrict__ B = S.B;
+  const int  c[3] = { i, j, k };
+  const Hit  h = neighbour(m, I, o, c);
+  const bool lin = h.rel < 0 && m.prolong == 1;
+  if (h.rel > 0 || lin)
+  {
+    if (lin)
+      extrapolate_linear_minmod(m, I, U, h, i, j, k, u);
+    else
+      for (int v = 0; v < 5; ++v)
+      {
+        const double * Un = U + (long(h.oct) * 5 + v) * ncell;
+        double         sum = 0;
+        for (int dk = 0; dk < (DIM == 3 ? 2 : 1); ++dk)
+          for (int dj = 0; dj < 2; ++dj)
+            for (int di = 0; di < 2; ++di)
+              sum += Un[IX::cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
+        u[v] = sum / double(1 << DIM);
+      }
+    // the cell-centred field from this block's ghosted faces (finer neighbour, and the
+    // linear prolongation: ConvertToPrimitivesVariablesFunctor.cpp:536-541, 623-628)
+    if (DIM == 2)
+    {
+      u[kA] = 0.5 * (gf(0, i, j, 0) + gf(0, i + 1, j, 0));
+      u[kB] = 0.5 * (gf(1, i, j, 0) + gf(1, i, j + 1, 0));
+      u[kC] = gf(2, i, j, 0);
+    }
+    else
+    {
+      u[kA] = 0.5 * (gf(0, i, j, k) + gf(0, i + 1, j, k));
+      u[kB] = 0.5 * (gf(1, i, j, k) + gf(1, i, j + 1, k));
+      u[kC] = 0.5 * (gf(2, i, j, k) + gf(2, i, j, k + 1));
+    }
+  }
+  else
+  {
+    const int cidx = IX::cell(h.l[0], h.l[1], h.l[2]);
+    for (int v = 0; v < 5; ++v)
+      u[v] = U[(long(h.oct) * 5 + v) * ncell + cidx];
+    const double * Bn = B + long(h.oct) * IX::fper();
+    const int      a = h.l[0], bb = h.l[1], cc = h.l[2];
+    if (DIM == 2)
+    {
+      u[kA] = 0.5 * (Bn[IX::face(0, a, bb, 0)] + Bn[IX::face(0, a + 1, bb, 0)]);
+      u[kB] = 0.5 * (Bn[IX::face(1, a, bb, 0)] + Bn[IX::face(1, a, bb + 1, 0)]);
+      u[kC] = 0.5 * (Bn[IX::face(2, a, bb, 0)] + Bn[IX::face(2, a, bb, 0)]);
+    }
+    else
+    {
+      u[kA] = 0.5 * (Bn[IX::face(0, a, bb, cc)] + Bn[IX::face(0, a + 1, bb, cc)]);
+      u[kB] = 0.5 * (Bn[IX::face(1, a, bb, cc)] + Bn[IX::face(1, a, bb + 1, cc)]);
+      u[kC] = 0.5 * (Bn[IX::face(2, a, bb, cc)] + Bn[IX::face(2, a, bb, cc + 1)]);
+    }
+  }
+}
+
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
 prim_item(const FusedSlot<T> & S, long r0)
@@ -255,7 +372,8 @@ prim_item(const FusedSlot<T> & S, long r0)
     int       g = int(r0) - o * G::PRIM;
     constexpr int n = G::N2, b = BS;
     int           i, j, k = 0;
-    if (g < G::NC)
+    const bool    inside = g < G::NC;
+    if (inside)
     {
       i = g % BS;
       j = (g / BS) % BS;
@@ -269,90 +387,16 @@ prim_item(const FusedSlot<T> & S, long r0)
       shell_cell<DIM, BS>(g, i, j, k);
     }
     const int ci = (i + 2) + n * ((j + 2) + (DIM == 3 ? n * (k + 2) : 0));
-    constexpr int ncell = IX::cells();
-    const double * __restrict__ U = S.U;
     const double * __restrict__ B = S.B;
-    double     u[8];
-    const bool inside = i >= 0 && i < b && j >= 0 && j < b && (DIM == 2 || (k >= 0 && k < b));
     auto       gfv = [&](int c, int a0, int a1, int a2) {
       const int f[3] = { a0, a1, a2 };
       return face_value_ghosted(m, I, B, o, c, f);
     };
+    double    u[8];
     if (inside)
-    {
-      const int      cidx = IX::cell(i, j, k);
-      const double * Bo = B + long(o) * IX::fper();
-      for (int v = 0; v < 5; ++v)
-        u[v] = U[(long(o) * 5 + v) * ncell + cidx];
-      if (DIM == 2)
-      {
-        u[kA] = 0.5 * (Bo[IX::face(0, i, j, 0)] + Bo[IX::face(0, i + 1, j, 0)]);
-        u[kB] = 0.5 * (Bo[IX::face(1, i, j, 0)] + Bo[IX::face(1, i, j + 1, 0)]);
-        u[kC] = 0.5 * (Bo[IX::face(2, i, j, 0)] + Bo[IX::face(2, i, j, 0)]);
-      }
-      else
-      {
-        u[kA] = 0.5 * (Bo[IX::face(0, i, j, k)] + Bo[IX::face(0, i + 1, j, k)]);
-        u[kB] = 0.5 * (Bo[IX::face(1, i, j, k)] + Bo[IX::face(1, i, j + 1, k)]);
-        u[kC] = 0.5 * (Bo[IX::face(2, i, j, k)] + Bo[IX::face(2, i, j, k + 1)]);
-      }
-    }
+      inside_state<T, DIM, BS>(S, o, i, j, k, u);
     else
-    {
-      const int c[3] = { i, j, k };
-      const Hit h = neighbour(m, I, o, c);
-      const bool lin = h.rel < 0 && m.prolong == 1;
-      if (h.rel > 0 || lin)
-      {
-        if (lin)
-          extrapolate_linear_minmod(m, I, U, h, i, j, k, u);
-        else
-          for (int v = 0; v < 5; ++v)
-          {
-            const double * Un = U + (long(h.oct) * 5 + v) * ncell;
-            double         sum = 0;
-            for (int dk = 0; dk < (DIM == 3 ? 2 : 1); ++dk)
-              for (int dj = 0; dj < 2; ++dj)
-                for (int di = 0; di < 2; ++di)
-                  sum += Un[IX::cell(h.l[0] + di, h.l[1] + dj, h.l[2] + dk)];
-            u[v] = sum / double(1 << DIM);
-          }
-        // the cell-centred field from this block's ghosted faces (finer neighbour, and the
-        // linear prolongation: ConvertToPrimitivesVariablesFunctor.cpp:536-541, 623-628)
-        if (DIM == 2)
-        {
-          u[kA] = 0.5 * (gfv(0, i, j, 0) + gfv(0, i + 1, j, 0));
-          u[kB] = 0.5 * (gfv(1, i, j, 0) + gfv(1, i, j + 1, 0));
-          u[kC] = gfv(2, i, j, 0);
-        }
-        else
-        {
-          u[kA] = 0.5 * (gfv(0, i, j, k) + gfv(0, i + 1, j, k));
-          u[kB] = 0.5 * (gfv(1, i, j, k) + gfv(1, i, j + 1, k));
-          u[kC] = 0.5 * (gfv(2, i, j, k) + gfv(2, i, j, k + 1));
-        }
-      }
-      else
-      {
-        const int cidx = IX::cell(h.l[0], h.l[1], h.l[2]);
-        for (int v = 0; v < 5; ++v)
-          u[v] = U[(long(h.oct) * 5 + v) * ncell + cidx];
-        const double * Bn = B + long(h.oct) * IX::fper();
-        const int      a = h.l[0], bb = h.l[1], cc = h.l[2];
-        if (DIM == 2)
-        {
-          u[kA] = 0.5 * (Bn[IX::face(0, a, bb, 0)] + Bn[IX::face(0, a + 1, bb, 0)]);
-          u[kB] = 0.5 * (Bn[IX::face(1, a, bb, 0)] + Bn[IX::face(1, a, bb + 1, 0)]);
-          u[kC] = 0.5 * (Bn[IX::face(2, a, bb, 0)] + Bn[IX::face(2, a, bb, 0)]);
-        }
-        else
-        {
-          u[kA] = 0.5 * (Bn[IX::face(0, a, bb, cc)] + Bn[IX::face(0, a + 1, bb, cc)]);
-          u[kB] = 0.5 * (Bn[IX::face(1, a, bb, cc)] + Bn[IX::face(1, a, bb + 1, cc)]);
-          u[kC] = 0.5 * (Bn[IX::face(2, a, bb, cc)] + Bn[IX::face(2, a, bb, cc + 1)]);
-        }
-      }
-    }
+      ghost_state<T, DIM, BS>(S, o, i, j, k, u, gfv);
     double       q[8];
     const double eint = to_primitive(u, q, S.gd);
     if (S.abort_eint && eint < 0)
@@ -406,53 +450,64 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
 // k_efield: edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235) on the
 // edge points [-1, b+1]^dim of each block, from the primitive state of the 2-ghost tile.
 // ---------------------------------------------------------------------------------------------
+// E = v x B at edge point (a, c, e) from the primitive state and the face field of src: out[0..2]
+// = E_x, E_y, E_z (3D), out[0] = E_z (2D)
+template <typename T, int DIM, class Src>
+__device__ __forceinline__ void
+efield_point(const Src & src, int a, int c, int e, T * out)
+{
+  auto Q = [&](int v, int x, int y, int z) -> T {
+    if (v < 5)
+      return src.qp(v, x, y, z);
+    const int fc = (v - 5) % 3;
+    const int hi = v >= 8;
+    if (DIM == 2 && fc == 2)
+      return src.bf(2, x, y, 0);
+    return src.bf(fc, x + (hi && fc == 0), y + (hi && fc == 1), z + (hi && fc == 2));
+  };
+  if (DIM == 3)
+  {
+    const T vv = T(0.25) * (Q(3, a, c - 1, e - 1) + Q(3, a, c - 1, e) + Q(3, a, c, e - 1) + Q(3, a, c, e));
+    const T ww = T(0.25) * (Q(4, a, c - 1, e - 1) + Q(4, a, c - 1, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
+    const T BB = T(0.5) * (Q(6, a, c, e - 1) + Q(6, a, c, e));
+    const T CC = T(0.5) * (Q(7, a, c - 1, e) + Q(7, a, c, e));
+    out[0] = vv * CC - ww * BB;
+  }
+  if (DIM == 3)
+  {
+    const T uu = T(0.25) * (Q(2, a - 1, c, e - 1) + Q(2, a - 1, c, e) + Q(2, a, c, e - 1) + Q(2, a, c, e));
+    const T ww = T(0.25) * (Q(4, a - 1, c, e - 1) + Q(4, a - 1, c, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
+    const T AA = T(0.5) * (Q(5, a, c, e - 1) + Q(5, a, c, e));
+    const T CC = T(0.5) * (Q(7, a - 1, c, e) + Q(7, a, c, e));
+    out[1] = ww * AA - uu * CC;
+  }
+  {
+    const T uu = T(0.25) * (Q(2, a - 1, c - 1, e) + Q(2, a - 1, c, e) + Q(2, a, c - 1, e) + Q(2, a, c, e));
+    const T vv = T(0.25) * (Q(3, a - 1, c - 1, e) + Q(3, a - 1, c, e) + Q(3, a, c - 1, e) + Q(3, a, c, e));
+    const T AA = T(0.5) * (Q(5, a, c - 1, e) + Q(5, a, c, e));
+    const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
+    out[DIM == 3 ? 2 : 0] = uu * BB - vv * AA;
+  }
+}
+
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
 efield_item(const FusedSlot<T> & S, long r0)
 {
-
   using G = Geo<DIM, BS>;
-
-    const int o = __builtin_amdgcn_readfirstlane(int(r0) / G::EFP);
-    const int r = int(r0) - o * G::EFP;
-    if (r >= G::NPE)
-      return;
-    constexpr int          np = G::NP;
-    const int              a = r % np - 1, c = (r / np) % np - 1, e = DIM == 3 ? r / (np * np) - 1 : 0;
-    const View<T, DIM, BS> V(S, o);
-    auto Q = [&](int v, int x, int y, int z) -> T {
-      if (v < 5)
-        return V.qp(v, x, y, z);
-      const int fc = (v - 5) % 3;
-      const int hi = v >= 8;
-      if (DIM == 2 && fc == 2)
-        return V.bf(2, x, y, 0);
-      return V.bf(fc, x + (hi && fc == 0), y + (hi && fc == 1), z + (hi && fc == 2));
-    };
-    T * Ed = S.Ed + long(o) * G::NCE * G::NPE + r;
-    if (DIM == 3)
-    {
-      const T vv = T(0.25) * (Q(3, a, c - 1, e - 1) + Q(3, a, c - 1, e) + Q(3, a, c, e - 1) + Q(3, a, c, e));
-      const T ww = T(0.25) * (Q(4, a, c - 1, e - 1) + Q(4, a, c - 1, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
-      const T BB = T(0.5) * (Q(6, a, c, e - 1) + Q(6, a, c, e));
-      const T CC = T(0.5) * (Q(7, a, c - 1, e) + Q(7, a, c, e));
-      store_nt(Ed, vv * CC - ww * BB);
-    }
-    if (DIM == 3)
-    {
-      const T uu = T(0.25) * (Q(2, a - 1, c, e - 1) + Q(2, a - 1, c, e) + Q(2, a, c, e - 1) + Q(2, a, c, e));
-      const T ww = T(0.25) * (Q(4, a - 1, c, e - 1) + Q(4, a - 1, c, e) + Q(4, a, c, e - 1) + Q(4, a, c, e));
-      const T AA = T(0.5) * (Q(5, a, c, e - 1) + Q(5, a, c, e));
-      const T CC = T(0.5) * (Q(7, a - 1, c, e) + Q(7, a, c, e));
-      store_nt(Ed + G::NPE, ww * AA - uu * CC);
-    }
-    {
-      const T uu = T(0.25) * (Q(2, a - 1, c - 1, e) + Q(2, a - 1, c, e) + Q(2, a, c - 1, e) + Q(2, a, c, e));
-      const T vv = T(0.25) * (Q(3, a - 1, c - 1, e) + Q(3, a - 1, c, e) + Q(3, a, c - 1, e) + Q(3, a, c, e));
-      const T AA = T(0.5) * (Q(5, a, c - 1, e) + Q(5, a, c, e));
-      const T BB = T(0.5) * (Q(6, a - 1, c, e) + Q(6, a, c, e));
-      store_nt(Ed + (DIM == 3 ? 2 * G::NPE : 0), uu * BB - vv * AA);
-    }
+  const int o = __builtin_amdgcn_readfirstlane(int(r0) / G::EFP);
+  const int r = int(r0) - o * G::EFP;
+  if (r >= G::NPE)
+    return;
+  constexpr int          np = G::NP;
+  const int              a = r % np - 1, c = (r / np) % np - 1, e = DIM == 3 ? r / (np * np) - 1 : 0;
+  const View<T, DIM, BS> V(S, o);
+  const GlobalSrc<T, DIM, BS> src(V, S, o);
+  T out[3];
+  efield_point<T, DIM>(src, a, c, e, out);
+  T * Ed = S.Ed + long(o) * G::NCE * G::NPE + r;
+  for (int comp = 0; comp < G::NCE; ++comp)
+    store_nt(Ed + comp * G::NPE, out[comp]);
 }
 
 template <typename T, int DIM, int BSEL>
@@ -478,32 +533,23 @@ k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __res
 // predictor's viscous term is compiled in (FP64 kernels of one block size: viscosity runs per
 // context, never in a batch of mixed block sizes, and the FP32 form refuses it).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM, int BS, bool HK, bool VISC>
+template <typename T, int DIM, int BS, bool HK, bool VISC, class Src>
 __device__ __forceinline__ void
-cell_item(const FusedSlot<T> & S, long r0)
+cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, int o, int i, int j, int k, int c1)
 {
-
   using G = Geo<DIM, BS>;
     const DevMesh &      m = S.m;
-    constexpr int             n = G::N1;
-    const int                 o = __builtin_amdgcn_readfirstlane(int(r0) / G::P1);
-    const int                 ci = int(r0) - o * G::P1;
-    if (ci >= G::G1)
-      return;
-    const View<T, DIM, BS>    V(S, o);
-    const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
     const T   st = S.gt.slope_type;
     // the 11-variable primitive state of cell (a, c, e): 5 hydro + low/high face fields
     auto Q = [&](int v, int a, int c, int e) -> T {
       if (v < 5)
-        return V.qp(v, a, c, e);
+        return src.qp(v, a, c, e);
       const int fc = (v - 5) % 3;
       const int hi = v >= 8;
       if (DIM == 2 && fc == 2)
-        return V.bf(2, a, c, 0);
-      return V.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
+        return src.bf(2, a, c, 0);
+      return src.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
     };
-    const int c1 = ci;
     T         sl[3][8];
     constexpr int nh = DIM == 3 ? 5 : 4;
     for (int d = 0; d < DIM; ++d)
@@ -622,14 +668,10 @@ cell_item(const FusedSlot<T> & S, long r0)
       q2o[6] = B + T(0.5) * sB0;
       q2o[7] = C + T(0.5) * sC0;
     }
-    // edge electric field v x B at t_n from k_efield
-    const auto r_ed = rsrc(S.Ed + long(o) * G::NCE * G::NPE, G::NCE * G::NPE * int(sizeof(T)));
-    auto       ep = [](int a, int c, int e) {
-      return (a + 1) + G::NP * ((c + 1) + (DIM == 3 ? G::NP * (e + 1) : 0));
-    };
-    auto Ex = [&](int a, int c, int e) -> T { return bload<T>(r_ed, ep(a, c, e)); };
-    auto Ey = [&](int a, int c, int e) -> T { return bload<T>(r_ed, G::NPE + ep(a, c, e)); };
-    auto Ez = [&](int a, int c, int e) -> T { return bload<T>(r_ed, (DIM == 3 ? 2 * G::NPE : 0) + ep(a, c, e)); };
+    // edge electric field v x B at t_n (k_efield, or the front tile)
+    auto Ex = [&](int a, int c, int e) -> T { return src.ed(0, a, c, e); };
+    auto Ey = [&](int a, int c, int e) -> T { return src.ed(1, a, c, e); };
+    auto Ez = [&](int a, int c, int e) -> T { return src.ed(DIM == 3 ? 2 : 0, a, c, e); };
     T sfo[3];
     if (DIM == 2)
     {
@@ -675,6 +717,22 @@ cell_item(const FusedSlot<T> & S, long r0)
       bstore<T>(V.r_fs, n2 * G::G1 + c1, fso[n2]);
 }
 
+template <typename T, int DIM, int BS, bool HK, bool VISC>
+__device__ __forceinline__ void
+cell_item(const FusedSlot<T> & S, long r0)
+{
+  using G = Geo<DIM, BS>;
+  constexpr int n = G::N1;
+  const int     o = __builtin_amdgcn_readfirstlane(int(r0) / G::P1);
+  const int     ci = int(r0) - o * G::P1;
+  if (ci >= G::G1)
+    return;
+  const View<T, DIM, BS>      V(S, o);
+  const GlobalSrc<T, DIM, BS> src(V, S, o);
+  const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
+  cell_core<T, DIM, BS, HK, VISC>(S, V, src, o, i, j, k, ci);
+}
+
 template <typename T, int DIM, int BSEL, bool HK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
@@ -688,6 +746,342 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
       continue;
     MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS, HK, (BSEL != 0 && sizeof(T) == sizeof(double))>(S, r0)));
   }
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// k_front: k_prim, k_efield and k_cell for one sub-tile of a block in one workgroup, with the
+// primitive state, the ghosted face field and the edge electric field in LDS.  A sub-tile is
+// 8 x 8 x TZ cells (3D: TZ = 4, or 2 in 8^3 blocks, so that two workgroups fit the LDS of a CU;
+// 2D: 8 x 8).  Along each axis the sub-tile writes the records of the cells [lo, hi] (its cells;
+// the first sub-tile of a block extends to -1 and the last to b, so the ghost-1 layer is covered
+// once), which read the primitive state and the face field of the cells [lo-1, hi+1] and E at
+// the edge points [lo, hi+1].  The face field of the cells [clo, chi] (its cells; the first
+// extends to -2 and the last to b+1) also goes to Bg, which the flux and edge stages read.  The arithmetic is
+// that of the three kernels, so the records are the same bit for bit; the primitive state and
+// E of the tile's halo are computed by each tile that reads them.  The viscous predictor term
+// reads Qp from HBM and keeps the three-kernel path.
+// ---------------------------------------------------------------------------------------------
+template <int DIM, int BS>
+struct FrontG
+{
+  static constexpr int NB1 = BS / kTS;
+  static constexpr int TZ = fused_front_depth(DIM, BS);             // sub-tile depth
+  static constexpr int NSZ = DIM == 3 ? BS / TZ : 1;               // sub-tiles along z per block
+  static constexpr int NSUB = NB1 * NB1 * NSZ;                     // sub-tiles per block
+  static constexpr int R = NB1 == 1 ? kTS + 2 : kTS + 1;          // most record cells along x, y
+  static constexpr int RZ = DIM == 3 ? TZ + 1 : 1;                 // ... along z
+  static constexpr int P = R + 2;                                  // primitive cells along x, y
+  static constexpr int PZ = DIM == 3 ? RZ + 2 : 1;                 // ... along z
+  static constexpr int PE = R + 1;                                 // edge points along x, y
+  static constexpr int PEZ = DIM == 3 ? RZ + 1 : 1;                // ... along z
+  static constexpr int NQ = P * P * PZ;                            // primitive cells
+  // faces of each component: one more along its own axis (2D: the cell-centred z field)
+  static constexpr int NF0 = (P + 1) * P * PZ;
+  static constexpr int NF1 = P * (P + 1) * PZ;
+  static constexpr int NF2 = P * P * (DIM == 3 ? PZ + 1 : 1);
+  static constexpr int OF1 = NF0, OF2 = NF0 + NF1, NFT = NF0 + NF1 + NF2;
+  static constexpr int NPT = PE * PE * PEZ;                        // edge points
+  static constexpr int NCE = DIM == 3 ? 3 : 1;
+  static_assert(DIM == 2 || BS % TZ == 0, "sub-tile depth");
+};
+
+// LDS of one k_front workgroup in doubles: the face field in FP64, then the primitive state and E
+// in the bulk type T
+template <typename T, int DIM, int BS>
+constexpr int
+front_lds_doubles()
+{
+  using F = FrontG<DIM, BS>;
+  return F::NFT + int(((5 * F::NQ + F::NCE * F::NPT) * sizeof(T) + 7) / 8);
+}
+
+// the largest of the block sizes a kernel serves (BSEL 0: 8, 16 and 32)
+template <typename T, int DIM, int BSEL>
+constexpr int
+front_lds_max()
+{
+  if constexpr (BSEL != 0)
+    return front_lds_doubles<T, DIM, BSEL>();
+  else
+  {
+    constexpr int a = front_lds_doubles<T, DIM, 8>(), b = front_lds_doubles<T, DIM, 16>(),
+                  c = front_lds_doubles<T, DIM, 32>();
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+  }
+}
+
+template <int DIM>
+constexpr int
+front_threads()
+{
+  return DIM == 3 ? 512 : 256;
+}
+
+// the LDS tile of k_front: origin (x0, y0, z0) = (lo-1) per axis; each grid is laid out with the
+// extents of the largest tile of the block size, so that a grid slot is a compile-time function
+// of its position
+template <typename T, int DIM, int BS>
+struct LdsSrc
+{
+  using F = FrontG<DIM, BS>;
+  double * lf; // [NFT]      face field: component c over [lo-1, hi+2] along c, [lo-1, hi+1] across
+  T *      lq; // [5][NQ]    primitive state of the cells [lo-1, hi+1]
+  T *      le; // [NCE][NPT] E at the edge points [lo, hi+1]
+  int      x0, y0, z0;
+  __device__ int cq(int a, int c, int e) const
+  {
+    return (a - x0) + F::P * ((c - y0) + (DIM == 3 ? F::P * (e - z0) : 0));
+  }
+  __device__ int cf(int fc, int a, int c, int e) const
+  {
+    const int e0 = F::P + (fc == 0), e1 = F::P + (fc == 1);
+    const int base = fc == 0 ? 0 : (fc == 1 ? F::OF1 : F::OF2);
+    return base + (a - x0) + e0 * ((c - y0) + (DIM == 3 ? e1 * (e - z0) : 0));
+  }
+  __device__ int ce(int comp, int a, int c, int e) const
+  {
+    return comp * F::NPT + (a - x0 - 1) + F::PE * ((c - y0 - 1) + (DIM == 3 ? F::PE * (e - z0 - 1) : 0));
+  }
+  __device__ T qp(int v, int a, int c, int e) const { return lq[v * F::NQ + cq(a, c, e)]; }
+  __device__ T bf(int fc, int a, int c, int e) const { return T(lf[cf(fc, a, c, e)]); }
+  __device__ T ed(int comp, int a, int c, int e) const { return le[ce(comp, a, c, e)]; }
+};
+
+template <typename T, int DIM, int BS, bool HK>
+__device__ __forceinline__ void
+front_item(const FusedSlot<T> & S, long r0, double * lds)
+{
+  using G = Geo<DIM, BS>;
+  using F = FrontG<DIM, BS>;
+  using IX = IdxC<DIM, BS>;
+  constexpr IX    I{};
+  constexpr int   NTHR = front_threads<DIM>();
+  constexpr int   nb1 = G::NB1;
+  constexpr int   P = F::P;
+  const DevMesh & m = S.m;
+  const int       o = int(r0) / F::NSUB;
+  const int       tl = int(r0) - o * F::NSUB;
+  // segment along each axis: 8-cell segments along x, y; TZ-cell segments along z
+  const int       seg[3] = { tl % nb1, (tl / nb1) % nb1, DIM == 3 ? tl / (nb1 * nb1) : 0 };
+  int             lo[3], hi[3], clo[3], chi[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    const int s = seg[d];
+    const int w = d == 2 ? F::TZ : kTS, ns = d == 2 ? F::NSZ : nb1;
+    lo[d] = s == 0 ? -1 : w * s;
+    hi[d] = s == ns - 1 ? BS : w * s + w - 1;
+    clo[d] = s == 0 ? -2 : w * s;
+    chi[d] = s == ns - 1 ? BS + 1 : w * s + w - 1;
+  }
+  if (DIM == 2)
+    lo[2] = hi[2] = clo[2] = chi[2] = 0;
+  T *                lq = reinterpret_cast<T *>(lds + F::NFT);
+  LdsSrc<T, DIM, BS> src{ lds, lq, lq + 5 * F::NQ, lo[0] - 1, lo[1] - 1, DIM == 3 ? lo[2] - 1 : 0 };
+  const double * __restrict__ Bo = S.B + long(o) * IX::fper();
+  const double * __restrict__ Uo = S.U + long(o) * 5 * IX::cells();
+  T * Go = S.Bg + long(o) * IX::gfper();
+  // phase 1a: the face field (component c over [lo-1, hi+2] along c, [lo-1, hi+1] across; the 2D
+  // cell-centred z field over [lo-1, hi+1]), FP64, and the faces of the cells [clo, chi] to Bg.
+  // Slot t of the face grid is face t; the faces inside the block are loaded first, all at once
+  // (unconditional loads, clamped to face 0 elsewhere), the ghost faces after.
+  {
+    constexpr int NF = F::NFT;
+    constexpr int FMAX = (NF + NTHR - 1) / NTHR;
+    struct Face
+    {
+      int  c, a0, a1, a2;
+      bool valid, inner, own;
+    };
+    auto at = [&](int t) {
+      Face       f;
+      f.c = t < F::OF1 ? 0 : (t < F::OF2 ? 1 : 2);
+      const int  r = t - (f.c == 0 ? 0 : (f.c == 1 ? F::OF1 : F::OF2));
+      const int  e0 = P + (f.c == 0), e1 = P + (f.c == 1);
+      const int  rel[3] = { r % e0, (r / e0) % e1, r / (e0 * e1) };
+      const bool cellc = DIM == 2 && f.c == 2;
+      int        a[3] = { 0, 0, 0 };
+      f.valid = t < NF;
+      f.inner = true;
+      f.own = true;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+      {
+        const int along = d == f.c && !cellc;
+        a[d] = lo[d] - 1 + rel[d];
+        f.valid = f.valid && a[d] <= hi[d] + 1 + along;
+        f.inner = f.inner && a[d] >= 0 && a[d] <= BS - 1 + along;
+        f.own = f.own && a[d] >= clo[d] && a[d] <= chi[d] + (along && chi[d] == BS + 1);
+      }
+      f.a0 = a[0];
+      f.a1 = a[1];
+      f.a2 = a[2];
+      f.inner = f.inner && f.valid;
+      return f;
+    };
+    double x[FMAX];
+#pragma unroll
+    for (int r = 0; r < FMAX; ++r)
+    {
+      const Face f = at(threadIdx.x + r * NTHR);
+      x[r] = Bo[f.inner ? IX::face(f.c, f.a0, f.a1, f.a2) : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < FMAX; ++r)
+    {
+      const int  t = threadIdx.x + r * NTHR;
+      const Face f = at(t);
+      if (f.valid && !f.inner)
+      {
+        // the component as a constant, so that the lookup's arrays indexed by it stay in registers
+        const int a[3] = { f.a0, f.a1, f.a2 };
+        if (f.c == 0)
+          x[r] = face_value_ghosted(m, I, S.B, o, 0, a);
+        else if (f.c == 1)
+          x[r] = face_value_ghosted(m, I, S.B, o, 1, a);
+        else
+          x[r] = face_value_ghosted(m, I, S.B, o, 2, a);
+      }
+      if (f.valid)
+      {
+        lds[t] = x[r];
+        if (f.own)
+          store_nt(Go + IX::gface(f.c, f.a0, f.a1, f.a2), T(x[r]));
+      }
+    }
+  }
+  __syncthreads();
+  // phase 1b: the primitive state of the cells [lo-1, hi+1] (slot t of the grid is cell t): the
+  // cells inside the block first (U loaded all at once, the cell-centred field from the face
+  // grid), then the ghost cells
+  {
+    constexpr int CMAX = (F::NQ + NTHR - 1) / NTHR;
+    struct Cell
+    {
+      int  a0, a1, a2;
+      bool valid, inside;
+    };
+    auto at = [&](int t) {
+      Cell      cl;
+      const int rel[3] = { t % P, (t / P) % P, DIM == 3 ? t / (P * P) : 0 };
+      int       a[3] = { 0, 0, 0 };
+      cl.valid = t < F::NQ;
+      cl.inside = true;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+      {
+        a[d] = lo[d] - 1 + rel[d];
+        cl.valid = cl.valid && a[d] <= hi[d] + 1;
+        cl.inside = cl.inside && a[d] >= 0 && a[d] < BS;
+      }
+      cl.a0 = a[0];
+      cl.a1 = a[1];
+      cl.a2 = a[2];
+      cl.inside = cl.inside && cl.valid;
+      return cl;
+    };
+    auto gf = [&](int c, int a0, int a1, int a2) { return lds[src.cf(c, a0, a1, a2)]; };
+    auto finish = [&](int t, const double * u) {
+      double       q[8];
+      const double eint = to_primitive(u, q, S.gd);
+      if (S.abort_eint && eint < 0)
+        atomicOr(S.red + 1, 1ull);
+      for (int v = 0; v < 5; ++v)
+        lq[v * F::NQ + t] = T(q[v]);
+    };
+    double u[CMAX][5];
+#pragma unroll
+    for (int r = 0; r < CMAX; ++r)
+    {
+      const Cell cl = at(threadIdx.x + r * NTHR);
+      const int  cidx = cl.inside ? IX::cell(cl.a0, cl.a1, cl.a2) : 0;
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        u[r][v] = Uo[v * IX::cells() + cidx];
+    }
+#pragma unroll
+    for (int r = 0; r < CMAX; ++r)
+    {
+      const int  t = threadIdx.x + r * NTHR;
+      const Cell cl = at(t);
+      if (cl.inside)
+      {
+        const int i = cl.a0, j = cl.a1, k = cl.a2;
+        double    w[8] = { u[r][0], u[r][1], u[r][2], u[r][3], u[r][4] };
+        if (DIM == 2)
+        {
+          w[kA] = 0.5 * (gf(0, i, j, 0) + gf(0, i + 1, j, 0));
+          w[kB] = 0.5 * (gf(1, i, j, 0) + gf(1, i, j + 1, 0));
+          w[kC] = 0.5 * (gf(2, i, j, 0) + gf(2, i, j, 0));
+        }
+        else
+        {
+          w[kA] = 0.5 * (gf(0, i, j, k) + gf(0, i + 1, j, k));
+          w[kB] = 0.5 * (gf(1, i, j, k) + gf(1, i, j + 1, k));
+          w[kC] = 0.5 * (gf(2, i, j, k) + gf(2, i, j, k + 1));
+        }
+        finish(t, w);
+      }
+    }
+#pragma unroll 1
+    for (int r = 0; r < CMAX; ++r)
+    {
+      const int  t = threadIdx.x + r * NTHR;
+      const Cell cl = at(t);
+      if (cl.valid && !cl.inside)
+      {
+        double w[8];
+        ghost_state<T, DIM, BS>(S, o, cl.a0, cl.a1, cl.a2, w, gf);
+        finish(t, w);
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2: E at the edge points [lo, hi+1]
+  {
+    const int ex = hi[0] - lo[0] + 2, ey = hi[1] - lo[1] + 2, ez = DIM == 3 ? hi[2] - lo[2] + 2 : 1;
+#pragma unroll 1
+    for (int t = threadIdx.x; t < ex * ey * ez; t += NTHR)
+    {
+      const int a = lo[0] + t % ex, c = lo[1] + (t / ex) % ey, e = DIM == 3 ? lo[2] + t / (ex * ey) : 0;
+      T         out[3];
+      efield_point<T, DIM>(src, a, c, e, out);
+      for (int comp = 0; comp < F::NCE; ++comp)
+        src.le[src.ce(comp, a, c, e)] = out[comp];
+    }
+  }
+  __syncthreads();
+  // phase 3: the records of the cells [lo, hi]
+  {
+    const View<T, DIM, BS> V(S, o);
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
+    static_assert(F::R * F::R * F::RZ <= NTHR, "one record cell per thread");
+    const int t = threadIdx.x;
+    if (t < ex * ey * ez)
+    {
+      const int i = lo[0] + t % ex, j = lo[1] + (t / ex) % ey, k = DIM == 3 ? lo[2] + t / (ex * ey) : 0;
+      cell_core<T, DIM, BS, HK, false>(S, V, src, o, i, j, k, View<T, DIM, BS>::c1(i, j, k));
+    }
+  }
+}
+
+// k_front: one workgroup per 8^dim tile of the flux-tile work list.  No grid-stride loop: with
+// the tile bounds known at compile time (one tile per block) the compiler would keep every
+// thread's tile indexing live across it, beyond the register budget of 1024-thread workgroups.
+template <typename T, int DIM, int BSEL, bool HK>
+__global__ void __launch_bounds__((front_threads<DIM>())) __attribute__((amdgpu_waves_per_eu(4)))
+k_front(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  __shared__ double lds[front_lds_max<T, DIM, BSEL>()];
+  const long        g = xcd_group();
+  if (g >= total)
+    return;
+  const int            s = slot_of(off, nslots, g);
+  const FusedSlot<T> & S = slots[s];
+  const long           r0 = g - off[s];
+  if (r0 >= S.cnt[kStageFront])
+    return;
+  MHDCT_BS_DISPATCH(S.m.b, (front_item<T, DIM, BS, HK>(S, r0, lds)));
 }
 
 
@@ -1419,13 +1813,25 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
   const int            tcap = bt.tune.tile_cap / kXcd * kXcd;
   auto                 grid = [&](int stage) { return grid_for(bt.tot[stage], 256, cap); };
   auto                 on = [&](int stage) { return which < 0 || which == stage; };
-  if (on(kStagePrim))
+  const bool           front = bt.tune.front != 0;
+  if (on(kStagePrim) && front)
+  {
+    // one workgroup per sub-tile (k_front has no grid-stride loop)
+    const unsigned gf = grid_for(bt.tot[kStageFront], 1, 1 << 30);
+    if (bt.rk2)
+      hipLaunchKernelGGL((k_front<T, DIM, BSEL, false>), gf, (front_threads<DIM>()), 0, st, sl, n,
+                         bt.d_off[kStageFront], bt.tot[kStageFront]);
+    else
+      hipLaunchKernelGGL((k_front<T, DIM, BSEL, true>), gf, (front_threads<DIM>()), 0, st, sl, n,
+                         bt.d_off[kStageFront], bt.tot[kStageFront]);
+  }
+  if (on(kStagePrim) && !front)
     hipLaunchKernelGGL((k_prim<T, DIM, BSEL>), grid(kStagePrim), 256, 0, st, sl, n, bt.d_off[kStagePrim],
                        bt.tot[kStagePrim]);
-  if (on(kStageEfield))
+  if (on(kStageEfield) && !front)
     hipLaunchKernelGGL((k_efield<T, DIM, BSEL>), grid(kStageEfield), 256, 0, st, sl, n, bt.d_off[kStageEfield],
                        bt.tot[kStageEfield]);
-  if (on(kStageCell))
+  if (on(kStageCell) && !front)
   {
     if (bt.rk2)
       hipLaunchKernelGGL((k_cell<T, DIM, BSEL, false>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
@@ -1467,13 +1873,60 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
 }
 
 template <typename T, int DIM, int BSEL>
+void
+launch_inv_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_inv_dt<T, DIM, BSEL>), grid_for(bt.tot[kStageInvDt], 256, 2048), 256, 0, st,
+                     bt.d_slots[parity], bt.nslots, bt.d_off[kStageInvDt], bt.tot[kStageInvDt]);
+}
+
+// the batch restricted to block-size class c: its range of slots, its work lists, its block size
+template <typename T>
+FusedBatch<T>
+class_view(const FusedBatch<T> & bt, int c)
+{
+  const FusedClass & k = bt.cls[c];
+  FusedBatch<T>      v = bt;
+  v.bsel = k.bs;
+  v.nslots = k.n;
+  v.ncls = 0;
+  for (int p = 0; p < 4; ++p)
+    v.d_slots[p] = bt.d_slots[p] ? bt.d_slots[p] + k.s0 : nullptr;
+  for (int s = 0; s < kStages; ++s)
+  {
+    v.d_off[s] = k.d_off[s];
+    v.tot[s] = k.tot[s];
+  }
+  v.clock = nullptr;
+  return v;
+}
+
+template <typename T>
+bool
+by_class(const FusedBatch<T> & bt)
+{
+  return bt.bsel == 0 && bt.ncls > 0 && bt.tune.per_class != 0;
+}
+
+template <typename T, int DIM, int BSEL>
 int
 launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
 {
   const int n = bt.nslots;
   hipLaunchKernelGGL((k_dt_clear<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[parity], n);
-  hipLaunchKernelGGL((k_inv_dt<T, DIM, BSEL>), grid_for(bt.tot[kStageInvDt], 256, 2048), 256, 0, st,
-                     bt.d_slots[parity], n, bt.d_off[kStageInvDt], bt.tot[kStageInvDt]);
+  if (BSEL == 0 && by_class(bt))
+    for (int c = 0; c < bt.ncls; ++c)
+    {
+      const FusedBatch<T> v = class_view(bt, c);
+      if (v.bsel == 8)
+        launch_inv_dt<T, DIM, 8>(v, parity, st);
+      else if (v.bsel == 16)
+        launch_inv_dt<T, DIM, 16>(v, parity, st);
+      else
+        launch_inv_dt<T, DIM, 32>(v, parity, st);
+    }
+  else
+    launch_inv_dt<T, DIM, BSEL>(bt, parity, st);
   hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], bt.d_slots[2],
                      bt.d_slots[3], n, bt.clock);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1509,6 +1962,7 @@ fused_items_per_block(int dim, int b, int stage)
   case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
   case 4: return (dim == 3 ? 3 : 1) * epdp;
+  case 9: return (dim == 3 ? nb1 * nb1 * (b / fused_front_depth(dim, b)) : nb1 * nb1);
   default: return dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
   }
 }
@@ -1526,6 +1980,14 @@ fused_launch_stage(const FusedBatch<T> & bt, int parity, int which, hipStream_t 
 {
   if (bt.nslots == 0)
     return 0;
+  if (by_class(bt))
+  {
+    // one launch per block-size class, with the kernels specialised to its block size
+    for (int c = 0; c < bt.ncls; ++c)
+      if (fused_launch_stage<T>(class_view(bt, c), parity, which, st))
+        return -1;
+    return 0;
+  }
   switch (bt.dim * 100 + bt.bsel)
   {
   case 200: return launch_stages<T, 2, 0>(bt, parity, st, which);

@@ -24,8 +24,16 @@ enum : int
   kStageEfield = 6,  // edge points of the E grid (padded to 64 per block)
   kStageFix = 7,     // corrected border edges (the mesh's gather list)
   kStageInvDt = 8,   // owned cells (dt reduction)
-  kStages = 9
+  kStageFront = 9,   // sub-tiles of k_front (8 x 8 x fused_front_depth cells)
+  kStages = 10
 };
+
+// depth along z of a k_front sub-tile (3D: 2 in 8^3 blocks, else 4; 2D: the 8 x 8 tile)
+constexpr int
+fused_front_depth(int dim, int b)
+{
+  return dim == 3 ? (b == 8 ? 2 : 4) : 1;
+}
 
 // One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
 template <typename T>
@@ -88,6 +96,20 @@ struct FusedTune
   int grid_cap = 1 << 20; // element kernels: at most this many workgroups (multiple of 8)
   int tile_cap = 1 << 20; // flux tiles: at most this many workgroups (multiple of 8)
   int tile_z = 8;         // 3D flux tile depth: 8 (8x8x8 cells, 576 threads) or 4 (8x8x4, 320 threads)
+  int front = 0;          // 1: k_front runs the prim, efield and cell stages (kStagePrim; the other
+                          // two launch nothing); 0: k_prim, k_efield and k_cell (viscosity needs them)
+  int per_class = 1;      // slots of several block sizes: 1 launches every stage once per block-size
+                          // class with the kernels specialised to it; 0 once with the kernels that
+                          // dispatch on the slot's block size
+};
+
+// Block-size classes of a batch of mixed block sizes: the slots are ordered by block size, class c
+// holds the slots [s0, s0 + n) of block size bs, with work lists of their own.
+struct FusedClass
+{
+  int    bs = 0, s0 = 0, n = 0;
+  long * d_off[kStages] = {};
+  long   tot[kStages] = {};
 };
 
 // The device work list of a batch: per stage, the prefix offsets of every slot's work items
@@ -107,6 +129,8 @@ struct FusedBatch
   long           tot[kStages] = {};
   double *       clock = nullptr; // [nslots][2]: simulated time, steps (device-side dt only)
   FusedTune      tune;
+  int            ncls = 0; // block-size classes (bsel 0 only)
+  FusedClass     cls[3];
 };
 
 // block sizes the fused kernels are instantiated for

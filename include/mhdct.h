@@ -242,7 +242,8 @@ int mhdct_batch_step(mhdct_batch * batch, int nsteps, void * stream);
 /* The same as mhdct_batch_step without graphs, with HIP events recorded around each stage on
  * the stream it runs on: stage_ms[MHDCT_NUM_STAGES] receives the average milliseconds per step
  * of each stage (names from mhdct_stage_name; with the overlap on, k_edge runs beside
- * k_border_flux and k_flux_tile).  HANCOCK batches only (-1 for an RK2_SSP batch).
+ * k_border_flux and k_flux_tile; with k_front selected, stage 2 is k_front and stages 3 and 4
+ * launch nothing).  HANCOCK batches only (-1 for an RK2_SSP batch).
  * Synchronises `stream`. */
 #define MHDCT_NUM_STAGES 10
 int          mhdct_batch_step_timed(mhdct_batch * batch, int nsteps, void * stream, double * stage_ms);
@@ -263,6 +264,20 @@ int mhdct_batch_set_overlap(mhdct_batch * batch, int enable);
  * kernel (multiples of 8; below the work size the kernels loop), and the 3D flux-tile depth
  * tile_z: 8 (8x8x8 cells, 576-thread workgroups) or 4 (8x8x4, 320 threads). */
 int mhdct_batch_set_tune(mhdct_batch * batch, int grid_cap, int tile_cap, int tile_z);
+/* A batch of several block sizes: enable = 1 (the default) launches every stage once per
+ * block-size class, over that class's slots, with the kernels specialised to its block size; 0
+ * launches each stage once over all slots with kernels that dispatch on each slot's block size
+ * (three times the code).  The results are the same bit for bit.  No counterpart in the reference
+ * (a launch choice of this library). */
+int mhdct_batch_set_class_launch(mhdct_batch * batch, int enable);
+/* How the primitive, electric-field and cell stages run: enable = 0 (the default) as three
+ * kernels (k_prim, k_efield, k_cell) through HBM scratch; 1 as one workgroup per 8 x 8 x 4 cell
+ * sub-tile (8 x 8 x 2 in 8^3 blocks; 8 x 8 in 2D), k_front, holding the sub-tile's primitive
+ * state, face field and edge electric field in LDS.  The results are the same bit for bit; on
+ * MI355X k_front is the slower of the two (it recomputes the halo of every sub-tile).  No
+ * counterpart in the reference: a launch choice of this library (GodunovImplemV0.cpp:128-156 runs
+ * these stages as separate device loops). */
+int mhdct_batch_set_front(mhdct_batch * batch, int enable);
 /* Time each candidate (grid_cap, tile_cap, tile_z) over `reps` steps that leave the slots'
  * states and clocks unchanged, keep the fastest and return it. */
 int mhdct_batch_autotune(mhdct_batch * batch, int reps, void * stream, int * grid_cap, int * tile_cap,

@@ -191,18 +191,21 @@ def test_batch_tunables_do_not_change_results():
 
     cases = meshes_3d()
     nsteps = 3
-    print("[batch] tile depth 4, small launch caps, side-stream overlap vs defaults", flush=True)
+    print("[batch] tile depth 4, small launch caps, side-stream overlap, one launch over all block "
+          "sizes vs defaults", flush=True)
     ref = run_batch(cases, nsteps)
-    bt = BatchGodunovMHD(3, IMPL_FUSED)
-    ids = [bt.add(m, U, B, settings_for(slope)) for (m, U, B, slope) in cases]
-    bt.set_tune(64, 16, 4)
-    bt.set_overlap(True)
-    bt.step(nsteps)
-    bt.check()
-    for n, sid in enumerate(ids):
-        Ug, Bg = bt.state(sid)
-        (Ur, Br), _, _ = ref[n]
-        assert np.array_equal(Ug, Ur) and np.array_equal(Bg, Br), f"slot {n}"
+    for class_launch in (True, False):
+        bt = BatchGodunovMHD(3, IMPL_FUSED)
+        ids = [bt.add(m, U, B, settings_for(slope)) for (m, U, B, slope) in cases]
+        bt.set_tune(64, 16, 4)
+        bt.set_overlap(True)
+        bt.set_class_launch(class_launch)
+        bt.step(nsteps)
+        bt.check()
+        for n, sid in enumerate(ids):
+            Ug, Bg = bt.state(sid)
+            (Ur, Br), _, _ = ref[n]
+            assert np.array_equal(Ug, Ur) and np.array_equal(Bg, Br), f"slot {n} (class launch {class_launch})"
 
 
 @pytest.mark.gpu
