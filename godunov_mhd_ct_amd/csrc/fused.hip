@@ -1477,8 +1477,10 @@ edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx
 
 // ---------------------------------------------------------------------------------------------
 // k_edge: one thread per edge of a block, every edge once.  Each block's edges are enumerated
-// per direction, each direction padded to whole wavefronts, so a wavefront's direction is
-// uniform and compiled in.
+// per direction (k slowest), each direction padded to whole wavefronts; in 3D the wavefronts of
+// the three directions alternate (wavefront w takes the (w/3)-th 64 edges of direction w % 3), so
+// that the x-, y- and z-edges around the same cells run together and share the cells' records
+// in the caches.  A wavefront's direction is uniform and compiled in.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
@@ -1489,8 +1491,13 @@ edge_item(const FusedSlot<T> & S, long r0)
     constexpr int        per = G::NCE * G::EPDP;
     const int o = __builtin_amdgcn_readfirstlane(int(r0) / per);
     int       r = int(r0) - o * per;
-    const int a = DIM == 3 ? __builtin_amdgcn_readfirstlane(r / G::EPDP) : 2;
-    r -= DIM == 3 ? a * G::EPDP : 0;
+    int       a = 2;
+    if (DIM == 3)
+    {
+      const int w = __builtin_amdgcn_readfirstlane(r / 64);
+      a = w % 3;
+      r = (w / 3) * 64 + (r - w * 64);
+    }
     if (r >= G::EPD)
       return;
     const View<T, DIM, BS> V(S, o);
