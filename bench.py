@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
     ap.add_argument("--front", default="off", choices=["on", "off"],
                     help="primitive, electric-field and cell stages as one LDS-tiled kernel (on) or three (off)")
+    ap.add_argument("--stage-threads", default="",
+                    help="CELL,EDGE: workgroup sizes of k_cell and k_edge (256, 512 or 1024; default: the "
+                         "library's choice)")
     ap.add_argument("--precision-steps", type=int, default=0,
                     help="with --impl fp32: run the FP32-bulk and the FP64 form this many steps from the same "
                          "state and report the per-field relative L1 error (the precision config: 500 on kh3d)")
@@ -220,6 +223,8 @@ def bench_batch(args, cases, impl):
     dim = cases[0][0].dim
     bt = BatchGodunovMHD(dim, impl)
     bt.set_front(args.front == "on")
+    if args.stage_threads:
+        bt.set_stage_threads(*(int(x) for x in args.stage_threads.split(",")))
     for (m, U, B, st) in cases:
         bt.add(m, U, B, st)
     cells = bt.cells()

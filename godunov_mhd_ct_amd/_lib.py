@@ -94,6 +94,7 @@ SIGNATURES = {
     "mhdct_batch_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "mhdct_batch_set_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "mhdct_batch_set_front": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhdct_batch_set_stage_threads": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mhdct_batch_set_class_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "mhdct_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),

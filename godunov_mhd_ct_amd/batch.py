@@ -155,6 +155,11 @@ class BatchGodunovMHD:
         self._check(lib().mhdct_batch_set_front(self._h, int(bool(enable))), "batch_set_front")
         self.front = bool(enable)
 
+    def set_stage_threads(self, cell_threads: int, edge_threads: int):
+        """Workgroup sizes (256, 512 or 1024) of k_cell and k_edge; same results."""
+        self._check(lib().mhdct_batch_set_stage_threads(self._h, int(cell_threads), int(edge_threads)),
+                    "batch_set_stage_threads")
+
     def autotune(self, reps: int = 2, stream=None):
         """Pick the launch caps and the flux-tile depth by timing; returns (grid_cap, tile_cap, tile_z)."""
         g, t, z = C.c_int(), C.c_int(), C.c_int()

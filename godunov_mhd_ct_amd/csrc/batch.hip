@@ -841,6 +841,25 @@ extern "C"
   }
 
   int
+  mhdct_batch_set_stage_threads(mhdct_batch * b, int cell_threads, int edge_threads)
+  {
+    if (!b)
+      return -1;
+    auto ok = [](int t) { return t == 256 || t == 512 || t == 1024; };
+    if (!ok(cell_threads) || !ok(edge_threads))
+    {
+      b->err = "mhdct_batch_set_stage_threads: 256, 512 or 1024 threads";
+      return -1;
+    }
+    b->tune.cell_threads = cell_threads;
+    b->tune.edge_threads = edge_threads;
+    b->bd.tune = b->tune;
+    b->bf.tune = b->tune;
+    drop_graphs(b);
+    return 0;
+  }
+
+  int
   mhdct_batch_set_front(mhdct_batch * b, int enable)
   {
     if (!b)

@@ -734,7 +734,7 @@ cell_item(const FusedSlot<T> & S, long r0)
 }
 
 template <typename T, int DIM, int BSEL, bool HK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -1521,7 +1521,7 @@ edge_item(const FusedSlot<T> & S, long r0)
 }
 
 template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -1819,6 +1819,9 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
   const int            cap = bt.tune.grid_cap / kXcd * kXcd;
   const int            tcap = bt.tune.tile_cap / kXcd * kXcd;
   auto                 grid = [&](int stage) { return grid_for(bt.tot[stage], 256, cap); };
+  // workgroup sizes of the cell and edge stages (k_cell, k_edge): a larger workgroup puts more
+  // consecutive wavefronts, which read the same cells, on one CU
+  const int            ct = bt.tune.cell_threads, et = bt.tune.edge_threads;
   auto                 on = [&](int stage) { return which < 0 || which == stage; };
   const bool           front = bt.tune.front != 0;
   if (on(kStagePrim) && front)
@@ -1841,10 +1844,12 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
   if (on(kStageCell) && !front)
   {
     if (bt.rk2)
-      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, false>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, false>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
+                         bt.d_off[kStageCell],
                          bt.tot[kStageCell]);
     else
-      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, true>), grid(kStageCell), 256, 0, st, sl, n, bt.d_off[kStageCell],
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, true>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
+                         bt.d_off[kStageCell],
                          bt.tot[kStageCell]);
   }
   if (on(kStageBorder) && bt.tot[kStageBorder] > 0)
@@ -1860,7 +1865,8 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
                          (tile_threads<DIM, 8>()), 0, st, sl, n, bt.d_off[kStageTile], bt.tot[kStageTile]);
   }
   if (on(kStageEdge))
-    hipLaunchKernelGGL((k_edge<T, DIM, BSEL>), grid(kStageEdge), 256, 0, st, sl, n, bt.d_off[kStageEdge],
+    hipLaunchKernelGGL((k_edge<T, DIM, BSEL>), grid_for(bt.tot[kStageEdge], et, cap), et, 0, st, sl, n,
+                       bt.d_off[kStageEdge],
                        bt.tot[kStageEdge]);
   if (on(kStageFix) && bt.tot[kStageFix] > 0)
   {

@@ -1,10 +1,13 @@
 // phys_fast.hpp -- the HLLD flux and the 2D-HLLD EMF of phys.hpp with their divisions and square
 // roots shared: the same mathematics (models/RiemannSolvers_MHD.h:311-551, 629-869) with every
-// repeated denominator inverted once and multiplied in, and the Alfven-speed maxima taken over
-// squares before one square root.  The fused form uses these: its FP64 kernels are bound by the
-// FP64 division and square-root sequences (about ten VALU instructions each), and this cuts the
-// 2D-HLLD EMF from 70 divisions and 28 square roots to 20 and 20, the HLLD flux from 27 divisions
-// to 11.  The results differ from the reference's operation order by a few ulps per evaluation
+// repeated denominator inverted once and multiplied in, the Alfven-speed maxima taken over
+// squares before one square root, and the largest of several fast speeds taken as the square
+// root of the largest radicand (the square root is monotonic).  The fused form uses these: its
+// FP64 kernels are bound by the FP64 division and square-root sequences, and this cuts the 2D-HLLD
+// EMF from 70 divisions and 28 square roots to 20 and 14, the HLLD flux from 27 divisions and 6
+// square roots to 13 and 5.  In FP64 each reciprocal and square root is a Newton-Raphson
+// refinement of the hardware estimate (rcp_nr, sqrt_nr): 5 and 10 instructions, against the 11
+// and 17 of the correctly rounded sequences.  The results differ from the reference's operation order by a few ulps per evaluation
 // (the fused form's parity bar is the north star's 1e-12 relative L-inf per field, momentum as
 // a vector); the plain form keeps phys.hpp's order, bit for bit the reference's.
 #pragma once
@@ -14,6 +17,56 @@
 namespace mhdct
 {
 
+// 1/x in FP64: the hardware estimate v_rcp_f64 refined by two Newton-Raphson steps
+// r <- r + r (1 - x r) (the quadratically convergent reciprocal iteration), each as two fused
+// multiply-adds.  Within about one ulp of 1/x for normal x of either sign; not correctly rounded.
+// The FP32 form keeps 1/x (the hardware-approximate FP32 division of its build).
+template <typename T>
+__host__ __device__ __forceinline__ T
+rcp_nr(T x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8)
+  {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+  }
+  else
+#endif
+    return T(1) / x;
+}
+
+// sqrt(x) in FP64: the hardware estimate y = v_rsq_f64(x) of 1/sqrt(x), one step of Goldschmidt's
+// coupled iteration for g = sqrt(x) and h = 1/(2 sqrt(x)) (r = 1/2 - g h; g <- g + g r;
+// h <- h + h r), then two residual corrections g <- g + (x - g^2) h.  Within about one ulp of
+// sqrt(x) for normal x >= 0 (sqrt(+-0) = +-0, NaN below zero); not correctly rounded.
+template <typename T>
+__host__ __device__ __forceinline__ T
+sqrt_nr(T x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8)
+  {
+    const double y = __builtin_amdgcn_rsq(x);
+    double       g = x * y;
+    double       h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x == 0.0 ? x : g;
+  }
+  else
+#endif
+    return sqrt(x);
+}
+
 // HLLD flux along x (riemann_hlld, models/RiemannSolvers_MHD.h:311-551), shared denominators:
 // 1/(rcr + rcl), 1/(sl - ustar), 1/(sr - ustar), 1/e*_L, 1/e*_R, 1/(sqrt rho*_L + sqrt rho*_R),
 // and one reciprocal density per state in the fast speeds.
@@ -21,7 +74,7 @@ template <typename T>
 __host__ __device__ inline void
 hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
 {
-  const T entho = T(1) / (g.gamma0 - T(1));
+  const T entho = rcp_nr(g.gamma0 - T(1));
   const T a = T(0.5) * (ql[kA] + qr[kA]);
   const T sgnm = (a >= T(0)) ? T(1) : T(-1);
   ql[kA] = a;
@@ -42,27 +95,26 @@ hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
   const T ptotr = pr + emagr;
   const T vdotbr = ur * a + vr * br + wr * cr;
 
-  // fast speeds along x: d2 = (B^2/rho + c^2)/2, c_f = sqrt(d2 + sqrt(d2^2 - c^2 Bx^2 / rho))
-  auto cfast = [&](T r, T p, T bb) {
-    const T ir = T(1) / r;
+  // fast speeds along x: d2 = (B^2/rho + c^2)/2, c_f = sqrt(d2 + sqrt(d2^2 - c^2 Bx^2 / rho)); the
+  // larger of the two is the square root of the larger radicand
+  auto cfast2 = [&](T r, T p, T bb) {
+    const T ir = rcp_nr(r);
     const T c2 = g.gamma0 * p * ir;
     const T d2 = T(0.5) * (bb * ir + c2);
-    return sqrt(d2 + sqrt(d2 * d2 - c2 * a2 * ir));
+    return d2 + sqrt_nr(d2 * d2 - c2 * a2 * ir);
   };
-  const T cfl_ = cfast(rl, pl, T(2) * emagl);
-  const T cfr_ = cfast(rr, pr, T(2) * emagr);
-  const T cmax = fmax(cfl_, cfr_);
+  const T cmax = sqrt_nr(fmax(cfast2(rl, pl, T(2) * emagl), cfast2(rr, pr, T(2) * emagr)));
   const T sl = fmin(ul, ur) - cmax;
   const T sr = fmax(ul, ur) + cmax;
 
   const T rcl = rl * (ul - sl);
   const T rcr = rr * (sr - ur);
-  const T irc = T(1) / (rcr + rcl);
+  const T irc = rcp_nr(rcr + rcl);
   const T ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) * irc;
   const T ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) * irc;
 
   // left star region
-  const T isl = T(1) / (sl - ustar);
+  const T isl = rcp_nr(sl - ustar);
   const T rstarl = rl * (sl - ul) * isl;
   const T estarl = rl * (sl - ul) * (sl - ustar) - a2;
   const T el = rl * (sl - ul) * (sl - ul) - a2;
@@ -76,7 +128,7 @@ hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
   }
   else
   {
-    const T ie = T(1) / estarl;
+    const T ie = rcp_nr(estarl);
     const T aus = a * (ustar - ul) * ie;
     const T elr = el * ie;
     vstarl = vl - bl * aus;
@@ -86,10 +138,10 @@ hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
   }
   const T vdotbstarl = ustar * a + vstarl * bstarl + wstarl * cstarl;
   const T etotstarl = ((sl - ul) * etotl - ptotl * ul + ptotstar * ustar + a * (vdotbl - vdotbstarl)) * isl;
-  const T sqrrstarl = sqrt(rstarl);
+  const T sqrrstarl = sqrt_nr(rstarl);
 
   // right star region
-  const T isr = T(1) / (sr - ustar);
+  const T isr = rcp_nr(sr - ustar);
   const T rstarr = rr * (sr - ur) * isr;
   const T estarr = rr * (sr - ur) * (sr - ustar) - a2;
   const T er = rr * (sr - ur) * (sr - ur) - a2;
@@ -103,7 +155,7 @@ hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
   }
   else
   {
-    const T ie = T(1) / estarr;
+    const T ie = rcp_nr(estarr);
     const T aus = a * (ustar - ur) * ie;
     const T err = er * ie;
     vstarr = vr - br * aus;
@@ -113,14 +165,14 @@ hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
   }
   const T vdotbstarr = ustar * a + vstarr * bstarr + wstarr * cstarr;
   const T etotstarr = ((sr - ur) * etotr - ptotr * ur + ptotstar * ustar + a * (vdotbr - vdotbstarr)) * isr;
-  const T sqrrstarr = sqrt(rstarr);
+  const T sqrrstarr = sqrt_nr(rstarr);
 
   // Alfven waves: u* -+ |Bx| / sqrt(rho*), compared with 0 as |Bx| / sqrt(rho*) against u*
-  const T sal = ustar - fabs(a) / sqrrstarl;
-  const T sar = ustar + fabs(a) / sqrrstarr;
+  const T sal = ustar - fabs(a) * rcp_nr(sqrrstarl);
+  const T sar = ustar + fabs(a) * rcp_nr(sqrrstarr);
 
   // double-star region
-  const T iden = T(1) / (sqrrstarl + sqrrstarr);
+  const T iden = rcp_nr(sqrrstarl + sqrrstarr);
   const T sq2 = sqrrstarl * sqrrstarr;
   const T vss = (sqrrstarl * vstarl + sqrrstarr * vstarr + sgnm * (bstarr - bstarl)) * iden;
   const T wss = (sqrrstarl * wstarl + sqrrstarr * wstarr + sgnm * (cstarr - cstarl)) * iden;
@@ -202,26 +254,29 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
   const T ELR = uLR * bLR - vLR * aLR;
   const T ERR = uRR * bRR - vRR * aRR;
 
-  // fast speeds along x and y with one reciprocal density per state
-  auto cfast = [&](T r, T p, T a, T b, T c, T & cx, T & cy) {
-    const T ir = T(1) / r;
+  // fast speeds along x and y with one reciprocal density per state: the radicands
+  // d2 + sqrt(d2^2 - c^2 B_n^2 / rho); the largest speed is the square root of the largest
+  auto cfast2 = [&](T r, T p, T a, T b, T c, T & cx, T & cy) {
+    const T ir = rcp_nr(r);
     const T c2 = g.gamma0 * p * ir;
     const T d2 = T(0.5) * ((a * a + b * b + c * c) * ir + c2);
     const T dd = d2 * d2;
     const T ci = c2 * ir;
-    cx = sqrt(d2 + sqrt(dd - ci * a * a));
-    cy = sqrt(d2 + sqrt(dd - ci * b * b));
+    cx = d2 + sqrt_nr(dd - ci * a * a);
+    cy = d2 + sqrt_nr(dd - ci * b * b);
   };
   T cLLx, cLLy, cLRx, cLRy, cRLx, cRLy, cRRx, cRRy;
-  cfast(rLL, pLL, aLL, bLL, cLL, cLLx, cLLy);
-  cfast(rLR, pLR, aLR, bLR, cLR, cLRx, cLRy);
-  cfast(rRL, pRL, aRL, bRL, cRL, cRLx, cRLy);
-  cfast(rRR, pRR, aRR, bRR, cRR, cRRx, cRRy);
+  cfast2(rLL, pLL, aLL, bLL, cLL, cLLx, cLLy);
+  cfast2(rLR, pLR, aLR, bLR, cLR, cLRx, cLRy);
+  cfast2(rRL, pRL, aRL, bRL, cRL, cRLx, cRLy);
+  cfast2(rRR, pRR, aRR, bRR, cRR, cRRx, cRRy);
+  const T cmx = sqrt_nr(max4(cLLx, cLRx, cRLx, cRRx));
+  const T cmy = sqrt_nr(max4(cLLy, cLRy, cRLy, cRRy));
 
-  const T SL = min4(uLL, uLR, uRL, uRR) - max4(cLLx, cLRx, cRLx, cRRx);
-  const T SR = max4(uLL, uLR, uRL, uRR) + max4(cLLx, cLRx, cRLx, cRRx);
-  const T SB = min4(vLL, vLR, vRL, vRR) - max4(cLLy, cLRy, cRLy, cRRy);
-  const T ST = max4(vLL, vLR, vRL, vRR) + max4(cLLy, cLRy, cRLy, cRRy);
+  const T SL = min4(uLL, uLR, uRL, uRR) - cmx;
+  const T SR = max4(uLL, uLR, uRL, uRR) + cmx;
+  const T SB = min4(vLL, vLR, vRL, vRR) - cmy;
+  const T ST = max4(vLL, vLR, vRL, vRR) + cmy;
 
   const T PtLL = pLL + T(0.5) * (aLL * aLL + bLL * bLL + cLL * cLL);
   const T PtLR = pLR + T(0.5) * (aLR * aLR + bLR * bLR + cLR * cLR);
@@ -233,14 +288,14 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
   const T rcLLy = rLL * (vLL - SB), rcLRy = rLR * (ST - vLR);
   const T rcRLy = rRL * (vRL - SB), rcRRy = rRR * (ST - vRR);
 
-  const T ustar = (rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR + (PtLL - PtRL + PtLR - PtRR)) /
-                  (rcLLx + rcLRx + rcRLx + rcRRx);
-  const T vstar = (rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR + (PtLL - PtLR + PtRL - PtRR)) /
-                  (rcLLy + rcLRy + rcRLy + rcRRy);
+  const T ustar = (rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR + (PtLL - PtRL + PtLR - PtRR)) *
+                  rcp_nr(rcLLx + rcLRx + rcRLx + rcRRx);
+  const T vstar = (rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR + (PtLL - PtLR + PtRL - PtRR)) *
+                  rcp_nr(rcLLy + rcLRy + rcRLy + rcRRy);
 
   // compression ratios across the outer waves: f = (S - u)/(S - u*)
-  const T iSL = T(1) / (SL - ustar), iSR = T(1) / (SR - ustar);
-  const T iSB = T(1) / (SB - vstar), iST = T(1) / (ST - vstar);
+  const T iSL = rcp_nr(SL - ustar), iSR = rcp_nr(SR - ustar);
+  const T iSB = rcp_nr(SB - vstar), iST = rcp_nr(ST - vstar);
   const T fLLx = (SL - uLL) * iSL, fLRx = (SL - uLR) * iSL, fRLx = (SR - uRL) * iSR, fRRx = (SR - uRR) * iSR;
   const T fLLy = (SB - vLL) * iSB, fRLy = (SB - vRL) * iSB, fLRy = (ST - vLR) * iST, fRRy = (ST - vRR) * iST;
 
@@ -257,20 +312,20 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
   // |A*LL|/sqrt(rho*LL), smallc) with rho*x = rho f_x, rho* = rho f_x f_y, A* = a f_y: the squares
   // are a^2 / (rho f_x) times 1 and f_y.  Likewise calfvenB with b, f_y and f_x.
   auto sq = [](T x) { return x * x; };
-  const T ixLL = T(1) / (rLL * fLLx), ixLR = T(1) / (rLR * fLRx), ixRL = T(1) / (rRL * fRLx), ixRR = T(1) / (rRR * fRRx);
-  const T iyLL = T(1) / (rLL * fLLy), iyLR = T(1) / (rLR * fLRy), iyRL = T(1) / (rRL * fRLy), iyRR = T(1) / (rRR * fRRy);
+  const T ixLL = rcp_nr(rLL * fLLx), ixLR = rcp_nr(rLR * fLRx), ixRL = rcp_nr(rRL * fRLx), ixRR = rcp_nr(rRR * fRRx);
+  const T iyLL = rcp_nr(rLL * fLLy), iyLR = rcp_nr(rLR * fLRy), iyRL = rcp_nr(rRL * fRLy), iyRR = rcp_nr(rRR * fRRy);
   const T sc2 = g.smallc * g.smallc;
-  const T caL = sqrt(max5(sq(aLR) * ixLR, sq(aLR) * ixLR * fLRy, sq(aLL) * ixLL, sq(aLL) * ixLL * fLLy, sc2));
-  const T caR = sqrt(max5(sq(aRR) * ixRR, sq(aRR) * ixRR * fRRy, sq(aRL) * ixRL, sq(aRL) * ixRL * fRLy, sc2));
-  const T caB = sqrt(max5(sq(bLL) * iyLL, sq(bLL) * iyLL * fLLx, sq(bRL) * iyRL, sq(bRL) * iyRL * fRLx, sc2));
-  const T caT = sqrt(max5(sq(bLR) * iyLR, sq(bLR) * iyLR * fLRx, sq(bRR) * iyRR, sq(bRR) * iyRR * fRRx, sc2));
+  const T caL = sqrt_nr(max5(sq(aLR) * ixLR, sq(aLR) * ixLR * fLRy, sq(aLL) * ixLL, sq(aLL) * ixLL * fLLy, sc2));
+  const T caR = sqrt_nr(max5(sq(aRR) * ixRR, sq(aRR) * ixRR * fRRy, sq(aRL) * ixRL, sq(aRL) * ixRL * fRLy, sc2));
+  const T caB = sqrt_nr(max5(sq(bLL) * iyLL, sq(bLL) * iyLL * fLLx, sq(bRL) * iyRL, sq(bRL) * iyRL * fRLx, sc2));
+  const T caT = sqrt_nr(max5(sq(bLR) * iyLR, sq(bLR) * iyLR * fLRx, sq(bRR) * iyRR, sq(bRR) * iyRR * fRRx, sc2));
 
   const T SAL = fmin(ustar - caL, T(0));
   const T SAR = fmax(ustar + caR, T(0));
   const T SAB = fmin(vstar - caB, T(0));
   const T SAT = fmax(vstar + caT, T(0));
-  const T iLR = T(1) / (SAR - SAL);
-  const T iBT = T(1) / (SAT - SAB);
+  const T iLR = rcp_nr(SAR - SAL);
+  const T iBT = rcp_nr(SAT - SAB);
 
   const T AsT = (SAR * AsRR - SAL * AsLR) * iLR;
   const T AsB = (SAR * AsRL - SAL * AsLL) * iLR;

@@ -192,14 +192,15 @@ def test_batch_tunables_do_not_change_results():
     cases = meshes_3d()
     nsteps = 3
     print("[batch] tile depth 4, small launch caps, side-stream overlap, one launch over all block "
-          "sizes vs defaults", flush=True)
+          "sizes, cell/edge workgroup sizes vs defaults", flush=True)
     ref = run_batch(cases, nsteps)
-    for class_launch in (True, False):
+    for class_launch, threads in ((True, (256, 1024)), (False, (512, 512))):
         bt = BatchGodunovMHD(3, IMPL_FUSED)
         ids = [bt.add(m, U, B, settings_for(slope)) for (m, U, B, slope) in cases]
         bt.set_tune(64, 16, 4)
         bt.set_overlap(True)
         bt.set_class_launch(class_launch)
+        bt.set_stage_threads(*threads)
         bt.step(nsteps)
         bt.check()
         for n, sid in enumerate(ids):
