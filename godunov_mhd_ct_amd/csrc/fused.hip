@@ -1521,7 +1521,7 @@ edge_item(const FusedSlot<T> & S, long r0)
 }
 
 template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
