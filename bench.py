@@ -136,7 +136,7 @@ def cpu_baseline(cases, steps):
 
 
 # the kernel sources whose counters profiles/hbm_traffic.json records
-COUNTED_SOURCES = ["fused.hip", "fused.hpp", "phys.hpp", "common.hpp", "batch.hip"]
+COUNTED_SOURCES = ["fused.hip", "fused.hpp", "phys.hpp", "phys_fast.hpp", "common.hpp", "batch.hip"]
 
 
 def sources_digest():
