@@ -2,14 +2,14 @@
 // roots shared: the same mathematics (models/RiemannSolvers_MHD.h:311-551, 629-869) with every
 // repeated denominator inverted once and multiplied in, the Alfven-speed maxima taken over
 // squares before one square root, and the largest of several fast speeds taken as the square
-// root of the largest radicand (the square root is monotonic).  The fused form uses these: its
-// FP64 kernels are bound by the FP64 division and square-root sequences, and this cuts the 2D-HLLD
-// EMF from 70 divisions and 28 square roots to 20 and 14, the HLLD flux from 27 divisions and 6
-// square roots to 13 and 5.  In FP64 each reciprocal and square root is a Newton-Raphson
-// refinement of the hardware estimate (rcp_nr, sqrt_nr): 5 and 10 instructions, against the 11
-// and 17 of the correctly rounded sequences.  The results differ from the reference's operation order by a few ulps per evaluation
-// (the fused form's parity bar is the north star's 1e-12 relative L-inf per field, momentum as
-// a vector); the plain form keeps phys.hpp's order, bit for bit the reference's.
+// root of the largest radicand (the square root is monotonic).  This cuts the 2D-HLLD EMF from 70
+// divisions and 28 square roots to 20 reciprocals and 14 square roots, the HLLD flux from 27
+// divisions and 6 square roots to 11 and 5.  In FP64 each reciprocal and square root is a
+// Newton-Raphson refinement of the hardware estimate (rcp_nr, sqrt_nr): 5 and 10 instructions,
+// against the 11 and 17 of the correctly rounded sequences.  The fused form uses these; its
+// results differ from the reference's operation order by a few ulps per evaluation (its parity
+// bar is the north star's 1e-12 relative L-inf per field, momentum as a vector).  The plain form
+// keeps phys.hpp's order, bit for bit the reference's.
 #pragma once
 
 #include "phys.hpp"
