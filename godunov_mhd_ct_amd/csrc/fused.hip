@@ -87,7 +87,8 @@ struct Geo
   static constexpr int NCE = DIM == 3 ? 3 : 1;            // EMF components
   static constexpr int NEC = I::ecells();                 // edge-grid points per component
   static constexpr int EPD = DIM == 3 ? BS * NE * NE : NE * NE; // edges per direction
-  static constexpr int EPDP = (EPD + 63) / 64 * 64;       // ... padded to whole wavefronts
+  static constexpr int NPOS = (BS + 2) * NE * BS;          // 3D y-, z-edge positions
+  static constexpr int NWD = fused_edge_waves(DIM, BS);   // k_edge wavefronts per direction
   static constexpr int P1 = (G1 + 63) / 64 * 64;          // k_cell items per block
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
   static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
@@ -1475,53 +1476,216 @@ edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx
   S.emf[(long(o) * G::NCE + comp) * G::NEC + IdxC<DIM, BS>::edge(i, j, k)] = E;
 }
 
+// Corner states by parts (ComputeEmfAndStoreFunctor.cpp:105-367).  The corner of KIND (0 LB,
+// 1 RT, 2 RB, 3 LT) reconstructed from cell c0 across the directions (D0, D1) is
+//   q = Q2(c0) + 1/2 (s0 dQ/dD0 + s1 dQ/dD1)      for rho, p, the velocity and the B component
+//                                                  along the edge (the "cell part"),
+//   B_D0 = Bf_D0(p0) + 1/2 b0 dB_D0/dD1 (p0),  B_D1 = Bf_D1(p1) + 1/2 b1 dB_D1/dD0 (p1)
+// (the "face part"), Bf the face field plus, for HANCOCK, the face-B source term, p0 = c0 + e_D0
+// for KIND 1, 2 and p1 = c0 + e_D1 for KIND 1, 3: edge_state's arithmetic, split.  The y-edges add
+// the dBy slope terms twice, as edge_state does.
+//
+// cell part of the corner of KIND at cell c0 (index in the 1-ghost record grid) of a 3D z-edge
+// (D1 = y) or y-edge (D1 = z), D0 = x: h[0..4] rho, p, u, v, w (floored as in the reference),
+// h[5] the B component that stays with the cell (z-edges: Bz; y-edges: By)
+template <typename T, int DIM, int BS, int KIND, int D1>
+__device__ __forceinline__ void
+edge_cell_part(const View<T, DIM, BS> & V, int c0, T * h)
+{
+  constexpr int D0 = 0;
+  constexpr int s0 = (KIND == 0 || KIND == 3) ? -1 : 1;
+  constexpr int s1 = (KIND == 0 || KIND == 2) ? -1 : 1;
+  const T       f0 = T(s0), f1 = T(s1);
+  auto          tr = [&](int v) { return T(0.5) * (f0 * V.sl(D0, v, c0) + f1 * V.sl(D1, v, c0)); };
+  for (int v = 0; v < 5; ++v)
+    h[v] = V.q2(v, c0) + T(0.5) * (f0 * V.sl(D0, v, c0) + f1 * V.sl(D1, v, c0));
+  if (D1 == 1)
+    h[5] = V.q2(kC, c0) + tr(kC);
+  else
+  {
+    const T b = V.q2(kB, c0) + tr(kB);
+    h[5] = b + tr(kB);
+  }
+  h[kD] = fmax(V.S.gt.smallr, h[kD]);
+  h[kP] = fmax(V.S.gt.smallp * h[kD], h[kP]);
+}
+
+// the corner state q[8] from its cell part h and its face values (bf0, fs0) = (Bf_x, dBx/dD1) at
+// p0 and (bf1, fs1) = (Bf_D1, dB_D1/dx) at p1
+template <typename T, int DIM, int KIND, int D1>
+__device__ __forceinline__ void
+edge_corner(const T * h, T bf0, T fs0, T bf1, T fs1, T * q)
+{
+  constexpr int b0 = (KIND == 0 || KIND == 2) ? -1 : 1;
+  constexpr int b1 = (KIND == 0 || KIND == 3) ? -1 : 1;
+  for (int v = 0; v < 5; ++v)
+    q[v] = h[v];
+  const T B0 = bf0 + T(0.5) * (T(b0) * fs0);
+  const T B1 = bf1 + T(0.5) * (T(b1) * fs1);
+  q[kA] = B0;
+  if (D1 == 1)
+  {
+    q[kB] = B1;
+    q[kC] = h[5];
+  }
+  else
+  {
+    q[kB] = h[5];
+    q[kC] = B1;
+  }
+}
+
+// Bf_c (plus the face-B source for HANCOCK) and the transverse slope of face component c along
+// d at the low face of cell (a, b, e)
+template <typename T, int DIM, int BS, int C, int D, bool HK>
+__device__ __forceinline__ void
+edge_face(const View<T, DIM, BS> & V, int a, int b, int e, T & bf, T & fs)
+{
+  using VW = View<T, DIM, BS>;
+  constexpr int fsi = DIM == 2 ? C : (C == 0 ? D - 1 : (C == 1 ? (D == 0 ? 2 : 3) : (D == 0 ? 4 : 5)));
+  const int     c1 = VW::c1(a, b, e);
+  bf = V.bf(C, a, b, e);
+  if (HK)
+    bf += V.sf(C, c1);
+  fs = V.fs(fsi, c1);
+}
+
+// value x of the lane whose byte address (4 x lane) is addr
+template <typename T>
+__device__ __forceinline__ T
+lane_take(T x, int addr)
+{
+  if constexpr (sizeof(T) == 8)
+  {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u))));
+    const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u >> 32))));
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  }
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
+}
+
+// The EMF along axis AX (2 or 1: the 3D edge directions across x) at position p of block o, lane
+// `lane` of its wavefront.  Positions run over rows of b + 2 along x, i = -1 .. b: the
+// corners of the edge at A = (i, j, k) are KIND 0 at A, KIND 3 at C = A - e_D1, KIND 2 at
+// A' = A - e_x and KIND 1 at C' = A' - e_D1, with Bf_x at A (KIND 0, 2) and at C (KIND 1, 3) and
+// Bf_D1 at A (KIND 0, 3) and at A' (KIND 1, 2).  A' and C' are the A and C of the position one
+// lane below, so every lane forms the cell parts of its A and C for the lane above (KIND 2, 1)
+// and takes its own from the lane below (cross-lane reads, no memory traffic).  The lanes at
+// i = -1 and lane 0 of each wavefront (the position before its first one) only pass theirs on,
+// so every load a wavefront issues is useful to all of its lanes: 2 cells' records per edge
+// instead of 4.  Same arithmetic as edge_at, the same states bit for bit.
+template <typename T, int DIM, int BS, int AX, bool HK>
+__device__ __forceinline__ void
+edge_at_rows(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int p, int lane, T dtdx)
+{
+  using G = Geo<DIM, BS>;
+  using VW = View<T, DIM, BS>;
+  static_assert(DIM == 3 && (AX == 1 || AX == 2), "edge_at_rows: 3D y- and z-edges");
+  constexpr int e0 = BS + 2;
+  constexpr int e1 = AX == 1 ? BS : G::NE;
+  constexpr int D1 = AX == 2 ? 1 : 2;
+  constexpr int NH = 6;
+  const int     i = p % e0 - 1, j = (p / e0) % e1, k = p / (e0 * e1);
+  int           c[3] = { i, j, k };
+  const int     cA = VW::c1(c[0], c[1], c[2]);
+  c[D1] -= 1;
+  const int cC = VW::c1(c[0], c[1], c[2]);
+  // the cell parts of A and C: KIND 0 and 3 for this edge, KIND 2 and 1 for the lane above (all
+  // four from one read of the two cells' records), and Bf_D1 at A, which both need
+  T hA0[NH], hC3[NH], h2[NH], h1[NH], bA1, fA1;
+  edge_cell_part<T, DIM, BS, 0, D1>(V, cA, hA0);
+  edge_cell_part<T, DIM, BS, 2, D1>(V, cA, h2);
+  edge_cell_part<T, DIM, BS, 3, D1>(V, cC, hC3);
+  edge_cell_part<T, DIM, BS, 1, D1>(V, cC, h1);
+  edge_face<T, DIM, BS, D1, 0, HK>(V, i, j, k, bA1, fA1);
+  const int addr = (lane > 0 ? lane - 1 : 0) * 4;
+  T         hAp[NH], hCp[NH];
+  for (int v = 0; v < NH; ++v)
+  {
+    hAp[v] = lane_take(h2[v], addr);
+    hCp[v] = lane_take(h1[v], addr);
+  }
+  const T bAp1 = lane_take(bA1, addr), fAp1 = lane_take(fA1, addr);
+  if (lane == 0 || i < 0)
+    return;
+  // Bf_x at A and C
+  T bA0, fA0, bC0, fC0;
+  edge_face<T, DIM, BS, 0, D1, HK>(V, i, j, k, bA0, fA0);
+  edge_face<T, DIM, BS, 0, D1, HK>(V, c[0], c[1], c[2], bC0, fC0);
+  // the reference's corner order (RT, RB, LT, LB) per axis: KIND 2 is RB for z-edges, LT for y
+  constexpr int slot3 = AX == 1 ? kRB : kLT;
+  constexpr int slot2 = AX == 1 ? kLT : kRB;
+  T             qe[4][8];
+  edge_corner<T, DIM, 0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
+  edge_corner<T, DIM, 3, D1>(hC3, bC0, fC0, bA1, fA1, qe[slot3]);
+  edge_corner<T, DIM, 2, D1>(hAp, bA0, fA0, bAp1, fAp1, qe[slot2]);
+  edge_corner<T, DIM, 1, D1>(hCp, bC0, fC0, bAp1, fAp1, qe[kRT]);
+  const T E = edge_emf_fast<AX>(qe, S.gt) * dtdx;
+  S.emf[(long(o) * G::NCE + AX) * G::NEC + IdxC<DIM, BS>::edge(i, j, k)] = E;
+}
+
 // ---------------------------------------------------------------------------------------------
 // k_edge: one thread per edge of a block, every edge once.  Each block's edges are enumerated
-// per direction (k slowest), each direction padded to whole wavefronts; in 3D the wavefronts of
-// the three directions alternate (wavefront w takes the (w/3)-th 64 edges of direction w % 3), so
-// that the x-, y- and z-edges around the same cells run together and share the cells' records
-// in the caches.  A wavefront's direction is uniform and compiled in.
+// per direction (k slowest) in NWD wavefronts per direction: in 3D the x-edges one per lane, the
+// y- and z-edges as positions of edge_at_rows, 63 per wavefront after a halo lane; in 2D the edges
+// one per lane (edge_at_rows's extra live values spill in the 2D kernels).  In 3D the wavefronts of the three directions alternate (wavefront w takes the (w/3)-th of
+// direction w % 3), so that the x-, y- and z-edges around the same cells run together and share
+// the cells' records in the caches.  A wavefront's direction is uniform and compiled in.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
 edge_item(const FusedSlot<T> & S, long r0)
 {
-
   using G = Geo<DIM, BS>;
-    constexpr int        per = G::NCE * G::EPDP;
-    const int o = __builtin_amdgcn_readfirstlane(int(r0) / per);
-    int       r = int(r0) - o * per;
-    int       a = 2;
-    if (DIM == 3)
-    {
-      const int w = __builtin_amdgcn_readfirstlane(r / 64);
-      a = w % 3;
-      r = (w / 3) * 64 + (r - w * 64);
-    }
-    if (r >= G::EPD)
+  constexpr int per = G::NCE * G::NWD * 64;
+  const int     o = __builtin_amdgcn_readfirstlane(int(r0) / per);
+  const int     r = int(r0) - o * per;
+  int           w = __builtin_amdgcn_readfirstlane(r / 64);
+  const int     lane = r - w * 64;
+  int           a = 2;
+  if (DIM == 3)
+  {
+    a = w % 3;
+    w = w / 3;
+  }
+  const View<T, DIM, BS> V(S, o);
+  const T                dtdx = T(S.dt / dxd<BS>(S.m, S.scaling, o));
+  // the integrator is uniform per slot, so per wavefront; each variant is compiled on its own
+  if (DIM == 2 || a == 0)
+  {
+    const int re = w * 64 + lane;
+    if (re >= G::EPD)
       return;
-    const View<T, DIM, BS> V(S, o);
-    const T                dtdx = T(S.dt / dxd<BS>(S.m, S.scaling, o));
-    // the integrator is uniform per slot, so per wavefront; each variant is compiled on its own
+    if (S.rk2)
+      edge_at<T, DIM, BS, DIM == 2 ? 2 : 0, false>(S, V, o, re, dtdx);
+    else
+      edge_at<T, DIM, BS, DIM == 2 ? 2 : 0, true>(S, V, o, re, dtdx);
+    return;
+  }
+  if constexpr (DIM == 3)
+  {
+    // positions of the y- and z-edges: lanes outside them feed no edge lane and leave at once
+    const int p = w * 63 - 1 + lane;
+    if (p < 0 || p >= G::NPOS)
+      return;
     if (S.rk2)
     {
-      if (DIM == 2 || a == 2)
-        edge_at<T, DIM, BS, 2, false>(S, V, o, r, dtdx);
-      else if (a == 1)
-        edge_at<T, DIM, BS, 1, false>(S, V, o, r, dtdx);
+      if (a == 2)
+        edge_at_rows<T, DIM, BS, 2, false>(S, V, o, p, lane, dtdx);
       else
-        edge_at<T, DIM, BS, 0, false>(S, V, o, r, dtdx);
+        edge_at_rows<T, DIM, BS, 1, false>(S, V, o, p, lane, dtdx);
     }
-    else if (DIM == 2 || a == 2)
-      edge_at<T, DIM, BS, 2, true>(S, V, o, r, dtdx);
-    else if (a == 1)
-      edge_at<T, DIM, BS, 1, true>(S, V, o, r, dtdx);
+    else if (a == 2)
+      edge_at_rows<T, DIM, BS, 2, true>(S, V, o, p, lane, dtdx);
     else
-      edge_at<T, DIM, BS, 0, true>(S, V, o, r, dtdx);
+      edge_at_rows<T, DIM, BS, 1, true>(S, V, o, p, lane, dtdx);
+  }
 }
 
 template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -1961,8 +2125,6 @@ fused_items_per_block(int dim, int b, int stage)
   const long nt = dim == 3 ? long(b) * b : long(b);
   const long nb1 = b / kTS;
   const long n1 = b + 1;
-  const long epd = dim == 3 ? long(b) * n1 * n1 : n1 * n1;
-  const long epdp = (epd + 63) / 64 * 64;
   const long nc = dim == 3 ? long(b) * b * b : long(b) * b;
   const long np = b + 3;
   const long npe = dim == 3 ? np * np * np : np * np;
@@ -1974,7 +2136,7 @@ fused_items_per_block(int dim, int b, int stage)
   case 1: return (g1 + 63) / 64 * 64;
   case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
-  case 4: return (dim == 3 ? 3 : 1) * epdp;
+  case 4: return (dim == 3 ? 3 : 1) * long(fused_edge_waves(dim, b)) * 64;
   case 9: return (dim == 3 ? nb1 * nb1 * (b / fused_front_depth(dim, b)) : nb1 * nb1);
   default: return dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
   }

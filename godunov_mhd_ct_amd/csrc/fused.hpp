@@ -19,7 +19,7 @@ enum : int
   kStageCell = 1,    // ghost-1 cells (padded to 64 per block)
   kStageBorder = 2,  // faces of the coarse-facing block sides (padded to 64 per side)
   kStageTile = 3,    // 8^dim flux tiles (one workgroup each)
-  kStageEdge = 4,    // edges, each direction padded to 64
+  kStageEdge = 4,    // edge wavefronts of every direction (fused_edge_waves per direction, x 64)
   kStageBupdate = 5, // faces
   kStageEfield = 6,  // edge points of the E grid (padded to 64 per block)
   kStageFix = 7,     // corrected border edges (the mesh's gather list)
@@ -33,6 +33,21 @@ constexpr int
 fused_front_depth(int dim, int b)
 {
   return dim == 3 ? (b == 8 ? 2 : 4) : 1;
+}
+
+// Wavefronts per edge direction of a block in k_edge: in 3D the x-edges one per lane, b (b+1)^2
+// of them, the y- and z-edges over rows of b + 2 positions (a halo column before each row of
+// b + 1 edges), 63 positions per wavefront after a halo lane, every direction padded to the
+// largest count; in 2D the (b+1)^2 edges one per lane.
+constexpr int
+fused_edge_waves(int dim, int b)
+{
+  const int n1 = b + 1;
+  if (dim == 2)
+    return (n1 * n1 + 63) / 64;
+  const int wp = ((b + 2) * n1 * b + 62) / 63;
+  const int wx = (b * n1 * n1 + 63) / 64;
+  return wp > wx ? wp : wx;
 }
 
 // One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
