@@ -148,6 +148,7 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
   }
   bt.tune = b->tune;
   bt.rk2 = b->integrator == MHDCT_TIME_RK2_SSP ? 1 : 0;
+  bt.dev_dt = 1;
   if (bt.nslots == 0)
   {
     b->dirty = false;
@@ -182,10 +183,13 @@ rebuild(mhdct_batch * b, FusedBatch<T> & bt, hipStream_t st)
       v.B = e.XB[c];
       v.B2 = bt.rk2 ? e.RB : e.XB[1 - c];
       v.dt = 0.0;
+      v.dt_red = 1; // the first (or only) stage reduces dt in its primitive stage
       hs[p][i] = v;
       if (bt.rk2)
       {
-        // the second stage: from the stage buffers into the other state buffers
+        // the second stage: from the stage buffers into the other state buffers, with the dt
+        // of the first
+        v.dt_red = 0;
         v.U = e.R;
         v.U2 = e.X[1 - c];
         v.B = e.RB;

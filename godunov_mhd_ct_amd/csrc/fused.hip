@@ -359,8 +359,29 @@ ghost_state(const FusedSlot<T> & S, int o, int i, int j, int k, double * u, cons
   }
 }
 
+// ComputeDtMHD's rate of one cell (ComputeDtMHDFunctor.cpp:86-151) from its primitive state q:
+// the sum over directions of (c_f + |v|)/dx (with gravity, cell_rate's bound)
+template <int DIM, int BS, typename T>
+__device__ __forceinline__ double
+cell_inv_dt(const double * q, const FusedSlot<T> & S, int o)
+{
+  const Gas<double> & g = S.gd;
+  const double        b2 = q[kA] * q[kA] + q[kB] * q[kB] + q[kC] * q[kC];
+  const double        c2 = g.gamma0 * q[kP] / q[kD];
+  const double        d2 = 0.5 * (b2 / q[kD] + c2);
+  const double        dx = dxd<BS>(S.m, S.scaling, o);
+  double              v[3];
+  v[0] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
+  v[1] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
+  if (DIM == 3)
+    v[2] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
+  return cell_rate<DIM>(v, dx, g);
+}
+
+// one item of k_prim; returns the cell's inverse time step for an owned cell of a slot that
+// reduces dt here (S.dt_red), else 0
 template <typename T, int DIM, int BS>
-__device__ __forceinline__ void
+__device__ __forceinline__ double
 prim_item(const FusedSlot<T> & S, long r0)
 {
 
@@ -384,7 +405,7 @@ prim_item(const FusedSlot<T> & S, long r0)
     {
       g -= G::NC;
       if (g >= G::GSH)
-        return;
+        return 0.0;
       shell_cell<DIM, BS>(g, i, j, k);
     }
     const int ci = (i + 2) + n * ((j + 2) + (DIM == 3 ? n * (k + 2) : 0));
@@ -429,23 +450,51 @@ prim_item(const FusedSlot<T> & S, long r0)
       if (k == b + 1)
         store_nt(Go + IX::gface(2, i, j, b + 2), T(gfv(2, i, j, b + 2)));
     }
+    return (inside && S.dt_red) ? cell_inv_dt<DIM, BS>(q, S, o) : 0.0;
 }
 
+// k_prim, and for a slot with S.dt_red the dt reduction of ComputeDtMHD over its owned cells
+// (the interior items read exactly the U and B the reduction needs): per 256 items a workgroup
+// maximum, folded into the slot's reduction word by one atomic only when it exceeds the value
+// there (which only grows), so that few of the workgroups write it; a NaN rate raises flag bit
+// 2.  A workgroup's 256 items lie in one slot (slot ranges are padded to 512 items).  (One
+// atomic check per wavefront instead, without the barriers, was slower: 2.40 against 2.35 ms on
+// the mixed-block mesh.)
 template <typename T, int DIM, int BSEL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
-  for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
+  __shared__ double wmax[4];
+  for (long g0 = xcd_group() * blockDim.x; g0 < total; g0 += long(gridDim.x) * blockDim.x)
   {
-    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, t));
+    const int            s = __builtin_amdgcn_readfirstlane(slot_of(off, nslots, g0));
     const FusedSlot<T> & S = slots[s];
-    const long           r0 = t - off[s];
-    if (r0 >= S.cnt[0])
+    const long           r0 = g0 + threadIdx.x - off[s];
+    double               rate = 0.0;
+    if (r0 < S.cnt[0])
+      MHDCT_BS_DISPATCH(S.m.b, (rate = prim_item<T, DIM, BS>(S, r0)));
+    if (!S.dt_red) // uniform across the workgroup
       continue;
-    MHDCT_BS_DISPATCH(S.m.b, (prim_item<T, DIM, BS>(S, r0)));
+    const bool nan = rate != rate;
+    double     c = nan ? 0.0 : rate;
+    for (int o = 32; o > 0; o >>= 1)
+      c = fmax(c, __shfl_xor(c, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+      wmax[wave] = c;
+    if (__any(nan) && lane == 0)
+      atomicOr(S.red + 1, 2ull);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+      const double mx = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+      if (bits > __hip_atomic_load(S.red, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(S.red, bits);
+    }
+    __syncthreads();
   }
 }
-
 
 // ---------------------------------------------------------------------------------------------
 // k_efield: edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235) on the
@@ -1767,16 +1816,7 @@ inv_dt_item(const FusedSlot<T> & S, long r0)
     u[kC] = 0.5 * (Bo[IX::face(2, i, j, k)] + Bo[IX::face(2, i, j, k + 1)]);
   }
   to_primitive(u, q, g);
-  const double b2 = q[kA] * q[kA] + q[kB] * q[kB] + q[kC] * q[kC];
-  const double c2 = g.gamma0 * q[kP] / q[kD];
-  const double d2 = 0.5 * (b2 / q[kD] + c2);
-  const double dx = dxd<BS>(S.m, S.scaling, o);
-  double       v[3];
-  v[0] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kA] * q[kA] / q[kD])) + fabs(q[kU]);
-  v[1] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kB] * q[kB] / q[kD])) + fabs(q[kV]);
-  if (DIM == 3)
-    v[2] = sqrt(d2 + sqrt(d2 * d2 - c2 * q[kC] * q[kC] / q[kD])) + fabs(q[kW]);
-  return cell_rate<DIM>(v, dx, g);
+  return cell_inv_dt<DIM, BS>(q, S, o);
 }
 
 template <typename T, int DIM, int BSEL>
@@ -1974,6 +2014,17 @@ k_gravity(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __re
   }
 }
 
+// a batch reduces dt in k_prim (k_front, which replaces k_prim, leaves it to k_inv_dt)
+template <typename T>
+bool
+dt_in_prim(const FusedBatch<T> & bt)
+{
+  return bt.dev_dt != 0 && bt.tune.front == 0;
+}
+
+template <typename T>
+void launch_dt_set(const FusedBatch<T> & bt, hipStream_t st);
+
 template <typename T, int DIM, int BSEL>
 int
 launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
@@ -2000,8 +2051,13 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
                          bt.d_off[kStageFront], bt.tot[kStageFront]);
   }
   if (on(kStagePrim) && !front)
+  {
     hipLaunchKernelGGL((k_prim<T, DIM, BSEL>), grid(kStagePrim), 256, 0, st, sl, n, bt.d_off[kStagePrim],
                        bt.tot[kStagePrim]);
+    // dt = cfl / inv_dt once the first stage's k_prim has reduced it (the predictor needs it)
+    if (dt_in_prim(bt) && parity < 2)
+      launch_dt_set(bt, st);
+  }
   if (on(kStageEfield) && !front)
     hipLaunchKernelGGL((k_efield<T, DIM, BSEL>), grid(kStageEfield), 256, 0, st, sl, n, bt.d_off[kStageEfield],
                        bt.tot[kStageEfield]);
@@ -2074,7 +2130,7 @@ class_view(const FusedBatch<T> & bt, int c)
     v.d_off[s] = k.d_off[s];
     v.tot[s] = k.tot[s];
   }
-  v.clock = nullptr;
+  v.clock = bt.clock ? bt.clock + 2 * k.s0 : nullptr;
   return v;
 }
 
@@ -2085,12 +2141,25 @@ by_class(const FusedBatch<T> & bt)
   return bt.bsel == 0 && bt.ncls > 0 && bt.tune.per_class != 0;
 }
 
+template <typename T>
+void
+launch_dt_set(const FusedBatch<T> & bt, hipStream_t st)
+{
+  const int n = bt.nslots;
+  hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], bt.d_slots[2],
+                     bt.d_slots[3], n, bt.clock);
+}
+
+// Clear the slots' reduction words; then, unless k_prim reduces dt (dt_in_prim), the reduction
+// and dt = cfl / inv_dt here
 template <typename T, int DIM, int BSEL>
 int
 launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
 {
   const int n = bt.nslots;
   hipLaunchKernelGGL((k_dt_clear<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[parity], n);
+  if (dt_in_prim(bt))
+    return hipGetLastError() == hipSuccess ? 0 : -1;
   if (BSEL == 0 && by_class(bt))
     for (int c = 0; c < bt.ncls; ++c)
     {
@@ -2104,8 +2173,7 @@ launch_dt(const FusedBatch<T> & bt, int parity, hipStream_t st)
     }
   else
     launch_inv_dt<T, DIM, BSEL>(bt, parity, st);
-  hipLaunchKernelGGL((k_dt_set<T>), dim3((n + 63) / 64), 64, 0, st, bt.d_slots[0], bt.d_slots[1], bt.d_slots[2],
-                     bt.d_slots[3], n, bt.clock);
+  launch_dt_set(bt, st);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

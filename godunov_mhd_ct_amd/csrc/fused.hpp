@@ -80,6 +80,7 @@ struct FusedSlot
   const int *          bside;  // sides (block*2*dim + 2*dir + side) whose face neighbour is coarser
   int                  nbside;
   unsigned long long * red; // [2]: inv_dt bits (max), error flags (bit 0 eint < 0, bit 1 NaN speed)
+  int                  dt_red; // 1: k_prim reduces the inverse time step of the slot's owned cells into red[0]
   long                 cnt[kStages]; // work items per stage
 };
 
@@ -140,6 +141,7 @@ struct FusedBatch
   int            bsel = 0; // the block size of every slot, or 0 when they differ
   int            rk2 = 0;  // every slot is an RK2_SSP stage (a batch holds one integrator)
   int            grav = 0; // some slot has gravity
+  int            dev_dt = 0; // dt on the device (a batch): reduced in k_prim and set before k_efield
   int            nslots = 0;
   FusedSlot<T> * d_slots[4] = { nullptr, nullptr, nullptr, nullptr };
   long *         d_off[kStages] = {};

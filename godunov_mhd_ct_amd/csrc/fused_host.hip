@@ -95,6 +95,7 @@ build(Context & c, const Mesh & mesh, FusedState & F, FusedBatch<T> & bt, FusedS
   sl.rk2 = c.st.time_integrator == 1;
   sl.visc_mu = (c.st.viscosity_enabled && c.st.viscosity_predictor && !sl.rk2) ? c.st.mu : -1.0;
   sl.red = c.red;
+  sl.dt_red = 0; // the per-mesh form takes dt from the host
   // viscous fluxes, 8 variables per flux cell of one direction, as the plain form lays them out
   if (c.st.viscosity_enabled)
   {
