@@ -783,7 +783,9 @@ cell_item(const FusedSlot<T> & S, long r0)
   cell_core<T, DIM, BS, HK, VISC>(S, V, src, o, i, j, k, ci);
 }
 
-template <typename T, int DIM, int BSEL, bool HK>
+// VISC: the viscous predictor term compiled in (only the per-mesh FP64 form runs viscosity; its
+// registers would otherwise be held by every cell kernel)
+template <typename T, int DIM, int BSEL, bool HK, bool VISC>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
@@ -794,7 +796,7 @@ k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
     const long           r0 = t - off[s];
     if (r0 >= S.cnt[1])
       continue;
-    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS, HK, (BSEL != 0 && sizeof(T) == sizeof(double))>(S, r0)));
+    MHDCT_BS_DISPATCH(S.m.b, (cell_item<T, DIM, BS, HK, (VISC && BSEL != 0 && sizeof(T) == sizeof(double))>(S, r0)));
   }
 }
 
@@ -2064,13 +2066,14 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
   if (on(kStageCell) && !front)
   {
     if (bt.rk2)
-      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, false>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
-                         bt.d_off[kStageCell],
-                         bt.tot[kStageCell]);
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, false, false>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
+                         bt.d_off[kStageCell], bt.tot[kStageCell]);
+    else if (bt.visc)
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, true, true>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
+                         bt.d_off[kStageCell], bt.tot[kStageCell]);
     else
-      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, true>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
-                         bt.d_off[kStageCell],
-                         bt.tot[kStageCell]);
+      hipLaunchKernelGGL((k_cell<T, DIM, BSEL, true, false>), grid_for(bt.tot[kStageCell], ct, cap), ct, 0, st, sl, n,
+                         bt.d_off[kStageCell], bt.tot[kStageCell]);
   }
   if (on(kStageBorder) && bt.tot[kStageBorder] > 0)
     hipLaunchKernelGGL((k_border_flux<T, DIM, BSEL>), grid(kStageBorder), 256, 0, st, sl, n,

@@ -142,6 +142,7 @@ struct FusedBatch
   int            rk2 = 0;  // every slot is an RK2_SSP stage (a batch holds one integrator)
   int            grav = 0; // some slot has gravity
   int            dev_dt = 0; // dt on the device (a batch): reduced in k_prim and set before k_efield
+  int            visc = 0;   // the slot has the viscous predictor term (per-mesh FP64 form only)
   int            nslots = 0;
   FusedSlot<T> * d_slots[4] = { nullptr, nullptr, nullptr, nullptr };
   long *         d_off[kStages] = {};

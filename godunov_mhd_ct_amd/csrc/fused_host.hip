@@ -167,6 +167,7 @@ build(Context & c, const Mesh & mesh, FusedState & F, FusedBatch<T> & bt, FusedS
   bt.bsel = b;
   bt.rk2 = sl.rk2;
   bt.grav = sl.gd.grav;
+  bt.visc = sl.visc_mu >= 0.0;
   bt.nslots = 1;
   for (int k = 0; k < kStages; ++k)
     bt.tot[k] = fused_pad_slot(sl.cnt[k]);
