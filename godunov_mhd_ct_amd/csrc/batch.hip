@@ -849,10 +849,10 @@ extern "C"
   {
     if (!b)
       return -1;
-    auto ok = [](int t) { return t == 256 || t == 512 || t == 1024; };
-    if (!ok(cell_threads) || !ok(edge_threads))
+    if (!(cell_threads == 256 || cell_threads == 512 || cell_threads == 768) ||
+        !(edge_threads == 256 || edge_threads == 512 || edge_threads == 1024))
     {
-      b->err = "mhdct_batch_set_stage_threads: 256, 512 or 1024 threads";
+      b->err = "mhdct_batch_set_stage_threads: k_cell 256, 512 or 768 threads, k_edge 256, 512 or 1024";
       return -1;
     }
     b->tune.cell_threads = cell_threads;

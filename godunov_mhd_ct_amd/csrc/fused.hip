@@ -600,25 +600,46 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
         return src.bf(2, a, c, 0);
       return src.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
     };
+    // the stencil, each value read once: the 11 values of the cell, and at its 2 DIM neighbours
+    // (qn[d][0] at +e_d, qn[d][1] at -e_d) the 5 hydro values and the low and high faces of the
+    // components across d, the only ones the slopes read
+    T qc[11], qn[3][2][11];
+    for (int v = 0; v < 11; ++v)
+      qc[v] = Q(v, i, j, k);
+    for (int d = 0; d < DIM; ++d)
+      for (int sd = 0; sd < 2; ++sd)
+      {
+        const int sg = sd == 0 ? 1 : -1;
+        const int a = i + sg * (d == 0), c = j + sg * (d == 1), e = k + sg * (d == 2);
+        for (int v = 0; v < 5; ++v)
+          qn[d][sd][v] = Q(v, a, c, e);
+        for (int mc = 0; mc < 3; ++mc)
+          if (mc != d)
+          {
+            qn[d][sd][5 + mc] = Q(5 + mc, a, c, e);
+            qn[d][sd][8 + mc] = Q(8 + mc, a, c, e);
+          }
+      }
+    // Q(v, cell + o e_d) for o in {0, +1, -1}, from the stencil
+    auto QS = [&](int v, int d, int off) -> T { return off == 0 ? qc[v] : qn[d][off > 0 ? 0 : 1][v]; };
     T         sl[3][8];
     constexpr int nh = DIM == 3 ? 5 : 4;
     for (int d = 0; d < DIM; ++d)
     {
-      const int di = d == 0, dj = d == 1, dk = d == 2;
       for (int v = 0; v < 8; ++v)
         sl[d][v] = T(0);
       for (int v = 0; v < nh; ++v)
-        sl[d][v] = limited_slope(Q(v, i, j, k), Q(v, i + di, j + dj, k + dk), Q(v, i - di, j - dj, k - dk), st);
+        sl[d][v] = limited_slope(QS(v, d, 0), QS(v, d, 1), QS(v, d, -1), st);
       for (int mc = 0; mc < 3; ++mc)
       {
         const int vL = 5 + mc, vR = 8 + mc;
         if (mc == d)
-          sl[d][5 + mc] = Q(vR, i, j, k) - Q(vL, i, j, k);
+          sl[d][5 + mc] = QS(vR, d, 0) - QS(vL, d, 0);
         else
         {
-          const T dc = (Q(vL, i, j, k) + Q(vR, i, j, k)) * T(0.5);
-          const T dp = (Q(vL, i + di, j + dj, k + dk) + Q(vR, i + di, j + dj, k + dk)) * T(0.5);
-          const T dm = (Q(vL, i - di, j - dj, k - dk) + Q(vR, i - di, j - dj, k - dk)) * T(0.5);
+          const T dc = (QS(vL, d, 0) + QS(vR, d, 0)) * T(0.5);
+          const T dp = (QS(vL, d, 1) + QS(vR, d, 1)) * T(0.5);
+          const T dm = (QS(vL, d, -1) + QS(vR, d, -1)) * T(0.5);
           sl[d][5 + mc] = limited_slope(dc, dp, dm, st);
         }
       }
@@ -631,11 +652,11 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
     const double dxo = dxd<BS>(m, S.scaling, o);
     const T      dtdx = T(S.dt / dxo), dtdy = dtdx, dtdz = dtdx;
     const T      gamma = S.gt.gamma0;
-    const T r = Q(0, i, j, k), p = Q(1, i, j, k);
-    T       u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
-    const T A = (Q(5, i, j, k) + Q(8, i, j, k)) * T(0.5);
-    const T B = (Q(6, i, j, k) + Q(9, i, j, k)) * T(0.5);
-    const T C = (Q(7, i, j, k) + Q(10, i, j, k)) * T(0.5);
+    const T r = qc[0], p = qc[1];
+    T       u = qc[2], v = qc[3], w = qc[4];
+    const T A = (qc[5] + qc[8]) * T(0.5);
+    const T B = (qc[6] + qc[9]) * T(0.5);
+    const T C = (qc[7] + qc[10]) * T(0.5);
     T q2o[8];
     if (!HK)
     {
@@ -704,8 +725,7 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
         for (int c = 0; c < DIM; ++c)
           for (int a = 0; a < DIM; ++a)
           {
-            const int e0 = a == 0, e1 = a == 1, e2 = a == 2;
-            const T   d2 = Q(2 + c, i + e0, j + e1, k + e2) + Q(2 + c, i - e0, j - e1, k - e2) - T(2) * Q(2 + c, i, j, k);
+            const T d2 = QS(2 + c, a, 1) + QS(2 + c, a, -1) - T(2) * QS(2 + c, a, 0);
             *vel[c] += T(0.5) * nu * d2 / T(dxo) * dtdx;
           }
       }
@@ -741,17 +761,17 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
     T fso[G::NFS];
     if (DIM == 2)
     {
-      fso[0] = limited_slope(Q(5, i, j, 0), Q(5, i, j + 1, 0), Q(5, i, j - 1, 0), st);
-      fso[1] = limited_slope(Q(6, i, j, 0), Q(6, i + 1, j, 0), Q(6, i - 1, j, 0), st);
+      fso[0] = limited_slope(QS(5, 1, 0), QS(5, 1, 1), QS(5, 1, -1), st);
+      fso[1] = limited_slope(QS(6, 0, 0), QS(6, 0, 1), QS(6, 0, -1), st);
     }
     else
     {
-      fso[0] = limited_slope(Q(5, i, j, k), Q(5, i, j + 1, k), Q(5, i, j - 1, k), st);
-      fso[1] = limited_slope(Q(5, i, j, k), Q(5, i, j, k + 1), Q(5, i, j, k - 1), st);
-      fso[2] = limited_slope(Q(6, i, j, k), Q(6, i + 1, j, k), Q(6, i - 1, j, k), st);
-      fso[3] = limited_slope(Q(6, i, j, k), Q(6, i, j, k + 1), Q(6, i, j, k - 1), st);
-      fso[4] = limited_slope(Q(7, i, j, k), Q(7, i + 1, j, k), Q(7, i - 1, j, k), st);
-      fso[5] = limited_slope(Q(7, i, j, k), Q(7, i, j + 1, k), Q(7, i, j - 1, k), st);
+      fso[0] = limited_slope(QS(5, 1, 0), QS(5, 1, 1), QS(5, 1, -1), st);
+      fso[1] = limited_slope(QS(5, 2, 0), QS(5, 2, 1), QS(5, 2, -1), st);
+      fso[2] = limited_slope(QS(6, 0, 0), QS(6, 0, 1), QS(6, 0, -1), st);
+      fso[3] = limited_slope(QS(6, 2, 0), QS(6, 2, 1), QS(6, 2, -1), st);
+      fso[4] = limited_slope(QS(7, 0, 0), QS(7, 0, 1), QS(7, 0, -1), st);
+      fso[5] = limited_slope(QS(7, 1, 0), QS(7, 1, 1), QS(7, 1, -1), st);
     }
     // every result is stored after the last load: buffer stores are not reordered across
     // buffer loads, so stores issued early would split the loads into dependent phases
@@ -786,7 +806,7 @@ cell_item(const FusedSlot<T> & S, long r0)
 // VISC: the viscous predictor term compiled in (only the per-mesh FP64 form runs viscosity; its
 // registers would otherwise be held by every cell kernel)
 template <typename T, int DIM, int BSEL, bool HK, bool VISC>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(768) __attribute__((amdgpu_waves_per_eu(3)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)

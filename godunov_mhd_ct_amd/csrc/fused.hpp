@@ -114,7 +114,7 @@ struct FusedTune
   int tile_z = 8;         // 3D flux tile depth: 8 (8x8x8 cells, 576 threads) or 4 (8x8x4, 320 threads)
   int front = 0;          // 1: k_front runs the prim, efield and cell stages (kStagePrim; the other
                           // two launch nothing); 0: k_prim, k_efield and k_cell (viscosity needs them)
-  int cell_threads = 1024; // workgroup size of k_cell: 256, 512 or 1024
+  int cell_threads = 256; // workgroup size of k_cell: 256, 512 or 768
   int edge_threads = 256; // workgroup size of k_edge: 256, 512 or 1024
   int per_class = 1;      // slots of several block sizes: 1 launches every stage once per block-size
                           // class with the kernels specialised to it; 0 once with the kernels that

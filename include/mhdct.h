@@ -278,9 +278,10 @@ int mhdct_batch_set_class_launch(mhdct_batch * batch, int enable);
  * counterpart in the reference: a launch choice of this library (GodunovImplemV0.cpp:128-156 runs
  * these stages as separate device loops). */
 int mhdct_batch_set_front(mhdct_batch * batch, int enable);
-/* Workgroup sizes of the cell stage (k_cell) and of the edge stage (k_edge): 256, 512 or 1024
- * threads each (a larger workgroup puts more consecutive wavefronts, which read the same cells,
- * on one CU).  Same results bit for bit.  No counterpart in the reference (a launch choice). */
+/* Workgroup sizes of the cell stage (k_cell: 256, 512 or 768 threads) and of the edge stage
+ * (k_edge: 256, 512 or 1024); a larger workgroup puts more consecutive wavefronts, which read the
+ * same cells, on one CU.  Same results bit for bit.  No counterpart in the reference (a launch
+ * choice). */
 int mhdct_batch_set_stage_threads(mhdct_batch * batch, int cell_threads, int edge_threads);
 /* Time each candidate (grid_cap, tile_cap, tile_z) over `reps` steps that leave the slots'
  * states and clocks unchanged, keep the fastest and return it. */
