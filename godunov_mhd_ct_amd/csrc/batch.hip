@@ -61,6 +61,8 @@ struct mhdct_batch
   bool                  use_graph = false;
   hipGraphExec_t        gx[2] = { nullptr, nullptr };
   hipStream_t           cap = nullptr; // capture stream (graphs cannot be captured on the null stream)
+  hipStream_t           last = nullptr; // the stream of the last steps (the clocks are final once it drains)
+  bool                  ran = false;
   bool                  fork = false;  // run k_edge on a side stream beside the flux stages
   hipStream_t           side = nullptr;
   hipEvent_t            ev_fork = nullptr, ev_join = nullptr;
@@ -265,8 +267,11 @@ save_clocks(mhdct_batch * b, FusedBatch<T> & bt)
   if (!bt.clock || bt.nslots == 0 || b->dirty)
     return 0;
   std::vector<double> clk(size_t(2) * bt.nslots);
-  if (hipDeviceSynchronize() != hipSuccess ||
-      hipMemcpy(clk.data(), bt.clock, clk.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+  // only this batch's steps write its clocks: wait for its stream(s), not for the whole device
+  // (another batch may be running beside it)
+  const bool drained = !b->ran || (hipStreamSynchronize(b->last) == hipSuccess &&
+                                   (!b->side || hipStreamSynchronize(b->side) == hipSuccess));
+  if (!drained || hipMemcpy(clk.data(), bt.clock, clk.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
   {
     b->err = "clock readback failed";
     return -1;
@@ -379,6 +384,8 @@ step(mhdct_batch * b, FusedBatch<T> & bt, int nsteps, hipStream_t st)
     return -1;
   if (bt.nslots == 0)
     return 0;
+  b->last = st;
+  b->ran = true;
   for (int n = 0; n < nsteps; ++n)
   {
     const int p = int(b->gstep & 1);
@@ -418,6 +425,8 @@ step_timed(mhdct_batch * b, FusedBatch<T> & bt, int nsteps, hipStream_t st, doub
     b->err = "timed steps are for HANCOCK batches (an RK2_SSP step runs every stage twice)";
     return -1;
   }
+  b->last = st;
+  b->ran = true;
   // per step: 2 events around each of the 10 stages, on the stream the stage runs on
   std::vector<hipEvent_t> ev(size_t(nsteps) * 2 * ns);
   for (auto & e : ev)

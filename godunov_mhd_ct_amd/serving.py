@@ -8,7 +8,8 @@ arrive as a seeded Poisson stream.  The loop
   * admits prepared requests into a batch (one per dimension) while their HBM footprint fits
     the budget -- state and scratch stay on the device until the request finishes,
   * advances every admitted request together, one batch step at a time in ticks no longer than
-    the shortest remaining request, so that requests join and leave between ticks,
+    the shortest remaining request, so that requests join and leave between ticks; the 2D and
+    the 3D batch tick on streams of their own, independently, so their work overlaps,
   * reads finished results back to the host and frees their slots.
 
 Latency is arrival -> result on the host.  A request's result does not depend on what else
@@ -92,7 +93,11 @@ def prepare(req: Request):
 
 class Server:
     """Continuous-batching loop.  `hbm_budget` bounds the summed footprint of admitted
-    requests (state + scratch); `max_tick` bounds the steps between admission points."""
+    requests (state + scratch); `max_tick` bounds the steps between admission points.
+
+    The 2D and the 3D batch run on streams of their own, so their ticks overlap on the device;
+    the loop polls each batch's tick by an event, retires and admits requests of a batch while
+    it is idle, and starts its next tick at once, without waiting for the other batch."""
 
     def __init__(self, hbm_budget: int, impl: int = IMPL_FUSED, max_tick: int = 20, use_graph: bool = False,
                  prep_workers: int = 4):
@@ -102,12 +107,16 @@ class Server:
         self.use_graph = use_graph
         self.prep_workers = prep_workers
         self.batches: dict[int, BatchGodunovMHD] = {}
+        self.streams: dict = {}
 
     def _batch(self, dim):
         if dim not in self.batches:
+            import torch
+
             bt = BatchGodunovMHD(dim, self.impl)
             bt.set_graph(self.use_graph)
             self.batches[dim] = bt
+            self.streams[dim] = torch.cuda.Stream()
         return self.batches[dim]
 
     def run(self, requests: list[Request], keep_results: bool = False, log=None) -> dict:
@@ -119,18 +128,47 @@ class Server:
         pending = list(reqs)         # not arrived yet
         ready = []                   # arrived; (req, future)
         active = {}                  # (dim, slot) -> [req, remaining, footprint, cells]
+        inflight = {}                # dim -> (event, tick) of the batch's running tick
         used = 0
         done = []
         cell_updates = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         while pending or ready or active:
+            progressed = False
             # arrivals: start host preparation
             while pending and pending[0].arrival <= now():
                 r = pending.pop(0)
                 futures[r.rid] = pool.submit(prepare, r)
                 ready.append(r)
-            # admissions, in arrival order, while the footprint fits
+                progressed = True
+            # finished ticks: every request of that batch advanced; retire the finished ones
+            for dim in [d for d, (ev, _) in inflight.items() if ev.query()]:
+                _, tick = inflight.pop(dim)
+                progressed = True
+                bt = self.batches[dim]
+                for key, a in active.items():
+                    if key[0] == dim:
+                        a[1] -= tick
+                        cell_updates += a[3] * tick
+                for key in [k for k, a in active.items() if k[0] == dim and a[1] == 0]:
+                    r, _, need, _ = active.pop(key)
+                    sid = key[1]
+                    if keep_results:
+                        r.result = bt.state(sid)
+                    else:
+                        # the state after r.nsteps steps is in buffer r.nsteps % 2: one value to
+                        # the host stands for the result's read-back
+                        U, _ = bt.slots[sid].bufs[r.nsteps & 1]
+                        _ = float(U[0].item())
+                    bt.remove(sid)
+                    used -= need
+                    r.done = now()
+                    done.append(r)
+                    if log:
+                        log(f"[serve] request {r.rid} {r.kind} n={r.n} b={r.b} latency {r.latency * 1e3:.1f} ms")
+            # admissions, in arrival order, into batches that are not running a tick, while the
+            # footprint fits
             still = []
             for r in ready:
                 f = futures[r.rid]
@@ -139,7 +177,7 @@ class Server:
                     continue
                 mesh, U, B, st = f.result()
                 need = BatchGodunovMHD.footprint_bytes(mesh, self.impl)
-                if active and used + need > self.budget:
+                if mesh.dim in inflight or (active and used + need > self.budget):
                     still.append(r)
                     continue
                 bt = self._batch(mesh.dim)
@@ -148,40 +186,27 @@ class Server:
                 active[(mesh.dim, sid)] = [r, r.nsteps, need, mesh.num_leaves * mesh.cells_per_block]
                 used += need
                 del futures[r.rid]
+                progressed = True
             ready = still
-            if not active:
-                # idle until the next arrival or preparation
-                wait = (pending[0].arrival - now()) if pending else 0.0
-                time.sleep(min(max(wait, 0.0), 0.002) if not ready else 0.0005)
-                continue
-            # one tick: every active request advances the same number of steps
-            tick = min(self.max_tick, min(a[1] for a in active.values()))
+            # a tick on every idle batch with requests: as many steps as its nearest finish
             for dim, bt in self.batches.items():
-                if bt.num_slots():
-                    bt.step(tick)
-            for key in list(active):
-                a = active[key]
-                a[1] -= tick
-                cell_updates += a[3] * tick
-            torch.cuda.synchronize()
-            # retire finished requests
-            for key in [k for k, a in active.items() if a[1] == 0]:
-                r, _, need, _ = active.pop(key)
-                dim, sid = key
-                bt = self.batches[dim]
-                if keep_results:
-                    r.result = bt.state(sid)
-                else:
-                    U, _ = bt.current(sid)
-                    _ = float(U[0].item())  # result on the host
-                bt.remove(sid)
-                used -= need
-                r.done = now()
-                done.append(r)
-                if log:
-                    log(f"[serve] request {r.rid} {r.kind} n={r.n} b={r.b} latency {r.latency * 1e3:.1f} ms")
+                if dim in inflight:
+                    continue
+                rem = [a[1] for k, a in active.items() if k[0] == dim]
+                if not rem:
+                    continue
+                tick = min(self.max_tick, min(rem))
+                s = self.streams[dim]
+                bt.step(tick, s)
+                ev = torch.cuda.Event()
+                ev.record(s)
+                inflight[dim] = (ev, tick)
+                progressed = True
+            if not progressed:
+                time.sleep(0.0002)
         wall = now()
         pool.shutdown()
+        torch.cuda.synchronize()
         for bt in self.batches.values():
             bt.check()
         lat = np.array([r.latency for r in done]) * 1e3

@@ -235,9 +235,12 @@ int mhdct_batch_create(int dim, int impl_version, mhdct_batch ** out);
  * allocates for the slot.  Synchronises the device. */
 int mhdct_batch_add(mhdct_batch * batch, mhdct_ctx * ctx, double * U, double * U2, double * B, double * B2,
                     int * slot_id);
-/* Remove a slot after the steps enqueued so far.  Synchronises the device. */
+/* Remove a slot after the steps enqueued so far: waits for the stream of the batch's last steps
+ * (and, for an RK2_SSP slot, for the device). */
 int mhdct_batch_remove(mhdct_batch * batch, int slot_id);
-/* Enqueue `nsteps` steps of every slot on `stream` (outside fill, dt, stages). */
+/* Enqueue `nsteps` steps of every slot on `stream` (outside fill, dt, stages).  A batch's steps
+ * go to one stream at a time: the calls that read its clocks (remove, slot_state) wait for the
+ * stream of its last steps only, so that batches on other streams keep running. */
 int mhdct_batch_step(mhdct_batch * batch, int nsteps, void * stream);
 /* The same as mhdct_batch_step without graphs, with HIP events recorded around each stage on
  * the stream it runs on: stage_ms[MHDCT_NUM_STAGES] receives the average milliseconds per step
