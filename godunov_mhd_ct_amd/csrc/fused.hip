@@ -87,8 +87,17 @@ struct Geo
   static constexpr int NCE = DIM == 3 ? 3 : 1;            // EMF components
   static constexpr int NEC = I::ecells();                 // edge-grid points per component
   static constexpr int EPD = DIM == 3 ? BS * NE * NE : NE * NE; // edges per direction
-  static constexpr int NPOS = (BS + 2) * NE * BS;          // 3D y-, z-edge positions
-  static constexpr int NWD = fused_edge_waves(DIM, BS);   // k_edge wavefronts per direction
+  // k_edge (3D): runs of at most MCH edges marching across NE planes (NCH runs), the plane
+  // wavefronts of the y- and z-edges (NPL positions, 63 per wavefront) and of the x-edges (8 x 8
+  // tiles, 7 new positions along y each); EWB: k_edge wavefronts per block (2D: one edge per
+  // lane)
+  static constexpr int MCH = kEdgeRun;
+  static constexpr int NCH = (NE + MCH - 1) / MCH;
+  static constexpr int NPL = (BS + 2) * BS;
+  static constexpr int NWP = (NPL + 62) / 63;
+  static constexpr int NWX = (BS / 8) * ((BS + 8) / 7);
+  static constexpr int EWB = fused_edge_waves(DIM, BS);
+  static_assert(DIM == 2 || EWB == NCH * (NWX + 2 * NWP), "k_edge wavefronts per block");
   static constexpr int P1 = (G1 + 63) / 64 * 64;          // k_cell items per block
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
   static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
@@ -1553,37 +1562,18 @@ edge_at(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int r, T dtdx
 //                                                  along the edge (the "cell part"),
 //   B_D0 = Bf_D0(p0) + 1/2 b0 dB_D0/dD1 (p0),  B_D1 = Bf_D1(p1) + 1/2 b1 dB_D1/dD0 (p1)
 // (the "face part"), Bf the face field plus, for HANCOCK, the face-B source term, p0 = c0 + e_D0
-// for KIND 1, 2 and p1 = c0 + e_D1 for KIND 1, 3: edge_state's arithmetic, split.  The y-edges add
-// the dBy slope terms twice, as edge_state does.
+// for KIND 1, 2 and p1 = c0 + e_D1 for KIND 1, 3: edge_state's arithmetic, split.  As edge_state
+// does, the y-edges add the dBy slope terms twice, and the x-edges add the cell's dBy slope terms
+// to their face value of By.
 //
-// cell part of the corner of KIND at cell c0 (index in the 1-ghost record grid) of a 3D z-edge
-// (D1 = y) or y-edge (D1 = z), D0 = x: h[0..4] rho, p, u, v, w (floored as in the reference),
-// h[5] the B component that stays with the cell (z-edges: Bz; y-edges: By)
-template <typename T, int DIM, int BS, int KIND, int D1>
-__device__ __forceinline__ void
-edge_cell_part(const View<T, DIM, BS> & V, int c0, T * h)
-{
-  constexpr int D0 = 0;
-  constexpr int s0 = (KIND == 0 || KIND == 3) ? -1 : 1;
-  constexpr int s1 = (KIND == 0 || KIND == 2) ? -1 : 1;
-  const T       f0 = T(s0), f1 = T(s1);
-  auto          tr = [&](int v) { return T(0.5) * (f0 * V.sl(D0, v, c0) + f1 * V.sl(D1, v, c0)); };
-  for (int v = 0; v < 5; ++v)
-    h[v] = V.q2(v, c0) + T(0.5) * (f0 * V.sl(D0, v, c0) + f1 * V.sl(D1, v, c0));
-  if (D1 == 1)
-    h[5] = V.q2(kC, c0) + tr(kC);
-  else
-  {
-    const T b = V.q2(kB, c0) + tr(kB);
-    h[5] = b + tr(kB);
-  }
-  h[kD] = fmax(V.S.gt.smallr, h[kD]);
-  h[kP] = fmax(V.S.gt.smallp * h[kD], h[kP]);
-}
-
-// the corner state q[8] from its cell part h and its face values (bf0, fs0) = (Bf_x, dBx/dD1) at
-// p0 and (bf1, fs1) = (Bf_D1, dB_D1/dx) at p1
-template <typename T, int DIM, int KIND, int D1>
+// The cell part of KIND at cell c0 across (D0, D1) = (x, y) (z-edges), (x, z) (y-edges) or (y, z)
+// (x-edges), edge_part_of: h[0..4] rho, p, u, v, w (floored as in the reference), h[5] the B
+// component that stays with the cell (z-edges: Bz; y-edges: By; x-edges: Bx), h[6] (x-edges) the
+// dBy slope term of the By face value.
+//
+// the corner state q[8] from its cell part h and its face values (bf0, fs0) = (Bf_D0, dB_D0/dD1)
+// at p0 and (bf1, fs1) = (Bf_D1, dB_D1/dD0) at p1
+template <typename T, int KIND, int D0, int D1>
 __device__ __forceinline__ void
 edge_corner(const T * h, T bf0, T fs0, T bf1, T fs1, T * q)
 {
@@ -1593,15 +1583,22 @@ edge_corner(const T * h, T bf0, T fs0, T bf1, T fs1, T * q)
     q[v] = h[v];
   const T B0 = bf0 + T(0.5) * (T(b0) * fs0);
   const T B1 = bf1 + T(0.5) * (T(b1) * fs1);
-  q[kA] = B0;
-  if (D1 == 1)
+  if (D0 == 0 && D1 == 1)
   {
+    q[kA] = B0;
     q[kB] = B1;
     q[kC] = h[5];
   }
+  else if (D0 == 0)
+  {
+    q[kA] = B0;
+    q[kB] = h[5];
+    q[kC] = B1;
+  }
   else
   {
-    q[kB] = h[5];
+    q[kA] = h[5];
+    q[kB] = B0 + h[6];
     q[kC] = B1;
   }
 }
@@ -1637,126 +1634,239 @@ lane_take(T x, int addr)
     return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
 }
 
-// The EMF along axis AX (2 or 1: the 3D edge directions across x) at position p of block o, lane
-// `lane` of its wavefront.  Positions run over rows of b + 2 along x, i = -1 .. b: the
-// corners of the edge at A = (i, j, k) are KIND 0 at A, KIND 3 at C = A - e_D1, KIND 2 at
-// A' = A - e_x and KIND 1 at C' = A' - e_D1, with Bf_x at A (KIND 0, 2) and at C (KIND 1, 3) and
-// Bf_D1 at A (KIND 0, 3) and at A' (KIND 1, 2).  A' and C' are the A and C of the position one
-// lane below, so every lane forms the cell parts of its A and C for the lane above (KIND 2, 1)
-// and takes its own from the lane below (cross-lane reads, no memory traffic).  The lanes at
-// i = -1 and lane 0 of each wavefront (the position before its first one) only pass theirs on,
-// so every load a wavefront issues is useful to all of its lanes: 2 cells' records per edge
-// instead of 4.  Same arithmetic as edge_at, the same states bit for bit.
+// What a marching lane reads of its cell c for the edges across (D0, D1): the values the cell
+// parts take (rho, p, u, v, w and the cell's B component, their slopes along D0 and D1, and for
+// the x-edges the slopes of By), Bf_D1 and Bf_D0 at c with their transverse slopes (edge_face;
+// Bf_D0 only on edge lanes).
+template <typename T, int D0, int D1>
+struct EdgeRaw
+{
+  static constexpr int NQ = 6;                   // q2: rho, p, u, v, w, the cell's B component
+  static constexpr int NS = D0 == 1 ? 7 : 6;     // slopes: the same and, x-edges, By
+  static constexpr int BC = (D0 == 0 && D1 == 1) ? kC : (D0 == 0 ? kB : kA);
+  T                    q[NQ], s0[NS], s1[NS], b1, f1, b0, f0;
+};
+
+template <typename T, int DIM, int BS, int D0, int D1, bool HK>
+__device__ __forceinline__ void
+edge_raw_load(const View<T, DIM, BS> & V, const int * c, bool edge, EdgeRaw<T, D0, D1> & R)
+{
+  using ER = EdgeRaw<T, D0, D1>;
+  const int cA = View<T, DIM, BS>::c1(c[0], c[1], c[2]);
+  for (int v = 0; v < 5; ++v)
+  {
+    R.q[v] = V.q2(v, cA);
+    R.s0[v] = V.sl(D0, v, cA);
+    R.s1[v] = V.sl(D1, v, cA);
+  }
+  R.q[5] = V.q2(ER::BC, cA);
+  R.s0[5] = V.sl(D0, ER::BC, cA);
+  R.s1[5] = V.sl(D1, ER::BC, cA);
+  if constexpr (ER::NS == 7)
+  {
+    R.s0[6] = V.sl(D0, kB, cA);
+    R.s1[6] = V.sl(D1, kB, cA);
+  }
+  edge_face<T, DIM, BS, D1, D0, HK>(V, c[0], c[1], c[2], R.b1, R.f1);
+  R.b0 = T(0);
+  R.f0 = T(0);
+  if (edge)
+    edge_face<T, DIM, BS, D0, D1, HK>(V, c[0], c[1], c[2], R.b0, R.f0);
+}
+
+// the cell part of KIND from the values read (edge_at's corner arithmetic, the same operations in
+// the same order)
+template <typename T, int KIND, int D0, int D1>
+__device__ __forceinline__ void
+edge_part_of(const EdgeRaw<T, D0, D1> & R, const Gas<T> & gt, T * h)
+{
+  constexpr int s0 = (KIND == 0 || KIND == 3) ? -1 : 1;
+  constexpr int s1 = (KIND == 0 || KIND == 2) ? -1 : 1;
+  const T       f0 = T(s0), f1 = T(s1);
+  auto          tr = [&](int n) { return T(0.5) * (f0 * R.s0[n] + f1 * R.s1[n]); };
+  for (int v = 0; v < 5; ++v)
+    h[v] = R.q[v] + T(0.5) * (f0 * R.s0[v] + f1 * R.s1[v]);
+  if (D0 == 0 && D1 == 1)
+    h[5] = R.q[5] + tr(5);
+  else if (D0 == 0)
+  {
+    const T b = R.q[5] + tr(5);
+    h[5] = b + tr(5);
+  }
+  else
+  {
+    h[5] = R.q[5] + tr(5);
+    h[6] = tr(6);
+  }
+  h[kD] = fmax(gt.smallr, h[kD]);
+  h[kP] = fmax(gt.smallp * h[kD], h[kP]);
+}
+
+// The EMFs along axis AX of a run of 3D edges that march along D1, D0 and D1 the two directions
+// across the edges (z-edges: x, y; y-edges: x, z; x-edges: y, z).  The edge with cell A has its
+// corners at KIND 0 at A, KIND 3 at C = A - e_D1, KIND 2 at A' = A - e_D0 and KIND 1 at C' =
+// A' - e_D1, with Bf_D0 at A (KIND 0, 2) and at C (KIND 1, 3) and Bf_D1 at A (KIND 0, 3) and at
+// A' (KIND 1, 2).  Each lane holds the cell c = (c[0], c[1], c[2]) with c[D1] = m for m = m0 - 1
+// .. m1 - 1 and forms, from one read of the cell's record, the cell parts of KIND 0 and 2 (A of
+// its own edge at m and of the edge one step up along D0) and of KIND 3 and 1 (C of the same two
+// edges at m + 1); KIND 2 and 1 come to the lane one step up along D0 by cross-lane reads (the
+// lane whose byte address is `src` holds A' and C'), KIND 3, 1 and Bf_D0 are carried to the next
+// m in registers.  So each edge reads one cell's record and two faces, and the reads for m + 1
+// are issued before the EMF at m is formed; lanes with `edge` false only pass their cell parts
+// on.  Same arithmetic as edge_at, the same states bit for bit.
 template <typename T, int DIM, int BS, int AX, bool HK>
 __device__ __forceinline__ void
-edge_at_rows(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int p, int lane, T dtdx)
+edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int m0, int m1, int src, bool edge,
+           T dtdx)
 {
+  static_assert(DIM == 3, "edge_march: 3D edges");
   using G = Geo<DIM, BS>;
-  using VW = View<T, DIM, BS>;
-  static_assert(DIM == 3 && (AX == 1 || AX == 2), "edge_at_rows: 3D y- and z-edges");
-  constexpr int e0 = BS + 2;
-  constexpr int e1 = AX == 1 ? BS : G::NE;
+  constexpr int D0 = AX == 0 ? 1 : 0;
   constexpr int D1 = AX == 2 ? 1 : 2;
-  constexpr int NH = 6;
-  const int     i = p % e0 - 1, j = (p / e0) % e1, k = p / (e0 * e1);
-  int           c[3] = { i, j, k };
-  const int     cA = VW::c1(c[0], c[1], c[2]);
-  c[D1] -= 1;
-  const int cC = VW::c1(c[0], c[1], c[2]);
-  // the cell parts of A and C: KIND 0 and 3 for this edge, KIND 2 and 1 for the lane above (all
-  // four from one read of the two cells' records), and Bf_D1 at A, which both need
-  T hA0[NH], hC3[NH], h2[NH], h1[NH], bA1, fA1;
-  edge_cell_part<T, DIM, BS, 0, D1>(V, cA, hA0);
-  edge_cell_part<T, DIM, BS, 2, D1>(V, cA, h2);
-  edge_cell_part<T, DIM, BS, 3, D1>(V, cC, hC3);
-  edge_cell_part<T, DIM, BS, 1, D1>(V, cC, h1);
-  edge_face<T, DIM, BS, D1, 0, HK>(V, i, j, k, bA1, fA1);
-  const int addr = (lane > 0 ? lane - 1 : 0) * 4;
-  T         hAp[NH], hCp[NH];
-  for (int v = 0; v < NH; ++v)
+  constexpr int NH = AX == 0 ? 7 : 6;
+  // the reference's corner order (RT, RB, LT, LB) per axis: KIND 2 is RB for x- and z-edges, LT
+  // for y-edges
+  constexpr int             slot3 = AX == 1 ? kRB : kLT;
+  constexpr int             slot2 = AX == 1 ? kLT : kRB;
+  T                         hC3[NH], hCp[NH], bC0 = T(0), fC0 = T(0);
+  EdgeRaw<T, D0, D1>        R;
+  c[D1] = m0 - 1;
+  edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, R);
+  for (int m = m0 - 1; m < m1; ++m)
   {
-    hAp[v] = lane_take(h2[v], addr);
-    hCp[v] = lane_take(h1[v], addr);
+    T hA0[NH], h2[NH], h3[NH], h1[NH];
+    edge_part_of<T, 0, D0, D1>(R, S.gt, hA0);
+    edge_part_of<T, 2, D0, D1>(R, S.gt, h2);
+    edge_part_of<T, 3, D0, D1>(R, S.gt, h3);
+    edge_part_of<T, 1, D0, D1>(R, S.gt, h1);
+    const T bA1 = R.b1, fA1 = R.f1, bA0 = R.b0, fA0 = R.f0;
+    T       hAp[NH], h1p[NH];
+    for (int v = 0; v < NH; ++v)
+    {
+      hAp[v] = lane_take(h2[v], src);
+      h1p[v] = lane_take(h1[v], src);
+    }
+    const T bAp1 = lane_take(bA1, src), fAp1 = lane_take(fA1, src);
+    // the next cell's reads, in flight while this edge's EMF is formed
+    c[D1] = m;
+    if (m + 1 < m1)
+    {
+      c[D1] = m + 1;
+      edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, R);
+      c[D1] = m;
+    }
+    if (edge && m >= m0)
+    {
+      T qe[4][8];
+      edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
+      edge_corner<T, 3, D0, D1>(hC3, bC0, fC0, bA1, fA1, qe[slot3]);
+      edge_corner<T, 2, D0, D1>(hAp, bA0, fA0, bAp1, fAp1, qe[slot2]);
+      edge_corner<T, 1, D0, D1>(hCp, bC0, fC0, bAp1, fAp1, qe[kRT]);
+      const T E = edge_emf_fast<AX>(qe, S.gt) * dtdx;
+      S.emf[(long(o) * G::NCE + AX) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
+    }
+    for (int v = 0; v < NH; ++v)
+    {
+      hC3[v] = h3[v];
+      hCp[v] = h1p[v];
+    }
+    bC0 = bA0;
+    fC0 = fA0;
   }
-  const T bAp1 = lane_take(bA1, addr), fAp1 = lane_take(fA1, addr);
-  if (lane == 0 || i < 0)
-    return;
-  // Bf_x at A and C
-  T bA0, fA0, bC0, fC0;
-  edge_face<T, DIM, BS, 0, D1, HK>(V, i, j, k, bA0, fA0);
-  edge_face<T, DIM, BS, 0, D1, HK>(V, c[0], c[1], c[2], bC0, fC0);
-  // the reference's corner order (RT, RB, LT, LB) per axis: KIND 2 is RB for z-edges, LT for y
-  constexpr int slot3 = AX == 1 ? kRB : kLT;
-  constexpr int slot2 = AX == 1 ? kLT : kRB;
-  T             qe[4][8];
-  edge_corner<T, DIM, 0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
-  edge_corner<T, DIM, 3, D1>(hC3, bC0, fC0, bA1, fA1, qe[slot3]);
-  edge_corner<T, DIM, 2, D1>(hAp, bA0, fA0, bAp1, fAp1, qe[slot2]);
-  edge_corner<T, DIM, 1, D1>(hCp, bC0, fC0, bAp1, fAp1, qe[kRT]);
-  const T E = edge_emf_fast<AX>(qe, S.gt) * dtdx;
-  S.emf[(long(o) * G::NCE + AX) * G::NEC + IdxC<DIM, BS>::edge(i, j, k)] = E;
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_edge: one thread per edge of a block, every edge once.  Each block's edges are enumerated
-// per direction (k slowest) in NWD wavefronts per direction: in 3D the x-edges one per lane, the
-// y- and z-edges as positions of edge_at_rows, 63 per wavefront after a halo lane; in 2D the edges
-// one per lane (edge_at_rows's extra live values spill in the 2D kernels).  In 3D the wavefronts of the three directions alternate (wavefront w takes the (w/3)-th of
-// direction w % 3), so that the x-, y- and z-edges around the same cells run together and share
-// the cells' records in the caches.  A wavefront's direction is uniform and compiled in.
+// k_edge: every edge of a block once.  In 3D the edges of each direction march along one
+// direction D1 across them (edge_march), in runs of at most G::MCH edges, each run's wavefronts
+// over a plane of the two other directions:
+//   * y- and z-edges: positions over rows of b + 2 along x (i = -1 .. b) and the edge's own axis,
+//     63 per wavefront after the position before them; lane s - 1 holds the edge one step back
+//     along x, the lanes at i = -1 and lane 0 only pass their cell parts on;
+//   * x-edges: tiles of 8 lanes along x by 8 positions along y (j = -1 .. b), 7 new after the
+//     last one of the tile before; lane s - 8 holds the edge one step back along y, the first
+//     row and the lanes at j = -1 only pass their cell parts on.
+// G::EWB wavefronts per block: x-edges first, then y, then z, each by plane wavefront and run.
+// A marching lane holds about 230 registers (the next cell's reads in flight while the EMF is
+// formed), so the 3D kernels run 2 wavefronts per SIMD.  In 2D a plane of lanes would be one row
+// of b + 2 positions, too short for a wavefront: the edges run one per lane (edge_at), 4
+// wavefronts per SIMD.  A wavefront's direction is uniform and compiled in.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
 edge_item(const FusedSlot<T> & S, long r0)
 {
   using G = Geo<DIM, BS>;
-  constexpr int per = G::NCE * G::NWD * 64;
+  constexpr int per = G::EWB * 64;
   const int     o = __builtin_amdgcn_readfirstlane(int(r0) / per);
   const int     r = int(r0) - o * per;
-  int           w = __builtin_amdgcn_readfirstlane(r / 64);
+  const int     w = __builtin_amdgcn_readfirstlane(r / 64);
   const int     lane = r - w * 64;
-  int           a = 2;
-  if (DIM == 3)
-  {
-    a = w % 3;
-    w = w / 3;
-  }
   const View<T, DIM, BS> V(S, o);
   const T                dtdx = T(S.dt / dxd<BS>(S.m, S.scaling, o));
   // the integrator is uniform per slot, so per wavefront; each variant is compiled on its own
-  if (DIM == 2 || a == 0)
+  if constexpr (DIM == 2)
   {
     const int re = w * 64 + lane;
     if (re >= G::EPD)
       return;
     if (S.rk2)
-      edge_at<T, DIM, BS, DIM == 2 ? 2 : 0, false>(S, V, o, re, dtdx);
+      edge_at<T, DIM, BS, 2, false>(S, V, o, re, dtdx);
     else
-      edge_at<T, DIM, BS, DIM == 2 ? 2 : 0, true>(S, V, o, re, dtdx);
-    return;
+      edge_at<T, DIM, BS, 2, true>(S, V, o, re, dtdx);
   }
-  if constexpr (DIM == 3)
+  else
   {
-    // positions of the y- and z-edges: lanes outside them feed no edge lane and leave at once
-    const int p = w * 63 - 1 + lane;
-    if (p < 0 || p >= G::NPOS)
-      return;
+    // x-edges, then y, then z, each by plane wavefront and run
+    constexpr int nx = G::NWX * G::NCH, np = G::NWP * G::NCH;
+    const int     a = w < nx ? 0 : (w < nx + np ? 1 : 2);
+    const int     wd = a == 0 ? w : (a == 1 ? w - nx : w - nx - np);
+    const int     ch = wd % G::NCH, pw = wd / G::NCH;
+    const int     m0 = ch * G::MCH, m1 = (m0 + G::MCH < G::NE ? m0 + G::MCH : G::NE);
+    int           c[3], src;
+    bool          edge;
+    constexpr int e0 = BS + 2;
+    if (a == 0)
+    {
+      const int it = pw % (BS / 8);
+      const int p = (pw / (BS / 8)) * 7 - 1 + (lane >> 3);
+      if (p < 0 || p >= e0)
+        return;
+      c[0] = 8 * it + (lane & 7);
+      c[1] = p - 1;
+      src = (lane >= 8 ? lane - 8 : lane) * 4;
+      edge = lane >= 8 && p >= 1;
+    }
+    else
+    {
+      // (i, k) for z-edges, (i, j) for y-edges
+      const int p = pw * 63 - 1 + lane;
+      if (p < 0 || p >= G::NPL)
+        return;
+      c[0] = p % e0 - 1;
+      c[a == 1 ? 1 : 2] = p / e0;
+      src = (lane > 0 ? lane - 1 : 0) * 4;
+      edge = lane > 0 && c[0] >= 0;
+    }
     if (S.rk2)
     {
       if (a == 2)
-        edge_at_rows<T, DIM, BS, 2, false>(S, V, o, p, lane, dtdx);
+        edge_march<T, DIM, BS, 2, false>(S, V, o, c, m0, m1, src, edge, dtdx);
+      else if (a == 1)
+        edge_march<T, DIM, BS, 1, false>(S, V, o, c, m0, m1, src, edge, dtdx);
       else
-        edge_at_rows<T, DIM, BS, 1, false>(S, V, o, p, lane, dtdx);
+        edge_march<T, DIM, BS, 0, false>(S, V, o, c, m0, m1, src, edge, dtdx);
     }
     else if (a == 2)
-      edge_at_rows<T, DIM, BS, 2, true>(S, V, o, p, lane, dtdx);
+      edge_march<T, DIM, BS, 2, true>(S, V, o, c, m0, m1, src, edge, dtdx);
+    else if (a == 1)
+      edge_march<T, DIM, BS, 1, true>(S, V, o, c, m0, m1, src, edge, dtdx);
     else
-      edge_at_rows<T, DIM, BS, 1, true>(S, V, o, p, lane, dtdx);
+      edge_march<T, DIM, BS, 0, true>(S, V, o, c, m0, m1, src, edge, dtdx);
   }
 }
 
 template <typename T, int DIM, int BSEL>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIM == 3 ? 2 : 4)))
 k_edge(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -2227,7 +2337,7 @@ fused_items_per_block(int dim, int b, int stage)
   case 1: return (g1 + 63) / 64 * 64;
   case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
-  case 4: return (dim == 3 ? 3 : 1) * long(fused_edge_waves(dim, b)) * 64;
+  case 4: return long(fused_edge_waves(dim, b)) * 64;
   case 9: return (dim == 3 ? nb1 * nb1 * (b / fused_front_depth(dim, b)) : nb1 * nb1);
   default: return dim == 3 ? 3 * n1 * b * b : 2 * n1 * b + long(b) * b;
   }

@@ -19,7 +19,7 @@ enum : int
   kStageCell = 1,    // ghost-1 cells (padded to 64 per block)
   kStageBorder = 2,  // faces of the coarse-facing block sides (padded to 64 per side)
   kStageTile = 3,    // 8^dim flux tiles (one workgroup each)
-  kStageEdge = 4,    // edge wavefronts of every direction (fused_edge_waves per direction, x 64)
+  kStageEdge = 4,    // k_edge wavefronts of a block (fused_edge_waves, x 64)
   kStageBupdate = 5, // faces
   kStageEfield = 6,  // edge points of the E grid (padded to 64 per block)
   kStageFix = 7,     // corrected border edges (the mesh's gather list)
@@ -35,19 +35,24 @@ fused_front_depth(int dim, int b)
   return dim == 3 ? (b == 8 ? 2 : 4) : 1;
 }
 
-// Wavefronts per edge direction of a block in k_edge: in 3D the x-edges one per lane, b (b+1)^2
-// of them, the y- and z-edges over rows of b + 2 positions (a halo column before each row of
-// b + 1 edges), 63 positions per wavefront after a halo lane, every direction padded to the
-// largest count; in 2D the (b+1)^2 edges one per lane.
+// k_edge in 3D: the edges of each direction march across the b + 1 planes along one direction,
+// in runs of at most kEdgeRun planes (the first plane of a run is read for the second), each run
+// with its plane of lanes: the y- and z-edges over rows of b + 2 positions (a halo column before
+// each row of b + 1 edges), 63 positions per wavefront after a halo lane; the x-edges in tiles of
+// 8 x 8 lanes, 7 new rows of the b + 2 along y per tile.  In 2D the (b+1)^2 edges one per lane.
+constexpr int kEdgeRun = 9;
+
+// wavefronts of k_edge per block
 constexpr int
 fused_edge_waves(int dim, int b)
 {
   const int n1 = b + 1;
   if (dim == 2)
     return (n1 * n1 + 63) / 64;
-  const int wp = ((b + 2) * n1 * b + 62) / 63;
-  const int wx = (b * n1 * n1 + 63) / 64;
-  return wp > wx ? wp : wx;
+  const int runs = (n1 + kEdgeRun - 1) / kEdgeRun;
+  const int wp = ((b + 2) * b + 62) / 63;
+  const int wx = (b / 8) * ((b + 8) / 7);
+  return runs * (wx + 2 * wp);
 }
 
 // One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
@@ -115,7 +120,7 @@ struct FusedTune
   int front = 0;          // 1: k_front runs the prim, efield and cell stages (kStagePrim; the other
                           // two launch nothing); 0: k_prim, k_efield and k_cell (viscosity needs them)
   int cell_threads = 256; // workgroup size of k_cell: 256, 512 or 768
-  int edge_threads = 256; // workgroup size of k_edge: 256, 512 or 1024
+  int edge_threads = 256; // workgroup size of k_edge: 256 or 512
   int per_class = 1;      // slots of several block sizes: 1 launches every stage once per block-size
                           // class with the kernels specialised to it; 0 once with the kernels that
                           // dispatch on the slot's block size

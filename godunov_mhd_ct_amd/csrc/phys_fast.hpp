@@ -5,11 +5,11 @@
 // root of the largest radicand (the square root is monotonic).  This cuts the 2D-HLLD EMF from 70
 // divisions and 28 square roots to 20 reciprocals and 14 square roots, the HLLD flux from 27
 // divisions and 6 square roots to 11 and 5.  In FP64 each reciprocal and square root is a
-// Newton-Raphson refinement of the hardware estimate (rcp_nr, sqrt_nr): 5 and 10 instructions,
-// against the 11 and 17 of the correctly rounded sequences.  The fused form uses these; its
-// results differ from the reference's operation order by a few ulps per evaluation (its parity
-// bar is the north star's 1e-12 relative L-inf per field, momentum as a vector).  The plain form
-// keeps phys.hpp's order, bit for bit the reference's.
+// Refinements of the hardware estimates (rcp_nr, sqrt_nr): 4 and 8 instructions, against the 11
+// and 17 of the correctly rounded sequences.  The fused form uses these; its results differ from
+// the reference's operation order by a few ulps per evaluation (its parity bar is the north
+// star's 1e-12 relative L-inf per field, momentum as a vector).  The plain form keeps phys.hpp's
+// order, bit for bit the reference's.
 #pragma once
 
 #include "phys.hpp"
@@ -17,10 +17,12 @@
 namespace mhdct
 {
 
-// 1/x in FP64: the hardware estimate v_rcp_f64 refined by two Newton-Raphson steps
-// r <- r + r (1 - x r) (the quadratically convergent reciprocal iteration), each as two fused
-// multiply-adds.  Within about one ulp of 1/x for normal x of either sign; not correctly rounded.
-// The FP32 form keeps 1/x (the hardware-approximate FP32 division of its build).
+// 1/x in FP64: the hardware estimate r = v_rcp_f64(x) (relative error below 2^-25 on MI355X)
+// refined by one third-order step of the reciprocal iteration: with e = 1 - x r,
+// 1/x = r / (1 - e) = r (1 + e + e^2 + ...), so r <- r + r (e + e^2) leaves a relative error of
+// e^3 (below 2^-75) before the final rounding; three fused multiply-adds.  Equal to the correctly
+// rounded 1/x on all of 4M random normal x over 2^-30 .. 2^31 on MI355X, though not guaranteed
+// to be.  The FP32 form keeps 1/x (the hardware-approximate FP32 division of its build).
 template <typename T>
 __host__ __device__ __forceinline__ T
 rcp_nr(T x)
@@ -28,11 +30,9 @@ rcp_nr(T x)
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (sizeof(T) == 8)
   {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, __builtin_fma(e, e, e), r);
   }
   else
 #endif
@@ -41,8 +41,9 @@ rcp_nr(T x)
 
 // sqrt(x) in FP64: the hardware estimate y = v_rsq_f64(x) of 1/sqrt(x), one step of Goldschmidt's
 // coupled iteration for g = sqrt(x) and h = 1/(2 sqrt(x)) (r = 1/2 - g h; g <- g + g r;
-// h <- h + h r), then two residual corrections g <- g + (x - g^2) h.  Within about one ulp of
-// sqrt(x) for normal x >= 0 (sqrt(+-0) = +-0, NaN below zero); not correctly rounded.
+// h <- h + h r; within a few tens of ulps), then one residual correction g <- g + (x - g^2) h.
+// Equal to the correctly rounded sqrt(x) on all of 4M random normal x over 2^-30 .. 2^31 on
+// MI355X, though not guaranteed to be; sqrt(+-0) = +-0, NaN below zero.
 template <typename T>
 __host__ __device__ __forceinline__ T
 sqrt_nr(T x)
@@ -56,9 +57,7 @@ sqrt_nr(T x)
     const double r = __builtin_fma(-g, h, 0.5);
     g = __builtin_fma(g, r, g);
     h = __builtin_fma(h, r, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
+    const double d = __builtin_fma(-g, g, x);
     g = __builtin_fma(d, h, g);
     return x == 0.0 ? x : g;
   }
