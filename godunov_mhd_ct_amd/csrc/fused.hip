@@ -87,17 +87,14 @@ struct Geo
   static constexpr int NCE = DIM == 3 ? 3 : 1;            // EMF components
   static constexpr int NEC = I::ecells();                 // edge-grid points per component
   static constexpr int EPD = DIM == 3 ? BS * NE * NE : NE * NE; // edges per direction
-  // k_edge (3D): runs of at most MCH edges marching across NE planes (NCH runs), the plane
-  // wavefronts of the y- and z-edges (NPL positions, 63 per wavefront) and of the x-edges (8 x 8
-  // tiles, 7 new positions along y each); EWB: k_edge wavefronts per block (2D: one edge per
-  // lane)
-  static constexpr int MCH = kEdgeRun;
-  static constexpr int NCH = (NE + MCH - 1) / MCH;
+  // k_edge (3D): the plane wavefronts of the marching y- and z-edges (NPL positions, 63 per
+  // wavefront) and of the x-edges (8 x 8 tiles, 7 new positions along y each); EWB: k_edge
+  // wavefronts per block (2D: one edge per lane)
   static constexpr int NPL = (BS + 2) * BS;
   static constexpr int NWP = (NPL + 62) / 63;
   static constexpr int NWX = (BS / 8) * ((BS + 8) / 7);
   static constexpr int EWB = fused_edge_waves(DIM, BS);
-  static_assert(DIM == 2 || EWB == NCH * (NWX + 2 * NWP), "k_edge wavefronts per block");
+  static_assert(DIM == 2 || EWB == NWX + 2 * NWP, "k_edge wavefronts per block");
   static constexpr int P1 = (G1 + 63) / 64 * 64;          // k_cell items per block
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
   static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
@@ -1702,12 +1699,12 @@ edge_part_of(const EdgeRaw<T, D0, D1> & R, const Gas<T> & gt, T * h)
   h[kP] = fmax(gt.smallp * h[kD], h[kP]);
 }
 
-// The EMFs along axis AX of a run of 3D edges that march along D1, D0 and D1 the two directions
+// The EMFs along axis AX of the b + 1 3D edges that march along D1, D0 and D1 the two directions
 // across the edges (z-edges: x, y; y-edges: x, z; x-edges: y, z).  The edge with cell A has its
 // corners at KIND 0 at A, KIND 3 at C = A - e_D1, KIND 2 at A' = A - e_D0 and KIND 1 at C' =
 // A' - e_D1, with Bf_D0 at A (KIND 0, 2) and at C (KIND 1, 3) and Bf_D1 at A (KIND 0, 3) and at
-// A' (KIND 1, 2).  Each lane holds the cell c = (c[0], c[1], c[2]) with c[D1] = m for m = m0 - 1
-// .. m1 - 1 and forms, from one read of the cell's record, the cell parts of KIND 0 and 2 (A of
+// A' (KIND 1, 2).  Each lane holds the cell c = (c[0], c[1], c[2]) with c[D1] = m for m = -1 .. b
+// and forms, from one read of the cell's record, the cell parts of KIND 0 and 2 (A of
 // its own edge at m and of the edge one step up along D0) and of KIND 3 and 1 (C of the same two
 // edges at m + 1); KIND 2 and 1 come to the lane one step up along D0 by cross-lane reads (the
 // lane whose byte address is `src` holds A' and C'), KIND 3, 1 and Bf_D0 are carried to the next
@@ -1716,8 +1713,7 @@ edge_part_of(const EdgeRaw<T, D0, D1> & R, const Gas<T> & gt, T * h)
 // on.  Same arithmetic as edge_at, the same states bit for bit.
 template <typename T, int DIM, int BS, int AX, bool HK>
 __device__ __forceinline__ void
-edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int m0, int m1, int src, bool edge,
-           T dtdx)
+edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int src, bool edge, T dtdx)
 {
   static_assert(DIM == 3, "edge_march: 3D edges");
   using G = Geo<DIM, BS>;
@@ -1730,9 +1726,9 @@ edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], 
   constexpr int             slot2 = AX == 1 ? kLT : kRB;
   T                         hC3[NH], hCp[NH], bC0 = T(0), fC0 = T(0);
   EdgeRaw<T, D0, D1>        R;
-  c[D1] = m0 - 1;
+  c[D1] = -1;
   edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, R);
-  for (int m = m0 - 1; m < m1; ++m)
+  for (int m = -1; m < G::NE; ++m)
   {
     T hA0[NH], h2[NH], h3[NH], h1[NH];
     edge_part_of<T, 0, D0, D1>(R, S.gt, hA0);
@@ -1749,13 +1745,13 @@ edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], 
     const T bAp1 = lane_take(bA1, src), fAp1 = lane_take(fA1, src);
     // the next cell's reads, in flight while this edge's EMF is formed
     c[D1] = m;
-    if (m + 1 < m1)
+    if (m + 1 < G::NE)
     {
       c[D1] = m + 1;
       edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, R);
       c[D1] = m;
     }
-    if (edge && m >= m0)
+    if (edge && m >= 0)
     {
       T qe[4][8];
       edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
@@ -1777,15 +1773,15 @@ edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], 
 
 // ---------------------------------------------------------------------------------------------
 // k_edge: every edge of a block once.  In 3D the edges of each direction march along one
-// direction D1 across them (edge_march), in runs of at most G::MCH edges, each run's wavefronts
-// over a plane of the two other directions:
+// direction D1 across them (edge_march: 16^3 blocks at 9 and at 17 planes per march in one
+// measurement, 1.86 and 1.80 ms), each wavefront over a plane of the two other directions:
 //   * y- and z-edges: positions over rows of b + 2 along x (i = -1 .. b) and the edge's own axis,
 //     63 per wavefront after the position before them; lane s - 1 holds the edge one step back
 //     along x, the lanes at i = -1 and lane 0 only pass their cell parts on;
 //   * x-edges: tiles of 8 lanes along x by 8 positions along y (j = -1 .. b), 7 new after the
 //     last one of the tile before; lane s - 8 holds the edge one step back along y, the first
 //     row and the lanes at j = -1 only pass their cell parts on.
-// G::EWB wavefronts per block: x-edges first, then y, then z, each by plane wavefront and run.
+// G::EWB wavefronts per block: x-edges first, then y, then z, each by plane wavefront.
 // A marching lane holds about 230 registers (the next cell's reads in flight while the EMF is
 // formed), so the 3D kernels run 2 wavefronts per SIMD.  In 2D a plane of lanes would be one row
 // of b + 2 positions, too short for a wavefront: the edges run one per lane (edge_at), 4
@@ -1816,12 +1812,9 @@ edge_item(const FusedSlot<T> & S, long r0)
   }
   else
   {
-    // x-edges, then y, then z, each by plane wavefront and run
-    constexpr int nx = G::NWX * G::NCH, np = G::NWP * G::NCH;
-    const int     a = w < nx ? 0 : (w < nx + np ? 1 : 2);
-    const int     wd = a == 0 ? w : (a == 1 ? w - nx : w - nx - np);
-    const int     ch = wd % G::NCH, pw = wd / G::NCH;
-    const int     m0 = ch * G::MCH, m1 = (m0 + G::MCH < G::NE ? m0 + G::MCH : G::NE);
+    // x-edges, then y, then z, each by plane wavefront
+    const int     a = w < G::NWX ? 0 : (w < G::NWX + G::NWP ? 1 : 2);
+    const int     pw = a == 0 ? w : (a == 1 ? w - G::NWX : w - G::NWX - G::NWP);
     int           c[3], src;
     bool          edge;
     constexpr int e0 = BS + 2;
@@ -1850,18 +1843,18 @@ edge_item(const FusedSlot<T> & S, long r0)
     if (S.rk2)
     {
       if (a == 2)
-        edge_march<T, DIM, BS, 2, false>(S, V, o, c, m0, m1, src, edge, dtdx);
+        edge_march<T, DIM, BS, 2, false>(S, V, o, c, src, edge, dtdx);
       else if (a == 1)
-        edge_march<T, DIM, BS, 1, false>(S, V, o, c, m0, m1, src, edge, dtdx);
+        edge_march<T, DIM, BS, 1, false>(S, V, o, c, src, edge, dtdx);
       else
-        edge_march<T, DIM, BS, 0, false>(S, V, o, c, m0, m1, src, edge, dtdx);
+        edge_march<T, DIM, BS, 0, false>(S, V, o, c, src, edge, dtdx);
     }
     else if (a == 2)
-      edge_march<T, DIM, BS, 2, true>(S, V, o, c, m0, m1, src, edge, dtdx);
+      edge_march<T, DIM, BS, 2, true>(S, V, o, c, src, edge, dtdx);
     else if (a == 1)
-      edge_march<T, DIM, BS, 1, true>(S, V, o, c, m0, m1, src, edge, dtdx);
+      edge_march<T, DIM, BS, 1, true>(S, V, o, c, src, edge, dtdx);
     else
-      edge_march<T, DIM, BS, 0, true>(S, V, o, c, m0, m1, src, edge, dtdx);
+      edge_march<T, DIM, BS, 0, true>(S, V, o, c, src, edge, dtdx);
   }
 }
 

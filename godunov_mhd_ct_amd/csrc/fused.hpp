@@ -35,24 +35,20 @@ fused_front_depth(int dim, int b)
   return dim == 3 ? (b == 8 ? 2 : 4) : 1;
 }
 
-// k_edge in 3D: the edges of each direction march across the b + 1 planes along one direction,
-// in runs of at most kEdgeRun planes (the first plane of a run is read for the second), each run
-// with its plane of lanes: the y- and z-edges over rows of b + 2 positions (a halo column before
-// each row of b + 1 edges), 63 positions per wavefront after a halo lane; the x-edges in tiles of
-// 8 x 8 lanes, 7 new rows of the b + 2 along y per tile.  In 2D the (b+1)^2 edges one per lane.
-constexpr int kEdgeRun = 9;
-
-// wavefronts of k_edge per block
+// Wavefronts of k_edge per block.  In 3D the edges of each direction march across the b + 1
+// planes along one direction, each wavefront with its plane of lanes: the y- and z-edges over
+// rows of b + 2 positions (a halo column before each row of b + 1 edges), 63 positions per
+// wavefront after a halo lane; the x-edges in tiles of 8 x 8 lanes, 7 new rows of the b + 2
+// along y per tile.  In 2D the (b+1)^2 edges one per lane.
 constexpr int
 fused_edge_waves(int dim, int b)
 {
   const int n1 = b + 1;
   if (dim == 2)
     return (n1 * n1 + 63) / 64;
-  const int runs = (n1 + kEdgeRun - 1) / kEdgeRun;
   const int wp = ((b + 2) * b + 62) / 63;
   const int wx = (b / 8) * ((b + 8) / 7);
-  return runs * (wx + 2 * wp);
+  return wx + 2 * wp;
 }
 
 // One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
