@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--front", default="off", choices=["on", "off"],
                     help="primitive, electric-field and cell stages as one LDS-tiled kernel (on) or three (off)")
     ap.add_argument("--stage-threads", default="",
-                    help="CELL,EDGE: workgroup sizes of k_cell (256, 512 or 768) and k_edge (256 or 512; default: the "
+                    help="CELL,EDGE: workgroup sizes of k_cell (256 or 512) and k_edge (256 or 512; default: the "
                          "library's choice)")
     ap.add_argument("--precision-steps", type=int, default=0,
                     help="with --impl fp32: run the FP32-bulk and the FP64 form this many steps from the same "

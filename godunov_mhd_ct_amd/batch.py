@@ -156,7 +156,7 @@ class BatchGodunovMHD:
         self.front = bool(enable)
 
     def set_stage_threads(self, cell_threads: int, edge_threads: int):
-        """Workgroup sizes of k_cell (256, 512 or 768) and k_edge (256 or 512); same results."""
+        """Workgroup sizes of k_cell (256 or 512) and k_edge (256 or 512); same results."""
         self._check(lib().mhdct_batch_set_stage_threads(self._h, int(cell_threads), int(edge_threads)),
                     "batch_set_stage_threads")
 

@@ -858,10 +858,10 @@ extern "C"
   {
     if (!b)
       return -1;
-    if (!(cell_threads == 256 || cell_threads == 512 || cell_threads == 768) ||
+    if (!(cell_threads == 256 || cell_threads == 512) ||
         !(edge_threads == 256 || edge_threads == 512))
     {
-      b->err = "mhdct_batch_set_stage_threads: k_cell 256, 512 or 768 threads, k_edge 256 or 512";
+      b->err = "mhdct_batch_set_stage_threads: k_cell and k_edge 256 or 512 threads";
       return -1;
     }
     b->tune.cell_threads = cell_threads;

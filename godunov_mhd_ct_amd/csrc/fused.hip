@@ -95,7 +95,13 @@ struct Geo
   static constexpr int NWX = (BS / 8) * ((BS + 8) / 7);
   static constexpr int EWB = fused_edge_waves(DIM, BS);
   static_assert(DIM == 2 || EWB == NWX + 2 * NWP, "k_edge wavefronts per block");
-  static constexpr int P1 = (G1 + 63) / 64 * 64;          // k_cell items per block
+  // k_cell: NCHK runs of CL cells along z per (i, j) column of the 1-ghost tile (CMARCH, 3D), else
+  // one cell per item; P1 items per block
+  static constexpr int  NCHK = fused_cell_runs(DIM, BS);
+  static constexpr bool CMARCH = NCHK > 0;
+  static constexpr int  CL = CMARCH ? N1 / NCHK : 1;
+  static_assert(!CMARCH || CL * NCHK == N1, "k_cell runs tile the column");
+  static constexpr int  P1 = CMARCH ? (N1 * N1 * NCHK + 63) / 64 * 64 : (G1 + 63) / 64 * 64;
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
   static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
   static constexpr int PRIM = NC + (GSH + 63) / 64 * 64;  // k_prim items: interior, then shell
@@ -589,43 +595,124 @@ k_efield(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __res
 // predictor's viscous term is compiled in (FP64 kernels of one block size: viscosity runs per
 // context, never in a batch of mixed block sizes, and the FP32 form refuses it).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int DIM, int BS, bool HK, bool VISC, class Src>
+// The stencil of one cell, each value read once: qc the 11 values of the cell (5 hydro, the low
+// and the high face of each component), qn[d][0] / qn[d][1] at the neighbours +e_d / -e_d the 5
+// hydro values and the low and high faces of the components across d (the only ones the slopes
+// read), e the edge E values the face source reads.
+template <typename T>
+struct CellStencil
+{
+  T qc[11], qn[3][2][11];
+  // Ex, Ey, Ez at (i, j, k); Ez at j + 1, Ey at k + 1, Ez at i + 1, Ex at k + 1, Ey at i + 1,
+  // Ex at j + 1 (2D: e[2], e[3], e[5] only)
+  T e[9];
+};
+
+// value v (0-4 hydro, 5-7 low and 8-10 high faces of Bx, By, Bz) of cell (a, c, e)
+template <typename T, int DIM, class Src>
+__device__ __forceinline__ T
+cell_q(const Src & src, int v, int a, int c, int e)
+{
+  if (v < 5)
+    return src.qp(v, a, c, e);
+  const int fc = (v - 5) % 3;
+  const int hi = v >= 8;
+  if (DIM == 2 && fc == 2)
+    return src.bf(2, a, c, 0);
+  return src.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
+}
+
+// the values of neighbour n = cell (i, j, k) + sg e_d that the slopes read
+template <typename T, int DIM, class Src>
 __device__ __forceinline__ void
-cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, int o, int i, int j, int k, int c1)
+cell_neighbour(const Src & src, int d, int sg, int i, int j, int k, T * qn)
+{
+  const int a = i + sg * (d == 0), c = j + sg * (d == 1), e = k + sg * (d == 2);
+  for (int v = 0; v < 5; ++v)
+    qn[v] = cell_q<T, DIM>(src, v, a, c, e);
+  for (int mc = 0; mc < 3; ++mc)
+    if (mc != d)
+    {
+      qn[5 + mc] = cell_q<T, DIM>(src, 5 + mc, a, c, e);
+      qn[8 + mc] = cell_q<T, DIM>(src, 8 + mc, a, c, e);
+    }
+}
+
+template <typename T, int DIM, class Src>
+__device__ __forceinline__ void
+cell_stencil(const Src & src, int i, int j, int k, CellStencil<T> & C)
+{
+  for (int v = 0; v < 11; ++v)
+    C.qc[v] = cell_q<T, DIM>(src, v, i, j, k);
+  for (int d = 0; d < DIM; ++d)
+  {
+    cell_neighbour<T, DIM>(src, d, 1, i, j, k, C.qn[d][0]);
+    cell_neighbour<T, DIM>(src, d, -1, i, j, k, C.qn[d][1]);
+  }
+  if (DIM == 2)
+  {
+    C.e[2] = src.ed(0, i, j, 0);
+    C.e[3] = src.ed(0, i, j + 1, 0);
+    C.e[5] = src.ed(0, i + 1, j, 0);
+  }
+  else
+  {
+    C.e[0] = src.ed(0, i, j, k);
+    C.e[1] = src.ed(1, i, j, k);
+    C.e[2] = src.ed(2, i, j, k);
+    C.e[3] = src.ed(2, i, j + 1, k);
+    C.e[4] = src.ed(1, i, j, k + 1);
+    C.e[5] = src.ed(2, i + 1, j, k);
+    C.e[6] = src.ed(0, i, j, k + 1);
+    C.e[7] = src.ed(1, i + 1, j, k);
+    C.e[8] = src.ed(0, i, j + 1, k);
+  }
+}
+
+// The stencil of cell (i, j, k + 1) from that of (i, j, k) (3D): the cell becomes the -z
+// neighbour, the +z neighbour the cell (its low Bz face is the old high one), E at k + 1 of the
+// old cell E at the new one; the rest is read.
+template <typename T, class Src>
+__device__ __forceinline__ void
+cell_stencil_up(const Src & src, int i, int j, int k, CellStencil<T> & C)
+{
+  constexpr int keep[9] = { 0, 1, 2, 3, 4, 5, 6, 8, 9 };
+  for (int n = 0; n < 9; ++n)
+  {
+    C.qn[2][1][keep[n]] = C.qc[keep[n]];
+    C.qc[keep[n]] = C.qn[2][0][keep[n]];
+  }
+  C.qc[7] = C.qc[10];
+  const int kn = k + 1;
+  C.qc[10] = cell_q<T, 3>(src, 10, i, j, kn);
+  cell_neighbour<T, 3>(src, 2, 1, i, j, kn, C.qn[2][0]);
+  for (int d = 0; d < 2; ++d)
+  {
+    cell_neighbour<T, 3>(src, d, 1, i, j, kn, C.qn[d][0]);
+    cell_neighbour<T, 3>(src, d, -1, i, j, kn, C.qn[d][1]);
+  }
+  C.e[0] = C.e[6];
+  C.e[1] = C.e[4];
+  C.e[2] = src.ed(2, i, j, kn);
+  C.e[3] = src.ed(2, i, j + 1, kn);
+  C.e[4] = src.ed(1, i, j, kn + 1);
+  C.e[5] = src.ed(2, i + 1, j, kn);
+  C.e[6] = src.ed(0, i, j, kn + 1);
+  C.e[7] = src.ed(1, i + 1, j, kn);
+  C.e[8] = src.ed(0, i, j + 1, kn);
+}
+
+// slopes, predictor, face source and face-B slopes of cell (i, j, k) from its stencil, stored as
+// record c1
+template <typename T, int DIM, int BS, bool HK, bool VISC>
+__device__ __forceinline__ void
+cell_finish(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const CellStencil<T> & CS, int o, int c1)
 {
   using G = Geo<DIM, BS>;
     const DevMesh &      m = S.m;
     const T   st = S.gt.slope_type;
-    // the 11-variable primitive state of cell (a, c, e): 5 hydro + low/high face fields
-    auto Q = [&](int v, int a, int c, int e) -> T {
-      if (v < 5)
-        return src.qp(v, a, c, e);
-      const int fc = (v - 5) % 3;
-      const int hi = v >= 8;
-      if (DIM == 2 && fc == 2)
-        return src.bf(2, a, c, 0);
-      return src.bf(fc, a + (hi && fc == 0), c + (hi && fc == 1), e + (hi && fc == 2));
-    };
-    // the stencil, each value read once: the 11 values of the cell, and at its 2 DIM neighbours
-    // (qn[d][0] at +e_d, qn[d][1] at -e_d) the 5 hydro values and the low and high faces of the
-    // components across d, the only ones the slopes read
-    T qc[11], qn[3][2][11];
-    for (int v = 0; v < 11; ++v)
-      qc[v] = Q(v, i, j, k);
-    for (int d = 0; d < DIM; ++d)
-      for (int sd = 0; sd < 2; ++sd)
-      {
-        const int sg = sd == 0 ? 1 : -1;
-        const int a = i + sg * (d == 0), c = j + sg * (d == 1), e = k + sg * (d == 2);
-        for (int v = 0; v < 5; ++v)
-          qn[d][sd][v] = Q(v, a, c, e);
-        for (int mc = 0; mc < 3; ++mc)
-          if (mc != d)
-          {
-            qn[d][sd][5 + mc] = Q(5 + mc, a, c, e);
-            qn[d][sd][8 + mc] = Q(8 + mc, a, c, e);
-          }
-      }
+    const T(&qc)[11] = CS.qc;
+    const T(&qn)[3][2][11] = CS.qn;
     // Q(v, cell + o e_d) for o in {0, +1, -1}, from the stencil
     auto QS = [&](int v, int d, int off) -> T { return off == 0 ? qc[v] : qn[d][off > 0 ? 0 : 1][v]; };
     T         sl[3][8];
@@ -745,22 +832,20 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
       q2o[7] = C + T(0.5) * sC0;
     }
     // edge electric field v x B at t_n (k_efield, or the front tile)
-    auto Ex = [&](int a, int c, int e) -> T { return src.ed(0, a, c, e); };
-    auto Ey = [&](int a, int c, int e) -> T { return src.ed(1, a, c, e); };
-    auto Ez = [&](int a, int c, int e) -> T { return src.ed(DIM == 3 ? 2 : 0, a, c, e); };
+    const T(&E)[9] = CS.e;
     T sfo[3];
     if (DIM == 2)
     {
-      const T e00 = Ez(i, j, 0);
-      sfo[0] = (Ez(i, j + 1, 0) - e00) * T(0.5) * dtdy;
-      sfo[1] = -(Ez(i + 1, j, 0) - e00) * T(0.5) * dtdx;
+      const T e00 = E[2];
+      sfo[0] = (E[3] - e00) * T(0.5) * dtdy;
+      sfo[1] = -(E[5] - e00) * T(0.5) * dtdx;
     }
     else
     {
-      const T ex = Ex(i, j, k), ey = Ey(i, j, k), ez = Ez(i, j, k);
-      sfo[0] = (Ez(i, j + 1, k) - ez) * T(0.5) * dtdy - (Ey(i, j, k + 1) - ey) * T(0.5) * dtdz;
-      sfo[1] = -(Ez(i + 1, j, k) - ez) * T(0.5) * dtdx + (Ex(i, j, k + 1) - ex) * T(0.5) * dtdz;
-      sfo[2] = (Ey(i + 1, j, k) - ey) * T(0.5) * dtdx - (Ex(i, j + 1, k) - ex) * T(0.5) * dtdy;
+      const T ex = E[0], ey = E[1], ez = E[2];
+      sfo[0] = (E[3] - ez) * T(0.5) * dtdy - (E[4] - ey) * T(0.5) * dtdz;
+      sfo[1] = -(E[5] - ez) * T(0.5) * dtdx + (E[6] - ex) * T(0.5) * dtdz;
+      sfo[2] = (E[7] - ey) * T(0.5) * dtdx - (E[8] - ex) * T(0.5) * dtdy;
     }
     // transverse slopes of the face field at this cell's low faces (dBx/dy, dBx/dz, dBy/dx,
     // dBy/dz, dBz/dx, dBz/dy; 2D: dBx/dy, dBy/dx)
@@ -793,6 +878,19 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
       bstore<T>(V.r_fs, n2 * G::G1 + c1, fso[n2]);
 }
 
+template <typename T, int DIM, int BS, bool HK, bool VISC, class Src>
+__device__ __forceinline__ void
+cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, int o, int i, int j, int k, int c1)
+{
+  CellStencil<T> C;
+  cell_stencil<T, DIM>(src, i, j, k, C);
+  cell_finish<T, DIM, BS, HK, VISC>(S, V, C, o, c1);
+}
+
+// One item per cell of the 1-ghost tile, or (G::CMARCH, 3D) per lane marching a run of G::CL
+// cells of a column along z: the stencil of each next cell from the last one's
+// (cell_stencil_up), 53 reads per cell after the first instead of 74, at 2 wavefronts per SIMD
+// (the stencil of the next cell and the results of this one are live together).
 template <typename T, int DIM, int BS, bool HK, bool VISC>
 __device__ __forceinline__ void
 cell_item(const FusedSlot<T> & S, long r0)
@@ -801,18 +899,38 @@ cell_item(const FusedSlot<T> & S, long r0)
   constexpr int n = G::N1;
   const int     o = __builtin_amdgcn_readfirstlane(int(r0) / G::P1);
   const int     ci = int(r0) - o * G::P1;
-  if (ci >= G::G1)
-    return;
   const View<T, DIM, BS>      V(S, o);
   const GlobalSrc<T, DIM, BS> src(V, S, o);
-  const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
-  cell_core<T, DIM, BS, HK, VISC>(S, V, src, o, i, j, k, ci);
+  if constexpr (DIM == 3 && G::CMARCH)
+  {
+    // lane position (i, j, run) over N1 x N1 x NCHK: a column of CL cells from k0
+    if (ci >= n * n * G::NCHK)
+      return;
+    const int i = ci % n - 1, j = (ci / n) % n - 1, k0 = (ci / (n * n)) * G::CL - 1;
+    CellStencil<T> C;
+    cell_stencil<T, DIM>(src, i, j, k0, C);
+    for (int t = 0; t < G::CL; ++t)
+    {
+      const int k = k0 + t;
+      cell_finish<T, DIM, BS, HK, VISC>(S, V, C, o, View<T, DIM, BS>::c1(i, j, k));
+      if (t + 1 < G::CL)
+        cell_stencil_up<T>(src, i, j, k, C);
+    }
+  }
+  else
+  {
+    if (ci >= G::G1)
+      return;
+    const int i = ci % n - 1, j = (ci / n) % n - 1, k = DIM == 3 ? ci / (n * n) - 1 : 0;
+    cell_core<T, DIM, BS, HK, VISC>(S, V, src, o, i, j, k, ci);
+  }
 }
 
 // VISC: the viscous predictor term compiled in (only the per-mesh FP64 form runs viscosity; its
 // registers would otherwise be held by every cell kernel)
 template <typename T, int DIM, int BSEL, bool HK, bool VISC>
-__global__ void __launch_bounds__(768) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu((DIM == 3 && (BSEL == 0 || fused_cell_runs(DIM, BSEL) > 0)) ? 2 : 3)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -2327,7 +2445,8 @@ fused_items_per_block(int dim, int b, int stage)
   case 0: return nc + (g2 - nc + 63) / 64 * 64;
   case 6: return (npe + 63) / 64 * 64;
   case 8: return nc;
-  case 1: return (g1 + 63) / 64 * 64;
+  case 1: return fused_cell_runs(dim, b) > 0 ? (long(b + 2) * (b + 2) * fused_cell_runs(dim, b) + 63) / 64 * 64
+                                           : (g1 + 63) / 64 * 64;
   case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
   case 4: return long(fused_edge_waves(dim, b)) * 64;

@@ -35,6 +35,16 @@ fused_front_depth(int dim, int b)
   return dim == 3 ? (b == 8 ? 2 : 4) : 1;
 }
 
+// k_cell in 3D: the runs along z of each (i, j) column of a block's 1-ghost tile (b + 2 cells),
+// each run marched by one lane: runs of 5, 3 and 2 cells for 8^3, 16^3 and 32^3 blocks, measured
+// on MI355X against the other run lengths (16^3: 2, 6 and 9 cells; 32^3: 17) and against one cell
+// per item at 3 wavefronts per SIMD, which was slower at 16^3 and 32^3 (8^3: within the noise).
+constexpr int
+fused_cell_runs(int dim, int b)
+{
+  return dim != 3 ? 0 : (b == 8 ? 2 : (b == 16 ? 6 : 17));
+}
+
 // Wavefronts of k_edge per block.  In 3D the edges of each direction march across the b + 1
 // planes along one direction, each wavefront with its plane of lanes: the y- and z-edges over
 // rows of b + 2 positions (a halo column before each row of b + 1 edges), 63 positions per
@@ -115,7 +125,7 @@ struct FusedTune
   int tile_z = 8;         // 3D flux tile depth: 8 (8x8x8 cells, 576 threads) or 4 (8x8x4, 320 threads)
   int front = 0;          // 1: k_front runs the prim, efield and cell stages (kStagePrim; the other
                           // two launch nothing); 0: k_prim, k_efield and k_cell (viscosity needs them)
-  int cell_threads = 256; // workgroup size of k_cell: 256, 512 or 768
+  int cell_threads = 256; // workgroup size of k_cell: 256 or 512
   int edge_threads = 256; // workgroup size of k_edge: 256 or 512
   int per_class = 1;      // slots of several block sizes: 1 launches every stage once per block-size
                           // class with the kernels specialised to it; 0 once with the kernels that

@@ -281,7 +281,7 @@ int mhdct_batch_set_class_launch(mhdct_batch * batch, int enable);
  * counterpart in the reference: a launch choice of this library (GodunovImplemV0.cpp:128-156 runs
  * these stages as separate device loops). */
 int mhdct_batch_set_front(mhdct_batch * batch, int enable);
-/* Workgroup sizes of the cell stage (k_cell: 256, 512 or 768 threads) and of the edge stage
+/* Workgroup sizes of the cell stage (k_cell: 256 or 512 threads) and of the edge stage
  * (k_edge: 256 or 512); a larger workgroup puts more consecutive wavefronts, which read the
  * same cells, on one CU.  Same results bit for bit.  No counterpart in the reference (a launch
  * choice). */

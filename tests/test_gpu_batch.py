@@ -194,7 +194,7 @@ def test_batch_tunables_do_not_change_results():
     print("[batch] tile depth 4, small launch caps, side-stream overlap, one launch over all block "
           "sizes, cell/edge workgroup sizes vs defaults", flush=True)
     ref = run_batch(cases, nsteps)
-    for class_launch, threads in ((True, (768, 512)), (False, (512, 512))):
+    for class_launch, threads in ((True, (512, 512)), (False, (256, 512))):
         bt = BatchGodunovMHD(3, IMPL_FUSED)
         ids = [bt.add(m, U, B, settings_for(slope)) for (m, U, B, slope) in cases]
         bt.set_tune(64, 16, 4)
