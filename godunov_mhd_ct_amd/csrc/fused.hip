@@ -95,13 +95,6 @@ struct Geo
   static constexpr int NWX = (BS / 8) * ((BS + 8) / 7);
   static constexpr int EWB = fused_edge_waves(DIM, BS);
   static_assert(DIM == 2 || EWB == NWX + 2 * NWP, "k_edge wavefronts per block");
-  // k_cell: NCHK runs of CL cells along z per (i, j) column of the 1-ghost tile (CMARCH, 3D), else
-  // one cell per item; P1 items per block
-  static constexpr int  NCHK = fused_cell_runs(DIM, BS);
-  static constexpr bool CMARCH = NCHK > 0;
-  static constexpr int  CL = CMARCH ? N1 / NCHK : 1;
-  static_assert(!CMARCH || CL * NCHK == N1, "k_cell runs tile the column");
-  static constexpr int  P1 = CMARCH ? (N1 * N1 * NCHK + 63) / 64 * 64 : (G1 + 63) / 64 * 64;
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
   static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
   static constexpr int PRIM = NC + (GSH + 63) / 64 * 64;  // k_prim items: interior, then shell
@@ -154,6 +147,22 @@ bstore(__amdgpu_buffer_rsrc_t r, int idx, T x)
                                           idx * 8, 0, kCacheNT);
   else
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, idx * 4, 0, kCacheNT);
+}
+
+// value x of the lane whose byte address (4 x lane) is addr
+template <typename T>
+__device__ __forceinline__ T
+lane_take(T x, int addr)
+{
+  if constexpr (sizeof(T) == 8)
+  {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u))));
+    const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u >> 32))));
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  }
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
 }
 
 template <typename T>
@@ -887,8 +896,22 @@ cell_core(const FusedSlot<T> & S, const View<T, DIM, BS> & V, const Src & src, i
   cell_finish<T, DIM, BS, HK, VISC>(S, V, C, o, c1);
 }
 
-// One item per cell of the 1-ghost tile, or (G::CMARCH, 3D) per lane marching a run of G::CL
-// cells of a column along z: the stencil of each next cell from the last one's
+// k_cell's work per block: NCHK runs of CL cells along z per (i, j) column of the 1-ghost tile
+// (CMARCH), else one cell per item; P1 items per block
+template <typename T, int DIM, int BS>
+struct CellG
+{
+  static constexpr int  N1 = BS + 2;
+  static constexpr int  G1 = DIM == 3 ? N1 * N1 * N1 : N1 * N1;
+  static constexpr int  NCHK = fused_cell_runs(DIM, BS, sizeof(T) == sizeof(double));
+  static constexpr bool CMARCH = NCHK > 0;
+  static constexpr int  CL = CMARCH ? N1 / NCHK : 1;
+  static_assert(!CMARCH || CL * NCHK == N1, "k_cell runs tile the column");
+  static constexpr int  P1 = CMARCH ? (N1 * N1 * NCHK + 63) / 64 * 64 : (G1 + 63) / 64 * 64;
+};
+
+// One item per cell of the 1-ghost tile, or (CellG::CMARCH, FP64 3D) per lane marching a run of
+// CellG::CL cells of a column along z: the stencil of each next cell from the last one's
 // (cell_stencil_up), 53 reads per cell after the first instead of 74, at 2 wavefronts per SIMD
 // (the stencil of the next cell and the results of this one are live together).
 template <typename T, int DIM, int BS, bool HK, bool VISC>
@@ -896,24 +919,25 @@ __device__ __forceinline__ void
 cell_item(const FusedSlot<T> & S, long r0)
 {
   using G = Geo<DIM, BS>;
+  using CG = CellG<T, DIM, BS>;
   constexpr int n = G::N1;
-  const int     o = __builtin_amdgcn_readfirstlane(int(r0) / G::P1);
-  const int     ci = int(r0) - o * G::P1;
+  const int     o = __builtin_amdgcn_readfirstlane(int(r0) / CG::P1);
+  const int     ci = int(r0) - o * CG::P1;
   const View<T, DIM, BS>      V(S, o);
   const GlobalSrc<T, DIM, BS> src(V, S, o);
-  if constexpr (DIM == 3 && G::CMARCH)
+  if constexpr (DIM == 3 && CG::CMARCH)
   {
     // lane position (i, j, run) over N1 x N1 x NCHK: a column of CL cells from k0
-    if (ci >= n * n * G::NCHK)
+    if (ci >= n * n * CG::NCHK)
       return;
-    const int i = ci % n - 1, j = (ci / n) % n - 1, k0 = (ci / (n * n)) * G::CL - 1;
+    const int i = ci % n - 1, j = (ci / n) % n - 1, k0 = (ci / (n * n)) * CG::CL - 1;
     CellStencil<T> C;
     cell_stencil<T, DIM>(src, i, j, k0, C);
-    for (int t = 0; t < G::CL; ++t)
+    for (int t = 0; t < CG::CL; ++t)
     {
       const int k = k0 + t;
       cell_finish<T, DIM, BS, HK, VISC>(S, V, C, o, View<T, DIM, BS>::c1(i, j, k));
-      if (t + 1 < G::CL)
+      if (t + 1 < CG::CL)
         cell_stencil_up<T>(src, i, j, k, C);
     }
   }
@@ -930,7 +954,7 @@ cell_item(const FusedSlot<T> & S, long r0)
 // registers would otherwise be held by every cell kernel)
 template <typename T, int DIM, int BSEL, bool HK, bool VISC>
 __global__ void __launch_bounds__(512)
-__attribute__((amdgpu_waves_per_eu((DIM == 3 && (BSEL == 0 || fused_cell_runs(DIM, BSEL) > 0)) ? 2 : 3)))
+__attribute__((amdgpu_waves_per_eu((DIM == 3 && sizeof(T) == sizeof(double) && (BSEL == 0 || fused_cell_runs(DIM, BSEL, true) > 0)) ? 2 : 3)))
 k_cell(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
 {
   for (long t = xcd_group() * blockDim.x + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x)
@@ -1281,27 +1305,30 @@ k_front(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __rest
 }
 
 
-// HLLD flux (times dt/dx) through the low face of cell (i,j,k) along DIR
-// (ComputeHydroFluxesAndStoreFunctor.cpp:218-503); out in the reference variable order.
-template <typename T, int DIM, int BS, int DIR>
+// HLLD fluxes (times dt/dx) through the faces along DIR (ComputeHydroFluxesAndStoreFunctor.cpp:
+// 218-503), in the reference variable order.
+//
+// The reconstructed state of cell c (index in the 1-ghost record grid) on its face on side SG
+// along DIR (+1: the high face, a left state; -1: the low face, a right state), floored; the
+// normal B component is left to the face value (face_hlld).
+template <typename T, int DIM, int BS, int DIR, int SG>
 __device__ __forceinline__ void
-face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
+face_side(const View<T, DIM, BS> & V, int c, T * q)
 {
-  using VW = View<T, DIM, BS>;
-  const int cR = VW::c1(i, j, k);
-  const int cL = VW::c1(i - (DIR == 0), j - (DIR == 1), k - (DIR == 2));
-  T         qL[8], qR[8];
   for (int v = 0; v < 8; ++v)
-  {
-    qL[v] = V.q2(v, cL) + T(0.5) * V.sl(DIR, v, cL);
-    qR[v] = V.q2(v, cR) + T(-0.5) * V.sl(DIR, v, cR);
-  }
+    if (v != kA + DIR)
+      q[v] = V.q2(v, c) + T(0.5 * SG) * V.sl(DIR, v, c);
   const Gas<T> & g = V.S.gt;
-  qL[kD] = fmax(g.smallr, qL[kD]);
-  qL[kP] = fmax(g.smallp * qL[kD], qL[kP]);
-  qR[kD] = fmax(g.smallr, qR[kD]);
-  qR[kP] = fmax(g.smallp * qR[kD], qR[kP]);
-  const T bn = V.bf(DIM == 2 && DIR == 2 ? 2 : DIR, i, j, k) + V.sf(DIR, cR);
+  q[kD] = fmax(g.smallr, q[kD]);
+  q[kP] = fmax(g.smallp * q[kD], q[kP]);
+}
+
+// HLLD flux along DIR between the states qL and qR with the face's normal field bn, times dt/dx
+// (qL and qR are modified)
+template <typename T, int DIR>
+__device__ __forceinline__ void
+face_hlld(T * qL, T * qR, T bn, T dtdS, const Gas<T> & g, T * f)
+{
   qR[kA + DIR] = bn;
   qL[kA + DIR] = bn;
   if (DIR != 0)
@@ -1323,6 +1350,21 @@ face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
   }
   for (int v = 0; v < 8; ++v)
     f[v] = f[v] * dtdS;
+}
+
+// HLLD flux through the low DIR face of cell (i, j, k), times dt/dx
+template <typename T, int DIM, int BS, int DIR>
+__device__ __forceinline__ void
+face_flux(const View<T, DIM, BS> & V, int i, int j, int k, T dtdS, T * f)
+{
+  using VW = View<T, DIM, BS>;
+  const int cR = VW::c1(i, j, k);
+  const int cL = VW::c1(i - (DIR == 0), j - (DIR == 1), k - (DIR == 2));
+  T         qL[8], qR[8];
+  face_side<T, DIM, BS, DIR, 1>(V, cL, qL);
+  face_side<T, DIM, BS, DIR, -1>(V, cR, qR);
+  const T bn = V.bf(DIM == 2 && DIR == 2 ? 2 : DIR, i, j, k) + V.sf(DIR, cR);
+  face_hlld<T, DIR>(qL, qR, bn, dtdS, V.S.gt, f);
 }
 
 // the six stored flux components: rho, E, mx, my, mz, Bz (2D, cell-centred Bz)
@@ -1424,6 +1466,38 @@ tile_threads()
   return TileG<DIM, TZ>::NTHR;
 }
 
+// The flux through the block-border face at cell c (c[DIR] = 0 or b, `side` 0 or 1) whose
+// neighbour is finer: the sum of the fine faces covering it (their border fluxes), divided by
+// 2^dim (as tile_faces has always formed it).
+template <typename T, int DIM, int BS, int DIR>
+__device__ __forceinline__ void
+fine_face_flux(const FusedSlot<T> & S, int o, const int * c, int side, T * out)
+{
+  constexpr int NV = DIM == 2 ? 6 : 5;
+  constexpr int t0 = DIR == 0 ? 1 : 0;
+  constexpr int t1 = DIR == 2 ? 1 : 2;
+  int           cc[3] = { c[0], c[1], c[2] };
+  cc[DIR] = side == 0 ? -1 : BS;
+  const Hit    h = neighbour(S.m, IdxC<DIM, BS>{}, o, cc);
+  const double vr = double(1 << DIM);
+  for (int vi = 0; vi < NV; ++vi)
+  {
+    double sum = 0;
+    if (DIM == 2)
+    {
+      for (int a = 0; a < 2; ++a)
+        sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, h.l[t0] + a)];
+    }
+    else
+    {
+      for (int q1 = 0; q1 < 2; ++q1)
+        for (int q0 = 0; q0 < 2; ++q0)
+          sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, (h.l[t0] + q0) + BS * (h.l[t1] + q1))];
+    }
+    out[vi] = T(sum / vr);
+  }
+}
+
 template <typename T, int DIM, int BS, int TZ, int DIR>
 __device__ __forceinline__ void
 tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int org[3], T dtdS,
@@ -1434,8 +1508,6 @@ tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int 
   constexpr int NV = DIM == 2 ? 6 : 5;
   constexpr int e0 = DIR == 0 ? TG::TX + 1 : TG::TX;
   constexpr int e1 = DIR == 1 ? TG::TY + 1 : TG::TY;
-  constexpr int t0 = DIR == 0 ? 1 : 0;
-  constexpr int t1 = DIR == 2 ? 1 : 2;
   const DevMesh & m = S.m;
   for (int fl = threadIdx.x; fl < NF; fl += TG::NTHR)
   {
@@ -1445,29 +1517,7 @@ tile_faces(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, const int 
     const bool border = c[DIR] == 0 || c[DIR] == BS;
     const int  side = c[DIR] == BS;
     if (border && m.conformal[long(o) * 2 * DIM + 2 * DIR + side] != 0)
-    {
-      // the neighbour is finer: sum of the fine faces covering this face, divided by 2^dim
-      int cc[3] = { c[0], c[1], c[2] };
-      cc[DIR] = side == 0 ? -1 : BS;
-      const Hit    h = neighbour(m, IdxC<DIM, BS>{}, o, cc);
-      const double vr = double(1 << DIM);
-      for (int vi = 0; vi < NV; ++vi)
-      {
-        double sum = 0;
-        if (DIM == 2)
-        {
-          for (int a = 0; a < 2; ++a)
-            sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, h.l[t0] + a)];
-        }
-        else
-        {
-          for (int q1 = 0; q1 < 2; ++q1)
-            for (int q0 = 0; q0 < 2; ++q0)
-              sum += S.BF[bf_index<DIM, BS>(h.oct, DIR, 1 - side, vi, (h.l[t0] + q0) + BS * (h.l[t1] + q1))];
-        }
-        out[vi] = T(sum / vr);
-      }
-    }
+      fine_face_flux<T, DIM, BS, DIR>(S, o, c, side, out);
     else
     {
       T f[8];
@@ -1564,6 +1614,195 @@ k_flux_tile(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __
     if (r0 >= S.cnt[3])
       continue;
     MHDCT_BS_DISPATCH(S.m.b, (flux_tile_item<T, DIM, BS, TZ>(S, r0, part, lf)));
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_flux_march (3D): the flux stage of an 8^3 tile with each cell's record read once per
+// direction instead of twice.  A workgroup of 4 wavefronts takes a tile:
+//   * wavefront 0 (z-faces) and 1 (y-faces): a lane per (x, other) column of the tile marches
+//     along the direction through the 10 cells from -1 to 8, forming from each cell's record its
+//     right state (the face below it) and its left state (the face above it, carried to the next
+//     cell); face a's flux with the carried one of face a - 1 gives cell a - 1's difference;
+//   * wavefronts 2 and 3 (x-faces): rows of the 10 cells from -1 to 8 along x, 6 rows per pass;
+//     each lane forms both states of its cell, takes the left state of its face from the lane
+//     before it and the flux of the face above from the lane after it (cross-lane reads).
+// The three differences of each cell go to LDS (FP64, 60 KB), and after one barrier every cell
+// is updated as k_flux_tile updates it: ((U + d_x) + d_y) + d_z.  Block-border faces with a finer
+// neighbour take the sum of its border fluxes (fine_face_flux), as in k_flux_tile.  Same fluxes,
+// the same U2 bit for bit.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int BS, int DIR>
+__device__ __forceinline__ void
+flux_march_dir(const FusedSlot<T> & S, const View<T, 3, BS> & V, int o, const int * org, int lane, T dtdS,
+               double (*ld)[512])
+{
+  static_assert(DIR == 1 || DIR == 2, "flux_march_dir: y or z faces");
+  using VW = View<T, 3, BS>;
+  constexpr int OT = DIR == 1 ? 2 : 1; // the transverse direction besides x
+  const int     li = lane & 7, le = lane >> 3;
+  const bool    fine_lo = S.m.conformal[long(o) * 6 + 2 * DIR] != 0;
+  const bool    fine_hi = S.m.conformal[long(o) * 6 + 2 * DIR + 1] != 0;
+  int           c[3];
+  c[0] = org[0] + li;
+  c[OT] = org[OT] + le;
+  T qLp[8], Fp[5];
+  for (int t = 0; t < kTS + 2; ++t)
+  {
+    const int a = t - 1; // the tile-relative cell along DIR, -1 .. 8
+    c[DIR] = org[DIR] + a;
+    const int cc = VW::c1(c[0], c[1], c[2]);
+    T         qR[8], qLn[8];
+    face_side<T, 3, BS, DIR, -1>(V, cc, qR);
+    if (t <= kTS)
+      face_side<T, 3, BS, DIR, 1>(V, cc, qLn);
+    if (t >= 1)
+    {
+      // the face below cell a (absolute c[DIR], uniform across the wavefront)
+      T          out[5];
+      const bool lo = c[DIR] == 0, hi = c[DIR] == BS;
+      if ((lo && fine_lo) || (hi && fine_hi))
+        fine_face_flux<T, 3, BS, DIR>(S, o, c, hi ? 1 : 0, out);
+      else
+      {
+        T       f[8];
+        const T bn = V.bf(DIR, c[0], c[1], c[2]) + V.sf(DIR, cc);
+        face_hlld<T, DIR>(qLp, qR, bn, dtdS, S.gt, f);
+        for (int vi = 0; vi < 5; ++vi)
+          out[vi] = f[kFluxVar[vi]];
+      }
+      if (t >= 2)
+      {
+        // cell a - 1: (0 + F_low) - F_high, as ReadFluxesAndConservativeUpdateFunctor forms it
+        const int cl = DIR == 1 ? li + kTS * ((a - 1) + kTS * le) : li + kTS * (le + kTS * (a - 1));
+        for (int vi = 0; vi < 5; ++vi)
+          ld[vi][cl] = double(Fp[vi]) - double(out[vi]);
+      }
+      for (int vi = 0; vi < 5; ++vi)
+        Fp[vi] = out[vi];
+    }
+    if (t <= kTS)
+      for (int v = 0; v < 8; ++v)
+        qLp[v] = qLn[v];
+  }
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void
+flux_rows_x(const FusedSlot<T> & S, const View<T, 3, BS> & V, int o, const int * org, int xw, int lane, T dtdS,
+            double (*ld)[512])
+{
+  using VW = View<T, 3, BS>;
+  constexpr int RPP = 6;                        // rows of 10 lanes per pass
+  constexpr int NPASS = (kTS * kTS + RPP - 1) / RPP; // 11 passes over the 64 rows
+  constexpr int P0 = (NPASS + 1) / 2;           // passes of the first x wavefront
+  const bool    fine_lo = S.m.conformal[long(o) * 6] != 0;
+  const bool    fine_hi = S.m.conformal[long(o) * 6 + 1] != 0;
+  const int     rr = lane / (kTS + 2), p = lane - rr * (kTS + 2);
+  const int     a = p - 1; // the tile-relative cell along x, -1 .. 8
+  const int     pass0 = xw == 0 ? 0 : P0, pass1 = xw == 0 ? P0 : NPASS;
+  const int     back = (lane > 0 ? lane - 1 : 0) * 4, next = (lane < 63 ? lane + 1 : 63) * 4;
+  for (int pass = pass0; pass < pass1; ++pass)
+  {
+    const int  row = pass * RPP + rr;
+    const bool valid = rr < RPP && row < kTS * kTS;
+    const int  lj = row % kTS, lk = row / kTS;
+    int        c[3] = { org[0] + a, org[1] + lj, org[2] + lk };
+    T          qR[8], qLn[8];
+    for (int v = 0; v < 8; ++v)
+    {
+      qR[v] = T(0);
+      qLn[v] = T(0);
+    }
+    const int cc = VW::c1(c[0], c[1], c[2]);
+    if (valid)
+    {
+      face_side<T, 3, BS, 0, -1>(V, cc, qR);
+      face_side<T, 3, BS, 0, 1>(V, cc, qLn);
+    }
+    // the left state of the face below this cell: the cell before it, one lane back
+    T qL[8];
+    for (int v = 0; v < 8; ++v)
+      qL[v] = lane_take(qLn[v], back);
+    T          out[5] = { T(0), T(0), T(0), T(0), T(0) };
+    const bool own = valid && p >= 1;
+    if (own)
+    {
+      const bool lo = c[0] == 0, hi = c[0] == BS;
+      if ((lo && fine_lo) || (hi && fine_hi))
+        fine_face_flux<T, 3, BS, 0>(S, o, c, hi ? 1 : 0, out);
+      else
+      {
+        T       f[8];
+        const T bn = V.bf(0, c[0], c[1], c[2]) + V.sf(0, cc);
+        face_hlld<T, 0>(qL, qR, bn, dtdS, S.gt, f);
+        for (int vi = 0; vi < 5; ++vi)
+          out[vi] = f[kFluxVar[vi]];
+      }
+    }
+    // cell a: (0 + F_low) - F_high, the high face's flux one lane on
+    T up[5];
+    for (int vi = 0; vi < 5; ++vi)
+      up[vi] = lane_take(out[vi], next);
+    if (own && a < kTS)
+    {
+      const int cl = a + kTS * (lj + kTS * lk);
+      for (int vi = 0; vi < 5; ++vi)
+        ld[vi][cl] = double(out[vi]) - double(up[vi]);
+    }
+  }
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void
+flux_march_item(const FusedSlot<T> & S, long r0, double (*ld)[5][512])
+{
+  using G = Geo<3, BS>;
+  using IX = IdxC<3, BS>;
+  const int              o = int(r0) / G::TPB;
+  const View<T, 3, BS>   V(S, o);
+  const int              tl = int(r0) - o * G::TPB;
+  constexpr int          nb1 = G::NB1;
+  const int              org[3] = { (tl % nb1) * kTS, ((tl / nb1) % nb1) * kTS, (tl / (nb1 * nb1)) * kTS };
+  const T                dtdS = T(S.dt / dxd<BS>(S.m, S.scaling, o));
+  const int              w = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int              lane = int(threadIdx.x) & 63;
+  if (w == 0)
+    flux_march_dir<T, BS, 2>(S, V, o, org, lane, dtdS, ld[2]);
+  else if (w == 1)
+    flux_march_dir<T, BS, 1>(S, V, o, org, lane, dtdS, ld[1]);
+  else
+    flux_rows_x<T, BS>(S, V, o, org, w - 2, lane, dtdS, ld[0]);
+  __syncthreads();
+  constexpr int ncell = IX::cells();
+  for (int cl = int(threadIdx.x); cl < kTS * kTS * kTS; cl += 256)
+  {
+    const int cidx = IX::cell(org[0] + cl % kTS, org[1] + (cl / kTS) % kTS, org[2] + cl / (kTS * kTS));
+    for (int v = 0; v < 5; ++v)
+    {
+      double acc = S.U[(long(o) * 5 + v) * ncell + cidx];
+      acc = acc + ld[0][v][cl];
+      acc = acc + ld[1][v][cl];
+      acc = acc + ld[2][v][cl];
+      S.U2[(long(o) * 5 + v) * ncell + cidx] = acc;
+    }
+  }
+  __syncthreads(); // the next tile of this workgroup reuses the LDS
+}
+
+template <typename T, int BSEL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_flux_march(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restrict__ off, long total)
+{
+  __shared__ double ld[3][5][kTS * kTS * kTS];
+  for (long g = xcd_group(); g < total; g += gridDim.x)
+  {
+    const int            s = slot_of(off, nslots, g);
+    const FusedSlot<T> & S = slots[s];
+    const long           r0 = g - off[s];
+    if (r0 >= S.cnt[3])
+      continue;
+    MHDCT_BS_DISPATCH(S.m.b, (flux_march_item<T, BS>(S, r0, ld)));
   }
 }
 
@@ -1733,21 +1972,6 @@ edge_face(const View<T, DIM, BS> & V, int a, int b, int e, T & bf, T & fs)
   fs = V.fs(fsi, c1);
 }
 
-// value x of the lane whose byte address (4 x lane) is addr
-template <typename T>
-__device__ __forceinline__ T
-lane_take(T x, int addr)
-{
-  if constexpr (sizeof(T) == 8)
-  {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
-    const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u))));
-    const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u >> 32))));
-    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
-  }
-  else
-    return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
-}
 
 // What a marching lane reads of its cell c for the edges across (D0, D1): the values the cell
 // parts take (rho, p, u, v, w and the cell's B component, their slopes along D0 and D1, and for
@@ -2268,6 +2492,23 @@ dt_in_prim(const FusedBatch<T> & bt)
 template <typename T>
 void launch_dt_set(const FusedBatch<T> & bt, hipStream_t st);
 
+// k_flux_march for the flux stage of the FP64 3D form at tile depth 8 (in FP32 its 60 KB of FP64
+// differences per tile cost more than the reads it saves: BL3D 1.01 -> 1.13 ms); false: not
+// launched
+template <typename T, int DIM, int BSEL>
+bool
+launch_flux_march(const FusedBatch<T> & bt, const FusedSlot<T> * sl, hipStream_t st, int tcap)
+{
+  if constexpr (DIM == 3 && sizeof(T) == sizeof(double))
+    if (bt.tune.tile_z == 8 && bt.tune.flux_march)
+    {
+      hipLaunchKernelGGL((k_flux_march<T, BSEL>), grid_for(bt.tot[kStageTile], 1, tcap), 256, 0, st, sl, bt.nslots,
+                         bt.d_off[kStageTile], bt.tot[kStageTile]);
+      return true;
+    }
+  return false;
+}
+
 template <typename T, int DIM, int BSEL>
 int
 launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
@@ -2321,7 +2562,10 @@ launch_stages(const FusedBatch<T> & bt, int parity, hipStream_t st, int which)
                        bt.d_off[kStageBorder], bt.tot[kStageBorder]);
   if (on(kStageTile))
   {
-    if (DIM == 3 && bt.tune.tile_z == 4)
+    if (launch_flux_march<T, DIM, BSEL>(bt, sl, st, tcap))
+    {
+    }
+    else if (DIM == 3 && bt.tune.tile_z == 4)
       hipLaunchKernelGGL((k_flux_tile<T, DIM, BSEL, 4>), grid_for(bt.tot[kStageTile] * 2, 1, tcap),
                          (tile_threads<DIM, 4>()), 0, st, sl, n, bt.d_off[kStageTile], bt.tot[kStageTile]);
     else
@@ -2430,7 +2674,7 @@ fused_supports(int dim, int b)
 }
 
 long
-fused_items_per_block(int dim, int b, int stage)
+fused_items_per_block(int dim, int b, int stage, bool fp64)
 {
   const long g2 = dim == 3 ? long(b + 4) * (b + 4) * (b + 4) : long(b + 4) * (b + 4);
   const long g1 = dim == 3 ? long(b + 2) * (b + 2) * (b + 2) : long(b + 2) * (b + 2);
@@ -2445,8 +2689,8 @@ fused_items_per_block(int dim, int b, int stage)
   case 0: return nc + (g2 - nc + 63) / 64 * 64;
   case 6: return (npe + 63) / 64 * 64;
   case 8: return nc;
-  case 1: return fused_cell_runs(dim, b) > 0 ? (long(b + 2) * (b + 2) * fused_cell_runs(dim, b) + 63) / 64 * 64
-                                           : (g1 + 63) / 64 * 64;
+  case 1: return fused_cell_runs(dim, b, fp64) > 0 ? (long(b + 2) * (b + 2) * fused_cell_runs(dim, b, fp64) + 63) / 64 * 64
+                                                 : (g1 + 63) / 64 * 64;
   case 2: return dim * 2 * ((nt + 63) / 64 * 64);
   case 3: return dim == 3 ? nb1 * nb1 * nb1 : nb1 * nb1;
   case 4: return long(fused_edge_waves(dim, b)) * 64;

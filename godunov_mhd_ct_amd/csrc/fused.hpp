@@ -35,14 +35,16 @@ fused_front_depth(int dim, int b)
   return dim == 3 ? (b == 8 ? 2 : 4) : 1;
 }
 
-// k_cell in 3D: the runs along z of each (i, j) column of a block's 1-ghost tile (b + 2 cells),
-// each run marched by one lane: runs of 5, 3 and 2 cells for 8^3, 16^3 and 32^3 blocks, measured
-// on MI355X against the other run lengths (16^3: 2, 6 and 9 cells; 32^3: 17) and against one cell
-// per item at 3 wavefronts per SIMD, which was slower at 16^3 and 32^3 (8^3: within the noise).
+// k_cell in the FP64 3D form: the runs along z of each (i, j) column of a block's 1-ghost tile
+// (b + 2 cells), each run marched by one lane: runs of 5, 3 and 2 cells for 8^3, 16^3 and 32^3
+// blocks, measured on MI355X against the other run lengths (16^3: 2, 6 and 9 cells; 32^3: 17) and
+// against one cell per item at 3 wavefronts per SIMD, which was slower at 16^3 and 32^3 (8^3:
+// within the noise).  The FP32 form keeps one cell per item (its march at 2 wavefronts per SIMD:
+// BL3D 1.06 -> 1.26 ms).  0: one cell per item.
 constexpr int
-fused_cell_runs(int dim, int b)
+fused_cell_runs(int dim, int b, bool fp64)
 {
-  return dim != 3 ? 0 : (b == 8 ? 2 : (b == 16 ? 6 : 17));
+  return (dim != 3 || !fp64) ? 0 : (b == 8 ? 2 : (b == 16 ? 6 : 17));
 }
 
 // Wavefronts of k_edge per block.  In 3D the edges of each direction march across the b + 1
@@ -123,6 +125,7 @@ struct FusedTune
   int grid_cap = 1 << 20; // element kernels: at most this many workgroups (multiple of 8)
   int tile_cap = 1 << 20; // flux tiles: at most this many workgroups (multiple of 8)
   int tile_z = 8;         // 3D flux tile depth: 8 (8x8x8 cells, 576 threads) or 4 (8x8x4, 320 threads)
+  int flux_march = 1;     // FP64 3D, depth 8: k_flux_march (each record read once per direction), else k_flux_tile
   int front = 0;          // 1: k_front runs the prim, efield and cell stages (kStagePrim; the other
                           // two launch nothing); 0: k_prim, k_efield and k_cell (viscosity needs them)
   int cell_threads = 256; // workgroup size of k_cell: 256 or 512
@@ -167,7 +170,7 @@ struct FusedBatch
 // block sizes the fused kernels are instantiated for
 bool fused_supports(int dim, int b);
 // work items of one block in stage `stage` (the launch geometry of the kernels above)
-long fused_items_per_block(int dim, int b, int stage);
+long fused_items_per_block(int dim, int b, int stage, bool fp64);
 // work items of one coarse-facing side in the border-flux stage
 long fused_border_items_per_side(int dim, int b);
 // a slot's work range, padded so that every workgroup and wavefront lies in one slot

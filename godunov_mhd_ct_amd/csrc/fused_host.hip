@@ -155,7 +155,7 @@ build(Context & c, const Mesh & mesh, FusedState & F, FusedBatch<T> & bt, FusedS
     else if (k == kStageFix)
       sl.cnt[k] = c.dm.nfix;
     else
-      sl.cnt[k] = nq * fused_items_per_block(dim, b, k);
+      sl.cnt[k] = nq * fused_items_per_block(dim, b, k, sizeof(T) == sizeof(double));
     if (sl.cnt[k] >= (1L << 31))
     {
       c.err = "mesh too large for one fused work list (2^31 items per stage)";
