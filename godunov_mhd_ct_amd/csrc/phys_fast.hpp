@@ -66,6 +66,51 @@ sqrt_nr(T x)
     return sqrt(x);
 }
 
+// 1/a and 1/b (rcp2), 1/a .. 1/d (rcp4) from one reciprocal of the product: Montgomery's batch
+// inversion, 1/a = b / (ab) and so on, 3 and 9 multiplies beside one rcp_nr.  In FP64 on the
+// device only, for nonzero factors whose product stays a normal number (densities, wave-speed
+// differences and their products here); a few ulps, not correctly rounded.  Elsewhere one
+// reciprocal each.
+template <typename T>
+__host__ __device__ __forceinline__ void
+rcp2(T a, T b, T & ia, T & ib)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8)
+  {
+    const T i = rcp_nr(a * b);
+    ia = i * b;
+    ib = i * a;
+    return;
+  }
+#endif
+  ia = rcp_nr(a);
+  ib = rcp_nr(b);
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void
+rcp4(T a, T b, T c, T d, T & ia, T & ib, T & ic, T & id)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8)
+  {
+    const T ab = a * b, cd = c * d;
+    const T i = rcp_nr(ab * cd);
+    const T iab = i * cd, icd = i * ab;
+    ia = iab * b;
+    ib = iab * a;
+    ic = icd * d;
+    id = icd * c;
+    return;
+  }
+#endif
+  ia = rcp_nr(a);
+  ib = rcp_nr(b);
+  ic = rcp_nr(c);
+  id = rcp_nr(d);
+}
+
 // HLLD flux along x (riemann_hlld, models/RiemannSolvers_MHD.h:311-551), shared denominators:
 // 1/(rcr + rcl), 1/(sl - ustar), 1/(sr - ustar), 1/e*_L, 1/e*_R, 1/(sqrt rho*_L + sqrt rho*_R),
 // and one reciprocal density per state in the fast speeds.
@@ -224,7 +269,7 @@ hlld_fast(T * ql, T * qr, T * f, const Gas<T> & g)
 // in x and y), 1/(SL - u*), 1/(SR - u*), 1/(SB - v*), 1/(ST - v*) for the star states,
 // 1/(SAR - SAL), 1/(SAT - SAB) for the averages and the result; each Alfven speed is the square
 // root of the largest of B_n^2 / rho* over its four terms (rho* B_n^2 scaled by the same ratios
-// as in the reference).
+// as in the reference).  The 20 reciprocals come from 6 (rcp4, rcp2: batch inversion in FP64).
 template <int AXIS, typename T>
 __host__ __device__ inline T
 edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
@@ -255,8 +300,9 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
 
   // fast speeds along x and y with one reciprocal density per state: the radicands
   // d2 + sqrt(d2^2 - c^2 B_n^2 / rho); the largest speed is the square root of the largest
-  auto cfast2 = [&](T r, T p, T a, T b, T c, T & cx, T & cy) {
-    const T ir = rcp_nr(r);
+  T irLL, irLR, irRL, irRR;
+  rcp4(rLL, rLR, rRL, rRR, irLL, irLR, irRL, irRR);
+  auto cfast2 = [&](T ir, T p, T a, T b, T c, T & cx, T & cy) {
     const T c2 = g.gamma0 * p * ir;
     const T d2 = T(0.5) * ((a * a + b * b + c * c) * ir + c2);
     const T dd = d2 * d2;
@@ -265,10 +311,10 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
     cy = d2 + sqrt_nr(dd - ci * b * b);
   };
   T cLLx, cLLy, cLRx, cLRy, cRLx, cRLy, cRRx, cRRy;
-  cfast2(rLL, pLL, aLL, bLL, cLL, cLLx, cLLy);
-  cfast2(rLR, pLR, aLR, bLR, cLR, cLRx, cLRy);
-  cfast2(rRL, pRL, aRL, bRL, cRL, cRLx, cRLy);
-  cfast2(rRR, pRR, aRR, bRR, cRR, cRRx, cRRy);
+  cfast2(irLL, pLL, aLL, bLL, cLL, cLLx, cLLy);
+  cfast2(irLR, pLR, aLR, bLR, cLR, cLRx, cLRy);
+  cfast2(irRL, pRL, aRL, bRL, cRL, cRLx, cRLy);
+  cfast2(irRR, pRR, aRR, bRR, cRR, cRRx, cRRy);
   const T cmx = sqrt_nr(max4(cLLx, cLRx, cRLx, cRRx));
   const T cmy = sqrt_nr(max4(cLLy, cLRy, cRLy, cRRy));
 
@@ -287,14 +333,14 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
   const T rcLLy = rLL * (vLL - SB), rcLRy = rLR * (ST - vLR);
   const T rcRLy = rRL * (vRL - SB), rcRRy = rRR * (ST - vRR);
 
-  const T ustar = (rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR + (PtLL - PtRL + PtLR - PtRR)) *
-                  rcp_nr(rcLLx + rcLRx + rcRLx + rcRRx);
-  const T vstar = (rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR + (PtLL - PtLR + PtRL - PtRR)) *
-                  rcp_nr(rcLLy + rcLRy + rcRLy + rcRRy);
+  T irx, iry;
+  rcp2(rcLLx + rcLRx + rcRLx + rcRRx, rcLLy + rcLRy + rcRLy + rcRRy, irx, iry);
+  const T ustar = (rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR + (PtLL - PtRL + PtLR - PtRR)) * irx;
+  const T vstar = (rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR + (PtLL - PtLR + PtRL - PtRR)) * iry;
 
   // compression ratios across the outer waves: f = (S - u)/(S - u*)
-  const T iSL = rcp_nr(SL - ustar), iSR = rcp_nr(SR - ustar);
-  const T iSB = rcp_nr(SB - vstar), iST = rcp_nr(ST - vstar);
+  T iSL, iSR, iSB, iST;
+  rcp4(SL - ustar, SR - ustar, SB - vstar, ST - vstar, iSL, iSR, iSB, iST);
   const T fLLx = (SL - uLL) * iSL, fLRx = (SL - uLR) * iSL, fRLx = (SR - uRL) * iSR, fRRx = (SR - uRR) * iSR;
   const T fLLy = (SB - vLL) * iSB, fRLy = (SB - vRL) * iSB, fLRy = (ST - vLR) * iST, fRRy = (ST - vRR) * iST;
 
@@ -311,8 +357,9 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
   // |A*LL|/sqrt(rho*LL), smallc) with rho*x = rho f_x, rho* = rho f_x f_y, A* = a f_y: the squares
   // are a^2 / (rho f_x) times 1 and f_y.  Likewise calfvenB with b, f_y and f_x.
   auto sq = [](T x) { return x * x; };
-  const T ixLL = rcp_nr(rLL * fLLx), ixLR = rcp_nr(rLR * fLRx), ixRL = rcp_nr(rRL * fRLx), ixRR = rcp_nr(rRR * fRRx);
-  const T iyLL = rcp_nr(rLL * fLLy), iyLR = rcp_nr(rLR * fLRy), iyRL = rcp_nr(rRL * fRLy), iyRR = rcp_nr(rRR * fRRy);
+  T ixLL, ixLR, ixRL, ixRR, iyLL, iyLR, iyRL, iyRR;
+  rcp4(rLL * fLLx, rLR * fLRx, rRL * fRLx, rRR * fRRx, ixLL, ixLR, ixRL, ixRR);
+  rcp4(rLL * fLLy, rLR * fLRy, rRL * fRLy, rRR * fRRy, iyLL, iyLR, iyRL, iyRR);
   const T sc2 = g.smallc * g.smallc;
   const T caL = sqrt_nr(max5(sq(aLR) * ixLR, sq(aLR) * ixLR * fLRy, sq(aLL) * ixLL, sq(aLL) * ixLL * fLLy, sc2));
   const T caR = sqrt_nr(max5(sq(aRR) * ixRR, sq(aRR) * ixRR * fRRy, sq(aRL) * ixRL, sq(aRL) * ixRL * fRLy, sc2));
@@ -323,8 +370,8 @@ edge_emf_fast(const T (&qe)[4][8], const Gas<T> & g)
   const T SAR = fmax(ustar + caR, T(0));
   const T SAB = fmin(vstar - caB, T(0));
   const T SAT = fmax(vstar + caT, T(0));
-  const T iLR = rcp_nr(SAR - SAL);
-  const T iBT = rcp_nr(SAT - SAB);
+  T iLR, iBT;
+  rcp2(SAR - SAL, SAT - SAB, iLR, iBT);
 
   const T AsT = (SAR * AsRR - SAL * AsLR) * iLR;
   const T AsB = (SAR * AsRL - SAL * AsLL) * iLR;
