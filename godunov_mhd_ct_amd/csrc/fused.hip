@@ -93,8 +93,10 @@ struct Geo
   static constexpr int NPL = (BS + 2) * BS;
   static constexpr int NWP = (NPL + 62) / 63;
   static constexpr int NWX = (BS / 8) * ((BS + 8) / 7);
+  static constexpr int NPLZ = (BS + 2) * (BS + 1);
+  static constexpr int NWZ = (NPLZ + 62) / 63;
   static constexpr int EWB = fused_edge_waves(DIM, BS);
-  static_assert(DIM == 2 || EWB == NWX + 2 * NWP, "k_edge wavefronts per block");
+  static_assert(DIM == 2 || EWB == NWX + NWP + NWZ, "k_edge wavefronts per block");
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
   static constexpr int GSH = G2 - NC;                      // ghost shell of the 2-ghost tile
   static constexpr int PRIM = NC + (GSH + 63) / 64 * 64;  // k_prim items: interior, then shell
@@ -1986,7 +1988,7 @@ struct EdgeRaw
   T                    q[NQ], s0[NS], s1[NS], b1, f1, b0, f0;
 };
 
-template <typename T, int DIM, int BS, int D0, int D1, bool HK>
+template <typename T, int DIM, int BS, int D0, int D1, bool HK, bool B1 = true>
 __device__ __forceinline__ void
 edge_raw_load(const View<T, DIM, BS> & V, const int * c, bool edge, EdgeRaw<T, D0, D1> & R)
 {
@@ -2006,7 +2008,10 @@ edge_raw_load(const View<T, DIM, BS> & V, const int * c, bool edge, EdgeRaw<T, D
     R.s0[6] = V.sl(D0, kB, cA);
     R.s1[6] = V.sl(D1, kB, cA);
   }
-  edge_face<T, DIM, BS, D1, D0, HK>(V, c[0], c[1], c[2], R.b1, R.f1);
+  R.b1 = T(0);
+  R.f1 = T(0);
+  if (B1)
+    edge_face<T, DIM, BS, D1, D0, HK>(V, c[0], c[1], c[2], R.b1, R.f1);
   R.b0 = T(0);
   R.f0 = T(0);
   if (edge)
@@ -2113,21 +2118,82 @@ edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], 
   }
 }
 
+// The z-edges of an (i, j) column, marching along z beside the x- and y-edges (which march along
+// z too), so that the three directions read the records of the same planes close together in
+// time and share them in the L2.  The z-edge at (i, j, k) has its four cells in plane k: each
+// iteration reads the records of A = (i, j, k) and of C = A - e_y and forms KIND 0 and 2 at A and
+// KIND 3 and 1 at C; KIND 2 and 1 of the lane one step back along x (A' and C') come by
+// cross-lane reads, as in edge_march.  The reads of the next plane are issued before this
+// plane's EMF is formed.  Same arithmetic as edge_at, the same states bit for bit.
+template <typename T, int DIM, int BS, bool HK>
+__device__ __forceinline__ void
+edge_column_z(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int src, bool edge, T dtdx)
+{
+  static_assert(DIM == 3, "edge_column_z: 3D edges");
+  using G = Geo<DIM, BS>;
+  constexpr int D0 = 0, D1 = 1, NH = 6;
+  EdgeRaw<T, D0, D1> RA, RC;
+  int                cc[3] = { c[0], c[1] - 1, 0 };
+  c[2] = 0;
+  edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, RA);
+  edge_raw_load<T, DIM, BS, D0, D1, HK, false>(V, cc, edge, RC);
+  for (int m = 0; m < BS; ++m)
+  {
+    T hA0[NH], h2[NH], h3[NH], h1[NH];
+    edge_part_of<T, 0, D0, D1>(RA, S.gt, hA0);
+    edge_part_of<T, 2, D0, D1>(RA, S.gt, h2);
+    edge_part_of<T, 3, D0, D1>(RC, S.gt, h3);
+    edge_part_of<T, 1, D0, D1>(RC, S.gt, h1);
+    const T bA1 = RA.b1, fA1 = RA.f1, bA0 = RA.b0, fA0 = RA.f0, bC0 = RC.b0, fC0 = RC.f0;
+    T       hAp[NH], hCp[NH];
+    for (int v = 0; v < NH; ++v)
+    {
+      hAp[v] = lane_take(h2[v], src);
+      hCp[v] = lane_take(h1[v], src);
+    }
+    const T bAp1 = lane_take(bA1, src), fAp1 = lane_take(fA1, src);
+    if (m + 1 < BS)
+    {
+      c[2] = m + 1;
+      cc[2] = m + 1;
+      edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, RA);
+      edge_raw_load<T, DIM, BS, D0, D1, HK, false>(V, cc, edge, RC);
+    }
+    c[2] = m;
+    if (edge)
+    {
+      T qe[4][8];
+      edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
+      edge_corner<T, 3, D0, D1>(h3, bC0, fC0, bA1, fA1, qe[kLT]);
+      edge_corner<T, 2, D0, D1>(hAp, bA0, fA0, bAp1, fAp1, qe[kRB]);
+      edge_corner<T, 1, D0, D1>(hCp, bC0, fC0, bAp1, fAp1, qe[kRT]);
+      const T E = edge_emf_fast<2>(qe, S.gt) * dtdx;
+      S.emf[(long(o) * G::NCE + 2) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
-// k_edge: every edge of a block once.  In 3D the edges of each direction march along one
-// direction D1 across them (edge_march: 16^3 blocks at 9 and at 17 planes per march in one
-// measurement, 1.86 and 1.80 ms), each wavefront over a plane of the two other directions:
-//   * y- and z-edges: positions over rows of b + 2 along x (i = -1 .. b) and the edge's own axis,
-//     63 per wavefront after the position before them; lane s - 1 holds the edge one step back
-//     along x, the lanes at i = -1 and lane 0 only pass their cell parts on;
-//   * x-edges: tiles of 8 lanes along x by 8 positions along y (j = -1 .. b), 7 new after the
-//     last one of the tile before; lane s - 8 holds the edge one step back along y, the first
-//     row and the lanes at j = -1 only pass their cell parts on.
-// G::EWB wavefronts per block: x-edges first, then y, then z, each by plane wavefront.
-// A marching lane holds about 230 registers (the next cell's reads in flight while the EMF is
-// formed), so the 3D kernels run 2 wavefronts per SIMD.  In 2D a plane of lanes would be one row
-// of b + 2 positions, too short for a wavefront: the edges run one per lane (edge_at), 4
-// wavefronts per SIMD.  A wavefront's direction is uniform and compiled in.
+// k_edge: every edge of a block once.  In 3D every wavefront marches along z, over a plane of
+// lanes in x and y:
+//   * y-edges (edge_march, D1 = z): positions over rows of b + 2 along x (i = -1 .. b) and the
+//     b rows along y, 63 per wavefront after the position before them; lane s - 1 holds the edge
+//     one step back along x, the lanes at i = -1 and lane 0 only pass their cell parts on;
+//   * x-edges (edge_march, D1 = z): tiles of 8 lanes along x by 8 positions along y (j = -1 ..
+//     b), 7 new after the last one of the tile before; lane s - 8 holds the edge one step back
+//     along y, the first row and the lanes at j = -1 only pass their cell parts on;
+//   * z-edges (edge_column_z): positions over rows of b + 2 along x and the b + 1 rows along y,
+//     as the y-edges.
+// Marching the z-edges along y instead (one cell read per edge, the previous row carried) read
+// 4.0 GB from HBM on the 16^3 class of MIX against 3.1 along z, where the three directions share
+// the planes in the L2 (BL3D k_edge 1.76 -> 1.73 ms; the 32^3 class, whose marches drift apart
+// by more planes than the L2 holds, is unchanged).  (16^3 blocks at 9 and at 17 planes per march
+// in one measurement: 1.86 and 1.80 ms.)  G::EWB wavefronts per block: x-edges first, then y,
+// then z, each by plane wavefront.  A marching lane holds about 230-256 registers (the next
+// cell's reads in flight while the EMF is formed), so the 3D kernels run 2 wavefronts per SIMD.
+// In 2D a plane of lanes would be one row of b + 2 positions, too short for a wavefront: the
+// edges run one per lane (edge_at), 4 wavefronts per SIMD.  A wavefront's direction is uniform
+// and compiled in.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
@@ -2157,6 +2223,22 @@ edge_item(const FusedSlot<T> & S, long r0)
     // x-edges, then y, then z, each by plane wavefront
     const int     a = w < G::NWX ? 0 : (w < G::NWX + G::NWP ? 1 : 2);
     const int     pw = a == 0 ? w : (a == 1 ? w - G::NWX : w - G::NWX - G::NWP);
+    if (a == 2)
+    {
+      // z-edges: (i, j) over rows of b + 2 along x, b + 1 rows, marching along z
+      constexpr int e0 = BS + 2;
+      const int     p = pw * 63 - 1 + lane;
+      if (p < 0 || p >= G::NPLZ)
+        return;
+      int        c[3] = { p % e0 - 1, p / e0, 0 };
+      const int  src = (lane > 0 ? lane - 1 : 0) * 4;
+      const bool edge = lane > 0 && c[0] >= 0;
+      if (S.rk2)
+        edge_column_z<T, DIM, BS, false>(S, V, o, c, src, edge, dtdx);
+      else
+        edge_column_z<T, DIM, BS, true>(S, V, o, c, src, edge, dtdx);
+      return;
+    }
     int           c[3], src;
     bool          edge;
     constexpr int e0 = BS + 2;

@@ -47,11 +47,10 @@ fused_cell_runs(int dim, int b, bool fp64)
   return (dim != 3 || !fp64) ? 0 : (b == 8 ? 2 : (b == 16 ? 6 : 17));
 }
 
-// Wavefronts of k_edge per block.  In 3D the edges of each direction march across the b + 1
-// planes along one direction, each wavefront with its plane of lanes: the y- and z-edges over
-// rows of b + 2 positions (a halo column before each row of b + 1 edges), 63 positions per
-// wavefront after a halo lane; the x-edges in tiles of 8 x 8 lanes, 7 new rows of the b + 2
-// along y per tile.  In 2D the (b+1)^2 edges one per lane.
+// Wavefronts of k_edge per block.  In 3D the edges march along z, each wavefront with its plane
+// of lanes: the y- and z-edges over rows of b + 2 positions (a halo column before each row of
+// b + 1 edges), 63 positions per wavefront after a halo lane; the x-edges in tiles of 8 x 8
+// lanes, 7 new rows of the b + 2 along y per tile.  In 2D the (b+1)^2 edges one per lane.
 constexpr int
 fused_edge_waves(int dim, int b)
 {
@@ -60,7 +59,8 @@ fused_edge_waves(int dim, int b)
     return (n1 * n1 + 63) / 64;
   const int wp = ((b + 2) * b + 62) / 63;
   const int wx = (b / 8) * ((b + 8) / 7);
-  return wx + 2 * wp;
+  const int wz = ((b + 2) * n1 + 62) / 63;
+  return wx + wp + wz;
 }
 
 // One mesh ("slot") of a batch: its tables, its state pointers and its scratch.
