@@ -2047,7 +2047,7 @@ edge_part_of(const EdgeRaw<T, D0, D1> & R, const Gas<T> & gt, T * h)
 }
 
 // The EMFs along axis AX of the b + 1 3D edges that march along D1, D0 and D1 the two directions
-// across the edges (z-edges: x, y; y-edges: x, z; x-edges: y, z).  The edge with cell A has its
+// across the edges (y-edges: x, z; x-edges: y, z; the z-edges march along z in edge_column_z).  The edge with cell A has its
 // corners at KIND 0 at A, KIND 3 at C = A - e_D1, KIND 2 at A' = A - e_D0 and KIND 1 at C' =
 // A' - e_D1, with Bf_D0 at A (KIND 0, 2) and at C (KIND 1, 3) and Bf_D1 at A (KIND 0, 3) and at
 // A' (KIND 1, 2).  Each lane holds the cell c = (c[0], c[1], c[2]) with c[D1] = m for m = -1 .. b
@@ -2062,7 +2062,7 @@ template <typename T, int DIM, int BS, int AX, bool HK>
 __device__ __forceinline__ void
 edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int src, bool edge, T dtdx)
 {
-  static_assert(DIM == 3, "edge_march: 3D edges");
+  static_assert(DIM == 3 && AX != 2, "edge_march: 3D x- and y-edges");
   using G = Geo<DIM, BS>;
   constexpr int D0 = AX == 0 ? 1 : 0;
   constexpr int D1 = AX == 2 ? 1 : 2;
@@ -2255,26 +2255,22 @@ edge_item(const FusedSlot<T> & S, long r0)
     }
     else
     {
-      // (i, k) for z-edges, (i, j) for y-edges
+      // y-edges: (i, j) over rows of b + 2 along x, b rows
       const int p = pw * 63 - 1 + lane;
       if (p < 0 || p >= G::NPL)
         return;
       c[0] = p % e0 - 1;
-      c[a == 1 ? 1 : 2] = p / e0;
+      c[1] = p / e0;
       src = (lane > 0 ? lane - 1 : 0) * 4;
       edge = lane > 0 && c[0] >= 0;
     }
     if (S.rk2)
     {
-      if (a == 2)
-        edge_march<T, DIM, BS, 2, false>(S, V, o, c, src, edge, dtdx);
-      else if (a == 1)
+      if (a == 1)
         edge_march<T, DIM, BS, 1, false>(S, V, o, c, src, edge, dtdx);
       else
         edge_march<T, DIM, BS, 0, false>(S, V, o, c, src, edge, dtdx);
     }
-    else if (a == 2)
-      edge_march<T, DIM, BS, 2, true>(S, V, o, c, src, edge, dtdx);
     else if (a == 1)
       edge_march<T, DIM, BS, 1, true>(S, V, o, c, src, edge, dtdx);
     else
