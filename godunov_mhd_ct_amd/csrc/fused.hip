@@ -92,7 +92,8 @@ struct Geo
   // wavefronts per block (2D: one edge per lane)
   static constexpr int NPL = (BS + 2) * BS;
   static constexpr int NWP = (NPL + 62) / 63;
-  static constexpr int NWX = (BS / 8) * ((BS + 8) / 7);
+  static constexpr int NPLX = BS * (BS + 1);
+  static constexpr int NWX = (NPLX + 63) / 64;
   static constexpr int NPLZ = (BS + 2) * (BS + 1);
   static constexpr int NWZ = (NPLZ + 62) / 63;
   static constexpr int EWB = fused_edge_waves(DIM, BS);
@@ -2173,6 +2174,53 @@ edge_column_z(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3
   }
 }
 
+// The x-edges of an (i, j) position, marching along z: each iteration reads the records of
+// A = (i, j, k) and of A' = A - e_y and forms KIND 0 at A and KIND 2 at A'; KIND 3 and 1 (at
+// C = A - e_z and C' = A' - e_z) and Bf_y at C are those formed at the plane before, carried in
+// registers.  No cross-lane reads: every lane holds an edge.  (With the next plane's reads in
+// flight the kernel spilled 77 VGPRs.)  Same arithmetic as edge_at, the same states bit for bit.
+template <typename T, int DIM, int BS, bool HK>
+__device__ __forceinline__ void
+edge_column_x(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], T dtdx)
+{
+  static_assert(DIM == 3, "edge_column_x: 3D edges");
+  using G = Geo<DIM, BS>;
+  constexpr int D0 = 1, D1 = 2, NH = 7;
+  int cp[3] = { c[0], c[1] - 1, -1 };
+  T   hC3[NH], hC1[NH], bC0 = T(0), fC0 = T(0);
+  for (int m = -1; m < G::NE; ++m)
+  {
+    EdgeRaw<T, D0, D1> RA, RP;
+    c[2] = m;
+    cp[2] = m;
+    edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, true, RA);
+    edge_raw_load<T, DIM, BS, D0, D1, HK>(V, cp, false, RP);
+    T hA0[NH], hA2[NH], h3[NH], h1[NH];
+    edge_part_of<T, 0, D0, D1>(RA, S.gt, hA0);
+    edge_part_of<T, 3, D0, D1>(RA, S.gt, h3);
+    edge_part_of<T, 2, D0, D1>(RP, S.gt, hA2);
+    edge_part_of<T, 1, D0, D1>(RP, S.gt, h1);
+    const T bA1 = RA.b1, fA1 = RA.f1, bA0 = RA.b0, fA0 = RA.f0, bP1 = RP.b1, fP1 = RP.f1;
+    if (m >= 0)
+    {
+      T qe[4][8];
+      edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
+      edge_corner<T, 3, D0, D1>(hC3, bC0, fC0, bA1, fA1, qe[kLT]);
+      edge_corner<T, 2, D0, D1>(hA2, bA0, fA0, bP1, fP1, qe[kRB]);
+      edge_corner<T, 1, D0, D1>(hC1, bC0, fC0, bP1, fP1, qe[kRT]);
+      const T E = edge_emf_fast<0>(qe, S.gt) * dtdx;
+      S.emf[(long(o) * G::NCE + 0) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
+    }
+    for (int v = 0; v < NH; ++v)
+    {
+      hC3[v] = h3[v];
+      hC1[v] = h1[v];
+    }
+    bC0 = bA0;
+    fC0 = fA0;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // k_edge: every edge of a block once.  In 3D every wavefront marches along z, over a plane of
 // lanes in x and y:
@@ -2244,14 +2292,16 @@ edge_item(const FusedSlot<T> & S, long r0)
     constexpr int e0 = BS + 2;
     if (a == 0)
     {
-      const int it = pw % (BS / 8);
-      const int p = (pw / (BS / 8)) * 7 - 1 + (lane >> 3);
-      if (p < 0 || p >= e0)
+      // x-edges: (i, j), b along x by b + 1 along y, one edge per lane
+      const int p = pw * 64 + lane;
+      if (p >= G::NPLX)
         return;
-      c[0] = 8 * it + (lane & 7);
-      c[1] = p - 1;
-      src = (lane >= 8 ? lane - 8 : lane) * 4;
-      edge = lane >= 8 && p >= 1;
+      int cx[3] = { p % BS, p / BS, 0 };
+      if (S.rk2)
+        edge_column_x<T, DIM, BS, false>(S, V, o, cx, dtdx);
+      else
+        edge_column_x<T, DIM, BS, true>(S, V, o, cx, dtdx);
+      return;
     }
     else
     {

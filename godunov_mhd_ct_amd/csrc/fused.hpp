@@ -58,7 +58,7 @@ fused_edge_waves(int dim, int b)
   if (dim == 2)
     return (n1 * n1 + 63) / 64;
   const int wp = ((b + 2) * b + 62) / 63;
-  const int wx = (b / 8) * ((b + 8) / 7);
+  const int wx = (b * n1 + 63) / 64;
   const int wz = ((b + 2) * n1 + 62) / 63;
   return wx + wp + wz;
 }
