@@ -87,15 +87,13 @@ struct Geo
   static constexpr int NCE = DIM == 3 ? 3 : 1;            // EMF components
   static constexpr int NEC = I::ecells();                 // edge-grid points per component
   static constexpr int EPD = DIM == 3 ? BS * NE * NE : NE * NE; // edges per direction
-  // k_edge (3D): the plane wavefronts of the marching y- and z-edges (NPL positions, 63 per
-  // wavefront) and of the x-edges (8 x 8 tiles, 7 new positions along y each); EWB: k_edge
-  // wavefronts per block (2D: one edge per lane)
-  static constexpr int NPL = (BS + 2) * BS;
-  static constexpr int NWP = (NPL + 62) / 63;
+  // k_edge (3D): the plane wavefronts of the x-, y- and z-edges, one (i, j) position per lane
+  // (NPLX, NPLY, NE x NE); EWB: k_edge wavefronts per block (2D: one edge per lane)
+  static constexpr int NPLY = (BS + 1) * BS;
+  static constexpr int NWP = (NPLY + 63) / 64;
   static constexpr int NPLX = BS * (BS + 1);
   static constexpr int NWX = (NPLX + 63) / 64;
-  static constexpr int NPLZ = (BS + 2) * (BS + 1);
-  static constexpr int NWZ = (NPLZ + 62) / 63;
+  static constexpr int NWZ = (NE * NE + 63) / 64;
   static constexpr int EWB = fused_edge_waves(DIM, BS);
   static_assert(DIM == 2 || EWB == NWX + NWP + NWZ, "k_edge wavefronts per block");
   static constexpr int NTP = (NT + 63) / 64 * 64;          // k_border_flux items per side
@@ -2047,147 +2045,25 @@ edge_part_of(const EdgeRaw<T, D0, D1> & R, const Gas<T> & gt, T * h)
   h[kP] = fmax(gt.smallp * h[kD], h[kP]);
 }
 
-// The EMFs along axis AX of the b + 1 3D edges that march along D1, D0 and D1 the two directions
-// across the edges (y-edges: x, z; x-edges: y, z; the z-edges march along z in edge_column_z).  The edge with cell A has its
-// corners at KIND 0 at A, KIND 3 at C = A - e_D1, KIND 2 at A' = A - e_D0 and KIND 1 at C' =
-// A' - e_D1, with Bf_D0 at A (KIND 0, 2) and at C (KIND 1, 3) and Bf_D1 at A (KIND 0, 3) and at
-// A' (KIND 1, 2).  Each lane holds the cell c = (c[0], c[1], c[2]) with c[D1] = m for m = -1 .. b
-// and forms, from one read of the cell's record, the cell parts of KIND 0 and 2 (A of
-// its own edge at m and of the edge one step up along D0) and of KIND 3 and 1 (C of the same two
-// edges at m + 1); KIND 2 and 1 come to the lane one step up along D0 by cross-lane reads (the
-// lane whose byte address is `src` holds A' and C'), KIND 3, 1 and Bf_D0 are carried to the next
-// m in registers.  So each edge reads one cell's record and two faces, and the reads for m + 1
-// are issued before the EMF at m is formed; lanes with `edge` false only pass their cell parts
-// on.  Same arithmetic as edge_at, the same states bit for bit.
+// The x- (AX 0) or y-edges (AX 1) of an (i, j) position, marching along z: each iteration reads
+// the records of A = (i, j, k) and of A' = A - e_D0 (D0 = y for x-edges, x for y-edges) and
+// forms KIND 0 at A and KIND 2 at A'; KIND 3 and 1 (at C = A - e_z and C' = A' - e_z) and Bf_D0
+// at C are those formed at the plane before, carried in registers.  No cross-lane reads: every
+// lane holds an edge.  (With the next plane's reads in flight the kernel spilled 77 VGPRs.)
+// Same arithmetic as edge_at, the same states bit for bit.
 template <typename T, int DIM, int BS, int AX, bool HK>
 __device__ __forceinline__ void
-edge_march(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int src, bool edge, T dtdx)
+edge_column(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], T dtdx)
 {
-  static_assert(DIM == 3 && AX != 2, "edge_march: 3D x- and y-edges");
+  static_assert(DIM == 3 && AX != 2, "edge_column: 3D x- and y-edges");
   using G = Geo<DIM, BS>;
-  constexpr int D0 = AX == 0 ? 1 : 0;
-  constexpr int D1 = AX == 2 ? 1 : 2;
-  constexpr int NH = AX == 0 ? 7 : 6;
-  // the reference's corner order (RT, RB, LT, LB) per axis: KIND 2 is RB for x- and z-edges, LT
-  // for y-edges
-  constexpr int             slot3 = AX == 1 ? kRB : kLT;
-  constexpr int             slot2 = AX == 1 ? kLT : kRB;
-  T                         hC3[NH], hCp[NH], bC0 = T(0), fC0 = T(0);
-  EdgeRaw<T, D0, D1>        R;
-  c[D1] = -1;
-  edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, R);
-  for (int m = -1; m < G::NE; ++m)
-  {
-    T hA0[NH], h2[NH], h3[NH], h1[NH];
-    edge_part_of<T, 0, D0, D1>(R, S.gt, hA0);
-    edge_part_of<T, 2, D0, D1>(R, S.gt, h2);
-    edge_part_of<T, 3, D0, D1>(R, S.gt, h3);
-    edge_part_of<T, 1, D0, D1>(R, S.gt, h1);
-    const T bA1 = R.b1, fA1 = R.f1, bA0 = R.b0, fA0 = R.f0;
-    T       hAp[NH], h1p[NH];
-    for (int v = 0; v < NH; ++v)
-    {
-      hAp[v] = lane_take(h2[v], src);
-      h1p[v] = lane_take(h1[v], src);
-    }
-    const T bAp1 = lane_take(bA1, src), fAp1 = lane_take(fA1, src);
-    // the next cell's reads, in flight while this edge's EMF is formed
-    c[D1] = m;
-    if (m + 1 < G::NE)
-    {
-      c[D1] = m + 1;
-      edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, R);
-      c[D1] = m;
-    }
-    if (edge && m >= 0)
-    {
-      T qe[4][8];
-      edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
-      edge_corner<T, 3, D0, D1>(hC3, bC0, fC0, bA1, fA1, qe[slot3]);
-      edge_corner<T, 2, D0, D1>(hAp, bA0, fA0, bAp1, fAp1, qe[slot2]);
-      edge_corner<T, 1, D0, D1>(hCp, bC0, fC0, bAp1, fAp1, qe[kRT]);
-      const T E = edge_emf_fast<AX>(qe, S.gt) * dtdx;
-      S.emf[(long(o) * G::NCE + AX) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
-    }
-    for (int v = 0; v < NH; ++v)
-    {
-      hC3[v] = h3[v];
-      hCp[v] = h1p[v];
-    }
-    bC0 = bA0;
-    fC0 = fA0;
-  }
-}
-
-// The z-edges of an (i, j) column, marching along z beside the x- and y-edges (which march along
-// z too), so that the three directions read the records of the same planes close together in
-// time and share them in the L2.  The z-edge at (i, j, k) has its four cells in plane k: each
-// iteration reads the records of A = (i, j, k) and of C = A - e_y and forms KIND 0 and 2 at A and
-// KIND 3 and 1 at C; KIND 2 and 1 of the lane one step back along x (A' and C') come by
-// cross-lane reads, as in edge_march.  The reads of the next plane are issued before this
-// plane's EMF is formed.  Same arithmetic as edge_at, the same states bit for bit.
-template <typename T, int DIM, int BS, bool HK>
-__device__ __forceinline__ void
-edge_column_z(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], int src, bool edge, T dtdx)
-{
-  static_assert(DIM == 3, "edge_column_z: 3D edges");
-  using G = Geo<DIM, BS>;
-  constexpr int D0 = 0, D1 = 1, NH = 6;
-  EdgeRaw<T, D0, D1> RA, RC;
-  int                cc[3] = { c[0], c[1] - 1, 0 };
-  c[2] = 0;
-  edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, RA);
-  edge_raw_load<T, DIM, BS, D0, D1, HK, false>(V, cc, edge, RC);
-  for (int m = 0; m < BS; ++m)
-  {
-    T hA0[NH], h2[NH], h3[NH], h1[NH];
-    edge_part_of<T, 0, D0, D1>(RA, S.gt, hA0);
-    edge_part_of<T, 2, D0, D1>(RA, S.gt, h2);
-    edge_part_of<T, 3, D0, D1>(RC, S.gt, h3);
-    edge_part_of<T, 1, D0, D1>(RC, S.gt, h1);
-    const T bA1 = RA.b1, fA1 = RA.f1, bA0 = RA.b0, fA0 = RA.f0, bC0 = RC.b0, fC0 = RC.f0;
-    T       hAp[NH], hCp[NH];
-    for (int v = 0; v < NH; ++v)
-    {
-      hAp[v] = lane_take(h2[v], src);
-      hCp[v] = lane_take(h1[v], src);
-    }
-    const T bAp1 = lane_take(bA1, src), fAp1 = lane_take(fA1, src);
-    if (m + 1 < BS)
-    {
-      c[2] = m + 1;
-      cc[2] = m + 1;
-      edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, edge, RA);
-      edge_raw_load<T, DIM, BS, D0, D1, HK, false>(V, cc, edge, RC);
-    }
-    c[2] = m;
-    if (edge)
-    {
-      T qe[4][8];
-      edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
-      edge_corner<T, 3, D0, D1>(h3, bC0, fC0, bA1, fA1, qe[kLT]);
-      edge_corner<T, 2, D0, D1>(hAp, bA0, fA0, bAp1, fAp1, qe[kRB]);
-      edge_corner<T, 1, D0, D1>(hCp, bC0, fC0, bAp1, fAp1, qe[kRT]);
-      const T E = edge_emf_fast<2>(qe, S.gt) * dtdx;
-      S.emf[(long(o) * G::NCE + 2) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
-    }
-  }
-}
-
-// The x-edges of an (i, j) position, marching along z: each iteration reads the records of
-// A = (i, j, k) and of A' = A - e_y and forms KIND 0 at A and KIND 2 at A'; KIND 3 and 1 (at
-// C = A - e_z and C' = A' - e_z) and Bf_y at C are those formed at the plane before, carried in
-// registers.  No cross-lane reads: every lane holds an edge.  (With the next plane's reads in
-// flight the kernel spilled 77 VGPRs.)  Same arithmetic as edge_at, the same states bit for bit.
-template <typename T, int DIM, int BS, bool HK>
-__device__ __forceinline__ void
-edge_column_x(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], T dtdx)
-{
-  static_assert(DIM == 3, "edge_column_x: 3D edges");
-  using G = Geo<DIM, BS>;
-  constexpr int D0 = 1, D1 = 2, NH = 7;
-  int cp[3] = { c[0], c[1] - 1, -1 };
-  T   hC3[NH], hC1[NH], bC0 = T(0), fC0 = T(0);
+  constexpr int D0 = AX == 0 ? 1 : 0, D1 = 2, NH = AX == 0 ? 7 : 6;
+  // the reference's corner order (RT, RB, LT, LB) per axis: KIND 2 is RB for x-edges, LT for
+  // y-edges
+  constexpr int slot3 = AX == 1 ? kRB : kLT;
+  constexpr int slot2 = AX == 1 ? kLT : kRB;
+  int           cp[3] = { c[0] - (D0 == 0), c[1] - (D0 == 1), -1 };
+  T             hC3[NH], hC1[NH], bC0 = T(0), fC0 = T(0);
   for (int m = -1; m < G::NE; ++m)
   {
     EdgeRaw<T, D0, D1> RA, RP;
@@ -2205,11 +2081,11 @@ edge_column_x(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3
     {
       T qe[4][8];
       edge_corner<T, 0, D0, D1>(hA0, bA0, fA0, bA1, fA1, qe[kLB]);
-      edge_corner<T, 3, D0, D1>(hC3, bC0, fC0, bA1, fA1, qe[kLT]);
-      edge_corner<T, 2, D0, D1>(hA2, bA0, fA0, bP1, fP1, qe[kRB]);
+      edge_corner<T, 3, D0, D1>(hC3, bC0, fC0, bA1, fA1, qe[slot3]);
+      edge_corner<T, 2, D0, D1>(hA2, bA0, fA0, bP1, fP1, qe[slot2]);
       edge_corner<T, 1, D0, D1>(hC1, bC0, fC0, bP1, fP1, qe[kRT]);
-      const T E = edge_emf_fast<0>(qe, S.gt) * dtdx;
-      S.emf[(long(o) * G::NCE + 0) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
+      const T E = edge_emf_fast<AX>(qe, S.gt) * dtdx;
+      S.emf[(long(o) * G::NCE + AX) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
     }
     for (int v = 0; v < NH; ++v)
     {
@@ -2221,27 +2097,60 @@ edge_column_x(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3
   }
 }
 
+// The z-edges of an (i, j) position, marching along z beside the x- and y-edges (which march
+// along z too), so that the three directions read the records of the same planes close together
+// in time and share them in the L2.  The z-edge at (i, j, k) has its four cells in plane k: A =
+// (i, j, k) (KIND 0), C = A - e_y (KIND 3), A' = A - e_x (KIND 2) and C' = A' - e_y (KIND 1), with
+// Bf_x at A and C and Bf_y at A and A'; each iteration reads the four records, no cross-lane
+// reads.  Same arithmetic as edge_at, the same states bit for bit.
+template <typename T, int DIM, int BS, bool HK>
+__device__ __forceinline__ void
+edge_column_z(const FusedSlot<T> & S, const View<T, DIM, BS> & V, int o, int c[3], T dtdx)
+{
+  static_assert(DIM == 3, "edge_column_z: 3D edges");
+  using G = Geo<DIM, BS>;
+  constexpr int D0 = 0, D1 = 1, NH = 6;
+  for (int m = 0; m < BS; ++m)
+  {
+    EdgeRaw<T, D0, D1> RA, RC, RAp, RCp;
+    int                cC[3] = { c[0], c[1] - 1, m }, cAp[3] = { c[0] - 1, c[1], m };
+    int                cCp[3] = { c[0] - 1, c[1] - 1, m };
+    c[2] = m;
+    edge_raw_load<T, DIM, BS, D0, D1, HK>(V, c, true, RA);
+    edge_raw_load<T, DIM, BS, D0, D1, HK, false>(V, cC, true, RC);
+    edge_raw_load<T, DIM, BS, D0, D1, HK>(V, cAp, false, RAp);
+    edge_raw_load<T, DIM, BS, D0, D1, HK, false>(V, cCp, false, RCp);
+    T hA0[NH], hAp[NH], hC3[NH], hCp[NH];
+    edge_part_of<T, 0, D0, D1>(RA, S.gt, hA0);
+    edge_part_of<T, 2, D0, D1>(RAp, S.gt, hAp);
+    edge_part_of<T, 3, D0, D1>(RC, S.gt, hC3);
+    edge_part_of<T, 1, D0, D1>(RCp, S.gt, hCp);
+    T qe[4][8];
+    edge_corner<T, 0, D0, D1>(hA0, RA.b0, RA.f0, RA.b1, RA.f1, qe[kLB]);
+    edge_corner<T, 3, D0, D1>(hC3, RC.b0, RC.f0, RA.b1, RA.f1, qe[kLT]);
+    edge_corner<T, 2, D0, D1>(hAp, RA.b0, RA.f0, RAp.b1, RAp.f1, qe[kRB]);
+    edge_corner<T, 1, D0, D1>(hCp, RC.b0, RC.f0, RAp.b1, RAp.f1, qe[kRT]);
+    const T E = edge_emf_fast<2>(qe, S.gt) * dtdx;
+    S.emf[(long(o) * G::NCE + 2) * G::NEC + IdxC<DIM, BS>::edge(c[0], c[1], c[2])] = E;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
-// k_edge: every edge of a block once.  In 3D every wavefront marches along z, over a plane of
-// lanes in x and y:
-//   * y-edges (edge_march, D1 = z): positions over rows of b + 2 along x (i = -1 .. b) and the
-//     b rows along y, 63 per wavefront after the position before them; lane s - 1 holds the edge
-//     one step back along x, the lanes at i = -1 and lane 0 only pass their cell parts on;
-//   * x-edges (edge_march, D1 = z): tiles of 8 lanes along x by 8 positions along y (j = -1 ..
-//     b), 7 new after the last one of the tile before; lane s - 8 holds the edge one step back
-//     along y, the first row and the lanes at j = -1 only pass their cell parts on;
-//   * z-edges (edge_column_z): positions over rows of b + 2 along x and the b + 1 rows along y,
-//     as the y-edges.
-// Marching the z-edges along y instead (one cell read per edge, the previous row carried) read
-// 4.0 GB from HBM on the 16^3 class of MIX against 3.1 along z, where the three directions share
-// the planes in the L2 (BL3D k_edge 1.76 -> 1.73 ms; the 32^3 class, whose marches drift apart
-// by more planes than the L2 holds, is unchanged).  (16^3 blocks at 9 and at 17 planes per march
-// in one measurement: 1.86 and 1.80 ms.)  G::EWB wavefronts per block: x-edges first, then y,
-// then z, each by plane wavefront.  A marching lane holds about 230-256 registers (the next
-// cell's reads in flight while the EMF is formed), so the 3D kernels run 2 wavefronts per SIMD.
-// In 2D a plane of lanes would be one row of b + 2 positions, too short for a wavefront: the
-// edges run one per lane (edge_at), 4 wavefronts per SIMD.  A wavefront's direction is uniform
-// and compiled in.
+// k_edge: every edge of a block once.  In 3D every wavefront marches along z, one (i, j) position
+// per lane, i fastest: the x- and y-edges read two cells per plane and carry the plane before's
+// corner parts (edge_column), the z-edges read their four cells in each plane (edge_column_z).
+// The three directions march along z together and share each plane's records in the L2:
+// marching the z-edges along y instead (one cell read per edge, the previous row carried) read
+// 4.0 GB from HBM on the 16^3 class of MIX against 3.1 along z (BL3D k_edge 1.76 -> 1.73 ms; the
+// 32^3 class, whose marches drift apart by more planes than the L2 holds, was unchanged).
+// Lanes sharing the cells of the row before by cross-lane reads (8 x 8 tiles for the x-edges,
+// rows of b + 2 with a halo lane for the y- and z-edges) were slower: MIX k_edge 3.92 (x), 3.79
+// (y) and 3.76 (z) ms against 3.78, 3.74 and 3.67 with one edge per lane.  G::EWB wavefronts per
+// block: x-edges first, then y, then z, each by plane wavefront.  A lane holds up to 256
+// registers, so the 3D kernels run 2 wavefronts per SIMD (at 3, with 30 VGPRs spilled: MIX
+// k_edge 3.67 -> 3.83 ms).  In 2D a plane of lanes would be one
+// row of b + 1 positions, too short for a wavefront: the edges run one per lane (edge_at), 4
+// wavefronts per SIMD.  A wavefront's direction is uniform and compiled in.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
@@ -2273,58 +2182,35 @@ edge_item(const FusedSlot<T> & S, long r0)
     const int     pw = a == 0 ? w : (a == 1 ? w - G::NWX : w - G::NWX - G::NWP);
     if (a == 2)
     {
-      // z-edges: (i, j) over rows of b + 2 along x, b + 1 rows, marching along z
-      constexpr int e0 = BS + 2;
-      const int     p = pw * 63 - 1 + lane;
-      if (p < 0 || p >= G::NPLZ)
-        return;
-      int        c[3] = { p % e0 - 1, p / e0, 0 };
-      const int  src = (lane > 0 ? lane - 1 : 0) * 4;
-      const bool edge = lane > 0 && c[0] >= 0;
-      if (S.rk2)
-        edge_column_z<T, DIM, BS, false>(S, V, o, c, src, edge, dtdx);
-      else
-        edge_column_z<T, DIM, BS, true>(S, V, o, c, src, edge, dtdx);
-      return;
-    }
-    int           c[3], src;
-    bool          edge;
-    constexpr int e0 = BS + 2;
-    if (a == 0)
-    {
-      // x-edges: (i, j), b along x by b + 1 along y, one edge per lane
+      // z-edges: (i, j), b + 1 by b + 1, one edge per lane
       const int p = pw * 64 + lane;
-      if (p >= G::NPLX)
+      if (p >= G::NE * G::NE)
         return;
-      int cx[3] = { p % BS, p / BS, 0 };
+      int c[3] = { p % G::NE, p / G::NE, 0 };
       if (S.rk2)
-        edge_column_x<T, DIM, BS, false>(S, V, o, cx, dtdx);
+        edge_column_z<T, DIM, BS, false>(S, V, o, c, dtdx);
       else
-        edge_column_x<T, DIM, BS, true>(S, V, o, cx, dtdx);
+        edge_column_z<T, DIM, BS, true>(S, V, o, c, dtdx);
       return;
     }
-    else
-    {
-      // y-edges: (i, j) over rows of b + 2 along x, b rows
-      const int p = pw * 63 - 1 + lane;
-      if (p < 0 || p >= G::NPL)
-        return;
-      c[0] = p % e0 - 1;
-      c[1] = p / e0;
-      src = (lane > 0 ? lane - 1 : 0) * 4;
-      edge = lane > 0 && c[0] >= 0;
-    }
+    // x-edges: (i, j), b along x by b + 1 along y; y-edges: b + 1 along x by b along y; one edge
+    // per lane
+    const int p = pw * 64 + lane;
+    if (p >= (a == 0 ? G::NPLX : G::NPLY))
+      return;
+    const int nx = a == 0 ? BS : BS + 1;
+    int       c[3] = { p % nx, p / nx, 0 };
     if (S.rk2)
     {
-      if (a == 1)
-        edge_march<T, DIM, BS, 1, false>(S, V, o, c, src, edge, dtdx);
+      if (a == 0)
+        edge_column<T, DIM, BS, 0, false>(S, V, o, c, dtdx);
       else
-        edge_march<T, DIM, BS, 0, false>(S, V, o, c, src, edge, dtdx);
+        edge_column<T, DIM, BS, 1, false>(S, V, o, c, dtdx);
     }
-    else if (a == 1)
-      edge_march<T, DIM, BS, 1, true>(S, V, o, c, src, edge, dtdx);
+    else if (a == 0)
+      edge_column<T, DIM, BS, 0, true>(S, V, o, c, dtdx);
     else
-      edge_march<T, DIM, BS, 0, true>(S, V, o, c, src, edge, dtdx);
+      edge_column<T, DIM, BS, 1, true>(S, V, o, c, dtdx);
   }
 }
 

@@ -47,19 +47,18 @@ fused_cell_runs(int dim, int b, bool fp64)
   return (dim != 3 || !fp64) ? 0 : (b == 8 ? 2 : (b == 16 ? 6 : 17));
 }
 
-// Wavefronts of k_edge per block.  In 3D the edges march along z, each wavefront with its plane
-// of lanes: the y- and z-edges over rows of b + 2 positions (a halo column before each row of
-// b + 1 edges), 63 positions per wavefront after a halo lane; the x-edges in tiles of 8 x 8
-// lanes, 7 new rows of the b + 2 along y per tile.  In 2D the (b+1)^2 edges one per lane.
+// Wavefronts of k_edge per block.  In 3D the edges march along z, one (i, j) position per lane:
+// b (b + 1) x-edges, (b + 1) b y-edges and (b + 1)^2 z-edges per plane.  In 2D the (b+1)^2
+// edges one per lane.
 constexpr int
 fused_edge_waves(int dim, int b)
 {
   const int n1 = b + 1;
   if (dim == 2)
     return (n1 * n1 + 63) / 64;
-  const int wp = ((b + 2) * b + 62) / 63;
+  const int wp = (n1 * b + 63) / 64;
   const int wx = (b * n1 + 63) / 64;
-  const int wz = ((b + 2) * n1 + 62) / 63;
+  const int wz = (n1 * n1 + 63) / 64;
   return wx + wp + wz;
 }
 
