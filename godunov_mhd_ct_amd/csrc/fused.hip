@@ -101,7 +101,7 @@ struct Geo
   static constexpr int PRIM = NC + (GSH + 63) / 64 * 64;  // k_prim items: interior, then shell
   static constexpr int NP = BS + 3;                       // edge points per axis of the E grid
   static constexpr int NPE = DIM == 3 ? NP * NP * NP : NP * NP;
-  static constexpr int EFP = (NPE + 63) / 64 * 64;        // k_efield items
+  static constexpr int EFP = DIM == 3 ? (NP * NP + 63) / 64 * 64 : (NPE + 63) / 64 * 64; // k_efield items
   static constexpr int NB1 = BS / kTS;                    // flux tiles per axis
   static constexpr int TPB = DIM == 3 ? NB1 * NB1 * NB1 : NB1 * NB1;
   static_assert(BS % kTS == 0, "block size must be a multiple of the flux tile");
@@ -518,6 +518,20 @@ k_prim(const FusedSlot<T> * __restrict__ slots, int nslots, const long * __restr
   }
 }
 
+// value v (0-4 hydro, 5-7 low and 8-10 high faces of Bx, By, Bz) of cell (x, y, z) from src
+template <typename T, int DIM, class Src>
+__device__ __forceinline__ T
+cell_q_src(const Src & src, int v, int x, int y, int z)
+{
+  if (v < 5)
+    return src.qp(v, x, y, z);
+  const int fc = (v - 5) % 3;
+  const int hi = v >= 8;
+  if (DIM == 2 && fc == 2)
+    return src.bf(2, x, y, 0);
+  return src.bf(fc, x + (hi && fc == 0), y + (hi && fc == 1), z + (hi && fc == 2));
+}
+
 // ---------------------------------------------------------------------------------------------
 // k_efield: edge electric field v x B at t_n (ComputeElectricFieldFunctor.cpp:97-235) on the
 // edge points [-1, b+1]^dim of each block, from the primitive state of the 2-ghost tile.
@@ -562,6 +576,67 @@ efield_point(const Src & src, int a, int c, int e, T * out)
   }
 }
 
+// E at the edge points of an (a, c) column, marching along z (3D): each plane e reads the 18
+// values of plane e that the three components take (velocities at the four cells (a-1 .. a,
+// c-1 .. c), the faces) and carries the nine that E_x and E_y take again at e + 1 as plane e - 1.
+// The same sums in the same order as efield_point, bit for bit.
+template <typename T, int BS>
+__device__ __forceinline__ void
+efield_column(const FusedSlot<T> & S, int o, int a, int c)
+{
+  using G = Geo<3, BS>;
+  const View<T, 3, BS>      V(S, o);
+  const GlobalSrc<T, 3, BS> src(V, S, o);
+  auto                      Q = [&](int v, int x, int y, int z) -> T { return cell_q_src<T, 3>(src, v, x, y, z); };
+  // plane e - 1: v(a, c-1), v(a, c), w(a, c-1), w(a, c), w(a-1, c), u(a-1, c), u(a, c), By(a, c),
+  // Bx(a, c)
+  int e = -1;
+  T   pv0 = Q(3, a, c - 1, e - 1), pv1 = Q(3, a, c, e - 1), pw0 = Q(4, a, c - 1, e - 1), pw1 = Q(4, a, c, e - 1);
+  T   pw2 = Q(4, a - 1, c, e - 1), pu0 = Q(2, a - 1, c, e - 1), pu1 = Q(2, a, c, e - 1);
+  T   pB = Q(6, a, c, e - 1), pA = Q(5, a, c, e - 1);
+  T * Ed = S.Ed + long(o) * G::NCE * G::NPE + (a + 1) + G::NP * (c + 1);
+  for (; e <= BS + 1; ++e)
+  {
+    const T v0 = Q(3, a, c - 1, e), v1 = Q(3, a, c, e), w0 = Q(4, a, c - 1, e), w1 = Q(4, a, c, e);
+    const T w2 = Q(4, a - 1, c, e), u0 = Q(2, a - 1, c, e), u1 = Q(2, a, c, e);
+    const T Bc = Q(6, a, c, e), Ac = Q(5, a, c, e);
+    T       out[3];
+    {
+      const T vv = T(0.25) * (pv0 + v0 + pv1 + v1);
+      const T ww = T(0.25) * (pw0 + w0 + pw1 + w1);
+      const T BB = T(0.5) * (pB + Bc);
+      const T CC = T(0.5) * (Q(7, a, c - 1, e) + Q(7, a, c, e));
+      out[0] = vv * CC - ww * BB;
+    }
+    {
+      const T uu = T(0.25) * (pu0 + u0 + pu1 + u1);
+      const T ww = T(0.25) * (pw2 + w2 + pw1 + w1);
+      const T AA = T(0.5) * (pA + Ac);
+      const T CC = T(0.5) * (Q(7, a - 1, c, e) + Q(7, a, c, e));
+      out[1] = ww * AA - uu * CC;
+    }
+    {
+      const T uu = T(0.25) * (Q(2, a - 1, c - 1, e) + u0 + Q(2, a, c - 1, e) + u1);
+      const T vv = T(0.25) * (Q(3, a - 1, c - 1, e) + Q(3, a - 1, c, e) + v0 + v1);
+      const T AA = T(0.5) * (Q(5, a, c - 1, e) + Ac);
+      const T BB = T(0.5) * (Q(6, a - 1, c, e) + Bc);
+      out[2] = uu * BB - vv * AA;
+    }
+    const int pe = G::NP * G::NP * (e + 1);
+    for (int comp = 0; comp < 3; ++comp)
+      store_nt(Ed + comp * G::NPE + pe, out[comp]);
+    pv0 = v0;
+    pv1 = v1;
+    pw0 = w0;
+    pw1 = w1;
+    pw2 = w2;
+    pu0 = u0;
+    pu1 = u1;
+    pB = Bc;
+    pA = Ac;
+  }
+}
+
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
 efield_item(const FusedSlot<T> & S, long r0)
@@ -569,6 +644,14 @@ efield_item(const FusedSlot<T> & S, long r0)
   using G = Geo<DIM, BS>;
   const int o = __builtin_amdgcn_readfirstlane(int(r0) / G::EFP);
   const int r = int(r0) - o * G::EFP;
+  if constexpr (DIM == 3)
+  {
+    // one (a, c) column of edge points per lane
+    if (r >= G::NP * G::NP)
+      return;
+    efield_column<T, BS>(S, o, r % G::NP - 1, r / G::NP - 1);
+    return;
+  }
   if (r >= G::NPE)
     return;
   constexpr int          np = G::NP;
@@ -2701,7 +2784,7 @@ fused_items_per_block(int dim, int b, int stage, bool fp64)
   switch (stage)
   {
   case 0: return nc + (g2 - nc + 63) / 64 * 64;
-  case 6: return (npe + 63) / 64 * 64;
+  case 6: return dim == 3 ? (np * np + 63) / 64 * 64 : (npe + 63) / 64 * 64;
   case 8: return nc;
   case 1: return fused_cell_runs(dim, b, fp64) > 0 ? (long(b + 2) * (b + 2) * fused_cell_runs(dim, b, fp64) + 63) / 64 * 64
                                                  : (g1 + 63) / 64 * 64;
