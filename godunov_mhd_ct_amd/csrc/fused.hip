@@ -294,7 +294,7 @@ dxd(const DevMesh & m, double scaling, int o)
 // gf(c, a0, a1, a2) is the ghosted face value face_value_ghosted gives.
 template <typename T, int DIM, int BS>
 __device__ __forceinline__ void
-inside_state(const FusedSlot<T> & S, int o, int i, int j, int k, double * u)
+inside_state(const FusedSlot<T> & S, int o, int i, int j, int k, double * u, double * lo)
 {
   using IX = IdxC<DIM, BS>;
   constexpr int  ncell = IX::cells();
@@ -302,17 +302,21 @@ inside_state(const FusedSlot<T> & S, int o, int i, int j, int k, double * u)
   const double * Bo = S.B + long(o) * IX::fper();
   for (int v = 0; v < 5; ++v)
     u[v] = S.U[(long(o) * 5 + v) * ncell + cidx];
+  // the low faces (also the cell's ghosted face values: an interior cell's faces are its own)
+  lo[0] = Bo[IX::face(0, i, j, DIM == 3 ? k : 0)];
+  lo[1] = Bo[IX::face(1, i, j, DIM == 3 ? k : 0)];
+  lo[2] = Bo[IX::face(2, i, j, DIM == 3 ? k : 0)];
   if (DIM == 2)
   {
-    u[kA] = 0.5 * (Bo[IX::face(0, i, j, 0)] + Bo[IX::face(0, i + 1, j, 0)]);
-    u[kB] = 0.5 * (Bo[IX::face(1, i, j, 0)] + Bo[IX::face(1, i, j + 1, 0)]);
-    u[kC] = 0.5 * (Bo[IX::face(2, i, j, 0)] + Bo[IX::face(2, i, j, 0)]);
+    u[kA] = 0.5 * (lo[0] + Bo[IX::face(0, i + 1, j, 0)]);
+    u[kB] = 0.5 * (lo[1] + Bo[IX::face(1, i, j + 1, 0)]);
+    u[kC] = 0.5 * (lo[2] + lo[2]);
   }
   else
   {
-    u[kA] = 0.5 * (Bo[IX::face(0, i, j, k)] + Bo[IX::face(0, i + 1, j, k)]);
-    u[kB] = 0.5 * (Bo[IX::face(1, i, j, k)] + Bo[IX::face(1, i, j + 1, k)]);
-    u[kC] = 0.5 * (Bo[IX::face(2, i, j, k)] + Bo[IX::face(2, i, j, k + 1)]);
+    u[kA] = 0.5 * (lo[0] + Bo[IX::face(0, i + 1, j, k)]);
+    u[kB] = 0.5 * (lo[1] + Bo[IX::face(1, i, j + 1, k)]);
+    u[kC] = 0.5 * (lo[2] + Bo[IX::face(2, i, j, k + 1)]);
   }
 }
 
@@ -436,9 +440,9 @@ prim_item(const FusedSlot<T> & S, long r0)
       const int f[3] = { a0, a1, a2 };
       return face_value_ghosted(m, I, B, o, c, f);
     };
-    double    u[8];
+    double    u[8], lo[3] = { 0.0, 0.0, 0.0 };
     if (inside)
-      inside_state<T, DIM, BS>(S, o, i, j, k, u);
+      inside_state<T, DIM, BS>(S, o, i, j, k, u, lo);
     else
       ghost_state<T, DIM, BS>(S, o, i, j, k, u, gfv);
     double       q[8];
@@ -459,6 +463,13 @@ prim_item(const FusedSlot<T> & S, long r0)
         store_nt(Go + IX::gface(0, b + 2, j, 0), T(gfv(0, b + 2, j, 0)));
       if (j == b + 1)
         store_nt(Go + IX::gface(1, i, b + 2, 0), T(gfv(1, i, b + 2, 0)));
+    }
+    else if (inside)
+    {
+      // an interior cell: its low faces as read, no high faces on the last ghost layer
+      store_nt(Go + IX::gface(0, i, j, k), T(lo[0]));
+      store_nt(Go + IX::gface(1, i, j, k), T(lo[1]));
+      store_nt(Go + IX::gface(2, i, j, k), T(lo[2]));
     }
     else
     {
