@@ -241,9 +241,13 @@ def bench_batch(args, cases, impl):
     bt.step(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    stage_ms = bt.step_timed(args.steps)       # synchronises at its end
+    bt.step(args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    bt.check()
+    # the per-stage HIP-event times from as many steps again, outside the timed region (the events
+    # between the stages cost about 0.1 ms per step)
+    stage_ms = bt.step_timed(args.steps)       # synchronises at its end
     bt.check()
     # the same steps replayed from HIP graphs
     bt.set_graph(True)
